@@ -1,0 +1,207 @@
+"""eating_one_amd — MI355X-native inner joins (host mirror of the C ABI).
+
+The reference drives its joins from per-program ``join()`` functions over
+global tables (joins/hashJoin_gpu/main_street/hj.cc:171, joins/blockJoin_gpu/
+main_street/nestLoopJoin.cc:180, joins/innerJoin_gpu/*/inner-idx-bnlj.cc:183).
+Here :class:`Context` exposes the same operators over explicit arrays:
+R (build side, reference "right") and S (probe side, reference "left") in,
+(R payload, S payload) pairs out.  Every call goes through libeojoin.so;
+there is no CPU fallback — a missing library raises ImportError.
+
+Tuples are int32 arrays of shape (n, 2): column 0 the join key (reference
+TUPLE.val), column 1 the payload / row id (reference TUPLE.key).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import EO_DEVICE, EO_HOST, JoinOpts, JoinStats
+
+__all__ = ["Context", "DeviceArray", "EOError", "CapacityError", "load_library"]
+
+
+class EOError(RuntimeError):
+    def __init__(self, status: int, where: str, message: str = ""):
+        self.status = status
+        L = _abi.load()
+        text = L.eo_status_string(status).decode()
+        super().__init__(f"{where}: {text} (status {status}){': ' + message if message else ''}")
+
+
+class CapacityError(EOError):
+    def __init__(self, status, where, count):
+        super().__init__(status, where, f"{count} matches")
+        self.count = count
+
+
+def load_library():
+    return _abi.load()
+
+
+def _check(st: int, where: str):
+    if st != _abi.EO_OK:
+        msg = _abi.load().eo_last_error().decode(errors="replace")
+        raise EOError(st, where, msg)
+
+
+def _tuples(a) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    if a.ndim != 2 or a.shape[1] != 2:
+        raise ValueError("tuples must be an int32 array of shape (n, 2)")
+    return a
+
+
+def _vp(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceArray:
+    """A device allocation made through eo_dev_alloc (freed on close/GC)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(ctx.lib.eo_dev_alloc(ctx.handle, self.nbytes, ctypes.byref(p)), "eo_dev_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx: "Context", a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.nbytes)
+        if a.nbytes:
+            _check(ctx.lib.eo_copy_h2d(ctx.handle, d.ptr, _vp(a), a.nbytes), "eo_copy_h2d")
+        return d
+
+    def to_host(self, dtype, shape, nbytes=None) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        n = out.nbytes if nbytes is None else nbytes
+        if n:
+            _check(self.ctx.lib.eo_copy_d2h(self.ctx.handle, _vp(out), self.ptr, n), "eo_copy_d2h")
+        return out
+
+    def close(self):
+        if self.ptr is not None and self.ctx.handle:
+            self.ctx.lib.eo_dev_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU context (eo_ctx): stream, scratch arena, device queries."""
+
+    def __init__(self, device: int = 0, arena_bytes: int = 0, hbm_budget: int = 0):
+        self.lib = _abi.load()
+        h = ctypes.c_void_p()
+        o = _abi.CtxOpts(ctypes.sizeof(_abi.CtxOpts), 0, arena_bytes, hbm_budget)
+        _check(self.lib.eo_ctx_create(ctypes.byref(h), device, ctypes.byref(o)), "eo_ctx_create")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            self.lib.eo_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(self.lib.eo_ctx_sync(self.handle), "eo_ctx_sync")
+
+    def device_info(self) -> dict:
+        info = _abi.DeviceInfo()
+        _check(self.lib.eo_device_query(self.handle, ctypes.byref(info)), "eo_device_query")
+        return {
+            "name": info.name.decode(), "arch": info.arch.decode(), "cus": info.cus, "xcds": info.xcds,
+            "wave_size": info.wave_size, "clock_khz": info.clock_khz, "lds_per_wg": info.lds_per_wg,
+            "hbm_bytes": info.hbm_bytes, "l2_bytes": info.l2_bytes,
+        }
+
+    # ---- options -----------------------------------------------------------
+    @staticmethod
+    def opts(where=EO_HOST, radix_bits=0, passes=0, lds_tuples=0, s_tile=0, stats: JoinStats | None = None):
+        o = JoinOpts()
+        o.struct_size = ctypes.sizeof(JoinOpts)
+        o.where = where
+        o.radix_bits = radix_bits
+        o.passes = passes
+        o.lds_tuples = lds_tuples
+        o.s_tile = s_tile
+        o.stats = ctypes.pointer(stats) if stats is not None else None
+        return o
+
+    # ---- primitives --------------------------------------------------------
+    def exclusive_scan(self, a: np.ndarray):
+        """presum(): exclusive prefix sum (uint32 or uint64), returns (out, total)."""
+        a = np.ascontiguousarray(a)
+        out = np.empty_like(a)
+        tot = ctypes.c_uint64()
+        fn = {np.dtype(np.uint32): self.lib.eo_exclusive_scan_u32,
+              np.dtype(np.uint64): self.lib.eo_exclusive_scan_u64}.get(a.dtype)
+        if fn is None:
+            raise TypeError("uint32 or uint64 array expected")
+        o = self.opts()
+        _check(fn(self.handle, _vp(a), _vp(out), len(a), ctypes.byref(tot), ctypes.byref(o)), "exclusive_scan")
+        return out, tot.value
+
+    def radix_partition(self, T: np.ndarray, bits: int, shift: int = 0):
+        """Stable radix partition by ((uint32)key >> shift) & (2^bits-1)."""
+        T = _tuples(T)
+        out = np.empty_like(T)
+        off = np.empty((1 << bits) + 1, dtype=np.uint64)
+        o = self.opts()
+        _check(self.lib.eo_radix_partition(self.handle, _vp(T), len(T), bits, shift, _vp(out), _vp(off),
+                                           ctypes.byref(o)), "eo_radix_partition")
+        return out, off
+
+    # ---- joins ---------------------------------------------------------------
+    def _join(self, fn, name, R, S, count_only, o):
+        R, S = _tuples(R), _tuples(S)
+        n = ctypes.c_uint64()
+        _check(fn(self.handle, _vp(R), len(R), _vp(S), len(S), None, 0, ctypes.byref(n), ctypes.byref(o)), name)
+        if count_only:
+            return n.value
+        out = np.empty((max(n.value, 1), 2), dtype=np.int32)
+        n2 = ctypes.c_uint64()
+        st = fn(self.handle, _vp(R), len(R), _vp(S), len(S), _vp(out), n.value, ctypes.byref(n2), ctypes.byref(o))
+        if st == _abi.EO_ECAPACITY:
+            raise CapacityError(st, name, n2.value)
+        _check(st, name)
+        assert n2.value == n.value
+        return out[: n.value]
+
+    def hash_join(self, R, S, count_only=False, **kw):
+        """Radix hash join (reference hashJoin_gpu join())."""
+        return self._join(self.lib.eo_hash_join, "eo_hash_join", R, S, count_only, self.opts(**kw))
+
+    def nlj_join(self, R, S, count_only=False, **kw):
+        """Block nested-loop join (reference blockJoin_gpu join())."""
+        return self._join(self.lib.eo_nlj_join, "eo_nlj_join", R, S, count_only, self.opts(**kw))
+
+    def hash_join_device(self, dR: DeviceArray, nR: int, dS: DeviceArray, nS: int, dout: DeviceArray | None,
+                         out_cap: int, **kw):
+        """Device-resident hash join; returns (n_out, status)."""
+        n = ctypes.c_uint64()
+        o = self.opts(where=EO_DEVICE, **kw)
+        st = self.lib.eo_hash_join(self.handle, dR.ptr, nR, dS.ptr, nS, dout.ptr if dout else None, out_cap,
+                                   ctypes.byref(n), ctypes.byref(o))
+        if st not in (_abi.EO_OK, _abi.EO_ECAPACITY):
+            _check(st, "eo_hash_join")
+        return n.value, st

@@ -1,0 +1,114 @@
+"""ctypes declarations of include/eojoin.h (libeojoin.so)."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libeojoin.so")
+
+P = ctypes.c_void_p
+U64 = ctypes.c_uint64
+U32 = ctypes.c_uint32
+I32 = ctypes.c_int32
+INT = ctypes.c_int
+PU64 = ctypes.POINTER(ctypes.c_uint64)
+
+EO_OK, EO_EINVAL, EO_ENOMEM, EO_ECAPACITY, EO_EHIP, EO_EBUDGET = 0, 1, 2, 3, 4, 5
+EO_ENODEV, EO_EAGAIN, EO_ECANCELED, EO_ETIMEDOUT, EO_EINTERNAL = 6, 7, 8, 9, 10
+EO_HOST, EO_DEVICE = 0, 1
+
+
+class JoinStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double),
+        ("ms_partition", ctypes.c_double),
+        ("ms_count", ctypes.c_double),
+        ("ms_write", ctypes.c_double),
+        ("ms_build", ctypes.c_double),
+        ("n_out", U64),
+        ("radix_bits", U32),
+        ("passes", U32),
+        ("tiles", U32),
+        ("lds_tuples", U32),
+    ]
+
+
+class JoinOpts(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", U32),
+        ("where", I32),
+        ("radix_bits", I32),
+        ("passes", I32),
+        ("lds_tuples", I32),
+        ("s_tile", I32),
+        ("flags", I32),
+        ("reserved", I32),
+        ("stream", P),
+        ("stats", ctypes.POINTER(JoinStats)),
+    ]
+
+
+class CtxOpts(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", U32),
+        ("flags", U32),
+        ("arena_bytes", U64),
+        ("hbm_budget", U64),
+    ]
+
+
+class DeviceInfo(ctypes.Structure):
+    _fields_ = [
+        ("name", ctypes.c_char * 64),
+        ("arch", ctypes.c_char * 32),
+        ("cus", I32),
+        ("xcds", I32),
+        ("wave_size", I32),
+        ("clock_khz", I32),
+        ("lds_per_wg", U64),
+        ("hbm_bytes", U64),
+        ("l2_bytes", U64),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "eo_abi_version": (U32, []),
+    "eo_status_string": (ctypes.c_char_p, [INT]),
+    "eo_last_error": (ctypes.c_char_p, []),
+    "eo_ctx_create": (INT, [ctypes.POINTER(P), INT, ctypes.POINTER(CtxOpts)]),
+    "eo_ctx_destroy": (INT, [P]),
+    "eo_ctx_sync": (INT, [P]),
+    "eo_device_query": (INT, [P, ctypes.POINTER(DeviceInfo)]),
+    "eo_dev_alloc": (INT, [P, U64, ctypes.POINTER(P)]),
+    "eo_dev_free": (INT, [P, P]),
+    "eo_copy_h2d": (INT, [P, P, P, U64]),
+    "eo_copy_d2h": (INT, [P, P, P, U64]),
+    "eo_copy_d2d": (INT, [P, P, P, U64]),
+    "eo_exclusive_scan_u32": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_exclusive_scan_u64": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),
+    "eo_hash_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_nlj_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libeojoin.so; raises if it is missing (there is no fallback path)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m eating_one_amd.build` "
+            "(the HIP library is required; there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L

@@ -1,0 +1,574 @@
+// C ABI of libeojoin: context management, device memory helpers, primitives
+// and the hash / nested-loop joins.  Orchestration mirrors the reference's
+// join() (joins/hashJoin_gpu/main_street/hj.cc:171-1269): partition S and R,
+// plan the probe, count, scan, write — with the per-launch synchronisation,
+// per-call allocation and host chunking loops of the reference replaced by
+// device-side planning and one arena reservation per call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "ctx.hpp"
+#include "device.hpp"
+#include "eojoin.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace eo {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+eo_status hip_fail(hipError_t e, const char *what) {
+  set_last_error(std::string(what) + ": " + hipGetErrorString(e));
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return EO_ENOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return EO_ENODEV;
+  return EO_EHIP;
+}
+
+hipError_t Arena::reserve(size_t bytes) {
+  if (bytes <= cap) return hipSuccess;
+  release();
+  size_t want = std::max<size_t>(bytes, (size_t)64 << 20);
+  want = (want + (1u << 21) - 1) & ~(size_t)((1u << 21) - 1);
+  hipError_t e = hipMalloc((void **)&base, want);
+  if (e != hipSuccess) {
+    base = nullptr;
+    cap = 0;
+    return e;
+  }
+  cap = want;
+  return hipSuccess;
+}
+
+void Arena::release() {
+  if (base) (void)hipFree(base);
+  base = nullptr;
+  cap = 0;
+}
+
+eo_status read_u64(eo_ctx *ctx, const uint64_t *dev, uint64_t *host, int n, hipStream_t stream) {
+  EO_TRY_HIP(hipMemcpyAsync(ctx->h_pinned, dev, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream), "readback");
+  EO_TRY_HIP(hipStreamSynchronize(stream), "readback sync");
+  std::memcpy(host, ctx->h_pinned, sizeof(uint64_t) * n);
+  return EO_OK;
+}
+
+eo_status check_device_errors(eo_ctx *ctx, hipStream_t stream) {
+  EO_TRY_HIP(hipMemcpyAsync(ctx->h_pinned + 63, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+             "error flag readback");
+  EO_TRY_HIP(hipStreamSynchronize(stream), "sync");
+  const uint32_t flags = (uint32_t)ctx->h_pinned[63];
+  if (flags) {
+    EO_TRY_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), stream), "error flag reset");
+    set_last_error("device error flags 0x" + std::to_string(flags));
+    return EO_EINTERNAL;
+  }
+  return EO_OK;
+}
+
+namespace {
+
+bool opts_ok(const eo_join_opts *o) {
+  if (!o) return true;
+  if (o->struct_size != 0 && o->struct_size < sizeof(eo_join_opts)) return false;
+  return o->where == EO_HOST || o->where == EO_DEVICE;
+}
+
+constexpr uint64_t kMaxRows = (1ull << 31) - 1;
+
+// Partition `n` tuples into `out` (LSD over 1 or 2 passes) and write the
+// partition starts (2^bits + 1 u32).  tmp is needed for 2 passes.
+eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
+                             eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s) {
+  if (hp.passes == 1) {
+    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, ctx->props.cus);
+    EO_TRY_HIP(partition_pass(pp, in, out, part_start, scratch, ctx->d_err, s), "partition pass");
+    return EO_OK;
+  }
+  PartPassPlan p1 = plan_partition_pass(n, hp.bits_pass1, 0, 0u, ctx->props.cus);
+  PartPassPlan p2 = plan_partition_pass(n, hp.bits_pass2, hp.bits_pass1, 0u, ctx->props.cus);
+  EO_TRY_HIP(partition_pass(p1, in, tmp, nullptr, scratch, ctx->d_err, s), "partition pass 1");
+  EO_TRY_HIP(partition_pass(p2, tmp, out, nullptr, scratch, ctx->d_err, s), "partition pass 2");
+  EO_TRY_HIP(partition_bounds(out, n, hp.bits, 0, 0u, part_start, s), "partition bounds");
+  return EO_OK;
+}
+
+size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus) {
+  size_t b = 0;
+  if (hp.passes >= 1)
+    b = std::max(b, partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus)));
+  if (hp.passes == 2)
+    b = std::max(b, partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass2, hp.bits_pass1, 0u, cus)));
+  return b;
+}
+
+}  // namespace
+}  // namespace eo
+
+using namespace eo;
+
+extern "C" {
+
+uint32_t eo_abi_version(void) { return EOJOIN_ABI_VERSION; }
+
+const char *eo_status_string(eo_status st) {
+  switch (st) {
+    case EO_OK: return "ok";
+    case EO_EINVAL: return "invalid argument";
+    case EO_ENOMEM: return "out of memory";
+    case EO_ECAPACITY: return "output capacity too small";
+    case EO_EHIP: return "HIP runtime error";
+    case EO_EBUDGET: return "HBM budget exceeded";
+    case EO_ENODEV: return "no usable GPU";
+    case EO_EAGAIN: return "not ready";
+    case EO_ECANCELED: return "cancelled";
+    case EO_ETIMEDOUT: return "deadline expired";
+    case EO_EINTERNAL: return "internal error";
+  }
+  return "unknown status";
+}
+
+const char *eo_last_error(void) { return g_last_error.c_str(); }
+
+eo_status eo_ctx_create(eo_ctx **out, int device, const eo_ctx_opts *opts) {
+  if (!out) return EO_EINVAL;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    set_last_error("no HIP device visible");
+    return EO_ENODEV;
+  }
+  if (device < 0 || device >= ndev) return EO_EINVAL;
+  DeviceGuard g(device);
+  eo_ctx *c = new eo_ctx();
+  c->device = device;
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, device) != hipSuccess) {
+    delete c;
+    return EO_ENODEV;
+  }
+  c->props.name = pr.name;
+  c->props.arch = pr.gcnArchName;
+  c->props.cus = pr.multiProcessorCount;
+  c->props.clock_khz = pr.clockRate;
+  c->props.lds_per_wg = pr.sharedMemPerBlock;
+  c->props.hbm = pr.totalGlobalMem;
+  c->props.l2 = pr.l2CacheSize;
+  c->hbm_budget = (opts && opts->hbm_budget) ? opts->hbm_budget : 200ull * 1000 * 1000 * 1000;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->d_err, 256);
+  if (e == hipSuccess) e = hipMemset(c->d_err, 0, 256);
+  if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(uint64_t), hipHostMallocDefault);
+  for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+  if (e == hipSuccess && opts && opts->arena_bytes) e = c->arena.reserve(opts->arena_bytes);
+  if (e != hipSuccess) {
+    eo_status st = hip_fail(e, "eo_ctx_create");
+    eo_ctx_destroy(c);
+    return st;
+  }
+  *out = c;
+  return EO_OK;
+}
+
+eo_status eo_ctx_destroy(eo_ctx *c) {
+  if (!c) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->arena.release();
+  for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (c->d_err) (void)hipFree(c->d_err);
+  if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return EO_OK;
+}
+
+eo_status eo_ctx_sync(eo_ctx *c) {
+  if (!c) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  EO_TRY_HIP(hipStreamSynchronize(c->stream), "eo_ctx_sync");
+  return EO_OK;
+}
+
+eo_status eo_device_query(eo_ctx *c, eo_device_info *info) {
+  if (!c || !info) return EO_EINVAL;
+  std::memset(info, 0, sizeof(*info));
+  std::strncpy(info->name, c->props.name.c_str(), sizeof(info->name) - 1);
+  std::strncpy(info->arch, c->props.arch.c_str(), sizeof(info->arch) - 1);
+  info->cus = c->props.cus;
+  info->xcds = kXcds;
+  info->wave_size = kWave;
+  info->clock_khz = c->props.clock_khz;
+  info->lds_per_wg = c->props.lds_per_wg;
+  info->hbm_bytes = c->props.hbm;
+  info->l2_bytes = c->props.l2;
+  return EO_OK;
+}
+
+eo_status eo_dev_alloc(eo_ctx *c, uint64_t bytes, void **dptr) {
+  if (!c || !dptr) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  *dptr = nullptr;
+  EO_TRY_HIP(hipMalloc(dptr, bytes ? bytes : 1), "eo_dev_alloc");
+  return EO_OK;
+}
+
+eo_status eo_dev_free(eo_ctx *c, void *dptr) {
+  if (!c) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  if (dptr) EO_TRY_HIP(hipFree(dptr), "eo_dev_free");
+  return EO_OK;
+}
+
+static eo_status copy_sync(eo_ctx *c, void *dst, const void *src, uint64_t bytes, hipMemcpyKind kind) {
+  if (!c || (bytes && (!dst || !src))) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  if (!bytes) return EO_OK;
+  EO_TRY_HIP(hipMemcpyAsync(dst, src, bytes, kind, c->stream), "copy");
+  EO_TRY_HIP(hipStreamSynchronize(c->stream), "copy sync");
+  return EO_OK;
+}
+
+eo_status eo_copy_h2d(eo_ctx *c, void *dst, const void *src, uint64_t bytes) {
+  return copy_sync(c, dst, src, bytes, hipMemcpyHostToDevice);
+}
+eo_status eo_copy_d2h(eo_ctx *c, void *dst, const void *src, uint64_t bytes) {
+  return copy_sync(c, dst, src, bytes, hipMemcpyDeviceToHost);
+}
+eo_status eo_copy_d2d(eo_ctx *c, void *dst, const void *src, uint64_t bytes) {
+  return copy_sync(c, dst, src, bytes, hipMemcpyDeviceToDevice);
+}
+
+// ---- primitives ---------------------------------------------------------------
+
+}  // extern "C"
+
+template <typename T>
+static eo_status scan_entry(eo_ctx *c, const T *in, T *out, uint64_t n, uint64_t *total, const eo_join_opts *o) {
+  if (!c || !opts_ok(o) || (n && (!in || !out))) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  const bool host = where_of(o) == EO_HOST;
+  const size_t sb = sizeof(T) == 4 ? scan_scratch_bytes_u32(n) : scan_scratch_bytes_u64(n);
+  Layout L;
+  const size_t o_data = L.add(host ? n * sizeof(T) : 0);
+  const size_t o_tot = L.add(sizeof(uint64_t));
+  const size_t o_scr = L.add(sb);
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+  T *dbuf = host ? (T *)(c->arena.base + o_data) : nullptr;
+  uint64_t *dtot = (uint64_t *)(c->arena.base + o_tot);
+  void *scr = c->arena.base + o_scr;
+  const T *din = in;
+  T *dout = out;
+  if (host) {
+    if (n) EO_TRY_HIP(hipMemcpyAsync(dbuf, in, n * sizeof(T), hipMemcpyHostToDevice, s), "h2d");
+    din = dbuf;
+    dout = dbuf;
+  }
+  hipError_t e = sizeof(T) == 4
+                     ? scan_u32((const uint32_t *)din, (uint32_t *)dout, n, dtot, scr, c->d_err, s)
+                     : scan_u64((const uint64_t *)din, (uint64_t *)dout, n, dtot, scr, c->d_err, s);
+  EO_TRY_HIP(e, "scan");
+  if (host && n) EO_TRY_HIP(hipMemcpyAsync(out, dbuf, n * sizeof(T), hipMemcpyDeviceToHost, s), "d2h");
+  uint64_t tot = 0;
+  eo_status st = read_u64(c, dtot, &tot, 1, s);
+  if (st != EO_OK) return st;
+  st = check_device_errors(c, s);
+  if (st != EO_OK) return st;
+  if (total) *total = tot;
+  return EO_OK;
+}
+
+extern "C" eo_status eo_exclusive_scan_u32(eo_ctx *c, const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *total,
+                                const eo_join_opts *o) {
+  return scan_entry<uint32_t>(c, in, out, n, total, o);
+}
+extern "C" eo_status eo_exclusive_scan_u64(eo_ctx *c, const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total,
+                                const eo_join_opts *o) {
+  return scan_entry<uint64_t>(c, in, out, n, total, o);
+}
+
+__global__ void k_widen(const uint32_t *in, uint64_t *out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+extern "C" {
+
+eo_status eo_radix_partition(eo_ctx *c, const eo_tuple *in, uint64_t n, int bits, int shift, eo_tuple *out,
+                             uint64_t *part_off, const eo_join_opts *o) {
+  if (!c || !opts_ok(o) || bits < 1 || bits > 16 || shift < 0 || shift + bits > 32 || n > kMaxRows)
+    return EO_EINVAL;
+  if (n && (!in || !out)) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  const bool host = where_of(o) == EO_HOST;
+  const int b1 = bits <= kMaxPassBits ? bits : bits / 2;
+  const int b2 = bits - b1;
+  const PartPassPlan p1 = plan_partition_pass(n, b1, shift, 0u, c->props.cus);
+  const PartPassPlan p2 = plan_partition_pass(n, b2 ? b2 : 1, shift + b1, 0u, c->props.cus);
+  const uint32_t F = 1u << bits;
+  Layout L;
+  const size_t o_in = L.add(host ? n * sizeof(eo_tuple) : 0);
+  const size_t o_out = L.add(host ? n * sizeof(eo_tuple) : 0);
+  const size_t o_tmp = L.add(b2 ? n * sizeof(eo_tuple) : 0);
+  const size_t o_ps = L.add((F + 1) * sizeof(uint32_t));
+  const size_t o_po = L.add((F + 1) * sizeof(uint64_t));
+  const size_t o_scr =
+      L.add(std::max(partition_pass_scratch_bytes(p1), b2 ? partition_pass_scratch_bytes(p2) : (size_t)0));
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+  char *A = c->arena.base;
+  const eo_tuple *din = host ? (eo_tuple *)(A + o_in) : in;
+  eo_tuple *dout = host ? (eo_tuple *)(A + o_out) : out;
+  eo_tuple *dtmp = (eo_tuple *)(A + o_tmp);
+  uint32_t *ps = (uint32_t *)(A + o_ps);
+  uint64_t *po = (uint64_t *)(A + o_po);
+  if (host && n) EO_TRY_HIP(hipMemcpyAsync((void *)din, in, n * sizeof(eo_tuple), hipMemcpyHostToDevice, s), "h2d");
+  if (n == 0) {
+    EO_TRY_HIP(fill_u32(ps, 0u, F + 1, s), "fill");
+  } else if (!b2) {
+    EO_TRY_HIP(partition_pass(p1, din, dout, ps, A + o_scr, c->d_err, s), "partition");
+  } else {
+    EO_TRY_HIP(partition_pass(p1, din, dtmp, nullptr, A + o_scr, c->d_err, s), "partition pass 1");
+    EO_TRY_HIP(partition_pass(p2, dtmp, dout, nullptr, A + o_scr, c->d_err, s), "partition pass 2");
+    EO_TRY_HIP(partition_bounds(dout, n, bits, shift, 0u, ps, s), "partition bounds");
+  }
+  hipLaunchKernelGGL(k_widen, dim3((F + 1 + 255) / 256), dim3(256), 0, s, ps, po, F + 1);
+  EO_TRY_HIP(hipGetLastError(), "widen");
+  if (host && n) EO_TRY_HIP(hipMemcpyAsync(out, dout, n * sizeof(eo_tuple), hipMemcpyDeviceToHost, s), "d2h");
+  if (part_off)
+    EO_TRY_HIP(hipMemcpyAsync(part_off, po, (F + 1) * sizeof(uint64_t),
+                              host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s),
+               "part_off copy");
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  return check_device_errors(c, s);
+}
+
+// ---- hash join ----------------------------------------------------------------
+
+eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
+                       uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o) {
+  if (!c || !n_out || !opts_ok(o) || nR > kMaxRows || nS > kMaxRows) return EO_EINVAL;
+  if ((nR && !R) || (nS && !S) || (out_cap && !out)) return EO_EINVAL;
+  *n_out = 0;
+  HashPlan hp;
+  eo_status st = plan_hash_join(nR, nS, o, &hp);
+  if (st != EO_OK) return st;
+  eo_join_stats *stats = o ? o->stats : nullptr;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (nR == 0 || nS == 0) return EO_OK;
+
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  const bool host = where_of(o) == EO_HOST;
+  const uint32_t P = 1u << hp.bits;
+
+  Layout L;
+  const size_t o_rin = L.add(host ? nR * sizeof(eo_tuple) : 0);
+  const size_t o_sin = L.add(host ? nS * sizeof(eo_tuple) : 0);
+  const size_t o_rp = L.add(hp.passes ? nR * sizeof(eo_tuple) : 0);
+  const size_t o_sp = L.add(hp.passes ? nS * sizeof(eo_tuple) : 0);
+  const size_t o_tmp = L.add(hp.passes == 2 ? std::max(nR, nS) * sizeof(eo_tuple) : 0);
+  const size_t o_rs = L.add((P + 1) * sizeof(uint32_t));
+  const size_t o_ss = L.add((P + 1) * sizeof(uint32_t));
+  const size_t o_ts = L.add((P + 1) * sizeof(uint32_t));
+  const size_t o_small = L.add(8 * sizeof(uint64_t));
+  const size_t o_pscr = L.add(std::max(partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus)));
+  const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
+  const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_tile + 1024;
+  const size_t o_tc = L.add(tiles_est * sizeof(uint64_t) + scan_scratch_bytes_u64(tiles_est));
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+  char *A = c->arena.base;
+  const eo_tuple *dR = host ? (eo_tuple *)(A + o_rin) : R;
+  const eo_tuple *dS = host ? (eo_tuple *)(A + o_sin) : S;
+  uint32_t *r_start = (uint32_t *)(A + o_rs);
+  uint32_t *s_start = (uint32_t *)(A + o_ss);
+  uint32_t *tile_start = (uint32_t *)(A + o_ts);
+  uint64_t *small = (uint64_t *)(A + o_small);
+  if (host) {
+    EO_TRY_HIP(hipMemcpyAsync((void *)dR, R, nR * sizeof(eo_tuple), hipMemcpyHostToDevice, s), "h2d R");
+    EO_TRY_HIP(hipMemcpyAsync((void *)dS, S, nS * sizeof(eo_tuple), hipMemcpyHostToDevice, s), "h2d S");
+  }
+
+  PhaseClock clk(c, s, stats != nullptr);
+  clk.mark();  // 0
+  const eo_tuple *Rp = dR, *Sp = dS;
+  if (hp.passes) {
+    eo_tuple *rp = (eo_tuple *)(A + o_rp), *sp = (eo_tuple *)(A + o_sp), *tmp = (eo_tuple *)(A + o_tmp);
+    st = partition_relation(c, dR, nR, hp, tmp, rp, r_start, A + o_pscr, s);
+    if (st != EO_OK) return st;
+    st = partition_relation(c, dS, nS, hp, tmp, sp, s_start, A + o_pscr, s);
+    if (st != EO_OK) return st;
+    Rp = rp;
+    Sp = sp;
+  } else {
+    EO_TRY_HIP(fill_u32(r_start, 0u, 1, s), "fill");
+    EO_TRY_HIP(fill_u32(r_start + 1, (uint32_t)nR, 1, s), "fill");
+    EO_TRY_HIP(fill_u32(s_start, 0u, 1, s), "fill");
+    EO_TRY_HIP(fill_u32(s_start + 1, (uint32_t)nS, 1, s), "fill");
+  }
+  clk.mark();  // 1
+
+  JoinTilePlan jp{P, hp.lds_cap, hp.s_tile, 0};
+  EO_TRY_HIP(join_plan_count(jp, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s), "tile plan");
+  uint64_t n_tiles = 0;
+  st = read_u64(c, small, &n_tiles, 1, s);
+  if (st != EO_OK) return st;
+  if (n_tiles >= (1ull << 31)) {
+    set_last_error("too many join tiles");
+    return EO_EINTERNAL;
+  }
+
+  // Per-tile counts / offsets: reserved up front for the usual tile count,
+  // allocated on the side only when skew produces more tiles than estimated.
+  char *B = A + o_tc;
+  bool side = false;
+  size_t o_cnt = 0, o_cscr = tiles_est * sizeof(uint64_t);
+  if (n_tiles > tiles_est) {
+    Layout L2;
+    o_cnt = L2.add(n_tiles * sizeof(uint64_t));
+    o_cscr = L2.add(scan_scratch_bytes_u64(n_tiles));
+    EO_TRY_HIP(hipMallocAsync((void **)&B, L2.total, s), "tile count buffer");
+    side = true;
+  }
+  uint64_t *tile_cnt = (uint64_t *)(B + o_cnt);
+  auto free_side = [&]() {
+    if (side) (void)hipFreeAsync(B, s);
+  };
+
+  EO_TRY_HIP(join_count(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt, s), "count");
+  EO_TRY_HIP(scan_u64(tile_cnt, tile_cnt, n_tiles, small + 1, B + o_cscr, c->d_err, s), "count scan");
+  clk.mark();  // 2
+  uint64_t total = 0;
+  st = read_u64(c, small + 1, &total, 1, s);
+  if (st == EO_OK) st = check_device_errors(c, s);
+  if (st != EO_OK) {
+    free_side();
+    return st;
+  }
+  *n_out = total;
+  if (stats) {
+    stats->n_out = total;
+    stats->radix_bits = (uint32_t)hp.bits;
+    stats->passes = (uint32_t)hp.passes;
+    stats->tiles = (uint32_t)n_tiles;
+    stats->lds_tuples = hp.lds_cap;
+  }
+  if (!out) {
+    free_side();
+    return EO_OK;
+  }
+  if (total > out_cap) {
+    free_side();
+    return EO_ECAPACITY;
+  }
+  eo_pair *dout = out;
+  if (host && total) {
+    eo_pair *tmp_out = nullptr;
+    hipError_t e = hipMallocAsync((void **)&tmp_out, total * sizeof(eo_pair), s);
+    if (e != hipSuccess) {
+      free_side();
+      return hip_fail(e, "output staging");
+    }
+    dout = tmp_out;
+  }
+  if (total) {
+    hipError_t e = join_write(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt, dout,
+                              total, s);
+    clk.mark();  // 3
+    if (e == hipSuccess && host)
+      e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    if (host) (void)hipFreeAsync(dout, s);
+    if (e != hipSuccess) {
+      free_side();
+      return hip_fail(e, "write");
+    }
+  } else {
+    clk.mark();
+  }
+  free_side();
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  st = check_device_errors(c, s);
+  if (stats) {
+    stats->ms_partition = clk.ms(0, 1);
+    stats->ms_count = clk.ms(1, 2);
+    stats->ms_write = clk.ms(2, 3);
+    stats->ms_total = clk.ms(0, 3);
+  }
+  return st;
+}
+
+// ---- nested-loop join -----------------------------------------------------------
+
+eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
+                      uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o) {
+  if (!c || !n_out || !opts_ok(o) || nR > kMaxRows || nS > kMaxRows) return EO_EINVAL;
+  if ((nR && !R) || (nS && !S) || (out_cap && !out)) return EO_EINVAL;
+  *n_out = 0;
+  eo_join_stats *stats = o ? o->stats : nullptr;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (nR == 0 || nS == 0) return EO_OK;
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  const bool host = where_of(o) == EO_HOST;
+  const NljPlan np = plan_nlj(nR, nS);
+  const uint64_t slots = nlj_count_slots(np);
+  Layout L;
+  const size_t o_rin = L.add(host ? nR * sizeof(eo_tuple) : 0);
+  const size_t o_sin = L.add(host ? nS * sizeof(eo_tuple) : 0);
+  const size_t o_cnt = L.add(slots * sizeof(uint64_t));
+  const size_t o_small = L.add(8 * sizeof(uint64_t));
+  const size_t o_scr = L.add(scan_scratch_bytes_u64(slots));
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+  char *A = c->arena.base;
+  const eo_tuple *dR = host ? (eo_tuple *)(A + o_rin) : R;
+  const eo_tuple *dS = host ? (eo_tuple *)(A + o_sin) : S;
+  uint64_t *cnt = (uint64_t *)(A + o_cnt);
+  uint64_t *small = (uint64_t *)(A + o_small);
+  if (host) {
+    EO_TRY_HIP(hipMemcpyAsync((void *)dR, R, nR * sizeof(eo_tuple), hipMemcpyHostToDevice, s), "h2d R");
+    EO_TRY_HIP(hipMemcpyAsync((void *)dS, S, nS * sizeof(eo_tuple), hipMemcpyHostToDevice, s), "h2d S");
+  }
+  PhaseClock clk(c, s, stats != nullptr);
+  clk.mark();
+  EO_TRY_HIP(nlj_count(np, dR, dS, cnt, s), "nlj count");
+  EO_TRY_HIP(scan_u64(cnt, cnt, slots, small, A + o_scr, c->d_err, s), "nlj scan");
+  clk.mark();
+  uint64_t total = 0;
+  eo_status st = read_u64(c, small, &total, 1, s);
+  if (st == EO_OK) st = check_device_errors(c, s);
+  if (st != EO_OK) return st;
+  *n_out = total;
+  if (stats) stats->n_out = total;
+  if (!out) return EO_OK;
+  if (total > out_cap) return EO_ECAPACITY;
+  if (total) {
+    eo_pair *dout = out;
+    if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * sizeof(eo_pair), s), "output staging");
+    hipError_t e = nlj_write(np, dR, dS, cnt, dout, total, s);
+    clk.mark();
+    if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    if (host) (void)hipFreeAsync(dout, s);
+    if (e != hipSuccess) return hip_fail(e, "nlj write");
+  } else {
+    clk.mark();
+  }
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  if (stats) {
+    stats->ms_count = clk.ms(0, 1);
+    stats->ms_write = clk.ms(1, 2);
+    stats->ms_total = clk.ms(0, 2);
+  }
+  return check_device_errors(c, s);
+}
+
+}  // extern "C"

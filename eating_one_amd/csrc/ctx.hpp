@@ -1,0 +1,102 @@
+// Library context: device, stream, scratch arena, error flag, readback buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "eojoin.h"
+
+namespace eo {
+
+void set_last_error(const std::string &msg);
+
+// Bump allocator over one device buffer that only grows between calls.
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes);  // may free and reallocate (caller has synchronised)
+  void release();
+};
+
+// Computes offsets of sub-buffers inside one arena reservation.
+struct Layout {
+  size_t total = 0;
+  size_t add(size_t bytes) {
+    const size_t off = total;
+    total += (bytes + 255) / 256 * 256;
+    return off;
+  }
+};
+
+struct DeviceProps {
+  std::string name, arch;
+  int cus = 0;
+  int clock_khz = 0;
+  size_t lds_per_wg = 0, hbm = 0, l2 = 0;
+};
+
+}  // namespace eo
+
+struct eo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  eo::DeviceProps props;
+  eo::Arena arena;
+  uint32_t *d_err = nullptr;     // device-side error flags
+  uint64_t *h_pinned = nullptr;  // 64 pinned words for small readbacks
+  uint64_t hbm_budget = 0;
+  hipEvent_t ev[8] = {};
+};
+
+namespace eo {
+
+// Scoped device selection.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+eo_status hip_fail(hipError_t e, const char *what);
+eo_status check_device_errors(eo_ctx *ctx, hipStream_t stream);
+eo_status read_u64(eo_ctx *ctx, const uint64_t *dev, uint64_t *host, int n, hipStream_t stream);
+
+inline hipStream_t pick_stream(eo_ctx *ctx, const eo_join_opts *o) {
+  return (o && o->stream) ? (hipStream_t)o->stream : ctx->stream;
+}
+inline int where_of(const eo_join_opts *o) { return o ? o->where : EO_HOST; }
+
+// Phase timing with events on the call's stream (only when stats requested).
+struct PhaseClock {
+  eo_ctx *ctx;
+  hipStream_t stream;
+  bool on;
+  int n = 0;
+  PhaseClock(eo_ctx *c, hipStream_t s, bool enabled) : ctx(c), stream(s), on(enabled) {}
+  void mark() {
+    if (on && n < 8) (void)hipEventRecord(ctx->ev[n++], stream);
+  }
+  double ms(int a, int b) const {
+    float v = 0.f;
+    if (!on || a >= n || b >= n) return 0.0;
+    if (hipEventElapsedTime(&v, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.0;
+    return (double)v;
+  }
+};
+
+}  // namespace eo
+
+#define EO_TRY_HIP(call, what)                      \
+  do {                                              \
+    hipError_t eo_e_ = (call);                      \
+    if (eo_e_ != hipSuccess) return eo::hip_fail(eo_e_, what); \
+  } while (0)

@@ -1,0 +1,158 @@
+// Device-side building blocks shared by every kernel of the library:
+// wave64 scans on DPP row shifts/broadcasts, ballot-based peer matching,
+// workgroup scans through LDS, and agent-scope (cross-XCD coherent) atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eo {
+
+constexpr int kWave = 64;
+constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, WGs dealt round-robin
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// ---- DPP wave scans ---------------------------------------------------------
+// GFX9 DPP controls: row_shr:n = 0x110+n (within 16-lane rows),
+// row_bcast:15 = 0x142 (lane 15 of row r -> all lanes of row r+1),
+// row_bcast:31 = 0x143 (lane 31 -> rows 2 and 3).  Lanes whose source is
+// outside the row, or whose row is masked off, receive `old` (= 0 here).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xf, false);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  x += dpp_u32<0x111, 0xf>(x);
+  x += dpp_u32<0x112, 0xf>(x);
+  x += dpp_u32<0x114, 0xf>(x);
+  x += dpp_u32<0x118, 0xf>(x);
+  x += dpp_u32<0x142, 0xa>(x);
+  x += dpp_u32<0x143, 0xc>(x);
+  return x;
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t x) {
+  uint32_t lo = dpp_u32<CTRL, ROWMASK>((uint32_t)x);
+  uint32_t hi = dpp_u32<CTRL, ROWMASK>((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x) {
+  x += dpp_u64<0x111, 0xf>(x);
+  x += dpp_u64<0x112, 0xf>(x);
+  x += dpp_u64<0x114, 0xf>(x);
+  x += dpp_u64<0x118, 0xf>(x);
+  x += dpp_u64<0x142, 0xa>(x);
+  x += dpp_u64<0x143, 0xc>(x);
+  return x;
+}
+
+__device__ __forceinline__ uint32_t read_lane(uint32_t x, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)x, lane);
+}
+__device__ __forceinline__ uint64_t read_lane(uint64_t x, int lane) {
+  uint32_t lo = read_lane((uint32_t)x, lane);
+  uint32_t hi = read_lane((uint32_t)(x >> 32), lane);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+  return read_lane(wave_incl_scan(x), kWave - 1);
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Lanes whose `digit` (BITS low bits) equals this lane's, among the lanes in
+// `active`.  BITS ballots; replaces a match_any instruction that CDNA lacks.
+template <int BITS>
+__device__ __forceinline__ uint64_t match_peers(uint32_t digit, uint64_t active) {
+  uint64_t peers = active;
+#pragma unroll
+  for (int b = 0; b < BITS; ++b) {
+    const bool bit = (digit >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+// ---- workgroup scans --------------------------------------------------------
+// Exclusive scan of one value per thread across a workgroup of NT threads.
+// `lds` needs NT/64 + 1 entries.  Returns the exclusive prefix; *total gets
+// the workgroup sum.  Ends with a barrier so `lds` may be reused.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_scan(T x, T *lds, T *total) {
+  constexpr int NW = NT / kWave;
+  const int w = threadIdx.x / kWave;
+  const int l = lane_id();
+  T inc = wave_incl_scan(x);
+  if (l == kWave - 1) lds[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    T v = (l < NW) ? lds[l] : T(0);
+    T vi = wave_incl_scan(v);
+    if (l < NW) lds[l] = vi - v;
+    if (l == NW - 1) lds[NW] = vi;
+  }
+  __syncthreads();
+  T res = lds[w] + inc - x;
+  *total = lds[NW];
+  __syncthreads();
+  return res;
+}
+
+template <int NT, typename T>
+__device__ __forceinline__ T block_sum(T x, T *lds) {
+  T total;
+  (void)block_excl_scan<NT, T>(x, lds, &total);
+  return total;
+}
+
+// ---- coherent atomics -------------------------------------------------------
+// Agent scope makes loads/stores coherent across the XCDs' separate L2s.
+__device__ __forceinline__ uint64_t ld_acquire(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint64_t *p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_add_agent(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t atomic_add_agent(uint64_t *p, uint64_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t atomic_or_agent(uint32_t *p, uint32_t v) {
+  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroup -> work-item mapping that keeps consecutive work items on one XCD:
+// the hardware deals workgroup b to XCD b % 8, so item (b % 8) * (G/8) + b / 8
+// puts items [x*G/8, (x+1)*G/8) on XCD x.  Bijective on [0, G) for G % 8 == 0.
+__device__ __forceinline__ uint32_t xcd_local_index(uint32_t b, uint32_t grid) {
+  const uint32_t per = grid / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
+// Multiplicative hash for LDS/global bucket tables (Knuth's golden ratio).
+__device__ __forceinline__ uint32_t bucket_hash(int32_t key, int log2nb) {
+  return log2nb == 0 ? 0u : (((uint32_t)key * 2654435769u) >> (32 - log2nb));
+}
+
+// Device error flags (bit set -> host reports EO_EHIP / EO_EINTERNAL).
+enum : uint32_t {
+  kErrSpin = 1u,       // a bounded wait gave up
+  kErrOverflow = 2u,   // a device-side capacity was exceeded
+};
+
+}  // namespace eo

@@ -1,0 +1,283 @@
+// Per-partition join of a radix-partitioned R and S.
+//
+// The reference's probe (count / join, joins/hashJoin_gpu/main_street/
+// join_gpu.cu:9-113) holds a <=256-tuple S chunk in shared memory and compares
+// every R tuple of the partition against all of it — a partitioned nested
+// loop, run twice (count, then write).  Here each work tile
+//   (partition p, R-chunk c of <= CAP tuples, S-range j of <= s_tile tuples)
+// builds a bucketised table of its R chunk in LDS (LDS counting sort by a
+// multiplicative hash: histogram with LDS atomics, workgroup scan, placement by
+// atomic decrement), then streams its S range from HBM and probes the table.
+// Output materialisation is count -> device scan -> write: the count kernel
+// stores one total per tile, a u64 scan turns them into tile output offsets,
+// and the write kernel re-probes and writes (R payload, S payload) pairs at
+// tile offset + workgroup-scanned per-thread offsets.
+//
+// Tiles of one partition are numbered (j, c) with c fastest and dealt so that
+// consecutive tiles share an XCD: the R-chunks probing one S range run side by
+// side on one XCD and meet its S tuples in that XCD's L2.
+#include <algorithm>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace eo {
+namespace {
+
+template <int NT_, int CAP_, int LOG2NB_>
+struct JoinCfg {
+  static constexpr int NT = NT_;
+  static constexpr int CAP = CAP_;
+  static constexpr int LOG2NB = LOG2NB_;
+  static constexpr int NB = 1 << LOG2NB_;
+  static constexpr int RPT = CAP / NT;   // R tuples per thread while building
+  static constexpr int BPT = NB / NT;    // buckets per thread in the bucket scan
+  static constexpr int U = 4;            // S tuples per thread per probe step
+};
+
+using Cfg4K = JoinCfg<512, 4096, 10>;
+using Cfg8K = JoinCfg<512, 8192, 11>;
+using Cfg16K = JoinCfg<1024, 16384, 12>;
+
+struct TileRef {
+  uint32_t r_beg, r_n, s_beg, s_n;
+};
+
+// Locates tile t: p = last partition with tile_start[p] <= t, then (j, c).
+template <int CAP>
+__device__ __forceinline__ bool resolve_tile(uint32_t t, const uint32_t *tile_start, uint32_t P,
+                                             const uint32_t *r_start, const uint32_t *s_start, uint32_t s_tile,
+                                             TileRef *ref) {
+  uint32_t lo = 0, hi = P;  // invariant: tile_start[lo] <= t < tile_start[hi]
+  if (t >= tile_start[P]) return false;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tile_start[mid] <= t) lo = mid; else hi = mid;
+  }
+  const uint32_t p = lo;
+  const uint32_t local = t - tile_start[p];
+  const uint32_t rb = r_start[p], rn = r_start[p + 1] - rb;
+  const uint32_t sb = s_start[p], sn = s_start[p + 1] - sb;
+  const uint32_t nc = (rn + CAP - 1) / CAP;
+  const uint32_t c = local % nc, j = local / nc;
+  ref->r_beg = rb + c * CAP;
+  ref->r_n = min((uint32_t)CAP, rn - c * CAP);
+  ref->s_beg = sb + j * s_tile;
+  ref->s_n = min(s_tile, sn - j * s_tile);
+  return true;
+}
+
+// Builds the LDS table of R[rb, rb+rn): ent[] grouped by bucket, bucket b in
+// ent[off[b], off[b+1]).  Ends with a barrier.
+template <typename C>
+__device__ __forceinline__ void build_table(const uint2 *__restrict__ R, uint32_t rb, uint32_t rn, uint2 *ent,
+                                            uint32_t *off, uint32_t *scan_tmp) {
+  for (int h = threadIdx.x; h <= C::NB; h += C::NT) off[h] = 0;
+  __syncthreads();
+  uint2 r[C::RPT];
+  uint32_t hh[C::RPT];
+#pragma unroll
+  for (int k = 0; k < C::RPT; ++k) {
+    const uint32_t i = k * C::NT + threadIdx.x;
+    if (i < rn) {
+      r[k] = R[rb + i];
+      hh[k] = bucket_hash((int32_t)r[k].x, C::LOG2NB);
+      atomicAdd(&off[hh[k]], 1u);
+    }
+  }
+  __syncthreads();
+  {  // inclusive scan of bucket sizes: off[b] = end of bucket b
+    uint32_t loc[C::BPT];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < C::BPT; ++k) {
+      loc[k] = off[threadIdx.x * C::BPT + k];
+      sum += loc[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_excl_scan<C::NT, uint32_t>(sum, scan_tmp, &tot);
+#pragma unroll
+    for (int k = 0; k < C::BPT; ++k) {
+      run += loc[k];
+      off[threadIdx.x * C::BPT + k] = run;
+    }
+    if (threadIdx.x == 0) off[C::NB] = rn;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C::RPT; ++k) {
+    const uint32_t i = k * C::NT + threadIdx.x;
+    if (i < rn) ent[atomicSub(&off[hh[k]], 1u) - 1] = r[k];
+  }
+  __syncthreads();
+}
+
+template <typename C>
+__global__ __launch_bounds__(C::NT) void k_join_count(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
+                                                      const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
+                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
+                                                      uint32_t s_tile, uint64_t *__restrict__ tile_count) {
+  __shared__ uint2 ent[C::CAP];
+  __shared__ uint32_t off[C::NB + 1];
+  __shared__ uint32_t scan_tmp[C::NT / kWave + 1];
+  __shared__ uint64_t red[C::NT / kWave + 1];
+  const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
+  TileRef ref;
+  if (!resolve_tile<C::CAP>(t, tile_start, P, r_start, s_start, s_tile, &ref)) return;
+  build_table<C>(R, ref.r_beg, ref.r_n, ent, off, scan_tmp);
+
+  uint32_t cnt = 0;
+  for (uint32_t i0 = 0; i0 < ref.s_n; i0 += C::NT * C::U) {
+    int32_t key[C::U];
+#pragma unroll
+    for (int u = 0; u < C::U; ++u) {
+      const uint32_t i = i0 + u * C::NT + threadIdx.x;
+      key[u] = i < ref.s_n ? (int32_t)S[ref.s_beg + i].x : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < C::U; ++u) {
+      const uint32_t i = i0 + u * C::NT + threadIdx.x;
+      if (i < ref.s_n) {
+        const uint32_t h = bucket_hash(key[u], C::LOG2NB);
+        const uint32_t e1 = off[h + 1];
+        for (uint32_t e = off[h]; e < e1; ++e) cnt += (int32_t)ent[e].x == key[u];
+      }
+    }
+  }
+  const uint64_t tot = block_sum<C::NT, uint64_t>((uint64_t)cnt, red);
+  if (threadIdx.x == 0) tile_count[t] = tot;
+}
+
+template <typename C>
+__global__ __launch_bounds__(C::NT) void k_join_write(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
+                                                      const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
+                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
+                                                      uint32_t s_tile, const uint64_t *__restrict__ tile_off,
+                                                      uint2 *__restrict__ out, uint64_t out_cap) {
+  __shared__ uint2 ent[C::CAP];
+  __shared__ uint32_t off[C::NB + 1];
+  __shared__ uint32_t scan_tmp[C::NT / kWave + 1];
+  __shared__ uint64_t red[C::NT / kWave + 1];
+  const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
+  TileRef ref;
+  if (!resolve_tile<C::CAP>(t, tile_start, P, r_start, s_start, s_tile, &ref)) return;
+  build_table<C>(R, ref.r_beg, ref.r_n, ent, off, scan_tmp);
+
+  uint64_t base = tile_off[t];
+  for (uint32_t i0 = 0; i0 < ref.s_n; i0 += C::NT * C::U) {
+    uint2 s[C::U];
+    uint32_t b0[C::U], b1[C::U];
+    uint32_t m = 0;
+#pragma unroll
+    for (int u = 0; u < C::U; ++u) {
+      const uint32_t i = i0 + u * C::NT + threadIdx.x;
+      s[u] = i < ref.s_n ? S[ref.s_beg + i] : make_uint2(0u, 0u);
+      b0[u] = b1[u] = 0;
+    }
+#pragma unroll
+    for (int u = 0; u < C::U; ++u) {
+      const uint32_t i = i0 + u * C::NT + threadIdx.x;
+      if (i < ref.s_n) {
+        const uint32_t h = bucket_hash((int32_t)s[u].x, C::LOG2NB);
+        b0[u] = off[h];
+        b1[u] = off[h + 1];
+        for (uint32_t e = b0[u]; e < b1[u]; ++e) m += ent[e].x == s[u].x;
+      }
+    }
+    uint64_t step_total;
+    uint64_t pos = base + block_excl_scan<C::NT, uint64_t>((uint64_t)m, red, &step_total);
+#pragma unroll
+    for (int u = 0; u < C::U; ++u) {
+      for (uint32_t e = b0[u]; e < b1[u]; ++e) {
+        const uint2 r = ent[e];
+        if (r.x == s[u].x) {
+          if (pos < out_cap) out[pos] = make_uint2(r.y, s[u].y);
+          ++pos;
+        }
+      }
+    }
+    base += step_total;
+  }
+}
+
+__global__ void k_plan_tiles(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t cap,
+                             uint32_t s_tile, uint32_t *tiles) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > P) return;
+  if (p == P) {
+    tiles[P] = 0;
+    return;
+  }
+  const uint32_t rn = r_start[p + 1] - r_start[p];
+  const uint32_t sn = s_start[p + 1] - s_start[p];
+  tiles[p] = (rn && sn) ? ((rn + cap - 1) / cap) * ((sn + s_tile - 1) / s_tile) : 0u;
+}
+
+template <typename C>
+hipError_t launch_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                        const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, uint64_t *tile_count,
+                        hipStream_t stream) {
+  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(k_join_count<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
+                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_count);
+  return hipGetLastError();
+}
+
+template <typename C>
+hipError_t launch_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                        const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                        const uint64_t *tile_off, eo_pair *out, uint64_t out_cap, hipStream_t stream) {
+  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(k_join_write<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
+                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_off,
+                     reinterpret_cast<uint2 *>(out), out_cap);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+uint32_t join_max_lds_cap() { return Cfg16K::CAP; }
+
+size_t join_plan_scratch_bytes(uint32_t parts) { return scan_scratch_bytes_u32((uint64_t)parts + 1); }
+
+hipError_t join_plan_count(const JoinTilePlan &jp, const uint32_t *r_start, const uint32_t *s_start,
+                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
+                           hipStream_t stream) {
+  const uint32_t n = jp.parts + 1;
+  hipLaunchKernelGGL(k_plan_tiles, dim3((n + 255) / 256), dim3(256), 0, stream, r_start, s_start, jp.parts,
+                     jp.lds_cap, jp.s_tile, tile_start);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return scan_u32(tile_start, tile_start, n, n_tiles_dev, scan_scratch, err, stream);
+}
+
+hipError_t join_plan_map(const JoinTilePlan &, const uint32_t *, uint32_t *, hipStream_t) { return hipSuccess; }
+
+hipError_t join_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, const uint32_t *,
+                      uint64_t *tile_count, hipStream_t stream) {
+  if (n_tiles == 0) return hipSuccess;
+  switch (jp.lds_cap) {
+    case Cfg4K::CAP: return launch_count<Cfg4K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, stream);
+    case Cfg8K::CAP: return launch_count<Cfg8K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, stream);
+    case Cfg16K::CAP: return launch_count<Cfg16K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, const uint32_t *,
+                      const uint64_t *tile_off, eo_pair *out, uint64_t out_cap, hipStream_t stream) {
+  if (n_tiles == 0) return hipSuccess;
+  switch (jp.lds_cap) {
+    case Cfg4K::CAP:
+      return launch_write<Cfg4K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_off, out, out_cap, stream);
+    case Cfg8K::CAP:
+      return launch_write<Cfg8K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_off, out, out_cap, stream);
+    case Cfg16K::CAP:
+      return launch_write<Cfg16K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_off, out, out_cap, stream);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace eo

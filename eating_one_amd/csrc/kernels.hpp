@@ -1,0 +1,86 @@
+// Host-side launchers of the library's HIP kernels.  Every launcher enqueues
+// on `stream` and returns the hipError_t of the launch; nothing here
+// synchronises or allocates.  Scratch buffers are carved by the caller from
+// the context arena; their sizes come from the *_scratch_bytes() helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "eojoin.h"
+
+namespace eo {
+
+// ---- scan.hip ---------------------------------------------------------------
+// Exclusive scan with decoupled look-back.  scratch: scan_scratch_bytes(n).
+// total_dev (optional, device) receives the sum.  out may equal in.
+size_t scan_scratch_bytes_u32(uint64_t n);
+size_t scan_scratch_bytes_u64(uint64_t n);
+hipError_t scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *total_dev,
+                    void *scratch, uint32_t *err, hipStream_t stream);
+hipError_t scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *total_dev,
+                    void *scratch, uint32_t *err, hipStream_t stream);
+hipError_t fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
+hipError_t fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
+
+// ---- partition.hip ----------------------------------------------------------
+// One stable radix-partition pass over `bits` (1..kMaxPassBits) digit bits.
+constexpr int kMaxPassBits = 11;
+struct PartPassPlan {
+  uint64_t n;
+  int bits;
+  int shift;
+  uint32_t xor_mask;   // applied to (uint32)key before the shift (sign flip for sorting)
+  uint32_t stripes;    // G: workgroups, each owning one contiguous stripe
+  uint64_t stripe_len; // tuples per stripe (multiple of the scatter tile)
+};
+PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus);
+size_t partition_pass_scratch_bytes(const PartPassPlan &p);
+// counts -> scan -> scatter.  part_start (optional, 2^bits+1 u32 entries, device)
+// receives the partition start offsets of this pass.
+hipError_t partition_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
+                          uint32_t *part_start, void *scratch, uint32_t *err,
+                          hipStream_t stream);
+// Partition start offsets of a tuple array already sorted by digit
+// ((uint32)key ^ xor_mask) >> shift & (2^bits - 1):  part_start[d] = first index
+// with digit >= d, part_start[2^bits] = n.
+hipError_t partition_bounds(const eo_tuple *sorted, uint64_t n, int bits, int shift,
+                            uint32_t xor_mask, uint32_t *part_start, hipStream_t stream);
+
+// ---- hashjoin.hip -----------------------------------------------------------
+struct JoinTilePlan {
+  uint32_t parts;      // number of partitions P
+  uint32_t lds_cap;    // R tuples per LDS table (C)
+  uint32_t s_tile;     // S tuples per tile
+  int log2_buckets;    // LDS bucket count = 2^log2_buckets
+};
+// Tile planning: tiles_per_part -> exclusive scan -> tile_part map.
+size_t join_plan_scratch_bytes(uint32_t parts);
+hipError_t join_plan_count(const JoinTilePlan &jp, const uint32_t *r_start, const uint32_t *s_start,
+                           uint32_t *tile_start /*P+1*/, uint64_t *n_tiles_dev, void *scan_scratch,
+                           uint32_t *err, hipStream_t stream);
+hipError_t join_plan_map(const JoinTilePlan &jp, const uint32_t *tile_start, uint32_t *tile_part,
+                         hipStream_t stream);
+hipError_t join_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                      const uint32_t *tile_part, uint64_t *tile_count, hipStream_t stream);
+hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                      const uint32_t *tile_part, const uint64_t *tile_off, eo_pair *out, uint64_t out_cap,
+                      hipStream_t stream);
+uint32_t join_max_lds_cap();
+
+// ---- nlj.hip ----------------------------------------------------------------
+struct NljPlan {
+  uint64_t nR, nS;
+  uint32_t r_tiles, s_tiles;  // grid = r_tiles x s_tiles workgroups
+  uint32_t r_tile, s_tile;    // tuples per tile
+};
+NljPlan plan_nlj(uint64_t nR, uint64_t nS);
+uint64_t nlj_count_slots(const NljPlan &p);  // per-wave count entries
+hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *wave_count,
+                     hipStream_t stream);
+hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off,
+                     eo_pair *out, uint64_t out_cap, hipStream_t stream);
+
+}  // namespace eo

@@ -1,0 +1,162 @@
+// Block nested-loop equi-join.
+//
+// The reference (joins/blockJoin_gpu/main_street/join_gpu.cu:28-118) holds
+// 512 R tuples in shared memory per block and one S tuple per thread, writes
+// one count per (thread, R slice) — 2^27 counts for a 2^18 x 2^18 tile — and
+// repeats every compare in the write kernel.  Here a workgroup of 4 waves
+// keeps an R tile of 4096 keys in LDS and 8 S keys per lane in VGPRs
+// (a 2048-tuple S tile), and reads R keys from LDS four at a time with one
+// wave-uniform (broadcast) load.  Each (R key, S register) compare is one
+// VALU compare whose 64-lane ballot is OR-ed into a scalar accumulator; only
+// a group of 8 R keys whose accumulator shows a hit is recounted exactly.
+// Counts are kept per wave, so the write pass (same loop, same order) writes
+// every pair at its wave's scanned offset in a deterministic order.
+#include <algorithm>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace eo {
+namespace {
+
+constexpr int kNljNT = 256;
+constexpr int kNljNW = kNljNT / kWave;
+constexpr int kNljK = 8;                     // S keys per lane
+constexpr int kNljSTile = kNljNT * kNljK;    // 2048
+constexpr int kNljRTile = 4096;
+constexpr int kNljGroup = 8;                 // R keys per OR-accumulated group
+
+struct SRegs {
+  int32_t key[kNljK];
+  int32_t pay[kNljK];
+  uint64_t valid[kNljK];  // ballot of lanes holding a real S tuple in register k
+};
+
+__device__ __forceinline__ void load_s(const uint2 *__restrict__ S, uint64_t nS, uint64_t s0, SRegs &sr,
+                                       bool need_pay) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < kNljK; ++k) {
+    // wave w owns S[s0 + w*64*K, +64*K); register k holds element k*64 + lane of it
+    const uint64_t i = s0 + (uint64_t)w * kWave * kNljK + (uint64_t)k * kWave + lane;
+    const bool ok = i < nS;
+    uint2 t = ok ? S[i] : make_uint2(0u, 0u);
+    sr.key[k] = (int32_t)t.x;
+    sr.pay[k] = need_pay ? (int32_t)t.y : 0;
+    sr.valid[k] = __ballot(ok);
+  }
+}
+
+template <bool WRITE>
+__device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpay, uint32_t rn, const SRegs &sr,
+                                         uint64_t *count, uint64_t base, uint2 *out, uint64_t out_cap) {
+  const uint64_t lt = lanemask_lt();
+  uint64_t running = 0;
+  uint32_t g = 0;
+  for (; g + kNljGroup <= rn; g += kNljGroup) {
+    int32_t rk[kNljGroup];
+    const int4 *q = reinterpret_cast<const int4 *>(rkey + g);
+    const int4 a = q[0], b = q[1];
+    rk[0] = a.x; rk[1] = a.y; rk[2] = a.z; rk[3] = a.w;
+    rk[4] = b.x; rk[5] = b.y; rk[6] = b.z; rk[7] = b.w;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int q2 = 0; q2 < kNljGroup; ++q2)
+#pragma unroll
+      for (int k = 0; k < kNljK; ++k) acc |= __ballot(sr.key[k] == rk[q2]);
+    if (acc) {
+#pragma unroll
+      for (int q2 = 0; q2 < kNljGroup; ++q2)
+#pragma unroll
+        for (int k = 0; k < kNljK; ++k) {
+          const uint64_t m = __ballot(sr.key[k] == rk[q2]) & sr.valid[k];
+          if (m) {
+            if (WRITE && sr.key[k] == rk[q2] && ((sr.valid[k] >> lane_id()) & 1)) {
+              const uint64_t pos = base + running + __popcll(m & lt);
+              if (pos < out_cap) out[pos] = make_uint2((uint32_t)rpay[g + q2], (uint32_t)sr.pay[k]);
+            }
+            running += __popcll(m);
+          }
+        }
+    }
+  }
+  for (; g < rn; ++g) {  // tail of the R tile
+    const int32_t r = rkey[g];
+#pragma unroll
+    for (int k = 0; k < kNljK; ++k) {
+      const uint64_t m = __ballot(sr.key[k] == r) & sr.valid[k];
+      if (m) {
+        if (WRITE && sr.key[k] == r && ((sr.valid[k] >> lane_id()) & 1)) {
+          const uint64_t pos = base + running + __popcll(m & lt);
+          if (pos < out_cap) out[pos] = make_uint2((uint32_t)rpay[g], (uint32_t)sr.pay[k]);
+        }
+        running += __popcll(m);
+      }
+    }
+  }
+  if (!WRITE && lane_id() == 0) *count = running;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uint64_t nR, const uint2 *__restrict__ S,
+                                                uint64_t nS, uint32_t s_tiles, uint64_t *wave_count,
+                                                const uint64_t *__restrict__ wave_off, uint2 *__restrict__ out,
+                                                uint64_t out_cap) {
+  __shared__ __attribute__((aligned(16))) int32_t rkey[kNljRTile];
+  __shared__ int32_t rpay[WRITE ? kNljRTile : 1];
+  const uint32_t b = blockIdx.x;
+  const uint32_t st = b % s_tiles, rt = b / s_tiles;
+  const uint64_t r0 = (uint64_t)rt * kNljRTile;
+  const uint32_t rn = (uint32_t)std::min<uint64_t>(kNljRTile, nR - r0);
+  for (uint32_t i = threadIdx.x; i < rn; i += kNljNT) {
+    const uint2 t = R[r0 + i];
+    rkey[i] = (int32_t)t.x;
+    if (WRITE) rpay[i] = (int32_t)t.y;
+  }
+  SRegs sr;
+  load_s(S, nS, (uint64_t)st * kNljSTile, sr, WRITE);
+  __syncthreads();
+  const uint64_t slot = (uint64_t)b * kNljNW + threadIdx.x / kWave;
+  if (WRITE)
+    nlj_tile<true>(rkey, rpay, rn, sr, nullptr, wave_off[slot], out, out_cap);
+  else
+    nlj_tile<false>(rkey, rpay, rn, sr, wave_count + slot, 0, nullptr, 0);
+}
+
+}  // namespace
+
+NljPlan plan_nlj(uint64_t nR, uint64_t nS) {
+  NljPlan p{};
+  p.nR = nR;
+  p.nS = nS;
+  p.r_tile = kNljRTile;
+  p.s_tile = kNljSTile;
+  p.r_tiles = (uint32_t)((nR + kNljRTile - 1) / kNljRTile);
+  p.s_tiles = (uint32_t)((nS + kNljSTile - 1) / kNljSTile);
+  return p;
+}
+
+uint64_t nlj_count_slots(const NljPlan &p) { return (uint64_t)p.r_tiles * p.s_tiles * kNljNW; }
+
+hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *wave_count,
+                     hipStream_t stream) {
+  const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
+  if (grid == 0) return hipSuccess;
+  if (grid >= (1ull << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_nlj<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
+                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0);
+  return hipGetLastError();
+}
+
+hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off, eo_pair *out,
+                     uint64_t out_cap, hipStream_t stream) {
+  const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
+  if (grid == 0) return hipSuccess;
+  if (grid >= (1ull << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_nlj<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
+                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, nullptr, wave_off,
+                     reinterpret_cast<uint2 *>(out), out_cap);
+  return hipGetLastError();
+}
+
+}  // namespace eo

@@ -1,0 +1,265 @@
+// Stable radix partitioning of 8-byte tuples.
+//
+// The reference partitions with one thread per 256 consecutive tuples,
+// 16-thread blocks and per-thread digit histograms (lcount_partitioning /
+// lpartitioning, joins/hashJoin_gpu/main_street/partitioning.cu:15-131),
+// 6 bits per pass.  Here one pass is:
+//   1. k_part_hist: one workgroup per contiguous stripe, LDS histogram with
+//      LDS atomics, counts written digit-major: counts[d * G + g];
+//   2. exclusive scan of the F*G counts (decoupled look-back, scan.hip): the
+//      result is, per digit and stripe, where that stripe's run of the digit
+//      starts in the output; partition starts fall out as counts[d * G + 0];
+//   3. k_part_scatter: each workgroup walks its stripe in tiles; every wave
+//      ranks its 64 tuples per step by ballot-matched peers (stable), the tile
+//      is placed digit-sorted into an LDS staging buffer and flushed so that
+//      consecutive lanes write consecutive addresses of one digit's run.
+// Stripes are dealt so that consecutive stripes share an XCD, which makes
+// neighbouring runs of one digit meet in the same L2.
+#include <algorithm>
+
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace eo {
+namespace {
+
+constexpr int kPartNT = 512;
+
+template <int BITS>
+struct PartCfg {
+  static constexpr int F = 1 << BITS;
+  static constexpr int NT = kPartNT;
+  static constexpr int NW = NT / kWave;
+  static constexpr int IPT = BITS <= 8 ? 16 : 8;   // tuples per thread per tile
+  static constexpr int TILE = NT * IPT;
+  static constexpr int WSPAN = TILE / NW;          // tuples per wave per tile
+  static constexpr int DPT = (F + NT - 1) / NT;    // digits per thread
+};
+
+__device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shift, uint32_t mask) {
+  return ((key ^ xm) >> shift) & mask;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__ in, uint64_t n, int shift,
+                                                       uint32_t xm, uint32_t G, uint64_t stripe_len,
+                                                       uint32_t *__restrict__ counts) {
+  using C = PartCfg<BITS>;
+  __shared__ uint32_t hist[C::F];
+  for (int d = threadIdx.x; d < C::F; d += C::NT) hist[d] = 0;
+  __syncthreads();
+  const uint32_t g = xcd_local_index(blockIdx.x, G);
+  const uint64_t beg = (uint64_t)g * stripe_len;
+  const uint64_t end = beg + stripe_len < n ? beg + stripe_len : n;
+  for (uint64_t i = beg + threadIdx.x; i < end; i += C::NT) {
+    const uint32_t d = digit_of(in[i].x, xm, shift, C::F - 1);
+    atomicAdd(&hist[d], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < C::F; d += C::NT) counts[(uint64_t)d * G + g] = hist[d];
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restrict__ in, uint64_t n, int shift,
+                                                          uint32_t xm, uint32_t G, uint64_t stripe_len,
+                                                          const uint32_t *__restrict__ offsets,
+                                                          uint2 *__restrict__ out) {
+  using C = PartCfg<BITS>;
+  constexpr int F = C::F;
+  __shared__ uint32_t cursor[F];
+  __shared__ uint32_t tbase[F + 1];
+  __shared__ uint16_t wcnt[C::NW][F];
+  __shared__ uint2 stage[C::TILE];
+  __shared__ uint32_t scan_tmp[C::NW + 1];
+
+  const uint32_t g = xcd_local_index(blockIdx.x, G);
+  for (int d = threadIdx.x; d < F; d += C::NT) cursor[d] = offsets[(uint64_t)d * G + g];
+  const uint64_t beg = (uint64_t)g * stripe_len;
+  const uint64_t end = beg + stripe_len < n ? beg + stripe_len : n;
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const uint64_t lt = lanemask_lt();
+
+  for (uint64_t t0 = beg; t0 < end; t0 += C::TILE) {
+    const uint32_t tn = (uint32_t)((end - t0) < (uint64_t)C::TILE ? (end - t0) : (uint64_t)C::TILE);
+    for (int i = threadIdx.x; i < C::NW * F / 2; i += C::NT) reinterpret_cast<uint32_t *>(&wcnt[0][0])[i] = 0;
+    __syncthreads();
+
+    uint2 v[C::IPT];
+    uint32_t dg[C::IPT];
+    uint32_t rk[C::IPT];
+#pragma unroll
+    for (int it = 0; it < C::IPT; ++it) {
+      const uint32_t li = (uint32_t)(w * C::WSPAN + it * kWave + lane);
+      const bool valid = li < tn;
+      v[it] = valid ? in[t0 + li] : make_uint2(0u, 0u);
+      dg[it] = digit_of(v[it].x, xm, shift, F - 1);
+    }
+#pragma unroll
+    for (int it = 0; it < C::IPT; ++it) {
+      const uint32_t li = (uint32_t)(w * C::WSPAN + it * kWave + lane);
+      const bool valid = li < tn;
+      const uint64_t peers = match_peers<BITS>(dg[it], __ballot(valid));
+      const uint32_t before = (uint32_t)__popcll(peers & lt);
+      uint32_t old = 0;
+      if (valid) old = wcnt[w][dg[it]];
+      rk[it] = old + before;
+      if (valid && before == 0) wcnt[w][dg[it]] = (uint16_t)(old + (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+
+    // Per digit: exclusive offsets over waves (in place) and the tile total.
+    for (int d = threadIdx.x; d < F; d += C::NT) {
+      uint32_t run = 0;
+#pragma unroll
+      for (int ww = 0; ww < C::NW; ++ww) {
+        const uint32_t c = wcnt[ww][d];
+        wcnt[ww][d] = (uint16_t)run;
+        run += c;
+      }
+      tbase[d] = run;
+    }
+    __syncthreads();
+    {
+      uint32_t loc[C::DPT];
+      uint32_t sum = 0;
+#pragma unroll
+      for (int j = 0; j < C::DPT; ++j) {
+        const int d = threadIdx.x * C::DPT + j;
+        loc[j] = d < F ? tbase[d] : 0u;
+        sum += loc[j];
+      }
+      uint32_t tot;
+      uint32_t ex = block_excl_scan<C::NT, uint32_t>(sum, scan_tmp, &tot);
+#pragma unroll
+      for (int j = 0; j < C::DPT; ++j) {
+        const int d = threadIdx.x * C::DPT + j;
+        if (d < F) tbase[d] = ex;
+        ex += loc[j];
+      }
+      if (threadIdx.x == 0) tbase[F] = tn;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int it = 0; it < C::IPT; ++it) {
+      const uint32_t li = (uint32_t)(w * C::WSPAN + it * kWave + lane);
+      if (li < tn) stage[tbase[dg[it]] + wcnt[w][dg[it]] + rk[it]] = v[it];
+    }
+    __syncthreads();
+
+    for (uint32_t i = threadIdx.x; i < tn; i += C::NT) {
+      const uint2 t = stage[i];
+      const uint32_t d = digit_of(t.x, xm, shift, F - 1);
+      out[(uint64_t)cursor[d] + (i - tbase[d])] = t;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < F; d += C::NT) cursor[d] += tbase[d + 1] - tbase[d];
+  }
+}
+
+__global__ void k_part_starts(const uint32_t *offsets, uint32_t G, uint32_t F, uint64_t n, uint32_t *part_start) {
+  const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d < F) part_start[d] = offsets[(uint64_t)d * G];
+  if (d == F) part_start[F] = (uint32_t)n;
+}
+
+__global__ void k_part_bounds(const uint2 *__restrict__ sorted, uint64_t n, int shift, uint32_t xm, uint32_t F,
+                              uint32_t *part_start) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const int64_t dprev = i == 0 ? -1 : (int64_t)digit_of(sorted[i - 1].x, xm, shift, F - 1);
+    const int64_t dcur = i == n ? (int64_t)F : (int64_t)digit_of(sorted[i].x, xm, shift, F - 1);
+    for (int64_t d = dprev + 1; d <= dcur; ++d) part_start[d] = (uint32_t)i;
+  }
+}
+
+template <int BITS>
+hipError_t launch_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out, uint32_t *counts,
+                       void *scan_scratch, uint32_t *err, hipStream_t stream) {
+  const uint2 *vin = reinterpret_cast<const uint2 *>(in);
+  uint2 *vout = reinterpret_cast<uint2 *>(out);
+  hipLaunchKernelGGL(k_part_hist<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift, p.xor_mask,
+                     p.stripes, p.stripe_len, counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t m = (uint64_t)p.stripes << BITS;
+  e = scan_u32(counts, counts, m, nullptr, scan_scratch, err, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_part_scatter<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
+                     p.xor_mask, p.stripes, p.stripe_len, counts, vout);
+  return hipGetLastError();
+}
+
+uint32_t tile_of_bits(int bits) {
+  switch (bits) {
+#define EO_TILE_CASE(B) \
+  case B:               \
+    return PartCfg<B>::TILE;
+    EO_TILE_CASE(1) EO_TILE_CASE(2) EO_TILE_CASE(3) EO_TILE_CASE(4) EO_TILE_CASE(5) EO_TILE_CASE(6)
+    EO_TILE_CASE(7) EO_TILE_CASE(8) EO_TILE_CASE(9) EO_TILE_CASE(10) EO_TILE_CASE(11)
+#undef EO_TILE_CASE
+    default:
+      return 0;
+  }
+}
+
+}  // namespace
+
+PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus) {
+  PartPassPlan p{};
+  p.n = n;
+  p.bits = bits;
+  p.shift = shift;
+  p.xor_mask = xor_mask;
+  const uint64_t tile = tile_of_bits(bits);
+  const uint64_t tiles = std::max<uint64_t>(1, (n + tile - 1) / tile);
+  // Enough stripes to fill the chip twice over, each several tiles long.
+  uint64_t g = std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 8) * 2);
+  g = (g + kXcds - 1) / kXcds * kXcds;
+  uint64_t len = (n + g - 1) / g;
+  len = std::max<uint64_t>(tile, (len + tile - 1) / tile * tile);
+  p.stripes = (uint32_t)g;
+  p.stripe_len = len;
+  return p;
+}
+
+size_t partition_pass_scratch_bytes(const PartPassPlan &p) {
+  const uint64_t m = (uint64_t)p.stripes << p.bits;
+  const size_t counts = (m * sizeof(uint32_t) + 255) / 256 * 256;
+  return counts + scan_scratch_bytes_u32(m);
+}
+
+hipError_t partition_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out, uint32_t *part_start,
+                          void *scratch, uint32_t *err, hipStream_t stream) {
+  const uint64_t m = (uint64_t)p.stripes << p.bits;
+  uint32_t *counts = (uint32_t *)scratch;
+  void *scan_scratch = (char *)scratch + (m * sizeof(uint32_t) + 255) / 256 * 256;
+  hipError_t e;
+  switch (p.bits) {
+#define EO_PASS_CASE(B)                                                    \
+  case B:                                                                  \
+    e = launch_pass<B>(p, in, out, counts, scan_scratch, err, stream); \
+    break;
+    EO_PASS_CASE(1) EO_PASS_CASE(2) EO_PASS_CASE(3) EO_PASS_CASE(4) EO_PASS_CASE(5) EO_PASS_CASE(6)
+    EO_PASS_CASE(7) EO_PASS_CASE(8) EO_PASS_CASE(9) EO_PASS_CASE(10) EO_PASS_CASE(11)
+#undef EO_PASS_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess || !part_start) return e;
+  const uint32_t F = 1u << p.bits;
+  hipLaunchKernelGGL(k_part_starts, dim3((F + 1 + 255) / 256), dim3(256), 0, stream, counts, p.stripes, F, p.n,
+                     part_start);
+  return hipGetLastError();
+}
+
+hipError_t partition_bounds(const eo_tuple *sorted, uint64_t n, int bits, int shift, uint32_t xor_mask,
+                            uint32_t *part_start, hipStream_t stream) {
+  const uint32_t F = 1u << bits;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 1 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_part_bounds, dim3(blocks), dim3(256), 0, stream, reinterpret_cast<const uint2 *>(sorted), n,
+                     shift, xor_mask, F, part_start);
+  return hipGetLastError();
+}
+
+}  // namespace eo

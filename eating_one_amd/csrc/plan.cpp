@@ -1,0 +1,64 @@
+#include "plan.hpp"
+
+#include "kernels.hpp"
+
+namespace eo {
+
+namespace {
+int ceil_log2(uint64_t x) {
+  int b = 0;
+  while ((1ull << b) < x) ++b;
+  return b;
+}
+bool valid_cap(int c) { return c == 4096 || c == 8192 || c == 16384; }
+}  // namespace
+
+eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPlan *out) {
+  HashPlan p;
+  const int cap_opt = o ? o->lds_tuples : 0;
+  if (cap_opt != 0 && !valid_cap(cap_opt)) return EO_EINVAL;
+  p.lds_cap = cap_opt ? (uint32_t)cap_opt : 16384u;
+
+  const int bits_opt = o ? o->radix_bits : 0;
+  if (bits_opt < -1 || bits_opt > kMaxJoinBits) return EO_EINVAL;
+  if (bits_opt == -1) {
+    p.bits = 0;
+  } else if (bits_opt > 0) {
+    p.bits = bits_opt;
+  } else if (nR <= p.lds_cap) {
+    p.bits = 0;
+  } else {
+    // Aim the mean R partition at 7/8 of the LDS table so that binomial
+    // spread rarely spills a partition into a second R chunk.
+    const uint64_t target = (uint64_t)p.lds_cap * 7 / 8;
+    p.bits = ceil_log2((nR + target - 1) / target);
+    if (p.bits > kMaxJoinBits) p.bits = kMaxJoinBits;
+  }
+
+  const int passes_opt = o ? o->passes : 0;
+  if (passes_opt < 0 || passes_opt > 2) return EO_EINVAL;
+  if (p.bits == 0) {
+    p.passes = 0;
+  } else if (passes_opt) {
+    p.passes = passes_opt;
+  } else {
+    p.passes = p.bits <= kMaxPassBits ? 1 : 2;
+  }
+  if (p.passes == 1) {
+    if (p.bits > kMaxPassBits) return EO_EINVAL;
+    p.bits_pass1 = p.bits;
+  } else if (p.passes == 2) {
+    if (p.bits < 2 || p.bits > 2 * kMaxPassBits) return EO_EINVAL;
+    p.bits_pass1 = p.bits / 2;
+    p.bits_pass2 = p.bits - p.bits_pass1;
+  }
+
+  const int st_opt = o ? o->s_tile : 0;
+  if (st_opt < 0) return EO_EINVAL;
+  p.s_tile = st_opt ? (uint32_t)st_opt : 16384u;
+  (void)nS;
+  *out = p;
+  return EO_OK;
+}
+
+}  // namespace eo

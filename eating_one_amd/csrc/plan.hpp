@@ -1,0 +1,31 @@
+// Host-only planning for the hash join: radix bits, passes, LDS table size
+// and S tile, per input size.  Pure functions (no HIP calls), so they run and
+// are tested without a GPU.
+//
+// The reference plans at joins/hashJoin_gpu/main_street/hj.cc:378-436:
+// p_num = |R| / PART_STANDARD(64) rounded to a power of 64 or of 2, then
+// 6 bits per pass.  Here partitions are sized to the LDS table instead, so
+// that each R partition is built once in LDS and probed by its S partition.
+#pragma once
+
+#include <stdint.h>
+
+#include "eojoin.h"
+
+namespace eo {
+
+struct HashPlan {
+  int bits = 0;        // total radix bits (0 = single partition)
+  int passes = 0;      // 0 (no partitioning), 1 or 2
+  int bits_pass1 = 0;  // LSD: first pass takes the low bits
+  int bits_pass2 = 0;
+  uint32_t lds_cap = 0;
+  uint32_t s_tile = 0;
+};
+
+constexpr int kMaxJoinBits = 22;
+
+// Returns EO_OK and fills *out, or EO_EINVAL for inconsistent overrides.
+eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *opts, HashPlan *out);
+
+}  // namespace eo

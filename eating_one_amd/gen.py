@@ -1,0 +1,141 @@
+"""Seeded synthetic join inputs for the five workload shapes.
+
+The reference generates unseeded data (srand(time(NULL)),
+joins/hashJoin_gpu/main_street/hj.cc:62-160); these generators are seeded
+numpy PCG64 streams so that the GPU path and the CPU oracle see identical
+inputs.  Tuples are int32 arrays (n, 2) = (key, payload = row index).
+
+  c1: R = 2^16 keys, a permutation of [0, 2^16); S = 2^20 keys uniform in
+      [0, 2^16)  -> every S tuple matches exactly once.
+  c2: R = permutation of [0, nR); S keys: with probability `selectivity` an
+      R key drawn by Zipf(theta) rank (rank r -> the key of R row r, R being
+      a random permutation), otherwise uniform in the disjoint [nR, 2^31).
+  c3: R, S keys uniform in [0, key_range) (duplicates on both sides).
+  c4: R = nR unique lognormal(0, sigma) keys scaled into [0, 2^31); S =
+      80 % keys sampled from R, 20 % uniform keys not in R.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+INT31 = (1 << 31) - 1
+
+
+def _tuples(keys: np.ndarray) -> np.ndarray:
+    out = np.empty((len(keys), 2), dtype=np.int32)
+    out[:, 0] = keys
+    out[:, 1] = np.arange(len(keys), dtype=np.int32)
+    return out
+
+
+def zipf_ranks(rng: np.random.Generator, n_items: int, theta: float, size: int) -> np.ndarray:
+    """Ranks in [0, n_items) with P(rank k) proportional to 1/(k+1)^theta.
+
+    theta == 0 is uniform.  Otherwise rejection-inversion sampling (Hoermann &
+    Derflinger, "Rejection-inversion to generate variates from monotone
+    discrete distributions", ACM TOMACS 6(3), 1996), vectorised.
+    """
+    if size == 0:
+        return np.empty(0, dtype=np.int64)
+    if theta == 0.0:
+        return rng.integers(0, n_items, size=size, dtype=np.int64)
+    s = float(theta)
+    one_m_s = 1.0 - s
+
+    def helper1(x):  # log1p(x)/x, continuous at 0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(np.abs(x) > 1e-8, np.log1p(x) / np.where(x == 0, 1, x), 1.0 - x * (0.5 - x / 3.0))
+        return r
+
+    def helper2(x):  # expm1(x)/x, continuous at 0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            r = np.where(np.abs(x) > 1e-8, np.expm1(x) / np.where(x == 0, 1, x), 1.0 + x * 0.5 * (1.0 + x / 3.0))
+        return r
+
+    def h(x):
+        return np.exp(-s * np.log(x))
+
+    def h_int(x):
+        lx = np.log(x)
+        return helper2(one_m_s * lx) * lx
+
+    def h_int_inv(x):
+        t = x * one_m_s
+        t = np.maximum(t, -1.0)
+        return np.exp(helper1(t) * x)
+
+    h_x1 = float(h_int(np.array(1.5))) - 1.0
+    h_n = float(h_int(np.array(n_items + 0.5)))
+    s_const = 2.0 - float(h_int_inv(np.array(float(h_int(np.array(2.5))) - float(h(np.array(2.0))))))
+
+    out = np.empty(size, dtype=np.int64)
+    todo = np.arange(size)
+    while len(todo):
+        u = h_n + rng.random(len(todo)) * (h_x1 - h_n)
+        x = h_int_inv(u)
+        k = np.clip(np.floor(x + 0.5), 1, n_items)
+        ok = (k - x <= s_const) | (u >= h_int(k + 0.5) - h(k))
+        out[todo[ok]] = k[ok].astype(np.int64) - 1
+        todo = todo[~ok]
+    return out
+
+
+def c1(seed: int = 0xE01C1, nR: int = 1 << 16, nS: int = 1 << 20):
+    rng = np.random.default_rng(seed)
+    R = _tuples(rng.permutation(nR).astype(np.int32))
+    S = _tuples(rng.integers(0, nR, size=nS, dtype=np.int32))
+    return R, S
+
+
+def c2(seed: int = 0xE01C2, nR: int = 1 << 27, nS: int = 1 << 28, theta: float = 0.0, selectivity: float = 1.0):
+    rng = np.random.default_rng(seed)
+    rkeys = rng.permutation(nR).astype(np.int32)
+    R = _tuples(rkeys)
+    hit = rng.random(nS) < selectivity if selectivity < 1.0 else np.ones(nS, dtype=bool)
+    nhit = int(hit.sum())
+    skeys = np.empty(nS, dtype=np.int32)
+    skeys[hit] = rkeys[zipf_ranks(rng, nR, theta, nhit)]
+    skeys[~hit] = rng.integers(nR, INT31, size=nS - nhit, dtype=np.int32)
+    return R, _tuples(skeys)
+
+
+def c3(seed: int = 0xE01C3, nR: int = 1 << 18, nS: int = 1 << 18, key_range: int = 1 << 16):
+    rng = np.random.default_rng(seed)
+    R = _tuples(rng.integers(0, key_range, size=nR, dtype=np.int32))
+    S = _tuples(rng.integers(0, key_range, size=nS, dtype=np.int32))
+    return R, S
+
+
+def c4(seed: int = 0xE01C4, nR: int = 1 << 26, nS: int = 1 << 27, sigma: float = 2.0, hit_frac: float = 0.8):
+    rng = np.random.default_rng(seed)
+    keys = np.empty(0, dtype=np.int64)
+    # lognormal draws scaled so that the 1 - 1e-6 quantile lands near 2^31
+    scale = INT31 / np.exp(sigma * 4.753424)
+    while len(keys) < nR:
+        draw = np.floor(rng.lognormal(0.0, sigma, size=nR) * scale)
+        draw = draw[(draw >= 0) & (draw <= INT31)].astype(np.int64)
+        keys = np.unique(np.concatenate([keys, draw]))
+        if len(keys) > nR:
+            keys = rng.choice(keys, size=nR, replace=False)
+            keys.sort()
+    rkeys = keys.astype(np.int32)
+    R = _tuples(rng.permutation(rkeys))
+    nhit = int(round(nS * hit_frac))
+    skeys = np.empty(nS, dtype=np.int32)
+    skeys[:nhit] = rkeys[rng.integers(0, nR, size=nhit)]
+    miss = np.empty(0, dtype=np.int64)
+    while len(miss) < nS - nhit:
+        cand = rng.integers(0, INT31, size=(nS - nhit) * 2, dtype=np.int64)
+        pos = np.searchsorted(keys, cand)
+        pos = np.minimum(pos, len(keys) - 1)
+        miss = np.concatenate([miss, cand[keys[pos] != cand]])
+    skeys[nhit:] = miss[: nS - nhit].astype(np.int32)
+    skeys = rng.permutation(skeys)
+    return R, _tuples(skeys)
+
+
+def uniform(seed: int, nR: int, nS: int, key_range: int, lo: int = 0):
+    rng = np.random.default_rng(seed)
+    R = _tuples(rng.integers(lo, lo + key_range, size=nR, dtype=np.int64).astype(np.int32))
+    S = _tuples(rng.integers(lo, lo + key_range, size=nS, dtype=np.int64).astype(np.int32))
+    return R, S

@@ -1,0 +1,180 @@
+/*
+ * eojoin.h — C ABI of the MI355X inner-join library (libeojoin.so).
+ *
+ * Data model (reference: joins/hashJoin_gpu/main_street/tuple.h:43-53,
+ * joins/CPU_JOIN/hash_cpu/tuple.h:16-26):
+ *   the reference TUPLE is {int key (row id), int val (join attribute)};
+ *   here eo_tuple.key is the join attribute (reference .val) and
+ *   eo_tuple.payload is the row id (reference .key).  An output eo_pair is
+ *   (R payload, S payload) = (RESULT.rkey, RESULT.lkey) of the reference.
+ *   R is the build side (reference "right"/rt), S the probe side ("left"/lt).
+ *
+ * Conventions (replacing the reference's printf+exit(1) error handling,
+ * joins/hashJoin_gpu/main_street/hj.cc:214-227):
+ *   - every entry point returns an eo_status; nothing calls exit();
+ *   - inputs are caller-owned; opts->where says whether R/S/out are host or
+ *     device pointers (EO_DEVICE: inputs already resident in HBM);
+ *   - two-phase sizing, mirroring count→scan→alloc→write of hj.cc:1040-1069:
+ *     out == NULL returns the match count in *n_out and writes nothing;
+ *     out_cap < count returns EO_ECAPACITY with *n_out set and nothing written;
+ *   - output order is unspecified: compare results as multisets;
+ *   - one eo_ctx per host thread; calls on one ctx are serialised by the caller;
+ *   - scratch memory comes from a ctx-owned device arena that only grows.
+ */
+#ifndef EOJOIN_H
+#define EOJOIN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EOJOIN_ABI_VERSION 1u
+
+typedef struct eo_tuple {
+  int32_t key;     /* join attribute (reference TUPLE.val) */
+  int32_t payload; /* row id        (reference TUPLE.key) */
+} eo_tuple;
+
+typedef struct eo_pair {
+  int32_t r_payload; /* reference RESULT.rkey */
+  int32_t s_payload; /* reference RESULT.lkey */
+} eo_pair;
+
+typedef enum eo_status {
+  EO_OK = 0,
+  EO_EINVAL = 1,     /* bad argument */
+  EO_ENOMEM = 2,     /* device or host allocation failed */
+  EO_ECAPACITY = 3,  /* out_cap < match count; *n_out holds the count */
+  EO_EHIP = 4,       /* HIP runtime error or device-side fault flag */
+  EO_EBUDGET = 5,    /* request exceeds the HBM admission budget */
+  EO_ENODEV = 6,     /* no usable GPU */
+  EO_EAGAIN = 7,     /* not ready yet (non-blocking poll) */
+  EO_ECANCELED = 8,  /* request cancelled */
+  EO_ETIMEDOUT = 9,  /* request deadline expired */
+  EO_EINTERNAL = 10  /* internal invariant violated */
+} eo_status;
+
+typedef enum eo_where { EO_HOST = 0, EO_DEVICE = 1 } eo_where;
+
+typedef struct eo_ctx eo_ctx;
+typedef struct eo_index eo_index;
+
+typedef struct eo_ctx_opts {
+  uint32_t struct_size;  /* sizeof(eo_ctx_opts) */
+  uint32_t flags;        /* reserved, 0 */
+  uint64_t arena_bytes;  /* initial scratch arena size (0 = grow on demand) */
+  uint64_t hbm_budget;   /* admission budget for batch/serve paths (0 = 200 GB) */
+} eo_ctx_opts;
+
+/* Per-phase device times of the last call (hipEvent based), in ms. */
+typedef struct eo_join_stats {
+  double ms_total;
+  double ms_partition;   /* histogram + scan + scatter, both relations */
+  double ms_count;       /* count pass (incl. per-tile scan) */
+  double ms_write;       /* write pass */
+  double ms_build;       /* index build (index joins) */
+  uint64_t n_out;        /* match count */
+  uint32_t radix_bits;   /* partition bits used */
+  uint32_t passes;       /* partition passes used */
+  uint32_t tiles;        /* join work tiles (partition x R-chunk x S-range) */
+  uint32_t lds_tuples;   /* LDS table capacity used */
+} eo_join_stats;
+
+typedef struct eo_join_opts {
+  uint32_t struct_size;  /* sizeof(eo_join_opts) */
+  int32_t where;         /* EO_HOST or EO_DEVICE for R, S and out */
+  int32_t radix_bits;    /* total radix bits of the partition (0 = autotune) */
+  int32_t passes;        /* partition passes, 1 or 2 (0 = autotune) */
+  int32_t lds_tuples;    /* R tuples per LDS table (0 = autotune) */
+  int32_t s_tile;        /* S tuples per join tile (0 = autotune) */
+  int32_t flags;         /* EO_F_* bits */
+  int32_t reserved;
+  void *stream;          /* hipStream_t to run on; NULL = the ctx stream */
+  eo_join_stats *stats;  /* optional out: per-phase times of this call */
+} eo_join_opts;
+
+#define EO_F_NO_SYNC_STATS 1  /* do not record per-phase events */
+
+typedef struct eo_device_info {
+  char name[64];
+  char arch[32];          /* e.g. "gfx950" */
+  int32_t cus;            /* compute units */
+  int32_t xcds;           /* accelerator complex dies (8 on MI355X) */
+  int32_t wave_size;      /* 64 */
+  int32_t clock_khz;
+  uint64_t lds_per_wg;    /* max LDS bytes per workgroup */
+  uint64_t hbm_bytes;     /* total device memory */
+  uint64_t l2_bytes;
+} eo_device_info;
+
+/* ---- library / context --------------------------------------------------- */
+
+uint32_t eo_abi_version(void);
+const char *eo_status_string(eo_status st);
+/* Message of the last failure on this host thread (HIP error text etc.). */
+const char *eo_last_error(void);
+
+/* Replaces cuInit/cuDeviceGet/cuCtxCreate + cuModuleLoad of hj.cc:213-289. */
+eo_status eo_ctx_create(eo_ctx **ctx, int device, const eo_ctx_opts *opts);
+eo_status eo_ctx_destroy(eo_ctx *ctx);
+eo_status eo_ctx_sync(eo_ctx *ctx);
+eo_status eo_device_query(eo_ctx *ctx, eo_device_info *info);
+
+/* ---- device memory (replaces cuMemAlloc / cuMemcpyHtoD / cuMemcpyDtoH of
+ *      hj.cc:324-354 and hj.cc:1124) -------------------------------------- */
+
+eo_status eo_dev_alloc(eo_ctx *ctx, uint64_t bytes, void **dptr);
+eo_status eo_dev_free(eo_ctx *ctx, void *dptr);
+eo_status eo_copy_h2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+eo_status eo_copy_d2h(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+eo_status eo_copy_d2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* ---- primitives ---------------------------------------------------------- */
+
+/* Exclusive prefix sum of n uint32 values; *total (optional) = sum.
+ * Replaces presum() (joins/hashJoin_gpu/main_street/scan_main.cpp:38-111,
+ * API scan_common.h:100-103) and getValue() (scan_common.h:112-115), without
+ * its allocate-and-free-the-input ownership transfer: out may equal in. */
+eo_status eo_exclusive_scan_u32(eo_ctx *ctx, const uint32_t *in, uint32_t *out,
+                                uint64_t n, uint64_t *total,
+                                const eo_join_opts *opts);
+eo_status eo_exclusive_scan_u64(eo_ctx *ctx, const uint64_t *in, uint64_t *out,
+                                uint64_t n, uint64_t *total,
+                                const eo_join_opts *opts);
+
+/* Stable radix partition of n tuples by digit ((uint32)key >> shift) & (2^bits-1),
+ * 1 <= bits <= 16.  part_off (2^bits + 1 entries, optional) receives the
+ * partition start offsets.  Replaces lcount_partitioning + presum +
+ * lpartitioning + countPartition (joins/hashJoin_gpu/main_street/
+ * partitioning.cu:15-131, :273-293; hj.cc:430-656); its output equals the
+ * stable partition of joins/CPU_JOIN/hash_cpu/hj.cc:132-181 for key >= 0. */
+eo_status eo_radix_partition(eo_ctx *ctx, const eo_tuple *in, uint64_t n,
+                             int bits, int shift, eo_tuple *out,
+                             uint64_t *part_off, const eo_join_opts *opts);
+
+/* ---- joins --------------------------------------------------------------- */
+
+/* Radix-partitioned hash join: replaces join() of
+ * joins/hashJoin_gpu/main_street/hj.cc:171-1269 (CPU oracle
+ * joins/CPU_JOIN/hash_cpu/hj.cc:116-228). */
+eo_status eo_hash_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
+                       const eo_tuple *S, uint64_t nS, eo_pair *out,
+                       uint64_t out_cap, uint64_t *n_out,
+                       const eo_join_opts *opts);
+
+/* Block nested-loop join: replaces join() of
+ * joins/blockJoin_gpu/main_street/nestLoopJoin.cc:180-655 (CPU oracle
+ * joins/CPU_JOIN/block_cpu/nestLoopJoin.cc:61-77). */
+eo_status eo_nlj_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
+                      const eo_tuple *S, uint64_t nS, eo_pair *out,
+                      uint64_t out_cap, uint64_t *n_out,
+                      const eo_join_opts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EOJOIN_H */

@@ -1,0 +1,75 @@
+"""CPU-side checks of the C ABI: the library loads, exports every symbol
+include/*.h declares, and its host-only logic behaves — no compute call."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from eating_one_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b(eo_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("eo_hash_join", "eo_nlj_join", "eo_radix_partition", "eo_exclusive_scan_u32", "eo_ctx_create"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_abi.LIB_PATH)
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, f"declared but not exported: {missing}"
+
+
+def test_ctypes_signatures_cover_header():
+    names = set(declared_symbols())
+    assert set(_abi.SIGNATURES) <= names
+
+
+def test_status_strings_and_version():
+    L = _abi.load()
+    assert L.eo_abi_version() == 1
+    for st in range(11):
+        s = L.eo_status_string(st).decode()
+        assert s and s != "unknown status"
+    assert L.eo_status_string(99).decode() == "unknown status"
+
+
+def test_struct_sizes_match_c_layout():
+    # eo_join_opts: 8 x int32 + 2 pointers
+    assert ctypes.sizeof(_abi.JoinOpts) == 8 * 4 + 2 * 8
+    assert ctypes.sizeof(_abi.CtxOpts) == 24
+    assert ctypes.sizeof(_abi.DeviceInfo) == 64 + 32 + 4 * 4 + 3 * 8
+
+
+def test_ctx_create_without_gpu_reports_nodev():
+    """On a machine without a GPU, context creation fails with EO_ENODEV (no exit, no crash)."""
+    L = _abi.load()
+    h = ctypes.c_void_p()
+    st = L.eo_ctx_create(ctypes.byref(h), 0, None)
+    if st == _abi.EO_OK:
+        L.eo_ctx_destroy(h)
+        pytest.skip("a GPU is visible")
+    assert st == _abi.EO_ENODEV
+    assert h.value is None
+    assert L.eo_ctx_destroy(None) == _abi.EO_EINVAL
+
+
+def test_invalid_arguments_rejected_before_device_use():
+    L = _abi.load()
+    n = ctypes.c_uint64()
+    assert L.eo_hash_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL
+    assert L.eo_nlj_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL

@@ -1,0 +1,145 @@
+"""GPU parity of the primitives and the hash / nested-loop joins, through the C ABI.
+
+Every result is compared with the CPU oracle on the same seeded inputs:
+bit-exact for scans and stable partitions, as a sorted multiset with an
+identical count for joins.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from eating_one_amd import CapacityError, gen
+from common import (brute_join, eo_to_ref, join_count_closed_form, join_ori_doc, libc_rand_table, ref_to_eo,
+                    same_multiset, sha256)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 4095, 4096, 4097, 100003, 1 << 20, (1 << 22) + 17])
+def test_scan_u32(ctx, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(0, 1 << 10, size=n).astype(np.uint32)
+    out, tot = ctx.exclusive_scan(a)
+    want, wtot = oracle.exclusive_scan(a)
+    assert np.array_equal(out, want)
+    assert tot == wtot
+
+
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 300001, 1 << 21])
+def test_scan_u64(ctx, n):
+    rng = np.random.default_rng(n + 1)
+    a = rng.integers(0, 1 << 40, size=n).astype(np.uint64)
+    out, tot = ctx.exclusive_scan(a)
+    want, wtot = oracle.exclusive_scan(a)
+    assert np.array_equal(out, want)
+    assert tot == wtot
+
+
+def test_partition_reproduces_join_ori_hash_part_files(ctx):
+    doc = join_ori_doc()
+    for table in ("R", "S"):
+        T = ref_to_eo(libc_rand_table(doc["tables"][table]["rows"]))
+        out, off = ctx.radix_partition(T, 2)
+        for p in range(4):
+            part = eo_to_ref(out[off[p]:off[p + 1]])
+            assert sha256(part) == doc["partitions"][f"{table}-{p}"]["sha256"]
+
+
+@pytest.mark.parametrize("bits,shift", [(1, 0), (4, 0), (6, 3), (8, 0), (9, 0), (11, 0), (11, 5), (12, 0), (14, 0),
+                                        (16, 0), (8, 24)])
+def test_partition_stable_bits(ctx, bits, shift):
+    rng = np.random.default_rng(bits * 100 + shift)
+    n = 200003
+    T = np.stack([rng.integers(-(1 << 31), (1 << 31) - 1, size=n, dtype=np.int64).astype(np.int32),
+                  np.arange(n, dtype=np.int32)], 1)
+    out, off = ctx.radix_partition(T, bits, shift)
+    want, woff = oracle.partition_bits(T, bits, shift)
+    assert np.array_equal(off, woff)
+    assert np.array_equal(out, want)
+
+
+def test_partition_skewed_and_tiny(ctx):
+    for n in (0, 1, 5, 4096, 4097):
+        T = np.stack([np.full(n, 7, dtype=np.int32), np.arange(n, dtype=np.int32)], 1)
+        out, off = ctx.radix_partition(T, 8)
+        want, woff = oracle.partition_bits(T, 8, 0)
+        assert np.array_equal(out, want) and np.array_equal(off, woff)
+
+
+def test_hash_join_c1_full(ctx):
+    R, S = gen.c1()
+    got = ctx.hash_join(R, S)
+    assert len(got) == 1 << 20
+    assert same_multiset(got, oracle.hash_join(R, S))
+
+
+@pytest.mark.parametrize("bits,passes,cap", [(-1, 0, 16384), (3, 1, 4096), (8, 1, 8192), (11, 1, 4096),
+                                             (12, 2, 4096), (14, 2, 16384), (0, 0, 8192)])
+def test_hash_join_plans(ctx, bits, passes, cap):
+    R, S = gen.c3(seed=bits + 100, nR=60000, nS=90000, key_range=1 << 15)
+    got = ctx.hash_join(R, S, radix_bits=bits, passes=passes, lds_tuples=cap)
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == join_count_closed_form(R, S)
+    assert same_multiset(got, want)
+
+
+def test_hash_join_edge_cases(ctx):
+    e = np.empty((0, 2), dtype=np.int32)
+    one = np.array([[5, 9]], dtype=np.int32)
+    assert len(ctx.hash_join(e, e)) == 0
+    assert len(ctx.hash_join(one, e)) == 0
+    assert len(ctx.hash_join(e, one)) == 0
+    assert same_multiset(ctx.hash_join(one, one), np.array([[9, 9]]))
+    # extreme and negative keys
+    R = np.array([[-(1 << 31), 0], [(1 << 31) - 1, 1], [0, 2], [-1, 3]], dtype=np.int32)
+    S = np.array([[(1 << 31) - 1, 0], [-(1 << 31), 1], [-1, 2], [7, 3]], dtype=np.int32)
+    assert same_multiset(ctx.hash_join(R, S), np.array([[1, 0], [0, 1], [3, 2]]))
+    assert same_multiset(ctx.hash_join(R, S, radix_bits=4), np.array([[1, 0], [0, 1], [3, 2]]))
+
+
+def test_hash_join_heavy_duplicates(ctx):
+    # one key everywhere: R partition larger than any LDS table -> several R chunks
+    R = np.stack([np.full(20000, 42), np.arange(20000)], 1).astype(np.int32)
+    S = np.stack([np.full(300, 42), np.arange(300)], 1).astype(np.int32)
+    got = ctx.hash_join(R, S, radix_bits=4)
+    assert len(got) == 20000 * 300
+    assert same_multiset(got, brute_join(R, S))
+
+
+def test_hash_join_count_only_and_capacity(ctx):
+    R, S = gen.c3(seed=5, nR=5000, nS=5000, key_range=1000)
+    n = ctx.hash_join(R, S, count_only=True)
+    assert n == join_count_closed_form(R, S)
+    import ctypes
+    from eating_one_amd import _abi
+    out = np.empty((n - 1, 2), dtype=np.int32)
+    cnt = ctypes.c_uint64()
+    o = ctx.opts()
+    st = ctx.lib.eo_hash_join(ctx.handle, R.ctypes.data, len(R), S.ctypes.data, len(S), out.ctypes.data, n - 1,
+                              ctypes.byref(cnt), ctypes.byref(o))
+    assert st == _abi.EO_ECAPACITY and cnt.value == n
+
+
+@pytest.mark.parametrize("theta,sel", [(0.0, 1.0), (0.5, 1.0), (0.9, 1.0), (0.0, 0.1), (0.9, 0.1)])
+def test_hash_join_c2_shape_small(ctx, theta, sel):
+    R, S = gen.c2(seed=17, nR=1 << 18, nS=1 << 19, theta=theta, selectivity=sel)
+    got = ctx.hash_join(R, S)
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert same_multiset(got, want)
+
+
+def test_nlj_small_and_edges(ctx):
+    R, S = gen.c3(seed=9, nR=3000, nS=5000, key_range=200)
+    assert same_multiset(ctx.nlj_join(R, S), oracle.nlj_join(R, S))
+    e = np.empty((0, 2), dtype=np.int32)
+    assert len(ctx.nlj_join(e, R)) == 0
+    # ragged tiles: sizes not multiples of the 4096 x 2048 tile
+    R, S = gen.c3(seed=10, nR=4097, nS=2049, key_range=50)
+    assert same_multiset(ctx.nlj_join(R, S), oracle.nlj_join(R, S))
+
+
+def test_nlj_c3_full(ctx):
+    R, S = gen.c3()
+    got = ctx.nlj_join(R, S)
+    assert len(got) == join_count_closed_form(R, S)
+    assert same_multiset(got, oracle.hash_join(R, S, nparts=1 << 12))
