@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import EO_DEVICE, EO_HOST, JoinOpts, JoinStats
+from ._abi import EO_DEVICE, EO_F_DETERMINISTIC, EO_HOST, JoinOpts, JoinStats
 
 __all__ = ["Context", "DeviceArray", "EOError", "CapacityError", "load_library"]
 
@@ -136,7 +136,8 @@ class Context:
 
     # ---- options -----------------------------------------------------------
     @staticmethod
-    def opts(where=EO_HOST, radix_bits=0, passes=0, lds_tuples=0, s_tile=0, stats: JoinStats | None = None):
+    def opts(where=EO_HOST, radix_bits=0, passes=0, lds_tuples=0, s_tile=0, stats: JoinStats | None = None,
+             deterministic: bool = False, flags: int = 0):
         o = JoinOpts()
         o.struct_size = ctypes.sizeof(JoinOpts)
         o.where = where
@@ -144,6 +145,7 @@ class Context:
         o.passes = passes
         o.lds_tuples = lds_tuples
         o.s_tile = s_tile
+        o.flags = flags | (EO_F_DETERMINISTIC if deterministic else 0)
         o.stats = ctypes.pointer(stats) if stats is not None else None
         return o
 

@@ -17,6 +17,7 @@ PU64 = ctypes.POINTER(ctypes.c_uint64)
 EO_OK, EO_EINVAL, EO_ENOMEM, EO_ECAPACITY, EO_EHIP, EO_EBUDGET = 0, 1, 2, 3, 4, 5
 EO_ENODEV, EO_EAGAIN, EO_ECANCELED, EO_ETIMEDOUT, EO_EINTERNAL = 6, 7, 8, 9, 10
 EO_HOST, EO_DEVICE = 0, 1
+EO_F_NO_SYNC_STATS, EO_F_DETERMINISTIC = 1, 2
 
 
 class JoinStats(ctypes.Structure):
@@ -86,6 +87,7 @@ SIGNATURES = {
     "eo_copy_h2d": (INT, [P, P, P, U64]),
     "eo_copy_d2h": (INT, [P, P, P, U64]),
     "eo_copy_d2d": (INT, [P, P, P, U64]),
+    "eo_bandwidth_probe": (INT, [P, U64, INT, ctypes.POINTER(ctypes.c_double)]),
     "eo_exclusive_scan_u32": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_exclusive_scan_u64": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),

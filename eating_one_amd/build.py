@@ -26,6 +26,7 @@ SOURCES = [
     "index.hip",
     "learned.hip",
     "gen.hip",
+    "util.hip",
     "batch.hip",
     "api.cpp",
     "api_index.cpp",

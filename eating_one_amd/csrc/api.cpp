@@ -80,12 +80,16 @@ bool opts_ok(const eo_join_opts *o) {
 
 constexpr uint64_t kMaxRows = (1ull << 31) - 1;
 
-// Partition `n` tuples into `out` (LSD over 1 or 2 passes) and write the
-// partition starts (2^bits + 1 u32).  tmp is needed for 2 passes.
+// Partition `n` tuples into `out` and write the partition starts
+// (2^bits + 1 u32).  One pass uses the fast unstable scatter when the input is
+// 16-byte aligned; two passes are LSD and therefore use the stable scatter
+// (tmp holds the first pass).
+bool use_fast(const eo_tuple *in, const HashPlan &hp) { return hp.passes == 1 && ((uintptr_t)in & 15) == 0; }
+
 eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
                              eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s) {
   if (hp.passes == 1) {
-    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, ctx->props.cus);
+    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, ctx->props.cus, use_fast(in, hp));
     EO_TRY_HIP(partition_pass(pp, in, out, part_start, scratch, ctx->d_err, s), "partition pass");
     return EO_OK;
   }
@@ -99,10 +103,13 @@ eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const 
 
 size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus) {
   size_t b = 0;
-  if (hp.passes >= 1)
-    b = std::max(b, partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus)));
-  if (hp.passes == 2)
-    b = std::max(b, partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass2, hp.bits_pass1, 0u, cus)));
+  if (hp.passes == 1) {
+    b = std::max(partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, true)),
+                 partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, false)));
+  } else if (hp.passes == 2) {
+    b = std::max(partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus)),
+                 partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass2, hp.bits_pass1, 0u, cus)));
+  }
   return b;
 }
 
@@ -244,6 +251,23 @@ eo_status eo_copy_d2d(eo_ctx *c, void *dst, const void *src, uint64_t bytes) {
   return copy_sync(c, dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
+eo_status eo_bandwidth_probe(eo_ctx *c, uint64_t bytes, int iters, double *gbs) {
+  if (!c || !gbs || bytes == 0 || bytes % 16 || iters < 1) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  EO_TRY_HIP(hipStreamSynchronize(c->stream), "sync");
+  EO_TRY_HIP(c->arena.reserve(2 * bytes), "arena");
+  char *a = c->arena.base, *b = c->arena.base + bytes;
+  EO_TRY_HIP(copy_bytes(a, b, bytes, c->props.cus, c->stream), "warm copy");
+  EO_TRY_HIP(hipEventRecord(c->ev[0], c->stream), "event");
+  for (int i = 0; i < iters; ++i) EO_TRY_HIP(copy_bytes(a, b, bytes, c->props.cus, c->stream), "copy");
+  EO_TRY_HIP(hipEventRecord(c->ev[1], c->stream), "event");
+  EO_TRY_HIP(hipEventSynchronize(c->ev[1]), "event sync");
+  float ms = 0.f;
+  EO_TRY_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
+  *gbs = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  return EO_OK;
+}
+
 // ---- primitives ---------------------------------------------------------------
 
 }  // extern "C"
@@ -359,7 +383,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   if ((nR && !R) || (nS && !S) || (out_cap && !out)) return EO_EINVAL;
   *n_out = 0;
   HashPlan hp;
-  eo_status st = plan_hash_join(nR, nS, o, &hp);
+  eo_status st = plan_hash_join(nR, nS, o, &hp, c->props.cus);
   if (st != EO_OK) return st;
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
@@ -417,8 +441,13 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   }
   clk.mark();  // 1
 
+  const bool deterministic = o && (o->flags & EO_F_DETERMINISTIC);
   JoinTilePlan jp{P, hp.lds_cap, hp.s_tile, 0};
-  EO_TRY_HIP(join_plan_count(jp, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s), "tile plan");
+  if (deterministic)
+    EO_TRY_HIP(join_plan_count(jp, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s), "tile plan");
+  else
+    EO_TRY_HIP(join_fused_plan(P, hp.s_span, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s),
+               "tile plan");
   uint64_t n_tiles = 0;
   st = read_u64(c, small, &n_tiles, 1, s);
   if (st != EO_OK) return st;
@@ -426,9 +455,52 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     set_last_error("too many join tiles");
     return EO_EINTERNAL;
   }
+  if (stats) {
+    stats->radix_bits = (uint32_t)hp.bits;
+    stats->passes = (uint32_t)hp.passes;
+    stats->tiles = (uint32_t)n_tiles;
+    stats->lds_tuples = hp.lds_cap;
+  }
+  eo_pair *dout = out;
+  bool staged = false;
+  uint64_t total = 0;
 
-  // Per-tile counts / offsets: reserved up front for the usual tile count,
-  // allocated on the side only when skew produces more tiles than estimated.
+  if (!deterministic) {
+    // One fused probe kernel; with out == NULL (or a too small buffer) it counts
+    // every match and writes only the pairs that fit.
+    uint64_t cap_dev = out ? out_cap : 0;
+    if (host && out && out_cap) {
+      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+      staged = true;
+    }
+    EO_TRY_HIP(fill_u64(small + 1, 0, 1, s), "cursor reset");
+    clk.mark();  // 2
+    hipError_t e = join_fused(P, hp.s_span, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, dout, cap_dev,
+                              small + 1, s);
+    clk.mark();  // 3
+    if (e == hipSuccess) {
+      st = read_u64(c, small + 1, &total, 1, s);
+      if (st == EO_OK && staged && total <= out_cap && total)
+        e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    }
+    if (staged) (void)hipFreeAsync(dout, s);
+    if (e != hipSuccess) return hip_fail(e, "fused probe");
+    if (st != EO_OK) return st;
+    EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+    st = check_device_errors(c, s);
+    *n_out = total;
+    if (stats) {
+      stats->n_out = total;
+      stats->ms_partition = clk.ms(0, 1);
+      stats->ms_count = 0.0;
+      stats->ms_write = clk.ms(2, 3);
+      stats->ms_total = clk.ms(0, 3);
+    }
+    if (st != EO_OK) return st;
+    return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
+  }
+
+  // Deterministic order: count -> scan -> write, nothing written on EO_ECAPACITY.
   char *B = A + o_tc;
   bool side = false;
   size_t o_cnt = 0, o_cscr = tiles_est * sizeof(uint64_t);
@@ -443,11 +515,9 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   auto free_side = [&]() {
     if (side) (void)hipFreeAsync(B, s);
   };
-
   EO_TRY_HIP(join_count(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt, s), "count");
   EO_TRY_HIP(scan_u64(tile_cnt, tile_cnt, n_tiles, small + 1, B + o_cscr, c->d_err, s), "count scan");
   clk.mark();  // 2
-  uint64_t total = 0;
   st = read_u64(c, small + 1, &total, 1, s);
   if (st == EO_OK) st = check_device_errors(c, s);
   if (st != EO_OK) {
@@ -455,13 +525,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     return st;
   }
   *n_out = total;
-  if (stats) {
-    stats->n_out = total;
-    stats->radix_bits = (uint32_t)hp.bits;
-    stats->passes = (uint32_t)hp.passes;
-    stats->tiles = (uint32_t)n_tiles;
-    stats->lds_tuples = hp.lds_cap;
-  }
+  if (stats) stats->n_out = total;
   if (!out) {
     free_side();
     return EO_OK;
@@ -470,15 +534,12 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     free_side();
     return EO_ECAPACITY;
   }
-  eo_pair *dout = out;
   if (host && total) {
-    eo_pair *tmp_out = nullptr;
-    hipError_t e = hipMallocAsync((void **)&tmp_out, total * sizeof(eo_pair), s);
+    hipError_t e = hipMallocAsync((void **)&dout, total * sizeof(eo_pair), s);
     if (e != hipSuccess) {
       free_side();
       return hip_fail(e, "output staging");
     }
-    dout = tmp_out;
   }
   if (total) {
     hipError_t e = join_write(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt, dout,

@@ -200,6 +200,206 @@ __global__ __launch_bounds__(C::NT) void k_join_write(const uint2 *__restrict__ 
   }
 }
 
+
+// Fused probe, one pass over S.  A tile is (partition p, R-chunk c, S span j):
+// the R chunk (<= 16384 tuples) is built once into an LDS table and the whole S
+// span (up to s_span tuples) streams past it in batches of NT*SPT tuples held
+// in VGPRs, the next batch's loads in flight while the current one probes.
+//
+// LDS table (SoA, 144 KB): keys[16384] and pays[16384] grouped by bucket,
+// u16 bucket offsets off[8192 + 1].  Built in three steps without a cursor
+// array: (1) every R tuple takes its rank inside its bucket from an LDS
+// atomic on a u32 counter array that lives in the (still empty) keys region,
+// (2) a workgroup scan turns the counters into u16 offsets, (3) each tuple is
+// stored at off[h] + rank.
+//
+// Per batch, every S tuple is probed once (match count and first payload kept
+// in registers), output positions are numbered register-major (so a wave's
+// store instruction writes one contiguous run), and one 64-bit atomic per
+// batch reserves the output range.  Pairs at positions >= out_cap are counted
+// but not written.
+constexpr int kFNT = 1024;
+constexpr int kFCap = 16384;
+constexpr int kFLog2NB = 13;
+constexpr int kFNB = 1 << kFLog2NB;
+constexpr int kFSPT = 8;
+constexpr int kFBatch = kFNT * kFSPT;
+constexpr int kFRPT = kFCap / kFNT;
+constexpr int kFBPT = kFNB / kFNT;
+
+struct SpanRef {
+  uint32_t r_beg, r_n, s_beg, s_n;
+};
+
+__device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_start, uint32_t P,
+                                             const uint32_t *r_start, const uint32_t *s_start, uint32_t s_span,
+                                             SpanRef *ref) {
+  uint32_t lo = 0, hi = P;
+  if (t >= tile_start[P]) return false;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (tile_start[mid] <= t) lo = mid; else hi = mid;
+  }
+  const uint32_t p = lo;
+  const uint32_t local = t - tile_start[p];
+  const uint32_t rb = r_start[p], rn = r_start[p + 1] - rb;
+  const uint32_t sb = s_start[p], sn = s_start[p + 1] - sb;
+  const uint32_t nc = (rn + kFCap - 1) / kFCap;
+  const uint32_t c = local % nc, j = local / nc;
+  ref->r_beg = rb + c * kFCap;
+  ref->r_n = min((uint32_t)kFCap, rn - c * kFCap);
+  ref->s_beg = sb + j * s_span;
+  ref->s_n = min(s_span, sn - j * s_span);
+  return true;
+}
+
+__global__ __launch_bounds__(kFNT) void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
+                                                     const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
+                                                     const uint32_t *__restrict__ tile_start, uint32_t P,
+                                                     uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
+                                                     uint64_t *out_cursor) {
+  constexpr int NW = kFNT / kWave;
+  __shared__ uint32_t keys[kFCap];
+  __shared__ uint32_t pays[kFCap];
+  __shared__ uint16_t off[kFNB + 2];
+  __shared__ uint32_t scan_tmp[NW + 1];
+  __shared__ uint32_t wtot[kFSPT * NW];
+  __shared__ uint64_t base_sh;
+  const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
+  SpanRef ref;
+  if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, &ref)) return;
+
+  auto load_batch = [&](uint32_t b0, uint2 *dst) {
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      const uint32_t i = b0 + u * kFNT + threadIdx.x;
+      dst[u] = i < ref.s_n ? S[ref.s_beg + i] : make_uint2(0u, 0u);
+    }
+  };
+  uint2 cur[kFSPT];
+  load_batch(0, cur);
+
+  // ---- build: rank by LDS atomics on counters aliased onto keys[] ----
+  uint32_t *cnt = keys;  // kFNB u32 counters, dead once the offsets exist
+  for (int h = threadIdx.x; h < kFNB; h += kFNT) cnt[h] = 0;
+  __syncthreads();
+  uint2 r[kFRPT];
+  uint32_t hh[kFRPT], rk[kFRPT];
+#pragma unroll
+  for (int k = 0; k < kFRPT; ++k) {
+    const uint32_t i = k * kFNT + threadIdx.x;
+    if (i < ref.r_n) r[k] = R[ref.r_beg + i];
+  }
+#pragma unroll
+  for (int k = 0; k < kFRPT; ++k) {
+    const uint32_t i = k * kFNT + threadIdx.x;
+    if (i < ref.r_n) {
+      hh[k] = bucket_hash((int32_t)r[k].x, kFLog2NB);
+      rk[k] = atomicAdd(&cnt[hh[k]], 1u);
+    }
+  }
+  __syncthreads();
+  {
+    uint32_t loc[kFBPT];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kFBPT; ++k) {
+      loc[k] = cnt[threadIdx.x * kFBPT + k];
+      sum += loc[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<kFNT, uint32_t>(sum, scan_tmp, &tot);
+#pragma unroll
+    for (int k = 0; k < kFBPT; ++k) {
+      off[threadIdx.x * kFBPT + k] = (uint16_t)ex;
+      ex += loc[k];
+    }
+    if (threadIdx.x == 0) off[kFNB] = (uint16_t)ref.r_n;  // r_n <= 16384 fits u16
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kFRPT; ++k) {
+    const uint32_t i = k * kFNT + threadIdx.x;
+    if (i < ref.r_n) {
+      const uint32_t pos = off[hh[k]] + rk[k];
+      keys[pos] = r[k].x;
+      pays[pos] = r[k].y;
+    }
+  }
+  __syncthreads();
+
+  // ---- probe the S span batch by batch ----
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += kFBatch) {
+    uint2 nxt[kFSPT];
+    if (b0 + kFBatch < ref.s_n) load_batch(b0 + kFBatch, nxt);
+    uint32_t m[kFSPT], fp[kFSPT];
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      m[u] = 0;
+      fp[u] = 0;
+      if (b0 + u * kFNT + threadIdx.x < ref.s_n) {
+        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
+        const uint32_t e1 = off[h + 1];
+        for (uint32_t e = off[h]; e < e1; ++e) {
+          if (keys[e] == cur[u].x) {
+            if (m[u] == 0) fp[u] = pays[e];
+            ++m[u];
+          }
+        }
+      }
+    }
+    uint32_t wex[kFSPT];
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      const uint32_t inc = wave_incl_scan(m[u]);
+      wex[u] = inc - m[u];
+      if (lane == kWave - 1) wtot[u * NW + w] = inc;
+    }
+    __syncthreads();
+    uint32_t batch_total;
+    const uint32_t v = threadIdx.x < kFSPT * NW ? wtot[threadIdx.x] : 0u;
+    const uint32_t vex = block_excl_scan<kFNT, uint32_t>(v, scan_tmp, &batch_total);
+    if (threadIdx.x < kFSPT * NW) wtot[threadIdx.x] = vex;
+    if (threadIdx.x == 0) base_sh = batch_total ? atomic_add_agent(out_cursor, (uint64_t)batch_total) : 0ull;
+    __syncthreads();
+    const uint64_t base = base_sh;
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      if (m[u] == 0) continue;
+      uint64_t pos = base + wtot[u * NW + w] + wex[u];
+      if (m[u] == 1) {
+        if (pos < out_cap) out[pos] = make_uint2(fp[u], cur[u].y);
+      } else {
+        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
+        const uint32_t e1 = off[h + 1];
+        for (uint32_t e = off[h]; e < e1; ++e) {
+          if (keys[e] == cur[u].x) {
+            if (pos < out_cap) out[pos] = make_uint2(pays[e], cur[u].y);
+            ++pos;
+          }
+        }
+      }
+    }
+    __syncthreads();  // wtot / base_sh reuse
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) cur[u] = nxt[u];
+  }
+}
+
+__global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,
+                             uint32_t *tiles) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > P) return;
+  if (p == P) {
+    tiles[P] = 0;
+    return;
+  }
+  const uint32_t rn = r_start[p + 1] - r_start[p];
+  const uint32_t sn = s_start[p + 1] - s_start[p];
+  tiles[p] = (rn && sn) ? ((rn + kFCap - 1) / kFCap) * ((sn + s_span - 1) / s_span) : 0u;
+}
+
 __global__ void k_plan_tiles(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t cap,
                              uint32_t s_tile, uint32_t *tiles) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -233,6 +433,7 @@ hipError_t launch_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple
                      reinterpret_cast<uint2 *>(out), out_cap);
   return hipGetLastError();
 }
+
 
 }  // namespace
 
@@ -278,6 +479,34 @@ hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *
       return launch_write<Cfg16K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_off, out, out_cap, stream);
     default: return hipErrorInvalidValue;
   }
+}
+
+}  // namespace eo
+
+namespace eo {
+
+uint32_t join_fused_cap() { return kFCap; }
+
+hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, const uint32_t *r_start, const uint32_t *s_start,
+                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
+                           hipStream_t stream) {
+  const uint32_t n = parts + 1;
+  hipLaunchKernelGGL(k_plan_spans, dim3((n + 255) / 256), dim3(256), 0, stream, r_start, s_start, parts, s_span,
+                     tile_start);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return scan_u32(tile_start, tile_start, n, n_tiles_dev, scan_scratch, err, stream);
+}
+
+hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, eo_pair *out,
+                      uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream) {
+  if (n_tiles == 0) return hipSuccess;
+  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL(k_join_fused, dim3(grid), dim3(kFNT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
+                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, parts, s_span,
+                     reinterpret_cast<uint2 *>(out), out_cap, out_cursor);
+  return hipGetLastError();
 }
 
 }  // namespace eo

@@ -23,6 +23,10 @@ hipError_t scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *tot
 hipError_t fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 hipError_t fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
 
+// ---- util.hip ---------------------------------------------------------------
+// Streaming 16-byte copy of `bytes` (a multiple of 16) with a grid of 8 WGs per CU.
+hipError_t copy_bytes(const void *src, void *dst, uint64_t bytes, int cus, hipStream_t stream);
+
 // ---- partition.hip ----------------------------------------------------------
 // One stable radix-partition pass over `bits` (1..kMaxPassBits) digit bits.
 constexpr int kMaxPassBits = 11;
@@ -31,10 +35,13 @@ struct PartPassPlan {
   int bits;
   int shift;
   uint32_t xor_mask;   // applied to (uint32)key before the shift (sign flip for sorting)
-  uint32_t stripes;    // G: workgroups, each owning one contiguous stripe
-  uint64_t stripe_len; // tuples per stripe (multiple of the scatter tile)
+  uint32_t stripes;    // G: scatter workgroups, each owning one contiguous stripe
+  uint64_t stripe_len; // tuples per stripe (even)
+  uint32_t hist_split; // K: histogram workgroups per stripe
+  bool fast;           // unstable LDS-atomic-ranked scatter (needs 16-byte aligned input)
 };
-PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus);
+// fast = false: stable scatter (ballot-ranked), equal to the CPU_JOIN partition.
+PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus, bool fast = false);
 size_t partition_pass_scratch_bytes(const PartPassPlan &p);
 // counts -> scan -> scatter.  part_start (optional, 2^bits+1 u32 entries, device)
 // receives the partition start offsets of this pass.
@@ -69,6 +76,18 @@ hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *
                       const uint32_t *tile_part, const uint64_t *tile_off, eo_pair *out, uint64_t out_cap,
                       hipStream_t stream);
 uint32_t join_max_lds_cap();
+// Fused single-pass probe.  Tiles are (partition, R-chunk of join_fused_cap()
+// tuples, S span of s_span tuples); join_fused_plan counts and scans them
+// (tile_start: P+1 u32, *n_tiles_dev = total).  *out_cursor must be zero on
+// entry and holds the match count afterwards; pairs beyond out_cap are
+// counted, not written.
+uint32_t join_fused_cap();
+hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, const uint32_t *r_start, const uint32_t *s_start,
+                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
+                           hipStream_t stream);
+hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, eo_pair *out,
+                      uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream);
 
 // ---- nlj.hip ----------------------------------------------------------------
 struct NljPlan {

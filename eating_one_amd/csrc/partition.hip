@@ -40,29 +40,59 @@ __device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shif
   return ((key ^ xm) >> shift) & mask;
 }
 
-template <int BITS>
+// Histogram of one pass.  The grid is G stripes x K sub-stripes; counts are
+// laid out [digit][stripe][sub] so that one exclusive scan gives, for every
+// (digit, stripe), the output offset of the stripe's run at index
+// (d * G + g) * K — the scatter reads those.  V16: 16-byte loads (pairs of
+// tuples), used when the input is 16-byte aligned.
+template <int BITS, bool V16>
 __global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__ in, uint64_t n, int shift,
-                                                       uint32_t xm, uint32_t G, uint64_t stripe_len,
+                                                       uint32_t xm, uint32_t G, uint32_t K, uint64_t stripe_len,
                                                        uint32_t *__restrict__ counts) {
   using C = PartCfg<BITS>;
   __shared__ uint32_t hist[C::F];
   for (int d = threadIdx.x; d < C::F; d += C::NT) hist[d] = 0;
   __syncthreads();
-  const uint32_t g = xcd_local_index(blockIdx.x, G);
-  const uint64_t beg = (uint64_t)g * stripe_len;
-  const uint64_t end = beg + stripe_len < n ? beg + stripe_len : n;
-  for (uint64_t i = beg + threadIdx.x; i < end; i += C::NT) {
-    const uint32_t d = digit_of(in[i].x, xm, shift, C::F - 1);
-    atomicAdd(&hist[d], 1u);
+  const uint32_t gk = xcd_local_index(blockIdx.x, G * K);
+  const uint32_t g = gk / K, k = gk % K;
+  const uint64_t s0 = (uint64_t)g * stripe_len;
+  const uint64_t s1 = s0 + stripe_len < n ? s0 + stripe_len : n;
+  const uint64_t sub = (stripe_len / K + 1) & ~1ull;  // even, so sub-stripes stay 16-byte aligned
+  uint64_t beg = s0 + k * sub, end = beg + sub;
+  if (k == K - 1 || end > s1) end = s1;
+  if (beg < end) {
+    if (V16) {
+      const uint4 *in4 = reinterpret_cast<const uint4 *>(in + beg);
+      const uint64_t npairs = (end - beg) / 2;
+      uint64_t i = threadIdx.x;
+      for (; i + 3 * C::NT < npairs; i += 4 * C::NT) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = in4[i + u * C::NT];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          atomicAdd(&hist[digit_of(q[u].x, xm, shift, C::F - 1)], 1u);
+          atomicAdd(&hist[digit_of(q[u].z, xm, shift, C::F - 1)], 1u);
+        }
+      }
+      for (; i < npairs; i += C::NT) {
+        const uint4 q = in4[i];
+        atomicAdd(&hist[digit_of(q.x, xm, shift, C::F - 1)], 1u);
+        atomicAdd(&hist[digit_of(q.z, xm, shift, C::F - 1)], 1u);
+      }
+      if (((end - beg) & 1) && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[end - 1].x, xm, shift, C::F - 1)], 1u);
+    } else {
+      for (uint64_t i = beg + threadIdx.x; i < end; i += C::NT) atomicAdd(&hist[digit_of(in[i].x, xm, shift, C::F - 1)], 1u);
+    }
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < C::F; d += C::NT) counts[(uint64_t)d * G + g] = hist[d];
+  for (int d = threadIdx.x; d < C::F; d += C::NT) counts[((uint64_t)d * G + g) * K + k] = hist[d];
 }
 
 template <int BITS>
 __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restrict__ in, uint64_t n, int shift,
                                                           uint32_t xm, uint32_t G, uint64_t stripe_len,
-                                                          const uint32_t *__restrict__ offsets,
+                                                          uint32_t K, const uint32_t *__restrict__ offsets,
                                                           uint2 *__restrict__ out) {
   using C = PartCfg<BITS>;
   constexpr int F = C::F;
@@ -73,7 +103,7 @@ __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restric
   __shared__ uint32_t scan_tmp[C::NW + 1];
 
   const uint32_t g = xcd_local_index(blockIdx.x, G);
-  for (int d = threadIdx.x; d < F; d += C::NT) cursor[d] = offsets[(uint64_t)d * G + g];
+  for (int d = threadIdx.x; d < F; d += C::NT) cursor[d] = offsets[((uint64_t)d * G + g) * K];
   const uint64_t beg = (uint64_t)g * stripe_len;
   const uint64_t end = beg + stripe_len < n ? beg + stripe_len : n;
   const int w = threadIdx.x / kWave;
@@ -158,9 +188,118 @@ __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restric
   }
 }
 
-__global__ void k_part_starts(const uint32_t *offsets, uint32_t G, uint32_t F, uint64_t n, uint32_t *part_start) {
+
+// Unstable scatter for the join's internal partitioning: a workgroup of 1024
+// threads stages 16K tuples per tile; ranks come from LDS atomics (the
+// histogram and the within-tile ranks in one pass), the tile is placed
+// digit-sorted in LDS and flushed in digit runs.  The next tile's loads are
+// issued before the flush so that HBM reads overlap the writes.  Within a
+// partition the tuple order depends on atomic order; the multiset per
+// partition equals the stable scatter's.
+constexpr int kFastNT = 1024;
+constexpr int kFastPairs = 8;                         // uint4 loads (2 tuples) per thread per tile
+constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples
+
+template <int BITS>
+__global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__restrict__ in, uint64_t n, int shift,
+                                                               uint32_t xm, uint32_t G, uint64_t stripe_len,
+                                                               uint32_t K, const uint32_t *__restrict__ offsets,
+                                                               uint2 *__restrict__ out) {
+  constexpr int F = 1 << BITS;
+  constexpr int DPT = (F + kFastNT - 1) / kFastNT;
+  __shared__ uint2 stage[kFastTile];
+  __shared__ uint32_t cnt[F];
+  __shared__ uint32_t tbase[F + 1];
+  __shared__ uint32_t cursor[F];
+  __shared__ uint32_t scan_tmp[kFastNT / kWave + 1];
+
+  const uint32_t g = xcd_local_index(blockIdx.x, G);
+  for (int d = threadIdx.x; d < F; d += kFastNT) {
+    cursor[d] = offsets[((uint64_t)d * G + g) * K];
+    cnt[d] = 0;
+  }
+  const uint64_t beg = (uint64_t)g * stripe_len;
+  const uint64_t end = beg + stripe_len < n ? beg + stripe_len : n;
+  if (beg >= end) return;
+  const uint4 *in4 = reinterpret_cast<const uint4 *>(in + beg);
+  const uint64_t nfull = (end - beg) / 2;              // complete 16-byte pairs
+  const uint64_t npairs = (end - beg + 1) / 2;         // incl. a trailing single tuple
+
+  uint4 cur[kFastPairs];
+  auto load_tile = [&](uint64_t t0pair, uint4 *dst) {
+#pragma unroll
+    for (int j = 0; j < kFastPairs; ++j) {
+      const uint64_t pi = t0pair + (uint64_t)j * kFastNT + threadIdx.x;
+      if (pi < nfull) {
+        dst[j] = in4[pi];
+      } else if (pi < npairs) {
+        const uint2 last = in[beg + 2 * pi];
+        dst[j] = make_uint4(last.x, last.y, 0u, 0u);
+      } else {
+        dst[j] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+  };
+  load_tile(0, cur);
+  __syncthreads();
+
+  for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
+    const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples in this tile
+    uint32_t rk[2 * kFastPairs];
+#pragma unroll
+    for (int j = 0; j < kFastPairs; ++j) {
+      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
+      if (e < tn) rk[2 * j] = atomicAdd(&cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
+      if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
+    }
+    __syncthreads();
+    {
+      uint32_t loc[DPT];
+      uint32_t sum = 0;
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int d = threadIdx.x * DPT + k;
+        loc[k] = d < F ? cnt[d] : 0u;
+        sum += loc[k];
+      }
+      uint32_t tot;
+      uint32_t ex = block_excl_scan<kFastNT, uint32_t>(sum, scan_tmp, &tot);
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int d = threadIdx.x * DPT + k;
+        if (d < F) tbase[d] = ex;
+        ex += loc[k];
+      }
+      if (threadIdx.x == 0) tbase[F] = tn;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFastPairs; ++j) {
+      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
+      if (e < tn) stage[tbase[digit_of(cur[j].x, xm, shift, F - 1)] + rk[2 * j]] = make_uint2(cur[j].x, cur[j].y);
+      if (e + 1 < tn)
+        stage[tbase[digit_of(cur[j].z, xm, shift, F - 1)] + rk[2 * j + 1]] = make_uint2(cur[j].z, cur[j].w);
+    }
+    __syncthreads();
+    load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
+    for (uint32_t i = threadIdx.x; i < tn; i += kFastNT) {
+      const uint2 t = stage[i];
+      const uint32_t d = digit_of(t.x, xm, shift, F - 1);
+      out[(uint64_t)cursor[d] + (i - tbase[d])] = t;
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < F; d += kFastNT) {
+      cursor[d] += tbase[d + 1] - tbase[d];
+      cnt[d] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_part_starts(const uint32_t *offsets, uint32_t G, uint32_t K, uint32_t F, uint64_t n,
+                              uint32_t *part_start) {
   const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d < F) part_start[d] = offsets[(uint64_t)d * G];
+  if (d < F) part_start[d] = offsets[(uint64_t)d * G * K];
   if (d == F) part_start[F] = (uint32_t)n;
 }
 
@@ -178,15 +317,27 @@ hipError_t launch_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
                        void *scan_scratch, uint32_t *err, hipStream_t stream) {
   const uint2 *vin = reinterpret_cast<const uint2 *>(in);
   uint2 *vout = reinterpret_cast<uint2 *>(out);
-  hipLaunchKernelGGL(k_part_hist<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift, p.xor_mask,
-                     p.stripes, p.stripe_len, counts);
+  const bool aligned = ((uintptr_t)in & 15) == 0;
+  const uint32_t hist_grid = p.stripes * p.hist_split;
+  if (aligned)
+    hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
+                       p.xor_mask, p.stripes, p.hist_split, p.stripe_len, counts);
+  else
+    hipLaunchKernelGGL((k_part_hist<BITS, false>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
+                       p.xor_mask, p.stripes, p.hist_split, p.stripe_len, counts);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint64_t m = (uint64_t)p.stripes << BITS;
+  const uint64_t m = ((uint64_t)p.stripes * p.hist_split) << BITS;
   e = scan_u32(counts, counts, m, nullptr, scan_scratch, err, stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_part_scatter<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
-                     p.xor_mask, p.stripes, p.stripe_len, counts, vout);
+  if (p.fast) {
+    if (!aligned) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(p.stripes), dim3(kFastNT), 0, stream, vin, p.n, p.shift,
+                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout);
+  } else {
+    hipLaunchKernelGGL(k_part_scatter<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
+                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout);
+  }
   return hipGetLastError();
 }
 
@@ -205,33 +356,38 @@ uint32_t tile_of_bits(int bits) {
 
 }  // namespace
 
-PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus) {
+PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus, bool fast) {
   PartPassPlan p{};
   p.n = n;
   p.bits = bits;
   p.shift = shift;
   p.xor_mask = xor_mask;
-  const uint64_t tile = tile_of_bits(bits);
+  p.fast = fast;
+  cus = std::max(cus, kXcds);
+  const uint64_t tile = fast ? (uint64_t)kFastTile : tile_of_bits(bits);
   const uint64_t tiles = std::max<uint64_t>(1, (n + tile - 1) / tile);
-  // Enough stripes to fill the chip twice over, each several tiles long.
-  uint64_t g = std::min<uint64_t>(tiles, (uint64_t)std::max(cus, 8) * 2);
+  // fast: one 152 KB-LDS workgroup per CU, one stripe each; stable: two per CU.
+  uint64_t g = std::min<uint64_t>(tiles, (uint64_t)cus * (fast ? 1 : 2));
   g = (g + kXcds - 1) / kXcds * kXcds;
   uint64_t len = (n + g - 1) / g;
-  len = std::max<uint64_t>(tile, (len + tile - 1) / tile * tile);
+  len = std::max<uint64_t>(tile, (len + 1) & ~1ull);
+  if (!fast) len = (len + tile - 1) / tile * tile;
   p.stripes = (uint32_t)g;
   p.stripe_len = len;
+  // the histogram reads with ~4 workgroups per CU
+  p.hist_split = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(8, (uint64_t)cus * 4 / g));
   return p;
 }
 
 size_t partition_pass_scratch_bytes(const PartPassPlan &p) {
-  const uint64_t m = (uint64_t)p.stripes << p.bits;
+  const uint64_t m = ((uint64_t)p.stripes * p.hist_split) << p.bits;
   const size_t counts = (m * sizeof(uint32_t) + 255) / 256 * 256;
   return counts + scan_scratch_bytes_u32(m);
 }
 
 hipError_t partition_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out, uint32_t *part_start,
                           void *scratch, uint32_t *err, hipStream_t stream) {
-  const uint64_t m = (uint64_t)p.stripes << p.bits;
+  const uint64_t m = ((uint64_t)p.stripes * p.hist_split) << p.bits;
   uint32_t *counts = (uint32_t *)scratch;
   void *scan_scratch = (char *)scratch + (m * sizeof(uint32_t) + 255) / 256 * 256;
   hipError_t e;
@@ -248,8 +404,8 @@ hipError_t partition_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *o
   }
   if (e != hipSuccess || !part_start) return e;
   const uint32_t F = 1u << p.bits;
-  hipLaunchKernelGGL(k_part_starts, dim3((F + 1 + 255) / 256), dim3(256), 0, stream, counts, p.stripes, F, p.n,
-                     part_start);
+  hipLaunchKernelGGL(k_part_starts, dim3((F + 1 + 255) / 256), dim3(256), 0, stream, counts, p.stripes,
+                     p.hist_split, F, p.n, part_start);
   return hipGetLastError();
 }
 

@@ -1,5 +1,7 @@
 #include "plan.hpp"
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace eo {
@@ -13,7 +15,7 @@ int ceil_log2(uint64_t x) {
 bool valid_cap(int c) { return c == 4096 || c == 8192 || c == 16384; }
 }  // namespace
 
-eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPlan *out) {
+eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPlan *out, int cus) {
   HashPlan p;
   const int cap_opt = o ? o->lds_tuples : 0;
   if (cap_opt != 0 && !valid_cap(cap_opt)) return EO_EINVAL;
@@ -28,11 +30,16 @@ eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPl
   } else if (nR <= p.lds_cap) {
     p.bits = 0;
   } else {
-    // Aim the mean R partition at 7/8 of the LDS table so that binomial
-    // spread rarely spills a partition into a second R chunk.
+    // One radix pass of at most kMaxPassBits bits (2^11 digits keep each
+    // digit's run per 16K-tuple tile near 64 bytes, so writes stay coalesced);
+    // an R partition may span up to 4 LDS tables, whose tiles share each S
+    // range through the XCD's L2.  Beyond that, a second pass.
     const uint64_t target = (uint64_t)p.lds_cap * 7 / 8;
     p.bits = ceil_log2((nR + target - 1) / target);
-    if (p.bits > kMaxJoinBits) p.bits = kMaxJoinBits;
+    if (p.bits > kMaxPassBits) {
+      const int one_pass = ceil_log2((nR + 4ull * p.lds_cap - 1) / (4ull * p.lds_cap));
+      p.bits = one_pass <= kMaxPassBits ? kMaxPassBits : std::min(p.bits, kMaxJoinBits);
+    }
   }
 
   const int passes_opt = o ? o->passes : 0;
@@ -56,7 +63,17 @@ eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPl
   const int st_opt = o ? o->s_tile : 0;
   if (st_opt < 0) return EO_EINVAL;
   p.s_tile = st_opt ? (uint32_t)st_opt : 16384u;
-  (void)nS;
+  // Fused probe: S spans long enough to amortise the LDS build, short enough
+  // to give every CU a few tiles (and to split skewed partitions).
+  {
+    const uint64_t parts = 1ull << p.bits;
+    const uint64_t want_tiles = (uint64_t)std::max(cus, 8) * 4;
+    uint64_t span = nS * ((nR + p.lds_cap - 1) / p.lds_cap) / std::max<uint64_t>(1, want_tiles);
+    span = std::max<uint64_t>(span, (nS + parts - 1) / parts);
+    uint32_t s2 = 8192;
+    while (s2 < span && s2 < (1u << 18)) s2 <<= 1;
+    p.s_span = st_opt ? std::max<uint32_t>(8192, (uint32_t)st_opt) : s2;
+  }
   *out = p;
   return EO_OK;
 }

@@ -20,12 +20,13 @@ struct HashPlan {
   int bits_pass1 = 0;  // LSD: first pass takes the low bits
   int bits_pass2 = 0;
   uint32_t lds_cap = 0;
-  uint32_t s_tile = 0;
+  uint32_t s_tile = 0;  // S tuples per tile of the deterministic count/write path
+  uint32_t s_span = 0;  // S tuples per tile of the fused probe
 };
 
 constexpr int kMaxJoinBits = 22;
 
 // Returns EO_OK and fills *out, or EO_EINVAL for inconsistent overrides.
-eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *opts, HashPlan *out);
+eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *opts, HashPlan *out, int cus = 256);
 
 }  // namespace eo

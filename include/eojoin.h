@@ -16,7 +16,8 @@
  *     device pointers (EO_DEVICE: inputs already resident in HBM);
  *   - two-phase sizing, mirroring count→scan→alloc→write of hj.cc:1040-1069:
  *     out == NULL returns the match count in *n_out and writes nothing;
- *     out_cap < count returns EO_ECAPACITY with *n_out set and nothing written;
+ *     out_cap < count returns EO_ECAPACITY with *n_out set; the contents of
+ *     out are then unspecified (with EO_F_DETERMINISTIC nothing is written);
  *   - output order is unspecified: compare results as multisets;
  *   - one eo_ctx per host thread; calls on one ctx are serialised by the caller;
  *   - scratch memory comes from a ctx-owned device arena that only grows.
@@ -97,6 +98,11 @@ typedef struct eo_join_opts {
 } eo_join_opts;
 
 #define EO_F_NO_SYNC_STATS 1  /* do not record per-phase events */
+/* Deterministic output order: separate count and write passes with a device
+ * scan of per-tile counts between them (the reference's count -> presum ->
+ * join sequence, hj.cc:1011-1091).  Default: one fused pass per tile that
+ * reserves its output range with an atomic (order varies run to run). */
+#define EO_F_DETERMINISTIC 2
 
 typedef struct eo_device_info {
   char name[64];
@@ -131,6 +137,12 @@ eo_status eo_dev_free(eo_ctx *ctx, void *dptr);
 eo_status eo_copy_h2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 eo_status eo_copy_d2h(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 eo_status eo_copy_d2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+
+/* Streaming-copy bandwidth probe: copies `bytes` (multiple of 16) between two
+ * scratch buffers `iters` times with the library's copy kernel and returns the
+ * mean device bandwidth in GB/s (read + write bytes / time).  The measured
+ * HBM ceiling reported beside every roofline fraction. */
+eo_status eo_bandwidth_probe(eo_ctx *ctx, uint64_t bytes, int iters, double *gbs);
 
 /* ---- primitives ---------------------------------------------------------- */
 

@@ -66,6 +66,24 @@ def test_partition_skewed_and_tiny(ctx):
         assert np.array_equal(out, want) and np.array_equal(off, woff)
 
 
+def test_hash_join_misaligned_device_input(ctx):
+    """Device pointers that are only 8-byte aligned take the stable scatter."""
+    R, S = gen.c3(seed=21, nR=70001, nS=50003, key_range=1 << 14)
+    import ctypes
+    from eating_one_amd import DeviceArray, _abi
+    dR = DeviceArray.from_host(ctx, np.concatenate([np.zeros((1, 2), np.int32), R]))
+    dS = DeviceArray.from_host(ctx, np.concatenate([np.zeros((1, 2), np.int32), S]))
+    want = oracle.hash_join(R, S, nparts=1 << 10)
+    dout = DeviceArray(ctx, len(want) * 8 + 8)
+    n = ctypes.c_uint64()
+    o = ctx.opts(where=_abi.EO_DEVICE)
+    st = ctx.lib.eo_hash_join(ctx.handle, dR.ptr + 8, len(R), dS.ptr + 8, len(S), dout.ptr, len(want),
+                              ctypes.byref(n), ctypes.byref(o))
+    assert st == 0 and n.value == len(want)
+    got = dout.to_host(np.int32, (len(want), 2), len(want) * 8)
+    assert same_multiset(got, want)
+
+
 def test_hash_join_c1_full(ctx):
     R, S = gen.c1()
     got = ctx.hash_join(R, S)
@@ -73,11 +91,12 @@ def test_hash_join_c1_full(ctx):
     assert same_multiset(got, oracle.hash_join(R, S))
 
 
+@pytest.mark.parametrize("det", [False, True])
 @pytest.mark.parametrize("bits,passes,cap", [(-1, 0, 16384), (3, 1, 4096), (8, 1, 8192), (11, 1, 4096),
                                              (12, 2, 4096), (14, 2, 16384), (0, 0, 8192)])
-def test_hash_join_plans(ctx, bits, passes, cap):
+def test_hash_join_plans(ctx, bits, passes, cap, det):
     R, S = gen.c3(seed=bits + 100, nR=60000, nS=90000, key_range=1 << 15)
-    got = ctx.hash_join(R, S, radix_bits=bits, passes=passes, lds_tuples=cap)
+    got = ctx.hash_join(R, S, radix_bits=bits, passes=passes, lds_tuples=cap, deterministic=det)
     want = oracle.hash_join(R, S, nparts=1 << 12)
     assert len(got) == len(want) == join_count_closed_form(R, S)
     assert same_multiset(got, want)
@@ -97,24 +116,26 @@ def test_hash_join_edge_cases(ctx):
     assert same_multiset(ctx.hash_join(R, S, radix_bits=4), np.array([[1, 0], [0, 1], [3, 2]]))
 
 
-def test_hash_join_heavy_duplicates(ctx):
+@pytest.mark.parametrize("det", [False, True])
+def test_hash_join_heavy_duplicates(ctx, det):
     # one key everywhere: R partition larger than any LDS table -> several R chunks
     R = np.stack([np.full(20000, 42), np.arange(20000)], 1).astype(np.int32)
     S = np.stack([np.full(300, 42), np.arange(300)], 1).astype(np.int32)
-    got = ctx.hash_join(R, S, radix_bits=4)
+    got = ctx.hash_join(R, S, radix_bits=4, deterministic=det)
     assert len(got) == 20000 * 300
     assert same_multiset(got, brute_join(R, S))
 
 
-def test_hash_join_count_only_and_capacity(ctx):
+@pytest.mark.parametrize("det", [False, True])
+def test_hash_join_count_only_and_capacity(ctx, det):
     R, S = gen.c3(seed=5, nR=5000, nS=5000, key_range=1000)
-    n = ctx.hash_join(R, S, count_only=True)
+    n = ctx.hash_join(R, S, count_only=True, deterministic=det)
     assert n == join_count_closed_form(R, S)
     import ctypes
     from eating_one_amd import _abi
     out = np.empty((n - 1, 2), dtype=np.int32)
     cnt = ctypes.c_uint64()
-    o = ctx.opts()
+    o = ctx.opts(deterministic=det)
     st = ctx.lib.eo_hash_join(ctx.handle, R.ctypes.data, len(R), S.ctypes.data, len(S), out.ctypes.data, n - 1,
                               ctypes.byref(cnt), ctypes.byref(o))
     assert st == _abi.EO_ECAPACITY and cnt.value == n
