@@ -81,20 +81,33 @@ bool opts_ok(const eo_join_opts *o) {
 constexpr uint64_t kMaxRows = (1ull << 31) - 1;
 
 // Partition `n` tuples into `out` and write the partition starts
-// (2^bits + 1 u32).  One pass uses the fast unstable scatter when the input is
-// 16-byte aligned; two passes are LSD and therefore use the stable scatter
-// (tmp holds the first pass).
-bool use_fast(const eo_tuple *in, const HashPlan &hp) { return hp.passes == 1 && ((uintptr_t)in & 15) == 0; }
+// (2^bits + 1 u32).  With a 16-byte aligned input: one fast pass, or two fast
+// passes MSD-first (high digit over the whole array, then the low digit inside
+// every first-pass partition).  Otherwise stable LSD passes (tmp holds the
+// first pass).
+bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
                              eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s) {
+  const int cus = ctx->props.cus;
   if (hp.passes == 1) {
-    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, ctx->props.cus, use_fast(in, hp));
+    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, aligned16(in));
     EO_TRY_HIP(partition_pass(pp, in, out, part_start, scratch, ctx->d_err, s), "partition pass");
     return EO_OK;
   }
-  PartPassPlan p1 = plan_partition_pass(n, hp.bits_pass1, 0, 0u, ctx->props.cus);
-  PartPassPlan p2 = plan_partition_pass(n, hp.bits_pass2, hp.bits_pass1, 0u, ctx->props.cus);
+  const int b_lo = hp.bits_pass1, b_hi = hp.bits_pass2;
+  if (aligned16(in) && aligned16(tmp)) {
+    const uint32_t F1 = 1u << b_hi;
+    uint32_t *start1 = (uint32_t *)scratch;
+    char *rest = (char *)scratch + ((F1 + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+    PartPassPlan p1 = plan_partition_pass(n, b_hi, b_lo, 0u, cus, true);
+    EO_TRY_HIP(partition_pass(p1, in, tmp, start1, rest, ctx->d_err, s), "partition pass 1");
+    SegPassPlan p2 = plan_seg_pass(n, F1, b_lo, 0, cus);
+    EO_TRY_HIP(partition_seg_pass(p2, tmp, out, start1, part_start, rest, ctx->d_err, s), "partition pass 2");
+    return EO_OK;
+  }
+  PartPassPlan p1 = plan_partition_pass(n, b_lo, 0, 0u, cus);
+  PartPassPlan p2 = plan_partition_pass(n, b_hi, b_lo, 0u, cus);
   EO_TRY_HIP(partition_pass(p1, in, tmp, nullptr, scratch, ctx->d_err, s), "partition pass 1");
   EO_TRY_HIP(partition_pass(p2, tmp, out, nullptr, scratch, ctx->d_err, s), "partition pass 2");
   EO_TRY_HIP(partition_bounds(out, n, hp.bits, 0, 0u, part_start, s), "partition bounds");
@@ -107,8 +120,12 @@ size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus) {
     b = std::max(partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, true)),
                  partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, false)));
   } else if (hp.passes == 2) {
-    b = std::max(partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus)),
-                 partition_pass_scratch_bytes(plan_partition_pass(n, hp.bits_pass2, hp.bits_pass1, 0u, cus)));
+    const int b_lo = hp.bits_pass1, b_hi = hp.bits_pass2;
+    const size_t head = (((1u << b_hi) + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+    b = std::max(partition_pass_scratch_bytes(plan_partition_pass(n, b_lo, 0, 0u, cus)),
+                 partition_pass_scratch_bytes(plan_partition_pass(n, b_hi, b_lo, 0u, cus)));
+    b = std::max(b, head + partition_pass_scratch_bytes(plan_partition_pass(n, b_hi, b_lo, 0u, cus, true)));
+    b = std::max(b, head + seg_pass_scratch_bytes(plan_seg_pass(n, 1u << b_hi, b_lo, 0, cus)));
   }
   return b;
 }

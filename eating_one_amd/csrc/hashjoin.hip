@@ -219,12 +219,12 @@ __global__ __launch_bounds__(C::NT) void k_join_write(const uint2 *__restrict__ 
 // batch reserves the output range.  Pairs at positions >= out_cap are counted
 // but not written.
 constexpr int kFNT = 1024;
-constexpr int kFCap = 16384;
+constexpr int kFCap = 17408;  // a 16K-tuple partition plus binomial spread
 constexpr int kFLog2NB = 13;
 constexpr int kFNB = 1 << kFLog2NB;
 constexpr int kFSPT = 8;
 constexpr int kFBatch = kFNT * kFSPT;
-constexpr int kFRPT = kFCap / kFNT;
+constexpr int kFRPT = (kFCap + kFNT - 1) / kFNT;
 constexpr int kFBPT = kFNB / kFNT;
 
 struct SpanRef {

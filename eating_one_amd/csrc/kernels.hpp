@@ -48,6 +48,23 @@ size_t partition_pass_scratch_bytes(const PartPassPlan &p);
 hipError_t partition_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
                           uint32_t *part_start, void *scratch, uint32_t *err,
                           hipStream_t stream);
+// Second pass of an MSD partition: the input is split into nseg segments
+// (seg_start: nseg+1 u32, device); each segment is partitioned by digit
+// ((uint32)key >> shift) & (2^bits - 1) with the fast scatter.  part_start
+// (nseg * 2^bits + 1 u32) receives the starts of the nseg * 2^bits partitions,
+// numbered seg * 2^bits + digit.  Input must be 16-byte aligned.
+struct SegPassPlan {
+  uint64_t n;
+  uint32_t nseg;
+  int bits, shift;
+  uint64_t stripe_len;
+  uint32_t max_stripes;
+};
+SegPassPlan plan_seg_pass(uint64_t n, uint32_t nseg, int bits, int shift, int cus);
+size_t seg_pass_scratch_bytes(const SegPassPlan &p);
+hipError_t partition_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple *out, const uint32_t *seg_start,
+                              uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream);
+
 // Partition start offsets of a tuple array already sorted by digit
 // ((uint32)key ^ xor_mask) >> shift & (2^bits - 1):  part_start[d] = first index
 // with digit >= d, part_start[2^bits] = n.

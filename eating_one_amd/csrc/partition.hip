@@ -40,30 +40,98 @@ __device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shif
   return ((key ^ xm) >> shift) & mask;
 }
 
+// Where a histogram / scatter workgroup's stripe lies and where its counts
+// go.  Uniform (seg_start == nullptr): stripe g = [g*len, (g+1)*len) of the
+// whole array, sub-stripe k of K; count of digit d at (d*G + g)*K + k.
+// Segmented (second pass of an MSD partition): the array is already split into
+// nseg segments [seg_start[i], seg_start[i+1]); segment i owns stripes
+// [seg_base[i], seg_base[i+1]) of length len, and the counts of segment i are
+// laid out [d][stripe] from seg_base[i] * F — one exclusive scan over all
+// counts then orders the output segment-major, digit, stripe.
+struct StripeGeo {
+  uint64_t n;
+  uint64_t len;
+  uint32_t G, K;
+  const uint32_t *seg_start;
+  const uint32_t *seg_base;
+  uint32_t nseg;
+};
+
+struct StripeRef {
+  uint64_t beg, end;
+  uint64_t cidx0;     // count index of digit 0
+  uint64_t cstride;   // distance between consecutive digits' counts
+  uint32_t seg;       // segment (segmented mode) or 0
+};
+
+// whole: the scatter's view — the full stripe, counts indexed at sub-stripe 0.
+template <int F>
+__device__ __forceinline__ bool stripe_of(const StripeGeo &geo, uint32_t g, uint32_t k, StripeRef *r,
+                                          bool whole = false) {
+  if (!geo.seg_start) {
+    const uint64_t s0 = (uint64_t)g * geo.len;
+    const uint64_t s1 = s0 + geo.len < geo.n ? s0 + geo.len : geo.n;
+    if (geo.K == 1 || whole) {
+      r->beg = s0;
+      r->end = s1;
+    } else {
+      const uint64_t sub = (geo.len / geo.K + 1) & ~1ull;  // even: sub-stripes stay 16-byte aligned
+      r->beg = s0 + k * sub;
+      r->end = r->beg + sub;
+      if (k == geo.K - 1 || r->end > s1) r->end = s1;
+    }
+    r->cidx0 = (uint64_t)g * geo.K + k;
+    r->cstride = (uint64_t)geo.G * geo.K;
+    r->seg = 0;
+    return true;
+  }
+  if (g >= geo.seg_base[geo.nseg]) return false;
+  uint32_t lo = 0, hi = geo.nseg;  // seg_base[lo] <= g < seg_base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (geo.seg_base[mid] <= g) lo = mid; else hi = mid;
+  }
+  const uint32_t sidx = g - geo.seg_base[lo];
+  const uint32_t k1 = geo.seg_base[lo + 1] - geo.seg_base[lo];
+  r->beg = geo.seg_start[lo] + (uint64_t)sidx * geo.len;
+  r->end = r->beg + geo.len < geo.seg_start[lo + 1] ? r->beg + geo.len : geo.seg_start[lo + 1];
+  r->cidx0 = (uint64_t)geo.seg_base[lo] * F + sidx;
+  r->cstride = k1;
+  r->seg = lo;
+  return true;
+}
+
 // Histogram of one pass.  The grid is G stripes x K sub-stripes; counts are
 // laid out [digit][stripe][sub] so that one exclusive scan gives, for every
 // (digit, stripe), the output offset of the stripe's run at index
 // (d * G + g) * K — the scatter reads those.  V16: 16-byte loads (pairs of
 // tuples), used when the input is 16-byte aligned.
 template <int BITS, bool V16>
-__global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__ in, uint64_t n, int shift,
-                                                       uint32_t xm, uint32_t G, uint32_t K, uint64_t stripe_len,
-                                                       uint32_t *__restrict__ counts) {
+__global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__ in, StripeGeo geo, int shift,
+                                                       uint32_t xm, uint32_t *__restrict__ counts) {
   using C = PartCfg<BITS>;
   __shared__ uint32_t hist[C::F];
+  uint32_t g, k;
+  if (geo.seg_start) {
+    g = xcd_local_index(blockIdx.x, gridDim.x);
+    k = 0;
+  } else {
+    const uint32_t gk = xcd_local_index(blockIdx.x, geo.G * geo.K);
+    g = gk / geo.K;
+    k = gk % geo.K;
+  }
+  StripeRef sr;
+  if (!stripe_of<C::F>(geo, g, k, &sr)) return;
   for (int d = threadIdx.x; d < C::F; d += C::NT) hist[d] = 0;
   __syncthreads();
-  const uint32_t gk = xcd_local_index(blockIdx.x, G * K);
-  const uint32_t g = gk / K, k = gk % K;
-  const uint64_t s0 = (uint64_t)g * stripe_len;
-  const uint64_t s1 = s0 + stripe_len < n ? s0 + stripe_len : n;
-  const uint64_t sub = (stripe_len / K + 1) & ~1ull;  // even, so sub-stripes stay 16-byte aligned
-  uint64_t beg = s0 + k * sub, end = beg + sub;
-  if (k == K - 1 || end > s1) end = s1;
+  const uint64_t beg = sr.beg, end = sr.end;
   if (beg < end) {
-    if (V16) {
-      const uint4 *in4 = reinterpret_cast<const uint4 *>(in + beg);
-      const uint64_t npairs = (end - beg) / 2;
+    // 16-byte loads from the first 16-byte aligned tuple on
+    const uint64_t a0 = (V16 && ((uintptr_t)(in + beg) & 15)) ? beg + 1 : beg;
+    if (V16 && a0 < end) {
+      if (a0 != beg && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[beg].x, xm, shift, C::F - 1)], 1u);
+      const uint4 *in4 = reinterpret_cast<const uint4 *>(in + a0);
+      const uint64_t npairs = (end - a0) / 2;
       uint64_t i = threadIdx.x;
       for (; i + 3 * C::NT < npairs; i += 4 * C::NT) {
         uint4 q[4];
@@ -80,20 +148,20 @@ __global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__
         atomicAdd(&hist[digit_of(q.x, xm, shift, C::F - 1)], 1u);
         atomicAdd(&hist[digit_of(q.z, xm, shift, C::F - 1)], 1u);
       }
-      if (((end - beg) & 1) && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[end - 1].x, xm, shift, C::F - 1)], 1u);
+      if (((end - a0) & 1) && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[end - 1].x, xm, shift, C::F - 1)], 1u);
     } else {
       for (uint64_t i = beg + threadIdx.x; i < end; i += C::NT) atomicAdd(&hist[digit_of(in[i].x, xm, shift, C::F - 1)], 1u);
     }
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < C::F; d += C::NT) counts[((uint64_t)d * G + g) * K + k] = hist[d];
+  for (int d = threadIdx.x; d < C::F; d += C::NT) counts[sr.cidx0 + (uint64_t)d * sr.cstride] = hist[d];
 }
 
 template <int BITS>
 __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restrict__ in, uint64_t n, int shift,
                                                           uint32_t xm, uint32_t G, uint64_t stripe_len,
                                                           uint32_t K, const uint32_t *__restrict__ offsets,
-                                                          uint2 *__restrict__ out) {
+                                                          uint2 *__restrict__ out, uint32_t *err) {
   using C = PartCfg<BITS>;
   constexpr int F = C::F;
   __shared__ uint32_t cursor[F];
@@ -181,7 +249,9 @@ __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restric
     for (uint32_t i = threadIdx.x; i < tn; i += C::NT) {
       const uint2 t = stage[i];
       const uint32_t d = digit_of(t.x, xm, shift, F - 1);
-      out[(uint64_t)cursor[d] + (i - tbase[d])] = t;
+      const uint64_t pos = (uint64_t)cursor[d] + (i - tbase[d]);
+      if (pos < n) out[pos] = t;
+      else atomic_or_agent(err, kErrOverflow);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < F; d += C::NT) cursor[d] += tbase[d + 1] - tbase[d];
@@ -201,10 +271,9 @@ constexpr int kFastPairs = 8;                         // uint4 loads (2 tuples) 
 constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples
 
 template <int BITS>
-__global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__restrict__ in, uint64_t n, int shift,
-                                                               uint32_t xm, uint32_t G, uint64_t stripe_len,
-                                                               uint32_t K, const uint32_t *__restrict__ offsets,
-                                                               uint2 *__restrict__ out) {
+__global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__restrict__ in, StripeGeo geo, int shift,
+                                                               uint32_t xm, const uint32_t *__restrict__ offsets,
+                                                               uint2 *__restrict__ out, uint32_t *err) {
   constexpr int F = 1 << BITS;
   constexpr int DPT = (F + kFastNT - 1) / kFastNT;
   __shared__ uint2 stage[kFastTile];
@@ -213,14 +282,19 @@ __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__re
   __shared__ uint32_t cursor[F];
   __shared__ uint32_t scan_tmp[kFastNT / kWave + 1];
 
-  const uint32_t g = xcd_local_index(blockIdx.x, G);
+  const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
+  StripeRef sr;
+  if (!stripe_of<F>(geo, g, 0, &sr, true)) return;
   for (int d = threadIdx.x; d < F; d += kFastNT) {
-    cursor[d] = offsets[((uint64_t)d * G + g) * K];
+    cursor[d] = offsets[sr.cidx0 + (uint64_t)d * sr.cstride];
     cnt[d] = 0;
   }
-  const uint64_t beg = (uint64_t)g * stripe_len;
-  const uint64_t end = beg + stripe_len < n ? beg + stripe_len : n;
-  if (beg >= end) return;
+  // a stripe whose first tuple is only 8-byte aligned is read from one tuple
+  // earlier (16-byte pairs); that tuple is not part of the stripe: skip it
+  const uint64_t skip = ((uintptr_t)(in + sr.beg) & 15) ? 1 : 0;
+  const uint64_t beg = sr.beg - skip;
+  const uint64_t end = sr.end;
+  if (sr.beg >= end) return;
   const uint4 *in4 = reinterpret_cast<const uint4 *>(in + beg);
   const uint64_t nfull = (end - beg) / 2;              // complete 16-byte pairs
   const uint64_t npairs = (end - beg + 1) / 2;         // incl. a trailing single tuple
@@ -244,12 +318,13 @@ __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__re
   __syncthreads();
 
   for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
-    const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples in this tile
+    const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples read in this tile
+    const uint32_t tskip = t0 == 0 ? (uint32_t)skip : 0u;                            // lead-in tuple, not ours
     uint32_t rk[2 * kFastPairs];
 #pragma unroll
     for (int j = 0; j < kFastPairs; ++j) {
       const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
-      if (e < tn) rk[2 * j] = atomicAdd(&cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
+      if (e >= tskip && e < tn) rk[2 * j] = atomicAdd(&cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
       if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
     }
     __syncthreads();
@@ -270,22 +345,25 @@ __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__re
         if (d < F) tbase[d] = ex;
         ex += loc[k];
       }
-      if (threadIdx.x == 0) tbase[F] = tn;
+      if (threadIdx.x == 0) tbase[F] = tn - tskip;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kFastPairs; ++j) {
       const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
-      if (e < tn) stage[tbase[digit_of(cur[j].x, xm, shift, F - 1)] + rk[2 * j]] = make_uint2(cur[j].x, cur[j].y);
+      if (e >= tskip && e < tn)
+        stage[tbase[digit_of(cur[j].x, xm, shift, F - 1)] + rk[2 * j]] = make_uint2(cur[j].x, cur[j].y);
       if (e + 1 < tn)
         stage[tbase[digit_of(cur[j].z, xm, shift, F - 1)] + rk[2 * j + 1]] = make_uint2(cur[j].z, cur[j].w);
     }
     __syncthreads();
     load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
-    for (uint32_t i = threadIdx.x; i < tn; i += kFastNT) {
+    for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
       const uint2 t = stage[i];
       const uint32_t d = digit_of(t.x, xm, shift, F - 1);
-      out[(uint64_t)cursor[d] + (i - tbase[d])] = t;
+      const uint64_t pos = (uint64_t)cursor[d] + (i - tbase[d]);
+      if (pos < geo.n) out[pos] = t;
+      else atomic_or_agent(err, kErrOverflow);  // inconsistent counts: report, never write outside
     }
     __syncthreads();
     for (int d = threadIdx.x; d < F; d += kFastNT) {
@@ -317,27 +395,86 @@ hipError_t launch_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
                        void *scan_scratch, uint32_t *err, hipStream_t stream) {
   const uint2 *vin = reinterpret_cast<const uint2 *>(in);
   uint2 *vout = reinterpret_cast<uint2 *>(out);
-  const bool aligned = ((uintptr_t)in & 15) == 0;
+  const bool aligned = ((uintptr_t)in & 7) == 0;
+  StripeGeo geo{p.n, p.stripe_len, p.stripes, p.hist_split, nullptr, nullptr, 0};
   const uint32_t hist_grid = p.stripes * p.hist_split;
   if (aligned)
-    hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
-                       p.xor_mask, p.stripes, p.hist_split, p.stripe_len, counts);
+    hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, geo, p.shift,
+                       p.xor_mask, counts);
   else
-    hipLaunchKernelGGL((k_part_hist<BITS, false>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
-                       p.xor_mask, p.stripes, p.hist_split, p.stripe_len, counts);
+    hipLaunchKernelGGL((k_part_hist<BITS, false>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, geo, p.shift,
+                       p.xor_mask, counts);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint64_t m = ((uint64_t)p.stripes * p.hist_split) << BITS;
   e = scan_u32(counts, counts, m, nullptr, scan_scratch, err, stream);
   if (e != hipSuccess) return e;
   if (p.fast) {
-    if (!aligned) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(p.stripes), dim3(kFastNT), 0, stream, vin, p.n, p.shift,
-                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout);
+    if ((uintptr_t)in & 15) return hipErrorInvalidValue;  // stripe 0 starts at the array start
+    hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(p.stripes), dim3(kFastNT), 0, stream, vin, geo, p.shift,
+                       p.xor_mask, counts, vout, err);
   } else {
     hipLaunchKernelGGL(k_part_scatter<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
-                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout);
+                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout, err);
   }
+  return hipGetLastError();
+}
+
+// ---- segmented (second MSD) pass ------------------------------------------------
+
+__global__ void k_seg_stripes(const uint32_t *seg_start, uint32_t nseg, uint64_t len, uint32_t *seg_base) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nseg) seg_base[i] = (uint32_t)(((uint64_t)(seg_start[i + 1] - seg_start[i]) + len - 1) / len);
+  if (i == nseg) seg_base[nseg] = 0;
+}
+
+// part_start[seg * F + d] = start of digit d inside segment seg.
+__global__ void k_seg_starts(const uint32_t *offsets, const uint32_t *seg_start, const uint32_t *seg_base,
+                             uint32_t nseg, uint32_t F, uint64_t n, uint32_t *part_start) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t tot = (uint64_t)nseg * F;
+  if (i < tot) {
+    const uint32_t sg = (uint32_t)(i / F), d = (uint32_t)(i % F);
+    const uint32_t k1 = seg_base[sg + 1] - seg_base[sg];
+    part_start[i] = k1 ? offsets[(uint64_t)seg_base[sg] * F + (uint64_t)d * k1] : seg_start[sg];
+  } else if (i == tot) {
+    part_start[tot] = (uint32_t)n;
+  }
+}
+
+template <int BITS>
+hipError_t launch_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple *out, const uint32_t *seg_start,
+                           uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
+  constexpr uint32_t F = 1u << BITS;
+  char *sc = (char *)scratch;
+  uint32_t *seg_base = (uint32_t *)sc;
+  sc += ((p.nseg + 1) * sizeof(uint32_t) + 255) / 256 * 256;
+  uint32_t *counts = (uint32_t *)sc;
+  const uint64_t m = (uint64_t)p.max_stripes * F;
+  sc += (m * sizeof(uint32_t) + 255) / 256 * 256;
+  void *scan_scratch = sc;
+
+  hipLaunchKernelGGL(k_seg_stripes, dim3((p.nseg + 1 + 255) / 256), dim3(256), 0, stream, seg_start, p.nseg,
+                     p.stripe_len, seg_base);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = scan_u32(seg_base, seg_base, p.nseg + 1, nullptr, scan_scratch, err, stream);
+  if (e == hipSuccess) e = fill_u32(counts, 0u, m, stream);  // counts of absent stripes stay 0
+  if (e != hipSuccess) return e;
+  const StripeGeo geo{p.n, p.stripe_len, 0, 1, seg_start, seg_base, p.nseg};
+  const uint2 *vin = reinterpret_cast<const uint2 *>(in);
+  const uint32_t grid = (p.max_stripes + kXcds - 1) / kXcds * kXcds;
+  hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(grid), dim3(kPartNT), 0, stream, vin, geo, p.shift, 0u,
+                     counts);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = scan_u32(counts, counts, m, nullptr, scan_scratch, err, stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(grid), dim3(kFastNT), 0, stream, vin, geo, p.shift, 0u, counts,
+                     reinterpret_cast<uint2 *>(out), err);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t tot = (uint64_t)p.nseg * F + 1;
+  hipLaunchKernelGGL(k_seg_starts, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, stream, counts, seg_start,
+                     seg_base, p.nseg, F, p.n, part_start);
   return hipGetLastError();
 }
 
@@ -416,6 +553,41 @@ hipError_t partition_bounds(const eo_tuple *sorted, uint64_t n, int bits, int sh
   hipLaunchKernelGGL(k_part_bounds, dim3(blocks), dim3(256), 0, stream, reinterpret_cast<const uint2 *>(sorted), n,
                      shift, xor_mask, F, part_start);
   return hipGetLastError();
+}
+
+SegPassPlan plan_seg_pass(uint64_t n, uint32_t nseg, int bits, int shift, int cus) {
+  SegPassPlan p{};
+  p.n = n;
+  p.nseg = nseg;
+  p.bits = bits;
+  p.shift = shift;
+  // stripes of ~n / (2 CUs) tuples, at least one fast tile
+  uint64_t len = std::max<uint64_t>(kFastTile, n / std::max<uint64_t>(1, (uint64_t)std::max(cus, 8) * 2));
+  len = (len + kFastTile - 1) / kFastTile * kFastTile;
+  p.stripe_len = len;
+  p.max_stripes = (uint32_t)(nseg + (n + len - 1) / len);
+  return p;
+}
+
+size_t seg_pass_scratch_bytes(const SegPassPlan &p) {
+  const uint64_t m = (uint64_t)p.max_stripes << p.bits;
+  return ((p.nseg + 1) * sizeof(uint32_t) + 255) / 256 * 256 + (m * sizeof(uint32_t) + 255) / 256 * 256 +
+         std::max(scan_scratch_bytes_u32(m), scan_scratch_bytes_u32(p.nseg + 1));
+}
+
+hipError_t partition_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple *out, const uint32_t *seg_start,
+                              uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
+  if (((uintptr_t)in & 15) || ((uintptr_t)out & 7)) return hipErrorInvalidValue;
+  switch (p.bits) {
+#define EO_SEG_CASE(B) \
+  case B:              \
+    return launch_seg_pass<B>(p, in, out, seg_start, part_start, scratch, err, stream);
+    EO_SEG_CASE(1) EO_SEG_CASE(2) EO_SEG_CASE(3) EO_SEG_CASE(4) EO_SEG_CASE(5) EO_SEG_CASE(6)
+    EO_SEG_CASE(7) EO_SEG_CASE(8) EO_SEG_CASE(9) EO_SEG_CASE(10) EO_SEG_CASE(11)
+#undef EO_SEG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace eo

@@ -30,16 +30,13 @@ eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPl
   } else if (nR <= p.lds_cap) {
     p.bits = 0;
   } else {
-    // One radix pass of at most kMaxPassBits bits (2^11 digits keep each
-    // digit's run per 16K-tuple tile near 64 bytes, so writes stay coalesced);
-    // an R partition may span up to 4 LDS tables, whose tiles share each S
-    // range through the XCD's L2.  Beyond that, a second pass.
-    const uint64_t target = (uint64_t)p.lds_cap * 7 / 8;
-    p.bits = ceil_log2((nR + target - 1) / target);
-    if (p.bits > kMaxPassBits) {
-      const int one_pass = ceil_log2((nR + 4ull * p.lds_cap - 1) / (4ull * p.lds_cap));
-      p.bits = one_pass <= kMaxPassBits ? kMaxPassBits : std::min(p.bits, kMaxJoinBits);
-    }
+    // Mean R partition at most lds_cap tuples, so that each partition is one
+    // LDS table (the table holds 17408: a full 16K partition plus binomial
+    // spread).  Up to 2^11 digits in one pass (each digit's run per 16K-tuple
+    // staging tile stays near 64 bytes, so writes coalesce); beyond that two
+    // passes, high digit first.
+    p.bits = ceil_log2((nR + p.lds_cap - 1) / p.lds_cap);
+    if (p.bits > kMaxJoinBits) p.bits = kMaxJoinBits;
   }
 
   const int passes_opt = o ? o->passes : 0;

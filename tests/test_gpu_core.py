@@ -66,7 +66,8 @@ def test_partition_skewed_and_tiny(ctx):
         assert np.array_equal(out, want) and np.array_equal(off, woff)
 
 
-def test_hash_join_misaligned_device_input(ctx):
+@pytest.mark.parametrize("bits", [0, 12])
+def test_hash_join_misaligned_device_input(ctx, bits):
     """Device pointers that are only 8-byte aligned take the stable scatter."""
     R, S = gen.c3(seed=21, nR=70001, nS=50003, key_range=1 << 14)
     import ctypes
@@ -76,7 +77,7 @@ def test_hash_join_misaligned_device_input(ctx):
     want = oracle.hash_join(R, S, nparts=1 << 10)
     dout = DeviceArray(ctx, len(want) * 8 + 8)
     n = ctypes.c_uint64()
-    o = ctx.opts(where=_abi.EO_DEVICE)
+    o = ctx.opts(where=_abi.EO_DEVICE, radix_bits=bits)
     st = ctx.lib.eo_hash_join(ctx.handle, dR.ptr + 8, len(R), dS.ptr + 8, len(S), dout.ptr, len(want),
                               ctypes.byref(n), ctypes.byref(o))
     assert st == 0 and n.value == len(want)
@@ -93,7 +94,8 @@ def test_hash_join_c1_full(ctx):
 
 @pytest.mark.parametrize("det", [False, True])
 @pytest.mark.parametrize("bits,passes,cap", [(-1, 0, 16384), (3, 1, 4096), (8, 1, 8192), (11, 1, 4096),
-                                             (12, 2, 4096), (14, 2, 16384), (0, 0, 8192)])
+                                             (12, 2, 4096), (14, 2, 16384), (0, 0, 8192), (2, 2, 16384),
+                                             (17, 2, 16384), (22, 2, 16384)])
 def test_hash_join_plans(ctx, bits, passes, cap, det):
     R, S = gen.c3(seed=bits + 100, nR=60000, nS=90000, key_range=1 << 15)
     got = ctx.hash_join(R, S, radix_bits=bits, passes=passes, lds_tuples=cap, deterministic=det)
@@ -123,6 +125,17 @@ def test_hash_join_heavy_duplicates(ctx, det):
     S = np.stack([np.full(300, 42), np.arange(300)], 1).astype(np.int32)
     got = ctx.hash_join(R, S, radix_bits=4, deterministic=det)
     assert len(got) == 20000 * 300
+    assert same_multiset(got, brute_join(R, S))
+
+
+def test_hash_join_overfull_subpartition_rounds(ctx):
+    """One key fills a whole partition: its R side spans many LDS tables."""
+    R = np.stack([np.full(1 << 17, 5), np.arange(1 << 17)], 1).astype(np.int32)
+    extra = np.stack([np.arange(1 << 15) * 4 + 2, (1 << 17) + np.arange(1 << 15)], 1).astype(np.int32)
+    R = np.concatenate([R, extra])
+    S = np.stack([np.concatenate([np.full(64, 5), np.arange(1000) * 4 + 2]), np.arange(1064)], 1).astype(np.int32)
+    got = ctx.hash_join(R, S, radix_bits=2)
+    assert len(got) == 64 * (1 << 17) + 1000
     assert same_multiset(got, brute_join(R, S))
 
 
