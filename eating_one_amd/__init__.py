@@ -20,7 +20,7 @@ import numpy as np
 from . import _abi
 from ._abi import EO_DEVICE, EO_F_DETERMINISTIC, EO_HOST, JoinOpts, JoinStats
 
-__all__ = ["Context", "DeviceArray", "EOError", "CapacityError", "load_library"]
+__all__ = ["Context", "DeviceArray", "Index", "EOError", "CapacityError", "load_library"]
 
 
 class EOError(RuntimeError):
@@ -207,3 +207,80 @@ class Context:
         if st not in (_abi.EO_OK, _abi.EO_ECAPACITY):
             _check(st, "eo_hash_join")
         return n.value, st
+
+    # ---- resident indexes -------------------------------------------------------
+    def hidx_build(self, R, **kw) -> "Index":
+        """Hash index on R (reference innerJoin_gpu createIndex), resident on the device."""
+        R = _tuples(R)
+        h = ctypes.c_void_p()
+        o = self.opts(**kw)
+        _check(self.lib.eo_hidx_build(self.handle, _vp(R), len(R), ctypes.byref(o), ctypes.byref(h)), "eo_hidx_build")
+        return Index(self, h.value)
+
+    def sidx_build(self, R, learned: bool = False, **kw) -> "Index":
+        """Sorted index on R (reference sort_index createIndex), optionally with the learned model."""
+        R = _tuples(R)
+        h = ctypes.c_void_p()
+        o = self.opts(**kw)
+        _check(self.lib.eo_sidx_build(self.handle, _vp(R), len(R), int(learned), ctypes.byref(o), ctypes.byref(h)),
+               "eo_sidx_build")
+        return Index(self, h.value)
+
+
+class Index:
+    """A resident index (eo_index) built on one Context."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def info(self) -> dict:
+        i = _abi.IndexInfo()
+        _check(self.ctx.lib.eo_index_query(self.handle, ctypes.byref(i)), "eo_index_query")
+        return {"kind": i.kind, "learned": bool(i.learned), "n": i.n, "bytes": i.bytes, "leaves": i.leaves,
+                "max_err": i.max_err, "mean_err": i.mean_err, "build_ms": i.build_ms}
+
+    def join(self, S, count_only: bool = False, binary_search: bool = False, **kw):
+        """Index join of S (reference inner-idx-bnlj.cc join())."""
+        S = _tuples(S)
+        flags = kw.pop("flags", 0) | (_abi.EO_F_BINARY_SEARCH if binary_search else 0)
+        o = self.ctx.opts(flags=flags, **kw)
+        L = self.ctx.lib
+        n = ctypes.c_uint64()
+        _check(L.eo_index_join(self.ctx.handle, self.handle, _vp(S), len(S), None, 0, ctypes.byref(n),
+                               ctypes.byref(o)), "eo_index_join")
+        if count_only:
+            return n.value
+        out = np.empty((max(n.value, 1), 2), dtype=np.int32)
+        n2 = ctypes.c_uint64()
+        st = L.eo_index_join(self.ctx.handle, self.handle, _vp(S), len(S), _vp(out), n.value, ctypes.byref(n2),
+                             ctypes.byref(o))
+        _check(st, "eo_index_join")
+        return out[: n.value]
+
+    def lookup(self, S, method: int = _abi.EO_LOOKUP_DEFAULT) -> np.ndarray:
+        """Per probe, the index position its search starts from (lower bound for sorted indexes)."""
+        S = _tuples(S)
+        pos = np.empty(len(S), dtype=np.uint64)
+        o = self.ctx.opts()
+        _check(self.ctx.lib.eo_index_lookup(self.ctx.handle, self.handle, _vp(S), len(S), method, _vp(pos),
+                                            ctypes.byref(o)), "eo_index_lookup")
+        return pos
+
+    def export(self):
+        n = self.info()["n"]
+        keys = np.empty(n, dtype=np.int32)
+        pays = np.empty(n, dtype=np.int32)
+        _check(self.ctx.lib.eo_index_export(self.ctx.handle, self.handle, _vp(keys), _vp(pays)), "eo_index_export")
+        return keys, pays
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.eo_index_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -73,6 +73,23 @@ class DeviceInfo(ctypes.Structure):
     ]
 
 
+class IndexInfo(ctypes.Structure):
+    _fields_ = [
+        ("kind", I32),
+        ("learned", I32),
+        ("n", U64),
+        ("bytes", U64),
+        ("leaves", I32),
+        ("max_err", I32),
+        ("mean_err", ctypes.c_double),
+        ("build_ms", ctypes.c_double),
+    ]
+
+
+EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
+EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
+EO_F_BINARY_SEARCH = 4
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "eo_abi_version": (U32, []),
@@ -93,6 +110,13 @@ SIGNATURES = {
     "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),
     "eo_hash_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_nlj_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_hidx_build": (INT, [P, P, U64, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
+    "eo_sidx_build": (INT, [P, P, U64, INT, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
+    "eo_index_join": (INT, [P, P, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_index_lookup": (INT, [P, P, P, U64, INT, P, ctypes.POINTER(JoinOpts)]),
+    "eo_index_export": (INT, [P, P, P, P]),
+    "eo_index_query": (INT, [P, ctypes.POINTER(IndexInfo)]),
+    "eo_index_free": (INT, [P]),
 }
 
 _lib = None

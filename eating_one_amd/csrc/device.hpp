@@ -59,6 +59,19 @@ __device__ __forceinline__ uint64_t read_lane(uint64_t x, int lane) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// FP64 wave scan (cross-lane shuffles; used off the hot path, e.g. model fits).
+__device__ __forceinline__ double wave_incl_scan(double x) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const double y = __shfl_up(x, o, kWave);
+    if (lane_id() >= o) x += y;
+  }
+  return x;
+}
+__device__ __forceinline__ double read_lane(double x, int lane) {
+  return __longlong_as_double((long long)read_lane((uint64_t)__double_as_longlong(x), lane));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
   return read_lane(wave_incl_scan(x), kWave - 1);

@@ -106,6 +106,25 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t n_tiles, const e
                       const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, eo_pair *out,
                       uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream);
 
+// ---- index.hip --------------------------------------------------------------
+// Resident indexes.  kind 0: hash (keys/pays grouped by bucket, off[NB+1]);
+// kind 1: sorted (binary search); kind 2: sorted with the learned model.
+uint64_t index_probe_tiles(uint64_t nS);
+int learned_leaves();
+int learned_knots();
+size_t learned_model_bytes();
+hipError_t hidx_build(const eo_tuple *R, uint64_t n, int log2nb, uint32_t *keys, uint32_t *pays, uint32_t *off,
+                      uint32_t *cursor, void *scan_scratch, uint32_t *err, hipStream_t stream);
+hipError_t sidx_split(const eo_tuple *sorted, uint64_t n, uint32_t *keys, uint32_t *pays, hipStream_t stream);
+// Fits the learned model over sorted keys; leaf_scratch: n u16.
+hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *leaf_scratch, hipStream_t stream);
+// Count (write = false: per-tile counts into tile_cnt) or write (write = true:
+// tile_cnt holds exclusive-scanned tile offsets).  lb_out (optional, nS u64):
+// the index position each probe starts from (lower bound for sorted kinds).
+hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
+                       const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
+                       uint64_t *tile_cnt, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream);
+
 // ---- nlj.hip ----------------------------------------------------------------
 struct NljPlan {
   uint64_t nR, nS;

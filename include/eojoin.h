@@ -185,6 +185,64 @@ eo_status eo_nlj_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
                       uint64_t out_cap, uint64_t *n_out,
                       const eo_join_opts *opts);
 
+/* ---- resident indexes ------------------------------------------------------ */
+
+typedef enum eo_index_kind { EO_INDEX_HASH = 0, EO_INDEX_SORTED = 1 } eo_index_kind;
+
+/* Lookup methods of eo_index_lookup. */
+enum { EO_LOOKUP_DEFAULT = 0, EO_LOOKUP_BINARY = 1, EO_LOOKUP_LEARNED = 2 };
+
+/* eo_index_join on a learned sorted index: use plain binary search instead. */
+#define EO_F_BINARY_SEARCH 4
+
+typedef struct eo_index_info {
+  int32_t kind;      /* eo_index_kind */
+  int32_t learned;   /* sorted index carries the learned model */
+  uint64_t n;        /* indexed R tuples */
+  uint64_t bytes;    /* device bytes held by the index */
+  int32_t leaves;    /* learned model: stage-2 linear models */
+  int32_t max_err;   /* learned model: largest last-mile window half-width */
+  double mean_err;   /* learned model: mean of err_lo + err_hi over leaves */
+  double build_ms;   /* device time of the build */
+} eo_index_info;
+
+/* Hash index on R, built on the device and kept resident until
+ * eo_index_free.  Replaces createIndex of joins/innerJoin_gpu/main_street/
+ * inner-idx-bnlj.cc:149-180 (a CPU counting sort into NB_BUCKET buckets) and
+ * its per-join HtoD copies (:279-330). */
+eo_status eo_hidx_build(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
+                        const eo_join_opts *opts, eo_index **index);
+
+/* Sorted index on R (device radix sort), optionally with the learned model
+ * (learned != 0).  Replaces qsort + createIndex of
+ * joins/innerJoin_gpu/sort_index/main_street/inner-idx-bnlj.cc:175-216. */
+eo_status eo_sidx_build(eo_ctx *ctx, const eo_tuple *R, uint64_t nR, int learned,
+                        const eo_join_opts *opts, eo_index **index);
+
+/* Index join of S against a resident index: replaces join() of
+ * joins/innerJoin_gpu/main_street/inner-idx-bnlj.cc:183-673 (hash) and
+ * joins/innerJoin_gpu/sort_index/main_street/inner-idx-bnlj.cc:219-658
+ * (sorted: binary search, or the learned model unless EO_F_BINARY_SEARCH).
+ * Same sizing and error conventions as eo_hash_join. */
+eo_status eo_index_join(eo_ctx *ctx, const eo_index *index, const eo_tuple *S,
+                        uint64_t nS, eo_pair *out, uint64_t out_cap,
+                        uint64_t *n_out, const eo_join_opts *opts);
+
+/* Per probe, the index position its search starts from: the lower bound of
+ * S[i].key (sorted index; method BINARY or LEARNED — equal for every key
+ * present in R) or the bucket start (hash index).  pos: nS uint64, host or
+ * device per opts->where.  Used to judge the learned model against binary
+ * search probe by probe. */
+eo_status eo_index_lookup(eo_ctx *ctx, const eo_index *index, const eo_tuple *S,
+                          uint64_t nS, int method, uint64_t *pos,
+                          const eo_join_opts *opts);
+
+/* Copies the index arrays (keys, payloads; index order) to host memory. */
+eo_status eo_index_export(eo_ctx *ctx, const eo_index *index, uint32_t *keys,
+                          uint32_t *pays);
+eo_status eo_index_query(const eo_index *index, eo_index_info *info);
+eo_status eo_index_free(eo_index *index);
+
 #ifdef __cplusplus
 }
 #endif
