@@ -1,17 +1,22 @@
-"""Headline benchmark: 2^27 x 2^28 radix hash join on one MI355X.
+"""Benchmarks of the MI355X inner joins.  Prints ONE JSON line.
 
-One step = one eo_hash_join over device-resident R and S (inputs already in
-HBM when the timed region starts; output pairs written to a device buffer).
-Prints one JSON line:
-  value      = (|R| + |S|) / seconds per join   (join tuples/s)
-  roofline   = the dominant kernel's algorithmic bytes per launch / its mean
-               launch time (HIP events on the join's stream) vs 8 TB/s
-  pipeline   = whole-join algorithmic bytes 24(|R|+|S|) + 8|out| / join time
-  cpu_baseline = the CPU oracle's hash join on a bounded sample, host cores.
+Default (the headline): C2, a 2^27 x 2^28 radix hash join on one GPU.  One
+step = one eo_hash_join over device-resident R and S (inputs already in HBM
+when the timed region starts; pairs written to a device buffer).
+  value        (|R| + |S|) / seconds per join        [join tuples/s]
+  roofline     the dominant kernel (largest device time per join, from HIP
+               events recorded around every launch on the join's stream):
+               algorithmic bytes per launch / its mean launch time vs 8 TB/s
+  pipeline     whole join: 24(|R|+|S|) + 8|out| bytes / join time vs 8 TB/s
+  cpu_baseline the CPU_JOIN hash join restated in oracle/ on a bounded sample
+
+Other configurations (parity cases; reported in the design notes, not the
+driver's line): --config c3-nlj | c3-hidx | c4.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -27,108 +32,132 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_log2_r: int, theta: float, sel: float, threads: int):
-    """CPU_JOIN hash join (oracle restatement) on a bounded sample of the same
-    workload, partitions scaled to keep the reference's tuples-per-partition."""
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+# Algorithmic bytes per launch of each kernel family of the hash join, as a
+# function of the tuples the launch processes (n) or of the whole join.
+def kernel_alg_bytes(name, nR, nS, n_out):
+    per_launch_tuples = (nR + nS) / 2  # each family runs once over R and once over S
+    if name in ("part_hist", "seg_hist"):
+        return 8 * per_launch_tuples  # read every tuple
+    if name in ("part_scatter", "seg_scatter", "part_scatter_stable"):
+        return 16 * per_launch_tuples  # read + write every tuple
+    if name == "probe":
+        return 8 * (nR + nS) + 8 * n_out  # read both partitioned relations, write the pairs
+    return None
+
+
+def cpu_baseline_hash(sample_log2_r: int, theta: float, sel: float, threads: int, RS=None):
+    """CPU_JOIN hash join (oracle restatement) on a bounded sample of C2, with
+    the reference's 128 R tuples per partition (2^20 partitions at |R| = 2^27).
+    RS: the GPU run's own inputs when the sample is the full workload."""
     import oracle
     from eating_one_amd import gen
 
     nR, nS = 1 << sample_log2_r, 1 << (sample_log2_r + 1)
-    R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS, theta=theta, selectivity=sel)
-    # CPU_JOIN uses NB_BKTENT = 2^20 partitions for |R| = 2^27: 128 R tuples per partition
+    if RS is not None and len(RS[0]) == nR and len(RS[1]) == nS:
+        R, S = RS
+    else:
+        R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS, theta=theta, selectivity=sel)
     nparts = max(1, nR // 128)
     t0 = time.perf_counter()
     if threads == 1:
-        n = oracle.hash_join(R, S, nparts=nparts, count_only=False)
+        n = oracle.hash_join(R, S, nparts=nparts, count_only=True)
     else:
-        n = oracle.hash_join_mt(R, S, nparts=nparts, threads=threads)
+        n = oracle.hash_join_mt(R, S, nparts=nparts, threads=threads, count_only=True)
     dt = time.perf_counter() - t0
+    model, ncpu = host_info()
     return {
         "value": (nR + nS) / dt,
         "unit": "tuples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"CPU_JOIN hash join restated (oracle/cpu_join.c), |R|=2^{sample_log2_r}, "
-                  f"|S|=2^{sample_log2_r + 1}, theta={theta}, selectivity={sel}, {nparts} partitions "
-                  f"(128 R tuples each, as 2^20 partitions at |R|=2^27), {len(n)} pairs, {dt:.2f} s",
+        "sample": f"oracle/cpu_join.c restatement of CPU_JOIN hash_cpu (partition + per-partition nested loop, "
+                  f"timer scope hj.cc:209-230), |R|=2^{sample_log2_r}, |S|=2^{sample_log2_r + 1}, "
+                  f"theta={theta}, selectivity={sel}, {nparts} partitions, {n} matches, {dt:.2f} s on "
+                  f"{threads} of {ncpu} host threads ({model})",
     }
 
 
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log2-r", type=int, default=27)
-    ap.add_argument("--theta", type=float, default=0.0)
-    ap.add_argument("--selectivity", type=float, default=1.0)
-    ap.add_argument("--radix-bits", type=int, default=0)
-    ap.add_argument("--passes", type=int, default=0)
-    ap.add_argument("--lds", type=int, default=0)
-    ap.add_argument("--s-tile", type=int, default=0)
-    ap.add_argument("--cpu-log2-r", type=int, default=24)
-    ap.add_argument("--cpu-threads", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    a = ap.parse_args(argv)
+def copy_ceiling(ctx):
+    g = ctypes.c_double()
+    st = ctx.lib.eo_bandwidth_probe(ctx.handle, 1 << 30, 10, ctypes.byref(g))
+    return g.value if st == 0 else None
 
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no multi-GPU path",
-              file=sys.stderr)
-        return 2
 
+def run_c2(a):
     import numpy as np
 
     import eating_one_amd as eo
     from eating_one_amd import _abi, gen
 
     nR, nS = 1 << a.log2_r, 1 << (a.log2_r + 1)
-    log(f"generating C2 workload |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1} theta={a.theta} sel={a.selectivity}")
+    log(f"generating C2 |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1} theta={a.theta} selectivity={a.selectivity}")
     t0 = time.perf_counter()
     R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS, theta=a.theta, selectivity=a.selectivity)
     log(f"generated in {time.perf_counter() - t0:.1f} s")
-
     ctx = eo.Context(0)
     info = ctx.device_info()
-    log(f"device {info}")
     dR = eo.DeviceArray.from_host(ctx, R)
     dS = eo.DeviceArray.from_host(ctx, S)
-    del R
-    cap = nS if a.selectivity >= 1.0 else int(nS * min(1.0, a.selectivity * 1.2 + 0.01))
-    n_expected = None
+    keep = (R, S) if (not a.no_cpu and a.cpu_log2_r == a.log2_r) else None
+    del R, S
     kw = dict(radix_bits=a.radix_bits, passes=a.passes, lds_tuples=a.lds, s_tile=a.s_tile)
-    n_out, st = ctx.hash_join_device(dR, nR, dS, nS, None, 0, **kw)
-    cap = max(cap, n_out)
-    dout = eo.DeviceArray(ctx, cap * 8)
-    log(f"count-only pass: {n_out} pairs")
-
+    n_out, _ = ctx.hash_join_device(dR, nR, dS, nS, None, 0, **kw)
+    dout = eo.DeviceArray(ctx, max(n_out, 1) * 8)
+    log(f"count: {n_out} pairs")
     stats = _abi.JoinStats()
 
     def step():
-        n, st = ctx.hash_join_device(dR, nR, dS, nS, dout, cap, stats=stats, **kw)
-        assert st == _abi.EO_OK, st
-        return n
+        n, st = ctx.hash_join_device(dR, nR, dS, nS, dout, n_out, stats=stats, **kw)
+        if st != _abi.EO_OK or n != n_out:
+            raise RuntimeError(f"join failed: status {st}, {n} pairs")
 
     for _ in range(a.warmup):
-        n_expected = step()
+        step()
     ctx.sync()
+    kern = {}
     phases = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        n = step()
-        assert n == n_expected
-        phases.append((stats.ms_partition, stats.ms_count, stats.ms_write, stats.ms_total))
+        step()
+        for k, (ms, nl) in stats.kernel_table().items():
+            tot = kern.setdefault(k, [0.0, 0])
+            tot[0] += ms
+            tot[1] += nl
+        phases.append(stats.ms_total)
     ctx.sync()
     dt = (time.perf_counter() - t0) / a.steps
-    ph = np.array(phases).mean(axis=0)
-    log(f"per join {dt * 1e3:.3f} ms; device phases partition {ph[0]:.3f} count {ph[1]:.3f} write {ph[2]:.3f} "
-        f"total {ph[3]:.3f} ms; bits={stats.radix_bits} passes={stats.passes} tiles={stats.tiles} "
-        f"lds={stats.lds_tuples}")
-
-    alg_bytes = 24 * (nR + nS) + 8 * n_expected
-    pipe_gbs = alg_bytes / dt / 1e9
-    # dominant phase: the partition of both relations (read + write both inputs: 16 B/tuple)
-    part_bytes = 16 * (nR + nS)
-    part_gbs = part_bytes / (ph[0] * 1e-3) / 1e9 if ph[0] > 0 else 0.0
+    log(f"per join {dt * 1e3:.3f} ms wall, {np.mean(phases):.3f} ms device; bits={stats.radix_bits} "
+        f"passes={stats.passes} tiles={stats.tiles}")
+    for k, (ms, nl) in sorted(kern.items(), key=lambda x: -x[1][0]):
+        log(f"  {k:20s} {ms / a.steps:8.3f} ms/join  {nl / a.steps:.0f} launches/join  {ms / nl:8.3f} ms/launch")
+    dom = max(kern.items(), key=lambda x: x[1][0])
+    dom_name, (dom_ms, dom_launches) = dom
+    dom_mean_ms = dom_ms / dom_launches
+    dom_bytes = kernel_alg_bytes(dom_name, nR, nS, n_out)
+    achieved = dom_bytes / (dom_mean_ms * 1e-3) / 1e9 if dom_bytes else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if os.path.exists(tfile):
+        try:
+            tj = json.load(open(tfile))
+            traffic = tj.get(dom_name, {}).get("bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    alg_bytes = 24 * (nR + nS) + 8 * n_out
+    pipe = alg_bytes / dt / 1e9
+    ceiling = copy_ceiling(ctx)
     res = {
         "metric": "join tuples/s & %HBM peak, 2^27 x 2^28 hash join",
         "value": (nR + nS) / dt,
@@ -141,21 +170,167 @@ def main(argv=None) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int32",
-        "data": "synthetic (seeded numpy generator, inputs resident in HBM)",
-        "config": {"workload": f"C2 radix hash join |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1} uniform R keys, "
-                               f"S zipf theta={a.theta}, selectivity={a.selectivity}",
-                   "radix_bits": stats.radix_bits, "passes": stats.passes, "lds_tuples": stats.lds_tuples,
-                   "n_out": int(n_expected)},
-        "roofline": {"bound": "hbm", "achieved": part_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": part_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "radix partition (hist+scan+scatter, R and S)"},
-        "pipeline": {"alg_bytes": alg_bytes, "achieved_gbs": pipe_gbs, "frac": pipe_gbs / HBM_PEAK_GBS},
+        "data": "synthetic: seeded numpy generator (R = permutation of [0,|R|), S keys drawn from R keys), "
+                "inputs resident in HBM",
+        "config": {"workload": f"C2 radix hash join |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1}, theta={a.theta}, "
+                               f"selectivity={a.selectivity}",
+                   "radix_bits": int(stats.radix_bits), "passes": int(stats.passes), "n_out": int(n_out),
+                   "device": info["arch"], "cus": info["cus"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                     "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "ms_per_launch": dom_mean_ms},
+        "pipeline": {"alg_bytes": alg_bytes, "achieved_gbs": pipe, "frac": pipe / HBM_PEAK_GBS,
+                     "kernels_ms_per_join": {k: v[0] / a.steps for k, v in kern.items()}},
+        "copy_ceiling_gbs": ceiling,
     }
+    ctx.close()
     if not a.no_cpu:
         log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(a.cpu_log2_r, a.theta, a.selectivity, a.cpu_threads)
-    print(json.dumps(res), flush=True)
+        res["cpu_baseline"] = cpu_baseline_hash(a.cpu_log2_r, a.theta, a.selectivity, a.cpu_threads, keep)
+    return res
+
+
+def run_c3_nlj(a):
+    import eating_one_amd as eo
+    from eating_one_amd import _abi, gen
+
+    R, S = gen.c3()
+    ctx = eo.Context(0)
+    info = ctx.device_info()
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    L = ctx.lib
+    n = ctypes.c_uint64()
+    o = ctx.opts(where=_abi.EO_DEVICE)
+    assert L.eo_nlj_join(ctx.handle, dR.ptr, len(R), dS.ptr, len(S), None, 0, ctypes.byref(n), ctypes.byref(o)) == 0
+    dout = eo.DeviceArray(ctx, n.value * 8)
+    stats = _abi.JoinStats()
+    o = ctx.opts(where=_abi.EO_DEVICE, stats=stats)
+    kern = {}
+    for it in range(a.warmup + a.steps):
+        st = L.eo_nlj_join(ctx.handle, dR.ptr, len(R), dS.ptr, len(S), dout.ptr, n.value, ctypes.byref(n),
+                           ctypes.byref(o))
+        assert st == 0
+        if it >= a.warmup:
+            for k, (ms, nl) in stats.kernel_table().items():
+                kern[k] = kern.get(k, 0.0) + ms
+    ms = {k: v / a.steps for k, v in kern.items()}
+    compares = len(R) * len(S)
+    peak = info["cus"] * 64 * info["clock_khz"] * 1e3  # one 32-bit compare per lane per clock
+    count_rate = compares / (ms["nlj_count"] * 1e-3)
+    total_ms = sum(ms.values())
+    res = {"metric": "NLJ compares/s, 2^18 x 2^18 (C3)", "value": compares / (total_ms * 1e-3),
+           "unit": "compares/s", "ms_per_step": total_ms, "n_out": n.value, "kernels_ms": ms,
+           "roofline": {"bound": "valu", "achieved": count_rate / 1e9, "peak": peak / 1e9, "unit": "Gcompares/s",
+                        "frac": count_rate / peak, "kernel": "nlj_count"}}
     ctx.close()
+    return res
+
+
+def run_c3_hidx(a):
+    import eating_one_amd as eo
+    from eating_one_amd import _abi, gen
+
+    R, S = gen.c3()
+    ctx = eo.Context(0)
+    stats = _abi.JoinStats()
+    ix = ctx.hidx_build(R, stats=stats)
+    build_ms = stats.ms_build
+    dS = eo.DeviceArray.from_host(ctx, S)
+    L = ctx.lib
+    n = ctypes.c_uint64()
+    o = ctx.opts(where=_abi.EO_DEVICE)
+    assert L.eo_index_join(ctx.handle, ix.handle, dS.ptr, len(S), None, 0, ctypes.byref(n), ctypes.byref(o)) == 0
+    dout = eo.DeviceArray(ctx, n.value * 8)
+    o = ctx.opts(where=_abi.EO_DEVICE, stats=stats)
+    t = []
+    for it in range(a.warmup + a.steps):
+        assert L.eo_index_join(ctx.handle, ix.handle, dS.ptr, len(S), dout.ptr, n.value, ctypes.byref(n),
+                               ctypes.byref(o)) == 0
+        if it >= a.warmup:
+            t.append(stats.ms_total)
+    ms = sum(t) / len(t)
+    alg = 8 * len(R) + 16 * len(R) + 8 * len(S) + 8 * n.value
+    res = {"metric": "hash-index join probes/s, 2^18 x 2^18 (C3)", "value": len(S) / (ms * 1e-3),
+           "unit": "probes/s", "ms_per_step": ms, "build_ms": build_ms, "n_out": n.value,
+           "pipeline": {"alg_bytes": alg, "achieved_gbs": alg / (ms + build_ms) / 1e6}}
+    ix.free()
+    ctx.close()
+    return res
+
+
+def run_c4(a):
+    import numpy as np
+
+    import eating_one_amd as eo
+    from eating_one_amd import _abi, gen
+
+    nR, nS = 1 << a.c4_log2_r, 1 << (a.c4_log2_r + 1)
+    log(f"generating C4 |R|=2^{a.c4_log2_r} lognormal(0,2) |S|=2^{a.c4_log2_r + 1}")
+    R, S = gen.c4(nR=nR, nS=nS)
+    ctx = eo.Context(0)
+    stats = _abi.JoinStats()
+    ix = ctx.sidx_build(R, learned=True, stats=stats)
+    info = ix.info()
+    dS = eo.DeviceArray.from_host(ctx, S)
+    L = ctx.lib
+    pos = eo.DeviceArray(ctx, nS * 8)
+    res = {"metric": "sorted-index probes/s, learned vs binary search (C4)", "unit": "probes/s",
+           "index": info, "n": {"R": nR, "S": nS}}
+    for method, name in ((_abi.EO_LOOKUP_BINARY, "binary"), (_abi.EO_LOOKUP_LEARNED, "learned")):
+        o = ctx.opts(where=_abi.EO_DEVICE, stats=stats)
+        t = []
+        for it in range(a.warmup + a.steps):
+            assert L.eo_index_lookup(ctx.handle, ix.handle, dS.ptr, nS, method, pos.ptr, ctypes.byref(o)) == 0
+            if it >= a.warmup:
+                t.append(stats.ms_total)
+        res[f"{name}_lookup_ms"] = float(np.median(t))
+        n = ctypes.c_uint64()
+        flags = _abi.EO_F_BINARY_SEARCH if method == _abi.EO_LOOKUP_BINARY else 0
+        o = ctx.opts(where=_abi.EO_DEVICE, flags=flags)
+        assert L.eo_index_join(ctx.handle, ix.handle, dS.ptr, nS, None, 0, ctypes.byref(n), ctypes.byref(o)) == 0
+        dout = eo.DeviceArray(ctx, max(n.value, 1) * 8)
+        o = ctx.opts(where=_abi.EO_DEVICE, flags=flags, stats=stats)
+        t = []
+        for it in range(a.warmup + a.steps):
+            assert L.eo_index_join(ctx.handle, ix.handle, dS.ptr, nS, dout.ptr, n.value, ctypes.byref(n),
+                                   ctypes.byref(o)) == 0
+            if it >= a.warmup:
+                t.append(stats.ms_total)
+        res[f"{name}_join_ms"] = float(np.median(t))
+        res["n_out"] = n.value
+        dout.close()
+    res["value"] = nS / (res["learned_lookup_ms"] * 1e-3)
+    res["speedup_lookup"] = res["binary_lookup_ms"] / res["learned_lookup_ms"]
+    res["speedup_join"] = res["binary_join_ms"] / res["learned_join_ms"]
+    ix.free()
+    ctx.close()
+    return res
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4"])
+    ap.add_argument("--log2-r", type=int, default=27)
+    ap.add_argument("--theta", type=float, default=0.0)
+    ap.add_argument("--selectivity", type=float, default=1.0)
+    ap.add_argument("--radix-bits", type=int, default=0)
+    ap.add_argument("--passes", type=int, default=0)
+    ap.add_argument("--lds", type=int, default=0)
+    ap.add_argument("--s-tile", type=int, default=0)
+    ap.add_argument("--c4-log2-r", type=int, default=26)
+    ap.add_argument("--cpu-log2-r", type=int, default=27)
+    ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): this build has no multi-GPU path; "
+              "the sharding design is described in DESIGN.md", file=sys.stderr)
+        return 2
+    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4}[a.config]
+    print(json.dumps(fn(a)), flush=True)
     return 0
 
 

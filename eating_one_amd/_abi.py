@@ -20,6 +20,13 @@ EO_HOST, EO_DEVICE = 0, 1
 EO_F_NO_SYNC_STATS, EO_F_DETERMINISTIC = 1, 2
 
 
+EO_MAX_KERNEL_STATS = 12
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", U32), ("reserved", U32)]
+
+
 class JoinStats(ctypes.Structure):
     _fields_ = [
         ("ms_total", ctypes.c_double),
@@ -32,7 +39,15 @@ class JoinStats(ctypes.Structure):
         ("passes", U32),
         ("tiles", U32),
         ("lds_tuples", U32),
+        ("n_kernels", U32),
+        ("reserved", U32),
+        ("kernels", KernelStat * EO_MAX_KERNEL_STATS),
     ]
+
+    def kernel_table(self) -> dict:
+        """{kernel family: (total ms, launches)} of the last call."""
+        return {self.kernels[i].name.decode(): (self.kernels[i].ms, self.kernels[i].launches)
+                for i in range(self.n_kernels)}
 
 
 class JoinOpts(ctypes.Structure):

@@ -70,6 +70,27 @@ eo_status check_device_errors(eo_ctx *ctx, hipStream_t stream) {
   return EO_OK;
 }
 
+void ScopedTimer::finish() {
+  if (!stats) return;
+  stats->n_kernels = 0;
+  for (int i = 0; i < ctx->timer.n; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->timer.ev[2 * i], ctx->timer.ev[2 * i + 1]) != hipSuccess) continue;
+    const char *nm = ctx->timer.name[i];
+    uint32_t k = 0;
+    while (k < stats->n_kernels && std::strncmp(stats->kernels[k].name, nm, sizeof(stats->kernels[k].name)))
+      ++k;
+    if (k == stats->n_kernels) {
+      if (k >= EO_MAX_KERNEL_STATS) continue;
+      std::memset(&stats->kernels[k], 0, sizeof(stats->kernels[k]));
+      std::strncpy(stats->kernels[k].name, nm, sizeof(stats->kernels[k].name) - 1);
+      ++stats->n_kernels;
+    }
+    stats->kernels[k].ms += ms;
+    stats->kernels[k].launches += 1;
+  }
+}
+
 namespace {
 
 bool opts_ok(const eo_join_opts *o) {
@@ -188,6 +209,7 @@ eo_status eo_ctx_create(eo_ctx **out, int device, const eo_ctx_opts *opts) {
   if (e == hipSuccess) e = hipMemset(c->d_err, 0, 256);
   if (e == hipSuccess) e = hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(uint64_t), hipHostMallocDefault);
   for (int i = 0; i < 8 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < 2 * kMaxTimedLaunches && e == hipSuccess; ++i) e = hipEventCreate(&c->timer.ev[i]);
   if (e == hipSuccess && opts && opts->arena_bytes) e = c->arena.reserve(opts->arena_bytes);
   if (e != hipSuccess) {
     eo_status st = hip_fail(e, "eo_ctx_create");
@@ -204,6 +226,8 @@ eo_status eo_ctx_destroy(eo_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->arena.release();
   for (auto &ev : c->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (auto &ev : c->timer.ev)
     if (ev) (void)hipEventDestroy(ev);
   if (c->d_err) (void)hipFree(c->d_err);
   if (c->h_pinned) (void)hipHostFree(c->h_pinned);
@@ -440,6 +464,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   }
 
   PhaseClock clk(c, s, stats != nullptr);
+  ScopedTimer tmr(c, stats);
   clk.mark();  // 0
   const eo_tuple *Rp = dR, *Sp = dS;
   if (hp.passes) {
@@ -512,6 +537,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
       stats->ms_count = 0.0;
       stats->ms_write = clk.ms(2, 3);
       stats->ms_total = clk.ms(0, 3);
+      tmr.finish();
     }
     if (st != EO_OK) return st;
     return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
@@ -580,6 +606,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     stats->ms_count = clk.ms(1, 2);
     stats->ms_write = clk.ms(2, 3);
     stats->ms_total = clk.ms(0, 3);
+    tmr.finish();
   }
   return st;
 }
@@ -617,6 +644,7 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
     EO_TRY_HIP(hipMemcpyAsync((void *)dS, S, nS * sizeof(eo_tuple), hipMemcpyHostToDevice, s), "h2d S");
   }
   PhaseClock clk(c, s, stats != nullptr);
+  ScopedTimer tmr(c, stats);
   clk.mark();
   EO_TRY_HIP(nlj_count(np, dR, dS, cnt, s), "nlj count");
   EO_TRY_HIP(scan_u64(cnt, cnt, slots, small, A + o_scr, c->d_err, s), "nlj scan");
@@ -645,6 +673,7 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
     stats->ms_count = clk.ms(0, 1);
     stats->ms_write = clk.ms(1, 2);
     stats->ms_total = clk.ms(0, 2);
+    tmr.finish();
   }
   return check_device_errors(c, s);
 }

@@ -291,6 +291,7 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   uint64_t *cnt = (uint64_t *)(A + o_cnt);
   uint64_t *small = (uint64_t *)(A + o_small);
   PhaseClock clk(c, s, stats != nullptr);
+  ScopedTimer tmr(c, stats);
   clk.mark();
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt, nullptr,
                          0, nullptr, s),
@@ -322,6 +323,7 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     stats->ms_count = clk.ms(0, 1);
     stats->ms_write = clk.ms(1, 2);
     stats->ms_total = clk.ms(0, 2);
+    tmr.finish();
   }
   return check_device_errors(c, s);
 }
@@ -349,6 +351,7 @@ eo_status eo_index_lookup(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint
   if (st != EO_OK) return st;
   uint64_t *dpos = host ? (uint64_t *)(A + o_pos) : pos;
   PhaseClock clk(c, s, o && o->stats);
+  ScopedTimer tmr(c, o ? o->stats : nullptr);
   clk.mark();
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, nullptr,
                          nullptr, 0, dpos, s),
@@ -360,6 +363,7 @@ eo_status eo_index_lookup(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint
     std::memset(o->stats, 0, sizeof(*o->stats));
     o->stats->ms_count = clk.ms(0, 1);
     o->stats->ms_total = clk.ms(0, 1);
+    tmr.finish();
   }
   return check_device_errors(c, s);
 }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "eojoin.h"
+#include "kernels.hpp"
 
 namespace eo {
 
@@ -49,6 +50,7 @@ struct eo_ctx {
   uint64_t *h_pinned = nullptr;  // 64 pinned words for small readbacks
   uint64_t hbm_budget = 0;
   hipEvent_t ev[8] = {};
+  eo::LaunchTimer timer;  // per-launch events, installed while a call collects stats
 };
 
 namespace eo {
@@ -74,6 +76,25 @@ inline hipStream_t pick_stream(eo_ctx *ctx, const eo_join_opts *o) {
   return (o && o->stream) ? (hipStream_t)o->stream : ctx->stream;
 }
 inline int where_of(const eo_join_opts *o) { return o ? o->where : EO_HOST; }
+
+// Installs the ctx's launch timer for the duration of a call when stats are
+// requested; finish() folds the launch times into stats->kernels[].
+struct ScopedTimer {
+  eo_ctx *ctx;
+  eo_join_stats *stats;
+  LaunchTimer *prev = nullptr;
+  ScopedTimer(eo_ctx *c, eo_join_stats *st) : ctx(c), stats(st) {
+    if (stats) {
+      ctx->timer.n = 0;
+      ctx->timer.overflow = false;
+      prev = set_launch_timer(&ctx->timer);
+    }
+  }
+  ~ScopedTimer() {
+    if (stats) set_launch_timer(prev);
+  }
+  void finish();  // call after the stream is synchronised
+};
 
 // Phase timing with events on the call's stream (only when stats requested).
 struct PhaseClock {

@@ -418,8 +418,8 @@ hipError_t launch_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple
                         const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, uint64_t *tile_count,
                         hipStream_t stream) {
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(k_join_count<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
-                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_count);
+  EO_TIMED("probe_count", stream, hipLaunchKernelGGL(k_join_count<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
+                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_count));
   return hipGetLastError();
 }
 
@@ -428,9 +428,9 @@ hipError_t launch_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple
                         const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
                         const uint64_t *tile_off, eo_pair *out, uint64_t out_cap, hipStream_t stream) {
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(k_join_write<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
+  EO_TIMED("probe_write", stream, hipLaunchKernelGGL(k_join_write<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
                      reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_off,
-                     reinterpret_cast<uint2 *>(out), out_cap);
+                     reinterpret_cast<uint2 *>(out), out_cap));
   return hipGetLastError();
 }
 
@@ -503,9 +503,9 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t n_tiles, const e
                       uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL(k_join_fused, dim3(grid), dim3(kFNT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
+  EO_TIMED("probe", stream, hipLaunchKernelGGL(k_join_fused, dim3(grid), dim3(kFNT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
                      reinterpret_cast<const uint2 *>(S), s_start, tile_start, parts, s_span,
-                     reinterpret_cast<uint2 *>(out), out_cap, out_cursor);
+                     reinterpret_cast<uint2 *>(out), out_cap, out_cursor));
   return hipGetLastError();
 }
 

@@ -330,12 +330,14 @@ hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint
   if (tiles == 0) return hipSuccess;
   if (tiles >= (1ull << 31)) return hipErrorInvalidValue;
   if (write)
-    hipLaunchKernelGGL((k_index_probe<KIND, true>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
-                       reinterpret_cast<const uint2 *>(S), nS, tile_cnt, reinterpret_cast<uint2 *>(out), out_cap,
-                       lb_out);
+    EO_TIMED("index_write", stream,
+             hipLaunchKernelGGL((k_index_probe<KIND, true>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
+                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, reinterpret_cast<uint2 *>(out),
+                                out_cap, lb_out));
   else
-    hipLaunchKernelGGL((k_index_probe<KIND, false>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
-                       reinterpret_cast<const uint2 *>(S), nS, tile_cnt, nullptr, 0, lb_out);
+    EO_TIMED("index_probe", stream,
+             hipLaunchKernelGGL((k_index_probe<KIND, false>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
+                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, nullptr, 0, lb_out));
   return hipGetLastError();
 }
 

@@ -11,6 +11,27 @@
 
 namespace eo {
 
+// ---- per-launch timing --------------------------------------------------------
+// When a timer is installed on the calling host thread, timed launch sites
+// record a HIP event pair around the kernel on its own stream; the caller
+// reads the elapsed times after synchronising.  No timer: no events.
+constexpr int kMaxTimedLaunches = 64;
+struct LaunchTimer {
+  hipEvent_t ev[2 * kMaxTimedLaunches] = {};
+  const char *name[kMaxTimedLaunches] = {};
+  int n = 0;
+  bool overflow = false;
+};
+LaunchTimer *set_launch_timer(LaunchTimer *t);  // returns the previous one
+void timer_begin(const char *name, hipStream_t s);
+void timer_end(hipStream_t s);
+#define EO_TIMED(name, stream, ...)  \
+  do {                               \
+    ::eo::timer_begin(name, stream); \
+    __VA_ARGS__;                     \
+    ::eo::timer_end(stream);         \
+  } while (0)
+
 // ---- scan.hip ---------------------------------------------------------------
 // Exclusive scan with decoupled look-back.  scratch: scan_scratch_bytes(n).
 // total_dev (optional, device) receives the sum.  out may equal in.

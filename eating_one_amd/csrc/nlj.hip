@@ -143,8 +143,8 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
   const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_nlj<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
-                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0);
+  EO_TIMED("nlj_count", stream, hipLaunchKernelGGL(k_nlj<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
+                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0));
   return hipGetLastError();
 }
 
@@ -153,9 +153,9 @@ hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, con
   const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_nlj<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
+  EO_TIMED("nlj_write", stream, hipLaunchKernelGGL(k_nlj<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
                      p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, nullptr, wave_off,
-                     reinterpret_cast<uint2 *>(out), out_cap);
+                     reinterpret_cast<uint2 *>(out), out_cap));
   return hipGetLastError();
 }
 

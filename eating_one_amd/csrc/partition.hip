@@ -399,11 +399,11 @@ hipError_t launch_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
   StripeGeo geo{p.n, p.stripe_len, p.stripes, p.hist_split, nullptr, nullptr, 0};
   const uint32_t hist_grid = p.stripes * p.hist_split;
   if (aligned)
-    hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, geo, p.shift,
-                       p.xor_mask, counts);
+    EO_TIMED("part_hist", stream, hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, geo, p.shift,
+                       p.xor_mask, counts));
   else
-    hipLaunchKernelGGL((k_part_hist<BITS, false>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, geo, p.shift,
-                       p.xor_mask, counts);
+    EO_TIMED("part_hist", stream, hipLaunchKernelGGL((k_part_hist<BITS, false>), dim3(hist_grid), dim3(kPartNT), 0, stream, vin, geo, p.shift,
+                       p.xor_mask, counts));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint64_t m = ((uint64_t)p.stripes * p.hist_split) << BITS;
@@ -411,11 +411,11 @@ hipError_t launch_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
   if (e != hipSuccess) return e;
   if (p.fast) {
     if ((uintptr_t)in & 15) return hipErrorInvalidValue;  // stripe 0 starts at the array start
-    hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(p.stripes), dim3(kFastNT), 0, stream, vin, geo, p.shift,
-                       p.xor_mask, counts, vout, err);
+    EO_TIMED("part_scatter", stream, hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(p.stripes), dim3(kFastNT), 0, stream, vin, geo, p.shift,
+                       p.xor_mask, counts, vout, err));
   } else {
-    hipLaunchKernelGGL(k_part_scatter<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
-                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout, err);
+    EO_TIMED("part_scatter_stable", stream, hipLaunchKernelGGL(k_part_scatter<BITS>, dim3(p.stripes), dim3(kPartNT), 0, stream, vin, p.n, p.shift,
+                       p.xor_mask, p.stripes, p.stripe_len, p.hist_split, counts, vout, err));
   }
   return hipGetLastError();
 }
@@ -463,13 +463,13 @@ hipError_t launch_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple *o
   const StripeGeo geo{p.n, p.stripe_len, 0, 1, seg_start, seg_base, p.nseg};
   const uint2 *vin = reinterpret_cast<const uint2 *>(in);
   const uint32_t grid = (p.max_stripes + kXcds - 1) / kXcds * kXcds;
-  hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(grid), dim3(kPartNT), 0, stream, vin, geo, p.shift, 0u,
-                     counts);
+  EO_TIMED("seg_hist", stream, hipLaunchKernelGGL((k_part_hist<BITS, true>), dim3(grid), dim3(kPartNT), 0, stream, vin, geo, p.shift, 0u,
+                     counts));
   e = hipGetLastError();
   if (e == hipSuccess) e = scan_u32(counts, counts, m, nullptr, scan_scratch, err, stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(grid), dim3(kFastNT), 0, stream, vin, geo, p.shift, 0u, counts,
-                     reinterpret_cast<uint2 *>(out), err);
+  EO_TIMED("seg_scatter", stream, hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(grid), dim3(kFastNT), 0, stream, vin, geo, p.shift, 0u, counts,
+                     reinterpret_cast<uint2 *>(out), err));
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint64_t tot = (uint64_t)p.nseg * F + 1;

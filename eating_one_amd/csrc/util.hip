@@ -24,6 +24,30 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
 
 }  // namespace
 
+static thread_local LaunchTimer *g_timer = nullptr;
+
+LaunchTimer *set_launch_timer(LaunchTimer *t) {
+  LaunchTimer *prev = g_timer;
+  g_timer = t;
+  return prev;
+}
+
+void timer_begin(const char *name, hipStream_t s) {
+  if (!g_timer) return;
+  if (g_timer->n >= kMaxTimedLaunches) {
+    g_timer->overflow = true;
+    return;
+  }
+  g_timer->name[g_timer->n] = name;
+  (void)hipEventRecord(g_timer->ev[2 * g_timer->n], s);
+}
+
+void timer_end(hipStream_t s) {
+  if (!g_timer || g_timer->n >= kMaxTimedLaunches) return;
+  (void)hipEventRecord(g_timer->ev[2 * g_timer->n + 1], s);
+  ++g_timer->n;
+}
+
 hipError_t copy_bytes(const void *src, void *dst, uint64_t bytes, int cus, hipStream_t stream) {
   const uint64_t n = bytes / 16;
   const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)cus * 8));

@@ -107,17 +107,21 @@ def c3(seed: int = 0xE01C3, nR: int = 1 << 18, nS: int = 1 << 18, key_range: int
 
 
 def c4(seed: int = 0xE01C4, nR: int = 1 << 26, nS: int = 1 << 27, sigma: float = 2.0, hit_frac: float = 0.8):
+    """Sorted-index workload: nR unique lognormal(0, sigma) keys scaled into [0, 2^31).
+
+    Draws are scaled so that the 1 - 1e-6 quantile lands near 2^31, sorted, and
+    made unique in one pass by k_i = cummax(d_i - i) + i: a key that collides
+    moves up to the next free integer (lognormal(0, 2) packs much of its mass
+    into a few hundred thousand small integers, which this fills densely).
+    S: hit_frac of the probes are keys sampled from R, the rest uniform keys in
+    [0, 2^31) that are not in R; shuffled."""
     rng = np.random.default_rng(seed)
-    keys = np.empty(0, dtype=np.int64)
-    # lognormal draws scaled so that the 1 - 1e-6 quantile lands near 2^31
     scale = INT31 / np.exp(sigma * 4.753424)
-    while len(keys) < nR:
-        draw = np.floor(rng.lognormal(0.0, sigma, size=nR) * scale)
-        draw = draw[(draw >= 0) & (draw <= INT31)].astype(np.int64)
-        keys = np.unique(np.concatenate([keys, draw]))
-        if len(keys) > nR:
-            keys = rng.choice(keys, size=nR, replace=False)
-            keys.sort()
+    draws = np.floor(rng.lognormal(0.0, sigma, size=nR) * scale)
+    draws = np.clip(draws, 0, INT31 - nR).astype(np.int64)
+    draws.sort()
+    idx = np.arange(nR, dtype=np.int64)
+    keys = np.maximum.accumulate(draws - idx) + idx
     rkeys = keys.astype(np.int32)
     R = _tuples(rng.permutation(rkeys))
     nhit = int(round(nS * hit_frac))
@@ -125,9 +129,8 @@ def c4(seed: int = 0xE01C4, nR: int = 1 << 26, nS: int = 1 << 27, sigma: float =
     skeys[:nhit] = rkeys[rng.integers(0, nR, size=nhit)]
     miss = np.empty(0, dtype=np.int64)
     while len(miss) < nS - nhit:
-        cand = rng.integers(0, INT31, size=(nS - nhit) * 2, dtype=np.int64)
-        pos = np.searchsorted(keys, cand)
-        pos = np.minimum(pos, len(keys) - 1)
+        cand = rng.integers(0, INT31, size=int((nS - nhit) * 1.1) + 16, dtype=np.int64)
+        pos = np.minimum(np.searchsorted(keys, cand), nR - 1)
         miss = np.concatenate([miss, cand[keys[pos] != cand]])
     skeys[nhit:] = miss[: nS - nhit].astype(np.int32)
     skeys = rng.permutation(skeys)

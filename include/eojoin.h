@@ -70,6 +70,17 @@ typedef struct eo_ctx_opts {
   uint64_t hbm_budget;   /* admission budget for batch/serve paths (0 = 200 GB) */
 } eo_ctx_opts;
 
+#define EO_MAX_KERNEL_STATS 12
+
+/* Device time of one kernel family during a call (HIP events around each
+ * launch on the call's stream), summed over its launches. */
+typedef struct eo_kernel_stat {
+  char name[32];
+  double ms;
+  uint32_t launches;
+  uint32_t reserved;
+} eo_kernel_stat;
+
 /* Per-phase device times of the last call (hipEvent based), in ms. */
 typedef struct eo_join_stats {
   double ms_total;
@@ -82,6 +93,9 @@ typedef struct eo_join_stats {
   uint32_t passes;       /* partition passes used */
   uint32_t tiles;        /* join work tiles (partition x R-chunk x S-range) */
   uint32_t lds_tuples;   /* LDS table capacity used */
+  uint32_t n_kernels;    /* valid entries of kernels[] */
+  uint32_t reserved;
+  eo_kernel_stat kernels[EO_MAX_KERNEL_STATS];
 } eo_join_stats;
 
 typedef struct eo_join_opts {

@@ -73,3 +73,26 @@ def test_invalid_arguments_rejected_before_device_use():
     n = ctypes.c_uint64()
     assert L.eo_hash_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL
     assert L.eo_nlj_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Compile the header with the host C compiler and compare every struct size with ctypes."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no host C compiler")
+    src = tmp_path / "sizes.c"
+    src.write_text('''#include <stdio.h>
+#include "eojoin.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(eo_join_opts), sizeof(eo_join_stats), sizeof(eo_kernel_stat),
+         sizeof(eo_ctx_opts), sizeof(eo_device_info), sizeof(eo_index_info), sizeof(eo_tuple));
+  return 0;
+}
+''')
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(t) for t in (_abi.JoinOpts, _abi.JoinStats, _abi.KernelStat, _abi.CtxOpts,
+                                       _abi.DeviceInfo, _abi.IndexInfo)] + [8]
+    assert got == want
