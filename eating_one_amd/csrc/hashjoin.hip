@@ -20,6 +20,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "tile_kernels.hpp"
 
 namespace eo {
 namespace {
@@ -218,18 +219,7 @@ __global__ __launch_bounds__(C::NT) void k_join_write(const uint2 *__restrict__ 
 // store instruction writes one contiguous run), and one 64-bit atomic per
 // batch reserves the output range.  Pairs at positions >= out_cap are counted
 // but not written.
-constexpr int kFNT = 1024;
-constexpr int kFCap = 17408;  // a 16K-tuple partition plus binomial spread
-constexpr int kFLog2NB = 13;
-constexpr int kFNB = 1 << kFLog2NB;
-constexpr int kFSPT = 8;
-constexpr int kFBatch = kFNT * kFSPT;
-constexpr int kFRPT = (kFCap + kFNT - 1) / kFNT;
-constexpr int kFBPT = kFNB / kFNT;
-
-struct SpanRef {
-  uint32_t r_beg, r_n, s_beg, s_n;
-};
+using namespace tile;
 
 __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_start, uint32_t P,
                                              const uint32_t *r_start, const uint32_t *s_start, uint32_t s_span,
@@ -258,133 +248,11 @@ __global__ __launch_bounds__(kFNT) void k_join_fused(const uint2 *__restrict__ R
                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
                                                      uint64_t *out_cursor) {
-  constexpr int NW = kFNT / kWave;
-  __shared__ uint32_t keys[kFCap];
-  __shared__ uint32_t pays[kFCap];
-  __shared__ uint16_t off[kFNB + 2];
-  __shared__ uint32_t scan_tmp[NW + 1];
-  __shared__ uint32_t wtot[kFSPT * NW];
-  __shared__ uint64_t base_sh;
+  __shared__ tile::ProbeShared sh;
   const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
   SpanRef ref;
   if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, &ref)) return;
-
-  auto load_batch = [&](uint32_t b0, uint2 *dst) {
-#pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
-      const uint32_t i = b0 + u * kFNT + threadIdx.x;
-      dst[u] = i < ref.s_n ? S[ref.s_beg + i] : make_uint2(0u, 0u);
-    }
-  };
-  uint2 cur[kFSPT];
-  load_batch(0, cur);
-
-  // ---- build: rank by LDS atomics on counters aliased onto keys[] ----
-  uint32_t *cnt = keys;  // kFNB u32 counters, dead once the offsets exist
-  for (int h = threadIdx.x; h < kFNB; h += kFNT) cnt[h] = 0;
-  __syncthreads();
-  uint2 r[kFRPT];
-  uint32_t hh[kFRPT], rk[kFRPT];
-#pragma unroll
-  for (int k = 0; k < kFRPT; ++k) {
-    const uint32_t i = k * kFNT + threadIdx.x;
-    if (i < ref.r_n) r[k] = R[ref.r_beg + i];
-  }
-#pragma unroll
-  for (int k = 0; k < kFRPT; ++k) {
-    const uint32_t i = k * kFNT + threadIdx.x;
-    if (i < ref.r_n) {
-      hh[k] = bucket_hash((int32_t)r[k].x, kFLog2NB);
-      rk[k] = atomicAdd(&cnt[hh[k]], 1u);
-    }
-  }
-  __syncthreads();
-  {
-    uint32_t loc[kFBPT];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kFBPT; ++k) {
-      loc[k] = cnt[threadIdx.x * kFBPT + k];
-      sum += loc[k];
-    }
-    uint32_t tot;
-    uint32_t ex = block_excl_scan<kFNT, uint32_t>(sum, scan_tmp, &tot);
-#pragma unroll
-    for (int k = 0; k < kFBPT; ++k) {
-      off[threadIdx.x * kFBPT + k] = (uint16_t)ex;
-      ex += loc[k];
-    }
-    if (threadIdx.x == 0) off[kFNB] = (uint16_t)ref.r_n;  // r_n <= 16384 fits u16
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kFRPT; ++k) {
-    const uint32_t i = k * kFNT + threadIdx.x;
-    if (i < ref.r_n) {
-      const uint32_t pos = off[hh[k]] + rk[k];
-      keys[pos] = r[k].x;
-      pays[pos] = r[k].y;
-    }
-  }
-  __syncthreads();
-
-  // ---- probe the S span batch by batch ----
-  const int w = threadIdx.x / kWave, lane = lane_id();
-  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += kFBatch) {
-    uint2 nxt[kFSPT];
-    if (b0 + kFBatch < ref.s_n) load_batch(b0 + kFBatch, nxt);
-    uint32_t m[kFSPT], fp[kFSPT];
-#pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
-      m[u] = 0;
-      fp[u] = 0;
-      if (b0 + u * kFNT + threadIdx.x < ref.s_n) {
-        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
-        const uint32_t e1 = off[h + 1];
-        for (uint32_t e = off[h]; e < e1; ++e) {
-          if (keys[e] == cur[u].x) {
-            if (m[u] == 0) fp[u] = pays[e];
-            ++m[u];
-          }
-        }
-      }
-    }
-    uint32_t wex[kFSPT];
-#pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
-      const uint32_t inc = wave_incl_scan(m[u]);
-      wex[u] = inc - m[u];
-      if (lane == kWave - 1) wtot[u * NW + w] = inc;
-    }
-    __syncthreads();
-    uint32_t batch_total;
-    const uint32_t v = threadIdx.x < kFSPT * NW ? wtot[threadIdx.x] : 0u;
-    const uint32_t vex = block_excl_scan<kFNT, uint32_t>(v, scan_tmp, &batch_total);
-    if (threadIdx.x < kFSPT * NW) wtot[threadIdx.x] = vex;
-    if (threadIdx.x == 0) base_sh = batch_total ? atomic_add_agent(out_cursor, (uint64_t)batch_total) : 0ull;
-    __syncthreads();
-    const uint64_t base = base_sh;
-#pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
-      if (m[u] == 0) continue;
-      uint64_t pos = base + wtot[u * NW + w] + wex[u];
-      if (m[u] == 1) {
-        if (pos < out_cap) out[pos] = make_uint2(fp[u], cur[u].y);
-      } else {
-        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
-        const uint32_t e1 = off[h + 1];
-        for (uint32_t e = off[h]; e < e1; ++e) {
-          if (keys[e] == cur[u].x) {
-            if (pos < out_cap) out[pos] = make_uint2(pays[e], cur[u].y);
-            ++pos;
-          }
-        }
-      }
-    }
-    __syncthreads();  // wtot / base_sh reuse
-#pragma unroll
-    for (int u = 0; u < kFSPT; ++u) cur[u] = nxt[u];
-  }
+  tile::probe_tile(sh, R, S, ref, out, out_cap, out_cursor);
 }
 
 __global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,

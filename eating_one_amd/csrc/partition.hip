@@ -19,9 +19,14 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "tile_kernels.hpp"
 
 namespace eo {
 namespace {
+
+using tile::digit_of;
+using tile::kFastNT;
+using tile::kFastTile;
 
 constexpr int kPartNT = 512;
 
@@ -36,9 +41,6 @@ struct PartCfg {
   static constexpr int DPT = (F + NT - 1) / NT;    // digits per thread
 };
 
-__device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shift, uint32_t mask) {
-  return ((key ^ xm) >> shift) & mask;
-}
 
 // Where a histogram / scatter workgroup's stripe lies and where its counts
 // go.  Uniform (seg_start == nullptr): stripe g = [g*len, (g+1)*len) of the
@@ -124,35 +126,7 @@ __global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__
   if (!stripe_of<C::F>(geo, g, k, &sr)) return;
   for (int d = threadIdx.x; d < C::F; d += C::NT) hist[d] = 0;
   __syncthreads();
-  const uint64_t beg = sr.beg, end = sr.end;
-  if (beg < end) {
-    // 16-byte loads from the first 16-byte aligned tuple on
-    const uint64_t a0 = (V16 && ((uintptr_t)(in + beg) & 15)) ? beg + 1 : beg;
-    if (V16 && a0 < end) {
-      if (a0 != beg && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[beg].x, xm, shift, C::F - 1)], 1u);
-      const uint4 *in4 = reinterpret_cast<const uint4 *>(in + a0);
-      const uint64_t npairs = (end - a0) / 2;
-      uint64_t i = threadIdx.x;
-      for (; i + 3 * C::NT < npairs; i += 4 * C::NT) {
-        uint4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = in4[i + u * C::NT];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          atomicAdd(&hist[digit_of(q[u].x, xm, shift, C::F - 1)], 1u);
-          atomicAdd(&hist[digit_of(q[u].z, xm, shift, C::F - 1)], 1u);
-        }
-      }
-      for (; i < npairs; i += C::NT) {
-        const uint4 q = in4[i];
-        atomicAdd(&hist[digit_of(q.x, xm, shift, C::F - 1)], 1u);
-        atomicAdd(&hist[digit_of(q.z, xm, shift, C::F - 1)], 1u);
-      }
-      if (((end - a0) & 1) && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[end - 1].x, xm, shift, C::F - 1)], 1u);
-    } else {
-      for (uint64_t i = beg + threadIdx.x; i < end; i += C::NT) atomicAdd(&hist[digit_of(in[i].x, xm, shift, C::F - 1)], 1u);
-    }
-  }
+  tile::hist_stripe<BITS, C::NT, V16>(hist, in, sr.beg, sr.end, shift, xm);
   __syncthreads();
   for (int d = threadIdx.x; d < C::F; d += C::NT) counts[sr.cidx0 + (uint64_t)d * sr.cstride] = hist[d];
 }
@@ -266,112 +240,19 @@ __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restric
 // issued before the flush so that HBM reads overlap the writes.  Within a
 // partition the tuple order depends on atomic order; the multiset per
 // partition equals the stable scatter's.
-constexpr int kFastNT = 1024;
-constexpr int kFastPairs = 8;                         // uint4 loads (2 tuples) per thread per tile
-constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples
-
 template <int BITS>
 __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__restrict__ in, StripeGeo geo, int shift,
                                                                uint32_t xm, const uint32_t *__restrict__ offsets,
                                                                uint2 *__restrict__ out, uint32_t *err) {
-  constexpr int F = 1 << BITS;
-  constexpr int DPT = (F + kFastNT - 1) / kFastNT;
-  __shared__ uint2 stage[kFastTile];
-  __shared__ uint32_t cnt[F];
-  __shared__ uint32_t tbase[F + 1];
-  __shared__ uint32_t cursor[F];
-  __shared__ uint32_t scan_tmp[kFastNT / kWave + 1];
-
+  __shared__ tile::ScatterShared<BITS> sh;
   const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
   StripeRef sr;
-  if (!stripe_of<F>(geo, g, 0, &sr, true)) return;
-  for (int d = threadIdx.x; d < F; d += kFastNT) {
-    cursor[d] = offsets[sr.cidx0 + (uint64_t)d * sr.cstride];
-    cnt[d] = 0;
+  if (!stripe_of<(1 << BITS)>(geo, g, 0, &sr, true)) return;
+  for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
+    sh.cursor[d] = offsets[sr.cidx0 + (uint64_t)d * sr.cstride];
+    sh.cnt[d] = 0;
   }
-  // a stripe whose first tuple is only 8-byte aligned is read from one tuple
-  // earlier (16-byte pairs); that tuple is not part of the stripe: skip it
-  const uint64_t skip = ((uintptr_t)(in + sr.beg) & 15) ? 1 : 0;
-  const uint64_t beg = sr.beg - skip;
-  const uint64_t end = sr.end;
-  if (sr.beg >= end) return;
-  const uint4 *in4 = reinterpret_cast<const uint4 *>(in + beg);
-  const uint64_t nfull = (end - beg) / 2;              // complete 16-byte pairs
-  const uint64_t npairs = (end - beg + 1) / 2;         // incl. a trailing single tuple
-
-  uint4 cur[kFastPairs];
-  auto load_tile = [&](uint64_t t0pair, uint4 *dst) {
-#pragma unroll
-    for (int j = 0; j < kFastPairs; ++j) {
-      const uint64_t pi = t0pair + (uint64_t)j * kFastNT + threadIdx.x;
-      if (pi < nfull) {
-        dst[j] = in4[pi];
-      } else if (pi < npairs) {
-        const uint2 last = in[beg + 2 * pi];
-        dst[j] = make_uint4(last.x, last.y, 0u, 0u);
-      } else {
-        dst[j] = make_uint4(0u, 0u, 0u, 0u);
-      }
-    }
-  };
-  load_tile(0, cur);
-  __syncthreads();
-
-  for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
-    const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples read in this tile
-    const uint32_t tskip = t0 == 0 ? (uint32_t)skip : 0u;                            // lead-in tuple, not ours
-    uint32_t rk[2 * kFastPairs];
-#pragma unroll
-    for (int j = 0; j < kFastPairs; ++j) {
-      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
-      if (e >= tskip && e < tn) rk[2 * j] = atomicAdd(&cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
-      if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
-    }
-    __syncthreads();
-    {
-      uint32_t loc[DPT];
-      uint32_t sum = 0;
-#pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const int d = threadIdx.x * DPT + k;
-        loc[k] = d < F ? cnt[d] : 0u;
-        sum += loc[k];
-      }
-      uint32_t tot;
-      uint32_t ex = block_excl_scan<kFastNT, uint32_t>(sum, scan_tmp, &tot);
-#pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const int d = threadIdx.x * DPT + k;
-        if (d < F) tbase[d] = ex;
-        ex += loc[k];
-      }
-      if (threadIdx.x == 0) tbase[F] = tn - tskip;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kFastPairs; ++j) {
-      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
-      if (e >= tskip && e < tn)
-        stage[tbase[digit_of(cur[j].x, xm, shift, F - 1)] + rk[2 * j]] = make_uint2(cur[j].x, cur[j].y);
-      if (e + 1 < tn)
-        stage[tbase[digit_of(cur[j].z, xm, shift, F - 1)] + rk[2 * j + 1]] = make_uint2(cur[j].z, cur[j].w);
-    }
-    __syncthreads();
-    load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
-    for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
-      const uint2 t = stage[i];
-      const uint32_t d = digit_of(t.x, xm, shift, F - 1);
-      const uint64_t pos = (uint64_t)cursor[d] + (i - tbase[d]);
-      if (pos < geo.n) out[pos] = t;
-      else atomic_or_agent(err, kErrOverflow);  // inconsistent counts: report, never write outside
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < F; d += kFastNT) {
-      cursor[d] += tbase[d + 1] - tbase[d];
-      cnt[d] = 0;
-    }
-    __syncthreads();
-  }
+  tile::scatter_stripe<BITS>(sh, in, sr.beg, sr.end, geo.n, shift, xm, out, err);
 }
 
 __global__ void k_part_starts(const uint32_t *offsets, uint32_t G, uint32_t K, uint32_t F, uint64_t n,

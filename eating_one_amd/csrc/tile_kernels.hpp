@@ -1,0 +1,318 @@
+// Device bodies shared by the single-join kernels and the batched persistent
+// kernels: the fused probe of one (R chunk, S span) tile, the fast radix
+// scatter of one stripe and the LDS histogram of one stripe.  Each works on a
+// caller-declared __shared__ struct so that a kernel can run several tiles or
+// stripes in a loop.
+#pragma once
+
+#include "device.hpp"
+
+namespace eo {
+namespace tile {
+
+// ---- fused probe tile ----------------------------------------------------------
+constexpr int kFNT = 1024;
+constexpr int kFCap = 17408;  // a 16K-tuple partition plus binomial spread
+constexpr int kFLog2NB = 13;
+constexpr int kFNB = 1 << kFLog2NB;
+constexpr int kFSPT = 8;
+constexpr int kFBatch = kFNT * kFSPT;
+constexpr int kFRPT = (kFCap + kFNT - 1) / kFNT;
+constexpr int kFBPT = kFNB / kFNT;
+
+struct SpanRef {
+  uint32_t r_beg, r_n, s_beg, s_n;
+};
+
+struct ProbeShared {
+  uint32_t keys[kFCap];
+  uint32_t pays[kFCap];
+  uint16_t off[kFNB + 2];
+  uint32_t scan_tmp[kFNT / kWave + 1];
+  uint32_t wtot[kFSPT * (kFNT / kWave)];
+  uint64_t base_sh;
+};
+
+// Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= kFCap) and probes the S
+// span S[ref.s_beg, +s_n) against it, writing (R payload, S payload) pairs at
+// positions reserved from *cursor; positions >= out_cap are counted only.
+// All threads of the workgroup call it; it leaves LDS free for reuse.
+__device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restrict__ R, const uint2 *__restrict__ S,
+                                           const SpanRef &ref, uint2 *__restrict__ out, uint64_t out_cap,
+                                           uint64_t *cursor) {
+  constexpr int NW = kFNT / kWave;
+  auto load_batch = [&](uint32_t b0, uint2 *dst) {
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      const uint32_t i = b0 + u * kFNT + threadIdx.x;
+      dst[u] = i < ref.s_n ? S[ref.s_beg + i] : make_uint2(0u, 0u);
+    }
+  };
+  uint2 cur[kFSPT];
+  load_batch(0, cur);
+
+  // ---- build: rank by LDS atomics on counters aliased onto sh.keys[] ----
+  uint32_t *cnt = sh.keys;  // kFNB u32 counters, dead once the offsets exist
+  for (int h = threadIdx.x; h < kFNB; h += kFNT) cnt[h] = 0;
+  __syncthreads();
+  uint2 r[kFRPT];
+  uint32_t hh[kFRPT], rk[kFRPT];
+#pragma unroll
+  for (int k = 0; k < kFRPT; ++k) {
+    const uint32_t i = k * kFNT + threadIdx.x;
+    if (i < ref.r_n) r[k] = R[ref.r_beg + i];
+  }
+#pragma unroll
+  for (int k = 0; k < kFRPT; ++k) {
+    const uint32_t i = k * kFNT + threadIdx.x;
+    if (i < ref.r_n) {
+      hh[k] = bucket_hash((int32_t)r[k].x, kFLog2NB);
+      rk[k] = atomicAdd(&cnt[hh[k]], 1u);
+    }
+  }
+  __syncthreads();
+  {
+    uint32_t loc[kFBPT];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kFBPT; ++k) {
+      loc[k] = cnt[threadIdx.x * kFBPT + k];
+      sum += loc[k];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_scan<kFNT, uint32_t>(sum, sh.scan_tmp, &tot);
+#pragma unroll
+    for (int k = 0; k < kFBPT; ++k) {
+      sh.off[threadIdx.x * kFBPT + k] = (uint16_t)ex;
+      ex += loc[k];
+    }
+    if (threadIdx.x == 0) sh.off[kFNB] = (uint16_t)ref.r_n;  // r_n <= 16384 fits u16
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kFRPT; ++k) {
+    const uint32_t i = k * kFNT + threadIdx.x;
+    if (i < ref.r_n) {
+      const uint32_t pos = sh.off[hh[k]] + rk[k];
+      sh.keys[pos] = r[k].x;
+      sh.pays[pos] = r[k].y;
+    }
+  }
+  __syncthreads();
+
+  // ---- probe the S span batch by batch ----
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += kFBatch) {
+    uint2 nxt[kFSPT];
+    if (b0 + kFBatch < ref.s_n) load_batch(b0 + kFBatch, nxt);
+    uint32_t m[kFSPT], fp[kFSPT];
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      m[u] = 0;
+      fp[u] = 0;
+      if (b0 + u * kFNT + threadIdx.x < ref.s_n) {
+        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
+        const uint32_t e1 = sh.off[h + 1];
+        for (uint32_t e = sh.off[h]; e < e1; ++e) {
+          if (sh.keys[e] == cur[u].x) {
+            if (m[u] == 0) fp[u] = sh.pays[e];
+            ++m[u];
+          }
+        }
+      }
+    }
+    uint32_t wex[kFSPT];
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      const uint32_t inc = wave_incl_scan(m[u]);
+      wex[u] = inc - m[u];
+      if (lane == kWave - 1) sh.wtot[u * NW + w] = inc;
+    }
+    __syncthreads();
+    uint32_t batch_total;
+    const uint32_t v = threadIdx.x < kFSPT * NW ? sh.wtot[threadIdx.x] : 0u;
+    const uint32_t vex = block_excl_scan<kFNT, uint32_t>(v, sh.scan_tmp, &batch_total);
+    if (threadIdx.x < kFSPT * NW) sh.wtot[threadIdx.x] = vex;
+    if (threadIdx.x == 0) sh.base_sh = batch_total ? atomic_add_agent(cursor, (uint64_t)batch_total) : 0ull;
+    __syncthreads();
+    const uint64_t base = sh.base_sh;
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) {
+      if (m[u] == 0) continue;
+      uint64_t pos = base + sh.wtot[u * NW + w] + wex[u];
+      if (m[u] == 1) {
+        if (pos < out_cap) out[pos] = make_uint2(fp[u], cur[u].y);
+      } else {
+        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
+        const uint32_t e1 = sh.off[h + 1];
+        for (uint32_t e = sh.off[h]; e < e1; ++e) {
+          if (sh.keys[e] == cur[u].x) {
+            if (pos < out_cap) out[pos] = make_uint2(sh.pays[e], cur[u].y);
+            ++pos;
+          }
+        }
+      }
+    }
+    __syncthreads();  // sh.wtot / sh.base_sh reuse
+#pragma unroll
+    for (int u = 0; u < kFSPT; ++u) cur[u] = nxt[u];
+  }
+  __syncthreads();
+}
+
+// ---- fast radix scatter of one stripe ---------------------------------------------
+// Unstable scatter: a workgroup of 1024 threads stages 16K tuples per tile;
+// ranks come from LDS atomics (the histogram and the within-tile ranks in one
+// pass), the tile is placed digit-sorted in LDS and flushed in digit runs; the
+// next tile's loads are issued before the flush so reads overlap writes.
+constexpr int kFastNT = 1024;
+constexpr int kFastPairs = 8;                         // uint4 loads (2 tuples) per thread per tile
+constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples
+
+__device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shift, uint32_t mask) {
+  return ((key ^ xm) >> shift) & mask;
+}
+
+template <int BITS>
+struct ScatterShared {
+  uint2 stage[kFastTile];
+  uint32_t cnt[1 << BITS];
+  uint32_t tbase[(1 << BITS) + 1];
+  uint32_t cursor[1 << BITS];
+  uint32_t scan_tmp[kFastNT / kWave + 1];
+};
+
+// Scatters in[sbeg, send) (in + sbeg 16-byte aligned, or sbeg > 0) by digit
+// into out, digit d starting at sh.cursor[d] (set by the caller, followed by a
+// barrier).  Positions >= nbound are never written (kErrOverflow is raised).
+template <int BITS>
+__device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const uint2 *__restrict__ in, uint64_t sbeg,
+                                               uint64_t send, uint64_t nbound, int shift, uint32_t xm,
+                                               uint2 *__restrict__ out, uint32_t *err) {
+  constexpr int F = 1 << BITS;
+  constexpr int DPT = (F + kFastNT - 1) / kFastNT;
+  // a stripe whose first tuple is only 8-byte aligned is read from one tuple
+  // earlier (16-byte pairs); that tuple is not part of the stripe: skip it
+  const uint64_t skip = ((uintptr_t)(in + sbeg) & 15) ? 1 : 0;
+  const uint64_t beg = sbeg - skip;
+  const uint64_t end = send;
+  if (sbeg >= end) return;
+  const uint4 *in4 = reinterpret_cast<const uint4 *>(in + beg);
+  const uint64_t nfull = (end - beg) / 2;              // complete 16-byte pairs
+  const uint64_t npairs = (end - beg + 1) / 2;         // incl. a trailing single tuple
+
+  uint4 cur[kFastPairs];
+  auto load_tile = [&](uint64_t t0pair, uint4 *dst) {
+#pragma unroll
+    for (int j = 0; j < kFastPairs; ++j) {
+      const uint64_t pi = t0pair + (uint64_t)j * kFastNT + threadIdx.x;
+      if (pi < nfull) {
+        dst[j] = in4[pi];
+      } else if (pi < npairs) {
+        const uint2 last = in[beg + 2 * pi];
+        dst[j] = make_uint4(last.x, last.y, 0u, 0u);
+      } else {
+        dst[j] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+  };
+  load_tile(0, cur);
+  __syncthreads();
+
+  for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
+    const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples read in this tile
+    const uint32_t tskip = t0 == 0 ? (uint32_t)skip : 0u;                            // lead-in tuple, not ours
+    uint32_t rk[2 * kFastPairs];
+#pragma unroll
+    for (int j = 0; j < kFastPairs; ++j) {
+      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
+      if (e >= tskip && e < tn) rk[2 * j] = atomicAdd(&sh.cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
+      if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&sh.cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
+    }
+    __syncthreads();
+    {
+      uint32_t loc[DPT];
+      uint32_t sum = 0;
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int d = threadIdx.x * DPT + k;
+        loc[k] = d < F ? sh.cnt[d] : 0u;
+        sum += loc[k];
+      }
+      uint32_t tot;
+      uint32_t ex = block_excl_scan<kFastNT, uint32_t>(sum, sh.scan_tmp, &tot);
+#pragma unroll
+      for (int k = 0; k < DPT; ++k) {
+        const int d = threadIdx.x * DPT + k;
+        if (d < F) sh.tbase[d] = ex;
+        ex += loc[k];
+      }
+      if (threadIdx.x == 0) sh.tbase[F] = tn - tskip;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kFastPairs; ++j) {
+      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
+      if (e >= tskip && e < tn)
+        sh.stage[sh.tbase[digit_of(cur[j].x, xm, shift, F - 1)] + rk[2 * j]] = make_uint2(cur[j].x, cur[j].y);
+      if (e + 1 < tn)
+        sh.stage[sh.tbase[digit_of(cur[j].z, xm, shift, F - 1)] + rk[2 * j + 1]] = make_uint2(cur[j].z, cur[j].w);
+    }
+    __syncthreads();
+    load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
+    for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
+      const uint2 t = sh.stage[i];
+      const uint32_t d = digit_of(t.x, xm, shift, F - 1);
+      const uint64_t pos = (uint64_t)sh.cursor[d] + (i - sh.tbase[d]);
+      if (pos < nbound) out[pos] = t;
+      else atomic_or_agent(err, kErrOverflow);  // inconsistent counts: report, never write outside
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < F; d += kFastNT) {
+      sh.cursor[d] += sh.tbase[d + 1] - sh.tbase[d];
+      sh.cnt[d] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- LDS histogram of one stripe --------------------------------------------------
+// Adds the digit counts of in[beg, end) to the LDS array hist[2^BITS] with LDS
+// atomics (the caller zeroes it).  V16: 16-byte loads from the first 16-byte
+// aligned tuple on.
+template <int BITS, int NT, bool V16>
+__device__ __forceinline__ void hist_stripe(uint32_t *hist, const uint2 *__restrict__ in, uint64_t beg, uint64_t end,
+                                            int shift, uint32_t xm) {
+  constexpr int F = 1 << BITS;
+  if (beg < end) {
+    // 16-byte loads from the first 16-byte aligned tuple on
+    const uint64_t a0 = (V16 && ((uintptr_t)(in + beg) & 15)) ? beg + 1 : beg;
+    if (V16 && a0 < end) {
+      if (a0 != beg && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[beg].x, xm, shift, F - 1)], 1u);
+      const uint4 *in4 = reinterpret_cast<const uint4 *>(in + a0);
+      const uint64_t npairs = (end - a0) / 2;
+      uint64_t i = threadIdx.x;
+      for (; i + 3 * NT < npairs; i += 4 * NT) {
+        uint4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = in4[i + u * NT];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          atomicAdd(&hist[digit_of(q[u].x, xm, shift, F - 1)], 1u);
+          atomicAdd(&hist[digit_of(q[u].z, xm, shift, F - 1)], 1u);
+        }
+      }
+      for (; i < npairs; i += NT) {
+        const uint4 q = in4[i];
+        atomicAdd(&hist[digit_of(q.x, xm, shift, F - 1)], 1u);
+        atomicAdd(&hist[digit_of(q.z, xm, shift, F - 1)], 1u);
+      }
+      if (((end - a0) & 1) && threadIdx.x == 0) atomicAdd(&hist[digit_of(in[end - 1].x, xm, shift, F - 1)], 1u);
+    } else {
+      for (uint64_t i = beg + threadIdx.x; i < end; i += NT) atomicAdd(&hist[digit_of(in[i].x, xm, shift, F - 1)], 1u);
+    }
+  }
+}
+
+}  // namespace tile
+}  // namespace eo
