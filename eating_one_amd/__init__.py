@@ -227,6 +227,117 @@ class Context:
         return Index(self, h.value)
 
 
+    # ---- resident tables and batches ---------------------------------------------
+    def table_create(self, R, **kw) -> "Table":
+        """Partition R once on the device and keep it resident (eo_table_create).
+        R is a host (n, 2) int32 array, or a DeviceArray with n= given."""
+        h = ctypes.c_void_p()
+        if isinstance(R, DeviceArray):
+            n = kw.pop("n")
+            o = self.opts(where=EO_DEVICE, **kw)
+            _check(self.lib.eo_table_create(self.handle, R.ptr, n, ctypes.byref(o), ctypes.byref(h)), "eo_table_create")
+        else:
+            R = _tuples(R)
+            o = self.opts(**kw)
+            _check(self.lib.eo_table_create(self.handle, _vp(R), len(R), ctypes.byref(o), ctypes.byref(h)),
+                   "eo_table_create")
+        return Table(self, h.value)
+
+    def batch(self) -> "Batch":
+        return Batch(self)
+
+
+class Table:
+    """A resident, partitioned build relation (eo_table)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def info(self) -> dict:
+        n = ctypes.c_uint64()
+        bits = ctypes.c_int32()
+        nbytes = ctypes.c_uint64()
+        _check(self.ctx.lib.eo_table_query(self.handle, ctypes.byref(n), ctypes.byref(bits), ctypes.byref(nbytes)),
+               "eo_table_query")
+        return {"n": n.value, "radix_bits": bits.value, "bytes": nbytes.value}
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.eo_table_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def request(R=None, nR=0, S=None, nS=0, out=None, out_cap=0, table=None, tag=0) -> "_abi.Request":
+    """An eo_request from device pointers (ints), DeviceArrays or a Table."""
+    def ptr(x):
+        if x is None:
+            return None
+        return x.ptr if isinstance(x, DeviceArray) else int(x)
+    return _abi.Request(ptr(R), int(nR), table.handle if table is not None else None, ptr(S), int(nS), ptr(out),
+                        int(out_cap), int(tag))
+
+
+class Batch:
+    """A batch of joins run by one persistent probe kernel (eo_batch)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(ctx.lib.eo_batch_create(ctx.handle, ctypes.byref(h)), "eo_batch_create")
+        self.handle = h.value
+        self._n = 0
+
+    def footprint(self, reqs) -> int:
+        arr = (_abi.Request * max(len(reqs), 1))(*reqs)
+        b = ctypes.c_uint64()
+        _check(self.ctx.lib.eo_batch_footprint(self.ctx.handle, arr, len(reqs), ctypes.byref(b)), "eo_batch_footprint")
+        return b.value
+
+    def submit(self, reqs, stats: JoinStats | None = None, **kw):
+        arr = (_abi.Request * max(len(reqs), 1))(*reqs)
+        o = self.ctx.opts(where=EO_DEVICE, stats=stats, **kw)
+        _check(self.ctx.lib.eo_batch_submit(self.handle, arr, len(reqs), ctypes.byref(o)), "eo_batch_submit")
+        self._n = len(reqs)
+
+    def replay(self):
+        _check(self.ctx.lib.eo_batch_replay(self.handle), "eo_batch_replay")
+
+    def poll(self) -> bool:
+        st = self.ctx.lib.eo_batch_poll(self.handle)
+        if st == _abi.EO_EAGAIN:
+            return False
+        _check(st, "eo_batch_poll")
+        return True
+
+    def wait(self):
+        """Returns a list of (tag, n_out, status)."""
+        res = (_abi.Result * max(self._n, 1))()
+        _check(self.ctx.lib.eo_batch_wait(self.handle, res, self._n), "eo_batch_wait")
+        return [(r.tag, r.n_out, r.status) for r in res[: self._n]]
+
+    def run(self, reqs, **kw):
+        self.submit(reqs, **kw)
+        return self.wait()
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.eo_batch_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Index:
     """A resident index (eo_index) built on one Context."""
 

@@ -101,6 +101,28 @@ class IndexInfo(ctypes.Structure):
     ]
 
 
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("R", P),
+        ("nR", U64),
+        ("table", P),
+        ("S", P),
+        ("nS", U64),
+        ("out", P),
+        ("out_cap", U64),
+        ("tag", U64),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("tag", U64),
+        ("n_out", U64),
+        ("status", I32),
+        ("reserved", U32),
+    ]
+
+
 EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
 EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
 EO_F_BINARY_SEARCH = 4
@@ -132,6 +154,16 @@ SIGNATURES = {
     "eo_index_export": (INT, [P, P, P, P]),
     "eo_index_query": (INT, [P, ctypes.POINTER(IndexInfo)]),
     "eo_index_free": (INT, [P]),
+    "eo_table_create": (INT, [P, P, U64, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
+    "eo_table_query": (INT, [P, PU64, ctypes.POINTER(I32), PU64]),
+    "eo_table_free": (INT, [P]),
+    "eo_batch_create": (INT, [P, ctypes.POINTER(P)]),
+    "eo_batch_free": (INT, [P]),
+    "eo_batch_footprint": (INT, [P, ctypes.POINTER(Request), U32, PU64]),
+    "eo_batch_submit": (INT, [P, ctypes.POINTER(Request), U32, ctypes.POINTER(JoinOpts)]),
+    "eo_batch_poll": (INT, [P]),
+    "eo_batch_wait": (INT, [P, ctypes.POINTER(Result), U32]),
+    "eo_batch_replay": (INT, [P]),
 }
 
 _lib = None

@@ -70,13 +70,12 @@ eo_status check_device_errors(eo_ctx *ctx, hipStream_t stream) {
   return EO_OK;
 }
 
-void ScopedTimer::finish() {
-  if (!stats) return;
+void fold_timer(const LaunchTimer &t, eo_join_stats *stats) {
   stats->n_kernels = 0;
-  for (int i = 0; i < ctx->timer.n; ++i) {
+  for (int i = 0; i < t.n; ++i) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ctx->timer.ev[2 * i], ctx->timer.ev[2 * i + 1]) != hipSuccess) continue;
-    const char *nm = ctx->timer.name[i];
+    if (hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]) != hipSuccess) continue;
+    const char *nm = t.name[i];
     uint32_t k = 0;
     while (k < stats->n_kernels && std::strncmp(stats->kernels[k].name, nm, sizeof(stats->kernels[k].name)))
       ++k;
@@ -91,22 +90,15 @@ void ScopedTimer::finish() {
   }
 }
 
-namespace {
+void ScopedTimer::finish() {
+  if (stats) fold_timer(ctx->timer, stats);
+}
 
 bool opts_ok(const eo_join_opts *o) {
   if (!o) return true;
   if (o->struct_size != 0 && o->struct_size < sizeof(eo_join_opts)) return false;
   return o->where == EO_HOST || o->where == EO_DEVICE;
 }
-
-constexpr uint64_t kMaxRows = (1ull << 31) - 1;
-
-// Partition `n` tuples into `out` and write the partition starts
-// (2^bits + 1 u32).  With a 16-byte aligned input: one fast pass, or two fast
-// passes MSD-first (high digit over the whole array, then the low digit inside
-// every first-pass partition).  Otherwise stable LSD passes (tmp holds the
-// first pass).
-bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
                              eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s) {
@@ -151,7 +143,6 @@ size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus) {
   return b;
 }
 
-}  // namespace
 }  // namespace eo
 
 using namespace eo;

@@ -31,14 +31,6 @@ namespace {
 
 using namespace eo;
 
-constexpr uint64_t kMaxRows = (1ull << 31) - 1;
-
-bool opts_ok(const eo_join_opts *o) {
-  if (!o) return true;
-  if (o->struct_size != 0 && o->struct_size < sizeof(eo_join_opts)) return false;
-  return o->where == EO_HOST || o->where == EO_DEVICE;
-}
-
 eo_status alloc_dev(void **p, size_t bytes, uint64_t *acc) {
   EO_TRY_HIP(hipMalloc(p, std::max<size_t>(bytes, 256)), "index allocation");
   *acc += bytes;

@@ -1,0 +1,237 @@
+// Batched execution of many joins of uneven sizes.
+//
+// The reference runs one join per process, with a host synchronisation after
+// every launch (joins/hashJoin_gpu/main_street/hj.cc:454-1105).  For a stream
+// of small and medium joins those launch and sync gaps dominate.  Here a batch
+// of joins runs as a fixed, host-sync-free sequence:
+//   1. partition: the relations of all joins that need the same number of
+//      radix bits form one group; one histogram launch, one u64 scan and one
+//      scatter launch partition every relation of the group (each stripe
+//      finds its relation by binary search in a descriptor table);
+//   2. plan: one thread per (join, partition) pair counts its probe tiles,
+//      a scan numbers them;
+//   3. probe: a persistent kernel (one workgroup per CU) drains a device work
+//      queue of (join, partition, R chunk, S span) tiles with an atomic
+//      ticket and runs the fused LDS probe on each, writing into the join's
+//      own output with the join's own output cursor.
+// The only host interaction is reading the per-join counts at the end, so
+// the whole sequence can also be captured once into a hipGraph and replayed.
+#include <algorithm>
+
+#include "device.hpp"
+#include "kernels.hpp"
+#include "tile_kernels.hpp"
+
+namespace eo {
+namespace {
+
+using tile::kFastNT;
+using tile::kFCap;
+using tile::kFNT;
+using tile::SpanRef;
+
+constexpr int kBHistNT = 512;
+
+__device__ __forceinline__ uint32_t find_seg(const BatchSeg *segs, uint32_t nseg, uint32_t g) {
+  uint32_t lo = 0, hi = nseg;  // segs[lo].stripe_base <= g < segs[hi].stripe_base
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].stripe_base <= g) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kBHistNT) void k_bseg_hist(const BatchSeg *__restrict__ segs, uint32_t nseg,
+                                                        uint32_t total_stripes, uint64_t len,
+                                                        uint64_t *__restrict__ counts) {
+  constexpr int F = 1 << BITS;
+  __shared__ uint32_t hist[F];
+  const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
+  if (g >= total_stripes) return;
+  const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+  const uint32_t s = g - sg.stripe_base;
+  const uint64_t beg = (uint64_t)s * len;
+  const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
+  for (int d = threadIdx.x; d < F; d += kBHistNT) hist[d] = 0;
+  __syncthreads();
+  tile::hist_stripe<BITS, kBHistNT, true>(hist, sg.in, beg, end, 0, 0u);
+  __syncthreads();
+  for (int d = threadIdx.x; d < F; d += kBHistNT) counts[sg.count_base + (uint64_t)d * sg.nstripes + s] = hist[d];
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__restrict__ segs, uint32_t nseg,
+                                                          uint32_t total_stripes, uint64_t len,
+                                                          const uint64_t *__restrict__ offsets, uint32_t *err) {
+  __shared__ tile::ScatterShared<BITS> sh;
+  const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
+  if (g >= total_stripes) return;
+  const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+  const uint32_t s = g - sg.stripe_base;
+  const uint64_t beg = (uint64_t)s * len;
+  const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
+  const uint64_t base = offsets[sg.count_base];  // tuples of the group's earlier relations
+  for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
+    sh.cursor[d] = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes + s] - base);
+    sh.cnt[d] = 0;
+  }
+  tile::scatter_stripe<BITS>(sh, sg.in, beg, end, sg.n, 0, 0u, sg.out, err);
+}
+
+// part_start[d] of every relation of the group, relative to the relation.
+__global__ void k_bseg_starts(const BatchSeg *__restrict__ segs, uint32_t nseg, uint32_t F,
+                              const uint64_t *__restrict__ offsets) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t per = (uint64_t)F + 1;
+  if (i >= (uint64_t)nseg * per) return;
+  const BatchSeg sg = segs[i / per];
+  const uint32_t d = (uint32_t)(i % per);
+  if (d == F) {
+    sg.part_start[F] = (uint32_t)sg.n;
+  } else {
+    const uint64_t base = offsets[sg.count_base];
+    sg.part_start[d] = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes] - base);
+  }
+}
+
+// pair_join[pair_base + p] = j for every partition p of join j (one workgroup per join).
+__global__ void k_bpairs(const BatchJoin *__restrict__ joins, uint32_t *__restrict__ pair_join) {
+  const uint32_t j = blockIdx.x;
+  const BatchJoin J = joins[j];
+  for (uint32_t p = threadIdx.x; p < J.P; p += blockDim.x) pair_join[J.pair_base + p] = j;
+}
+
+// tiles[q] for every (join, partition) pair q; tiles[npairs] = 0.
+__global__ void k_bplan(const BatchJoin *__restrict__ joins, const uint32_t *__restrict__ pair_join, uint32_t npairs,
+                        uint32_t *tiles) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q > npairs) return;
+  if (q == npairs) {
+    tiles[npairs] = 0;
+    return;
+  }
+  const BatchJoin J = joins[pair_join[q]];
+  const uint32_t p = q - J.pair_base;
+  const uint32_t rn = J.r_start[p + 1] - J.r_start[p];
+  const uint32_t sn = J.s_start[p + 1] - J.s_start[p];
+  tiles[q] = (rn && sn) ? ((rn + kFCap - 1) / kFCap) * ((sn + J.s_span - 1) / J.s_span) : 0u;
+}
+
+// Largest q in [0, n) with ts[q] <= t, for a non-decreasing ts with ts[0] <= t:
+// a 64-ary search by one wave (one load per lane per level, 4 levels for 2^22).
+__device__ __forceinline__ uint32_t wave_find(const uint32_t *__restrict__ ts, uint32_t n, uint32_t t) {
+  uint32_t lo = 0;
+  while (n > 1) {
+    const uint32_t step = (n + kWave - 1) / kWave;
+    const uint32_t idx = lo + (uint32_t)lane_id() * step;
+    const bool le = idx < lo + n && ts[idx] <= t;
+    const uint64_t m = __ballot(le);  // a prefix of the lanes; lane 0 always set
+    const uint32_t k = 63u - (uint32_t)__clzll(m);
+    const uint32_t nlo = lo + k * step;
+    n = min(step, lo + n - nlo);
+    lo = nlo;
+  }
+  return lo;
+}
+
+// Persistent probe: every workgroup takes tiles from the queue until it is
+// empty; every workgroup leaves the loop once its ticket passes the total.
+__global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ joins,
+                                                 const uint32_t *__restrict__ pair_join,
+                                                 const uint32_t *__restrict__ tile_start, uint32_t npairs,
+                                                 uint32_t *queue_head) {
+  __shared__ tile::ProbeShared sh;
+  __shared__ uint32_t item[2];
+  const uint32_t total = tile_start[npairs];
+  for (;;) {
+    if (threadIdx.x < kWave) {
+      uint32_t t = 0;
+      if (threadIdx.x == 0) t = atomic_add_agent(queue_head, 1u);
+      t = __shfl(t, 0);
+      uint32_t q = 0;
+      if (t < total) q = wave_find(tile_start, npairs, t);
+      if (threadIdx.x == 0) {
+        item[0] = t;
+        item[1] = q;
+      }
+    }
+    __syncthreads();
+    const uint32_t t = item[0], q = item[1];
+    __syncthreads();
+    if (t >= total) break;
+    const BatchJoin J = joins[pair_join[q]];
+    const uint32_t p = q - J.pair_base;
+    const uint32_t local = t - tile_start[q];
+    const uint32_t rb = J.r_start[p], rn = J.r_start[p + 1] - rb;
+    const uint32_t sb = J.s_start[p], sn = J.s_start[p + 1] - sb;
+    const uint32_t nc = (rn + kFCap - 1) / kFCap;
+    const uint32_t c = local % nc, j = local / nc;
+    SpanRef ref;
+    ref.r_beg = rb + c * kFCap;
+    ref.r_n = min((uint32_t)kFCap, rn - c * kFCap);
+    ref.s_beg = sb + j * J.s_span;
+    ref.s_n = min(J.s_span, sn - j * J.s_span);
+    tile::probe_tile(sh, J.R, J.S, ref, J.out, J.out_cap, J.cursor);
+  }
+}
+
+template <int BITS>
+hipError_t launch_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err, hipStream_t stream) {
+  const uint32_t grid = (g.total_stripes + kXcds - 1) / kXcds * kXcds;
+  EO_TIMED("batch_hist", stream,
+           hipLaunchKernelGGL(k_bseg_hist<BITS>, dim3(grid), dim3(kBHistNT), 0, stream, g.segs, g.nseg,
+                              g.total_stripes, g.stripe_len, counts));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = scan_u64(counts, counts, g.ncounts, nullptr, scan_scratch, err, stream);
+  if (e != hipSuccess) return e;
+  EO_TIMED("batch_scatter", stream,
+           hipLaunchKernelGGL(k_bseg_scatter<BITS>, dim3(grid), dim3(kFastNT), 0, stream, g.segs, g.nseg,
+                              g.total_stripes, g.stripe_len, counts, err));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const uint64_t nst = (uint64_t)g.nseg * ((1u << BITS) + 1);
+  hipLaunchKernelGGL(k_bseg_starts, dim3((uint32_t)((nst + 255) / 256)), dim3(256), 0, stream, g.segs, g.nseg,
+                     1u << BITS, counts);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+uint64_t batch_stripe_len() { return 1ull << 16; }
+
+hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
+                                 hipStream_t stream) {
+  if (g.nseg == 0) return hipSuccess;
+  switch (g.bits) {
+#define EO_BG_CASE(B) \
+  case B:             \
+    return launch_group<B>(g, counts, scan_scratch, err, stream);
+    EO_BG_CASE(1) EO_BG_CASE(2) EO_BG_CASE(3) EO_BG_CASE(4) EO_BG_CASE(5) EO_BG_CASE(6)
+    EO_BG_CASE(7) EO_BG_CASE(8) EO_BG_CASE(9) EO_BG_CASE(10) EO_BG_CASE(11)
+#undef EO_BG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_join, uint32_t npairs,
+                      uint32_t *tile_start, void *scan_scratch, uint32_t *err, hipStream_t stream) {
+  if (njoins == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bpairs, dim3(njoins), dim3(256), 0, stream, joins, pair_join);
+  hipLaunchKernelGGL(k_bplan, dim3((npairs + 1 + 255) / 256), dim3(256), 0, stream, joins, pair_join, npairs,
+                     tile_start);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return scan_u32(tile_start, tile_start, (uint64_t)npairs + 1, nullptr, scan_scratch, err, stream);
+}
+
+hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,
+                       uint32_t *queue_head, int cus, hipStream_t stream) {
+  if (npairs == 0) return hipSuccess;
+  EO_TIMED("batch_probe", stream,
+           hipLaunchKernelGGL(k_bprobe, dim3((uint32_t)std::max(cus, 1)), dim3(kFNT), 0, stream, joins, pair_join,
+                              tile_start, npairs, queue_head));
+  return hipGetLastError();
+}
+
+}  // namespace eo

@@ -72,6 +72,22 @@ eo_status hip_fail(hipError_t e, const char *what);
 eo_status check_device_errors(eo_ctx *ctx, hipStream_t stream);
 eo_status read_u64(eo_ctx *ctx, const uint64_t *dev, uint64_t *host, int n, hipStream_t stream);
 
+constexpr uint64_t kMaxRows = (1ull << 31) - 1;
+
+struct HashPlan;
+bool opts_ok(const eo_join_opts *o);
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+// Partition `n` tuples into `out` and write the partition starts
+// (2^bits + 1 u32) without any host synchronisation.  With a 16-byte aligned
+// input: one fast pass, or two fast passes MSD-first (high digit over the
+// whole array, then the low digit inside every first-pass partition).
+// Otherwise stable LSD passes (tmp holds the first pass).
+eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
+                             eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s);
+size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus);
+// Folds a launch timer's events into stats->kernels[] (stream synchronised).
+void fold_timer(const LaunchTimer &t, eo_join_stats *stats);
+
 inline hipStream_t pick_stream(eo_ctx *ctx, const eo_join_opts *o) {
   return (o && o->stream) ? (hipStream_t)o->stream : ctx->stream;
 }

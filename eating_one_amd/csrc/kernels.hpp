@@ -146,6 +146,48 @@ hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint3
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
                        uint64_t *tile_cnt, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream);
 
+// ---- batch.hip --------------------------------------------------------------
+// One relation of one join to partition in a batch group (all of a group's
+// relations use the same number of radix bits, digit = key & (2^bits - 1)).
+struct BatchSeg {
+  const uint2 *in;           // 16-byte aligned
+  uint2 *out;
+  uint64_t n;
+  uint32_t stripe_base;      // first stripe of this relation in the group
+  uint32_t nstripes;
+  uint64_t count_base;       // first count of this relation: counts laid out [rel][digit][stripe]
+  uint32_t *part_start;      // 2^bits + 1 partition starts (output)
+};
+struct BatchGroup {
+  int bits;
+  const BatchSeg *segs;      // device table, ordered by stripe_base
+  uint32_t nseg;
+  uint32_t total_stripes;
+  uint64_t stripe_len;
+  uint64_t ncounts;          // sum over relations of 2^bits * nstripes
+};
+// One join of a batch, as the persistent probe sees it.
+struct BatchJoin {
+  const uint2 *R, *S;        // partitioned (or raw when P == 1)
+  const uint32_t *r_start, *s_start;  // P + 1 each
+  uint32_t P, s_span;
+  uint2 *out;
+  uint64_t out_cap;
+  uint64_t *cursor;          // match count / output reservation (zero on entry)
+  uint32_t pair_base;        // first (join, partition) pair
+  uint32_t pad;
+};
+uint64_t batch_stripe_len();
+hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
+                                 hipStream_t stream);
+// Writes pair_join (npairs u32: the join of every (join, partition) pair) and
+// tile_start (npairs + 1 u32: scanned probe tile numbers of each pair).
+hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_join, uint32_t npairs,
+                      uint32_t *tile_start, void *scan_scratch, uint32_t *err, hipStream_t stream);
+// *queue_head must be zero on entry; grid = one workgroup per CU
+hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,
+                       uint32_t *queue_head, int cus, hipStream_t stream);
+
 // ---- nlj.hip ----------------------------------------------------------------
 struct NljPlan {
   uint64_t nR, nS;

@@ -257,6 +257,70 @@ eo_status eo_index_export(eo_ctx *ctx, const eo_index *index, uint32_t *keys,
 eo_status eo_index_query(const eo_index *index, eo_index_info *info);
 eo_status eo_index_free(eo_index *index);
 
+/* ---- resident build tables and batched joins ------------------------------ */
+
+typedef struct eo_table eo_table;
+typedef struct eo_batch eo_batch;
+
+/* A build relation partitioned once on the device (radix bits chosen from
+ * |R| as eo_hash_join would, or opts->radix_bits) and kept resident until
+ * eo_table_free, so that later joins against it partition only S.  The
+ * reference rebuilds and re-partitions R for every join
+ * (joins/hashJoin_gpu/main_street/hj.cc:498-602). */
+eo_status eo_table_create(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
+                          const eo_join_opts *opts, eo_table **table);
+eo_status eo_table_query(const eo_table *table, uint64_t *n, int32_t *radix_bits,
+                         uint64_t *bytes);
+eo_status eo_table_free(eo_table *table);
+
+/* One join of a batch.  All pointers are device pointers. */
+typedef struct eo_request {
+  const eo_tuple *R;      /* build side; ignored when table != NULL */
+  uint64_t nR;
+  const eo_table *table;  /* resident build side, or NULL */
+  const eo_tuple *S;      /* probe side */
+  uint64_t nS;
+  eo_pair *out;           /* NULL: count only */
+  uint64_t out_cap;
+  uint64_t tag;           /* caller's id, echoed in the result */
+} eo_request;
+
+typedef struct eo_result {
+  uint64_t tag;
+  uint64_t n_out;   /* match count (all of it, also when above out_cap) */
+  int32_t status;   /* EO_OK, or EO_ECAPACITY when out != NULL and n_out > out_cap
+                       (out then holds out_cap of the pairs, which ones unspecified) */
+  uint32_t reserved;
+} eo_result;
+
+/* A batch runs many joins of any sizes as one host-sync-free sequence on one
+ * stream: the relations that need the same radix bits are partitioned
+ * together (one histogram, scan and scatter launch per bit count), then one
+ * persistent kernel drains a device queue of (join, partition, R chunk,
+ * S span) probe tiles.  It replaces the reference's one-join-per-process
+ * driver with a host synchronisation after every launch (hj.cc:454-1105).
+ * Results equal those of eo_hash_join on each request alone.  A batch object
+ * owns its scratch arena and stream; one batch is in flight per object. */
+eo_status eo_batch_create(eo_ctx *ctx, eo_batch **batch);
+eo_status eo_batch_free(eo_batch *batch);
+/* Device scratch bytes a batch of these requests needs (outputs excluded). */
+eo_status eo_batch_footprint(eo_ctx *ctx, const eo_request *reqs, uint32_t n,
+                             uint64_t *bytes);
+/* Enqueues the batch and returns without waiting.  opts (optional): stream,
+ * stats (filled by eo_batch_wait), radix_bits / passes / lds_tuples / s_tile
+ * overrides for requests without a table. */
+eo_status eo_batch_submit(eo_batch *batch, const eo_request *reqs, uint32_t n,
+                          const eo_join_opts *opts);
+/* EO_OK when the batch in flight has finished, EO_EAGAIN while it runs. */
+eo_status eo_batch_poll(eo_batch *batch);
+/* Waits for the batch in flight and writes one result per request (n must
+ * equal the submitted count). */
+eo_status eo_batch_wait(eo_batch *batch, eo_result *results, uint32_t n);
+/* Runs the last submitted batch again over the same buffers (whose contents
+ * may have changed) from a hipGraph captured on first use; complete it with
+ * eo_batch_wait.  A new eo_batch_submit discards the graph. */
+eo_status eo_batch_replay(eo_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

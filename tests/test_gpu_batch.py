@@ -1,0 +1,200 @@
+"""GPU parity of resident tables and batched joins (persistent probe kernel).
+
+Every request of a batch must give the oracle's result (same count, same
+sorted multiset of pairs), which is also what the request gives alone through
+eo_hash_join; graph replays must repeat the eager result every time.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import eating_one_amd as eo
+import oracle
+from eating_one_amd import DeviceArray, gen
+from common import same_multiset
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(R, S):
+    return oracle.hash_join(R, S, nparts=max(1, len(R) // 64))
+
+
+def _mixed_inputs(seed, n_req, lo=10, hi=20):
+    """Sizes log-uniform in [2^lo, 2^hi]; keys uniform; selectivity U[0.1, 1]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_req):
+        nR = int(2 ** rng.uniform(lo, hi))
+        nS = int(2 ** rng.uniform(lo, hi))
+        sel = float(rng.uniform(0.1, 1.0))
+        R, S = gen.c2(seed=seed * 1000 + i, nR=nR, nS=nS, selectivity=sel)
+        out.append((R, S))
+    return out
+
+
+class _Req:
+    """Device buffers of one request."""
+
+    def __init__(self, ctx, R, S, cap=None, table=None, s_offset=0):
+        self.R, self.S = R, S
+        self.want = _oracle(R, S)
+        self.cap = len(self.want) if cap is None else cap
+        self.dR = None if table is not None else DeviceArray.from_host(ctx, R)
+        # s_offset tuples of padding in front of S: a device pointer that is only 8-byte aligned
+        pad = np.zeros((s_offset, 2), dtype=np.int32)
+        self.dS = DeviceArray.from_host(ctx, np.concatenate([pad, S]) if s_offset else S)
+        self.s_ptr = self.dS.ptr + 8 * s_offset
+        self.dout = DeviceArray(ctx, max(self.cap, 1) * 8) if self.cap else None
+        self.table = table
+
+    def request(self, tag, count_only=False):
+        return eo.request(R=self.dR, nR=len(self.R), S=self.s_ptr, nS=len(self.S),
+                          out=None if count_only or not self.dout else self.dout, out_cap=0 if count_only else self.cap,
+                          table=self.table, tag=tag)
+
+    def pairs(self, n):
+        return self.dout.to_host(np.int32, (n, 2)) if n else np.empty((0, 2), np.int32)
+
+
+def test_batch_mixed_sizes_equal_oracle_and_solo(ctx):
+    inputs = _mixed_inputs(51, 24)
+    reqs = [_Req(ctx, R, S) for R, S in inputs]
+    b = ctx.batch()
+    res = b.run([r.request(tag=100 + i) for i, r in enumerate(reqs)])
+    for i, (r, (tag, n, st)) in enumerate(zip(reqs, res)):
+        assert tag == 100 + i
+        assert st == 0
+        assert n == len(r.want), (i, len(r.R), len(r.S))
+        got = r.pairs(n)
+        assert same_multiset(got, r.want), i
+        # the same request alone
+        solo = ctx.hash_join(r.R, r.S)
+        assert same_multiset(solo, got), i
+    b.free()
+
+
+def test_batch_resident_tables_and_count_only(ctx):
+    rng = np.random.default_rng(7)
+    tabs = []
+    for k, nR in enumerate([3000, 40000, 300000]):
+        R, _ = gen.c2(seed=70 + k, nR=nR, nS=1)
+        tabs.append((R, ctx.table_create(R)))
+    assert tabs[0][1].info()["radix_bits"] == 0
+    assert tabs[2][1].info()["radix_bits"] > 0
+    reqs = []
+    for i in range(9):
+        R, t = tabs[i % 3]
+        nS = int(2 ** rng.uniform(8, 19))
+        keys = rng.integers(0, 2 * len(R), size=nS).astype(np.int32)
+        S = np.stack([keys, np.arange(nS, dtype=np.int32)], 1)
+        reqs.append(_Req(ctx, R, S, table=t))
+    b = ctx.batch()
+    res = b.run([r.request(tag=i, count_only=(i % 4 == 3)) for i, r in enumerate(reqs)])
+    for i, (r, (tag, n, st)) in enumerate(zip(reqs, res)):
+        assert st == 0 and n == len(r.want), i
+        if i % 4 != 3:
+            assert same_multiset(r.pairs(n), r.want), i
+    for _, t in tabs:
+        t.free()
+    b.free()
+
+
+def test_batch_edge_cases(ctx):
+    """Empty sides, capacity overflow, a misaligned S (individual stable
+    partition), a forced two-pass request and duplicate-heavy keys."""
+    R1, S1 = gen.c2(seed=901, nR=50000, nS=70000)
+    R2, S2 = gen.uniform(902, 30000, 40000, key_range=1 << 10)  # ~1.2M pairs, partitions overflow one LDS table
+    R3, S3 = gen.c2(seed=903, nR=100000, nS=5000)
+    empty = np.empty((0, 2), np.int32)
+    reqs = [
+        _Req(ctx, R1, S1),
+        _Req(ctx, R2, S2),
+        _Req(ctx, R1, S1, cap=1000),           # too small: EO_ECAPACITY, full count
+        _Req(ctx, R3, S3, s_offset=1),         # S only 8-byte aligned
+        _Req(ctx, R1[:1], S1[:10]),
+    ]
+    b = ctx.batch()
+    q = [r.request(tag=i) for i, r in enumerate(reqs)]
+    # empty relations
+    q.append(eo.request(R=reqs[0].dR, nR=0, S=reqs[0].dS, nS=len(S1), tag=99))
+    q.append(eo.request(R=reqs[0].dR, nR=len(R1), S=reqs[0].dS, nS=0, tag=98))
+    res = b.run(q)
+    assert res[5] == (99, 0, 0) and res[6] == (98, 0, 0)
+    for i in (0, 1, 3, 4):
+        tag, n, st = res[i]
+        assert st == 0 and n == len(reqs[i].want), i
+        assert same_multiset(reqs[i].pairs(n), reqs[i].want), i
+    tag, n, st = res[2]
+    assert st == eo._abi.EO_ECAPACITY and n == len(reqs[2].want)
+    # forced two radix passes (14 bits) for every request without a table
+    res2 = b.run(q, radix_bits=14)
+    for i in (0, 1, 3):
+        tag, n, st = res2[i]
+        assert st == 0 and n == len(reqs[i].want), i
+        assert same_multiset(reqs[i].pairs(n), reqs[i].want), i
+    b.free()
+
+
+def test_batch_rejects_bad_requests(ctx):
+    b = ctx.batch()
+    bad = eo.request(R=None, nR=10, S=None, nS=10)
+    with pytest.raises(eo.EOError):
+        b.run([bad])
+    # the batch object is still usable
+    R, S = gen.c2(seed=5, nR=2000, nS=3000)
+    r = _Req(ctx, R, S)
+    (tag, n, st), = b.run([r.request(tag=1)])
+    assert n == len(r.want)
+    b.free()
+
+
+def test_batch_graph_replay_matches_eager(ctx):
+    inputs = _mixed_inputs(61, 10, lo=10, hi=18)
+    reqs = [_Req(ctx, R, S) for R, S in inputs]
+    b = ctx.batch()
+    eager = b.run([r.request(tag=i) for i, r in enumerate(reqs)])
+    eager_pairs = [r.pairs(n) for r, (_, n, _) in zip(reqs, eager)]
+    R0, S0 = gen.c2(seed=62, nR=5000, nS=9000)
+    for rep in range(4):
+        b.replay()
+        res = b.wait()
+        assert res == eager, rep
+        for r, (_, n, _), want in zip(reqs, res, eager_pairs):
+            assert same_multiset(r.pairs(n), want), rep
+        # eager work on the same device between replays
+        assert len(ctx.hash_join(R0, S0)) == len(_oracle(R0, S0))
+    # new S contents in the same buffers: the replay sees them
+    rng = np.random.default_rng(63)
+    for r in reqs:
+        keys = rng.integers(0, len(r.R), size=len(r.S)).astype(np.int32)
+        S2 = np.stack([keys, np.arange(len(r.S), dtype=np.int32)], 1)
+        r.S = S2
+        r.want = _oracle(r.R, S2)
+        ctx.lib.eo_copy_h2d(ctx.handle, r.dS.ptr, ctypes.c_void_p(S2.ctypes.data), S2.nbytes)
+    b.replay()
+    res = b.wait()
+    for i, (r, (_, n, st)) in enumerate(zip(reqs, res)):
+        if len(r.want) > r.cap:
+            assert st == eo._abi.EO_ECAPACITY and n == len(r.want)
+        else:
+            assert st == 0 and n == len(r.want)
+            assert same_multiset(r.pairs(n), r.want), i
+    b.free()
+
+
+def test_batch_stats_and_footprint(ctx):
+    inputs = _mixed_inputs(71, 6, lo=12, hi=18)
+    reqs = [_Req(ctx, R, S) for R, S in inputs]
+    q = [r.request(tag=i) for i, r in enumerate(reqs)]
+    b = ctx.batch()
+    fp = b.footprint(q)
+    assert fp > sum(8 * (len(r.S)) for r in reqs if len(r.R) > 16384)
+    st = eo.JoinStats()
+    res = b.run(q, stats=st)
+    assert st.n_out == sum(n for _, n, _ in res)
+    names = st.kernel_table()
+    assert "batch_probe" in names
+    assert st.ms_total > 0
+    b.free()
