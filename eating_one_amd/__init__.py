@@ -246,6 +246,35 @@ class Context:
     def batch(self) -> "Batch":
         return Batch(self)
 
+    def serve(self, reqs, arrival_ms=None, mode=_abi.EO_SERVE_BATCHED, max_batch=0, slots=0, hbm_budget=0):
+        """Serve a request stream (eo_serve_run).  Returns (results, latency_ms, stats dict)."""
+        n = len(reqs)
+        arr = (_abi.Request * max(n, 1))(*reqs)
+        res = (_abi.Result * max(n, 1))()
+        lat = np.zeros(max(n, 1), dtype=np.float64)
+        a = None
+        if arrival_ms is not None:
+            a_np = np.ascontiguousarray(arrival_ms, dtype=np.float64)
+            a = a_np.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        o = _abi.ServeOpts(ctypes.sizeof(_abi.ServeOpts), mode, max_batch, slots, hbm_budget)
+        st = _abi.ServeStats()
+        _check(self.lib.eo_serve_run(self.handle, arr, a, n, ctypes.byref(o), res,
+                                     lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(st)),
+               "eo_serve_run")
+        stats = {f: getattr(st, f) for f, _ in _abi.ServeStats._fields_}
+        return [(r.tag, r.n_out, r.status) for r in res[:n]], lat[:n], stats
+
+    def gen_build(self, dR: DeviceArray, n: int, seed: int, offset: int = 0):
+        """Seeded build relation on the device: key = permutation of [0, n), payload = row."""
+        o = self.opts(where=EO_DEVICE)
+        _check(self.lib.eo_gen_build(self.handle, dR.ptr + offset, n, seed, ctypes.byref(o)), "eo_gen_build")
+
+    def gen_probe(self, dS: DeviceArray, n: int, nR: int, selectivity: float, seed: int, offset: int = 0):
+        """Seeded probe relation on the device: a fraction `selectivity` of keys in [0, nR)."""
+        o = self.opts(where=EO_DEVICE)
+        _check(self.lib.eo_gen_probe(self.handle, dS.ptr + offset, n, nR, selectivity, seed, ctypes.byref(o)),
+               "eo_gen_probe")
+
 
 class Table:
     """A resident, partitioned build relation (eo_table)."""

@@ -123,6 +123,32 @@ class Result(ctypes.Structure):
     ]
 
 
+class ServeOpts(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", U32),
+        ("mode", I32),
+        ("max_batch", U32),
+        ("slots", U32),
+        ("hbm_budget", U64),
+    ]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("wall_ms", ctypes.c_double),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("max_ms", ctypes.c_double),
+        ("mean_ms", ctypes.c_double),
+        ("tuples", U64),
+        ("n_out", U64),
+        ("batches", U32),
+        ("largest_batch", U32),
+        ("peak_admitted", U64),
+    ]
+
+
+EO_SERVE_BATCHED, EO_SERVE_PER_JOIN = 0, 1
 EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
 EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
 EO_F_BINARY_SEARCH = 4
@@ -164,6 +190,11 @@ SIGNATURES = {
     "eo_batch_poll": (INT, [P]),
     "eo_batch_wait": (INT, [P, ctypes.POINTER(Result), U32]),
     "eo_batch_replay": (INT, [P]),
+    "eo_serve_run": (INT, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), U32,
+                           ctypes.POINTER(ServeOpts), ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ServeStats)]),
+    "eo_gen_build": (INT, [P, P, U64, U64, ctypes.POINTER(JoinOpts)]),
+    "eo_gen_probe": (INT, [P, P, U64, U64, ctypes.c_double, U64, ctypes.POINTER(JoinOpts)]),
 }
 
 _lib = None

@@ -300,6 +300,26 @@ eo_status eo_bandwidth_probe(eo_ctx *c, uint64_t bytes, int iters, double *gbs) 
   return EO_OK;
 }
 
+eo_status eo_gen_build(eo_ctx *c, eo_tuple *R, uint64_t n, uint64_t seed, const eo_join_opts *o) {
+  if (!c || (n && !R) || n > kMaxRows || !opts_ok(o)) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  EO_TRY_HIP(gen_build(R, n, seed, s), "gen_build");
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  return EO_OK;
+}
+
+eo_status eo_gen_probe(eo_ctx *c, eo_tuple *S, uint64_t n, uint64_t nR, double selectivity, uint64_t seed,
+                       const eo_join_opts *o) {
+  if (!c || (n && !S) || n > kMaxRows || nR > kMaxRows || !opts_ok(o) || !(selectivity >= 0.0 && selectivity <= 1.0))
+    return EO_EINVAL;
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  EO_TRY_HIP(gen_probe(S, n, nR, selectivity, seed, s), "gen_probe");
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  return EO_OK;
+}
+
 // ---- primitives ---------------------------------------------------------------
 
 }  // extern "C"

@@ -20,16 +20,8 @@
 #include "eojoin.h"
 #include "kernels.hpp"
 #include "plan.hpp"
+#include "table.hpp"
 
-struct eo_table {
-  eo_ctx *ctx = nullptr;
-  uint64_t n = 0;
-  eo::HashPlan hp;
-  eo_tuple *part = nullptr;   // partitioned tuples (R itself copied when bits == 0)
-  uint32_t *start = nullptr;  // 2^bits + 1 partition starts
-  void *mem = nullptr;
-  uint64_t bytes = 0;
-};
 
 namespace eo {
 namespace {

@@ -188,6 +188,10 @@ hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_jo
 hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,
                        uint32_t *queue_head, int cus, hipStream_t stream);
 
+// ---- gen.hip ----------------------------------------------------------------
+hipError_t gen_build(eo_tuple *out, uint64_t n, uint64_t seed, hipStream_t s);
+hipError_t gen_probe(eo_tuple *out, uint64_t n, uint64_t nR, double selectivity, uint64_t seed, hipStream_t s);
+
 // ---- nlj.hip ----------------------------------------------------------------
 struct NljPlan {
   uint64_t nR, nS;

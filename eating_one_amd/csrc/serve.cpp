@@ -1,0 +1,280 @@
+// Request loop: continuous batching of a timed request stream.
+//
+// The reference has no serving path: every join is a separate process run
+// (joins/hashJoin_gpu/main_street/hj.cc:1189-1269).  eo_serve_run replays a
+// stream of requests with given arrival times against one device:
+//   - a request becomes eligible at its arrival time (host steady clock);
+//   - admission: a request's bytes (its inputs, partition scratch and output
+//     buffer) count against the HBM budget while it is in flight; a batch
+//     holds at most budget / slots, so the slots' scratch arenas never
+//     exceed the budget together; a request larger than that runs alone once
+//     the device is idle, and one larger than the whole budget is refused
+//     (EO_EBUDGET);
+//   - whenever a batch slot is free and requests are waiting, all waiting
+//     requests that fit are submitted as one batch (continuous batching);
+//     `slots` batches run at once on their own streams;
+//   - latency = completion (event observed by the loop) - arrival.
+// Mode EO_SERVE_PER_JOIN serves the same stream one eo_hash_join call per
+// request, the per-join launch sequence the batched mode is measured against.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "ctx.hpp"
+#include "eojoin.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+#include "table.hpp"
+
+
+namespace eo {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_since(Clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+// Admission size of one request: inputs, partition copies + starts, output.
+uint64_t request_bytes(const eo_request &q, int cus) {
+  const uint64_t nR = q.table ? q.table->n : q.nR;
+  uint64_t b = 8 * q.nS + 8 * q.out_cap + 256;
+  if (!q.table) b += 8 * nR;
+  if (nR == 0 || q.nS == 0) return b;
+  HashPlan hp;
+  if (q.table) {
+    eo_join_opts o{};
+    o.radix_bits = q.table->hp.bits ? q.table->hp.bits : -1;
+    o.passes = q.table->hp.passes;
+    if (plan_hash_join(nR, q.nS, &o, &hp, cus) != EO_OK) return b;
+  } else if (plan_hash_join(nR, q.nS, nullptr, &hp, cus) != EO_OK) {
+    return b;
+  }
+  if (hp.bits == 0) return b;
+  const uint64_t P = 1ull << hp.bits;
+  const uint64_t len = batch_stripe_len();
+  b += 8 * q.nS + 4 * (P + 1) + 8 * (((q.nS + len - 1) / len) << hp.bits) + 8 * P;
+  if (!q.table) b += 8 * nR + 4 * (P + 1) + 8 * (((nR + len - 1) / len) << hp.bits);
+  if (hp.passes == 2) b += 8 * std::max(nR, q.nS);
+  return b;
+}
+
+struct Slot {
+  eo_batch *b = nullptr;
+  bool busy = false;
+  bool exclusive = false;
+  uint64_t bytes = 0;
+  std::vector<uint32_t> idx;
+  std::vector<eo_request> reqs;
+  std::vector<eo_result> res;
+};
+
+void percentiles(std::vector<double> v, eo_serve_stats *st) {
+  if (v.empty()) return;
+  std::sort(v.begin(), v.end());
+  auto pct = [&](double p) {
+    // nearest-rank percentile
+    size_t r = (size_t)std::ceil(p / 100.0 * (double)v.size());
+    if (r < 1) r = 1;
+    return v[std::min(r, v.size()) - 1];
+  };
+  st->p50_ms = pct(50);
+  st->p99_ms = pct(99);
+  st->max_ms = v.back();
+  double s = 0;
+  for (double x : v) s += x;
+  st->mean_ms = s / (double)v.size();
+}
+
+}  // namespace
+}  // namespace eo
+
+using namespace eo;
+
+extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const double *arrival_ms, uint32_t n,
+                                  const eo_serve_opts *o, eo_result *results, double *latency_ms,
+                                  eo_serve_stats *stats) {
+  if (!c || (n && (!reqs || !results))) return EO_EINVAL;
+  if (o && o->struct_size != 0 && o->struct_size < sizeof(eo_serve_opts)) return EO_EINVAL;
+  const int mode = o ? o->mode : EO_SERVE_BATCHED;
+  if (mode != EO_SERVE_BATCHED && mode != EO_SERVE_PER_JOIN) return EO_EINVAL;
+  for (uint32_t i = 1; arrival_ms && i < n; ++i)
+    if (!(arrival_ms[i] >= arrival_ms[i - 1])) return EO_EINVAL;
+  const uint32_t max_batch = (o && o->max_batch) ? o->max_batch : 512u;
+  const uint32_t nslots = (o && o->slots) ? std::min<uint32_t>(o->slots, 8u) : 2u;
+  const uint64_t budget = (o && o->hbm_budget) ? o->hbm_budget : c->hbm_budget;
+  const uint64_t batch_cap = budget / nslots;
+  DeviceGuard g(c->device);
+  std::vector<double> lat(n, 0.0);
+  eo_serve_stats st{};
+  auto arrival = [&](uint32_t i) { return arrival_ms ? arrival_ms[i] : 0.0; };
+
+  std::vector<Slot> slots(mode == EO_SERVE_BATCHED ? nslots : 0);
+  auto cleanup = [&]() {
+    for (auto &s : slots)
+      if (s.b) {
+        eo_batch_free(s.b);
+        s.b = nullptr;
+      }
+  };
+  for (auto &s : slots) {
+    eo_status e = eo_batch_create(c, &s.b);
+    if (e != EO_OK) {
+      cleanup();
+      return e;
+    }
+  }
+  std::vector<uint64_t> rbytes(n);
+  for (uint32_t i = 0; i < n; ++i) rbytes[i] = request_bytes(reqs[i], c->props.cus);
+
+  const Clock::time_point t0 = Clock::now();
+  uint32_t next = 0, done = 0;
+  uint64_t in_flight_bytes = 0;
+  eo_status fail = EO_OK;
+
+  if (mode == EO_SERVE_PER_JOIN) {
+    eo_join_opts jo{};
+    jo.struct_size = sizeof(jo);
+    jo.where = EO_DEVICE;
+    for (uint32_t i = 0; i < n && fail == EO_OK; ++i) {
+      while (ms_since(t0) < arrival(i)) std::this_thread::yield();
+      const eo_request &q = reqs[i];
+      eo_result r{};
+      r.tag = q.tag;
+      if (rbytes[i] > budget) {
+        r.status = EO_EBUDGET;
+      } else {
+        const eo_tuple *R = q.table ? q.table->part : q.R;
+        const uint64_t nR = q.table ? q.table->n : q.nR;
+        uint64_t cnt = 0;
+        eo_status e = eo_hash_join(c, R, nR, q.S, q.nS, q.out, q.out ? q.out_cap : 0, &cnt, &jo);
+        if (e != EO_OK && e != EO_ECAPACITY) fail = e;
+        r.n_out = cnt;
+        r.status = e == EO_ECAPACITY ? EO_ECAPACITY : EO_OK;
+      }
+      lat[i] = ms_since(t0) - arrival(i);
+      results[i] = r;
+      st.n_out += r.n_out;
+      st.batches += 1;
+      st.largest_batch = 1;
+      st.peak_admitted = std::max(st.peak_admitted, rbytes[i]);
+    }
+    done = n;
+  }
+
+  while (done < n && fail == EO_OK) {
+    bool progress = false;
+    // completions
+    for (auto &s : slots) {
+      if (!s.busy) continue;
+      eo_status p = eo_batch_poll(s.b);
+      if (p == EO_EAGAIN) continue;
+      if (p == EO_OK) {
+        s.res.resize(s.idx.size());
+        p = eo_batch_wait(s.b, s.res.data(), (uint32_t)s.idx.size());
+      }
+      const double now = ms_since(t0);
+      s.busy = false;
+      in_flight_bytes -= s.bytes;
+      if (p != EO_OK) {
+        fail = p;
+        break;
+      }
+      for (size_t k = 0; k < s.idx.size(); ++k) {
+        const uint32_t i = s.idx[k];
+        results[i] = s.res[k];
+        lat[i] = now - arrival(i);
+        st.n_out += s.res[k].n_out;
+      }
+      done += (uint32_t)s.idx.size();
+      if (s.exclusive) {  // give back an arena grown past the per-batch share
+        eo_batch_free(s.b);
+        s.b = nullptr;
+        fail = eo_batch_create(c, &s.b);
+        s.exclusive = false;
+      }
+      progress = true;
+    }
+    if (fail != EO_OK) break;
+    // refuse requests that can never fit
+    while (next < n && rbytes[next] > budget && arrival(next) <= ms_since(t0)) {
+      eo_result r{};
+      r.tag = reqs[next].tag;
+      r.status = EO_EBUDGET;
+      results[next] = r;
+      lat[next] = 0.0;
+      ++next;
+      ++done;
+      progress = true;
+    }
+    // admission into a free slot
+    bool any_busy = false;
+    for (auto &s : slots) any_busy |= s.busy;
+    for (auto &s : slots) {
+      if (s.busy || next >= n) continue;
+      const double now = ms_since(t0);
+      if (arrival(next) > now) break;
+      if (rbytes[next] > budget) break;  // refused above once it has arrived
+      s.idx.clear();
+      s.reqs.clear();
+      s.bytes = 0;
+      s.exclusive = false;
+      if (rbytes[next] > batch_cap) {
+        if (any_busy) break;  // wait until the device is idle
+        s.exclusive = true;
+        s.idx.push_back(next);
+        s.reqs.push_back(reqs[next]);
+        s.bytes = rbytes[next];
+        ++next;
+      } else {
+        while (next < n && arrival(next) <= now && s.idx.size() < max_batch && rbytes[next] <= batch_cap &&
+               s.bytes + rbytes[next] <= batch_cap && in_flight_bytes + s.bytes + rbytes[next] <= budget) {
+          s.idx.push_back(next);
+          s.reqs.push_back(reqs[next]);
+          s.bytes += rbytes[next];
+          ++next;
+        }
+      }
+      if (s.idx.empty()) break;
+      eo_status e = eo_batch_submit(s.b, s.reqs.data(), (uint32_t)s.reqs.size(), nullptr);
+      if (e != EO_OK) {
+        fail = e;
+        break;
+      }
+      s.busy = true;
+      any_busy = true;
+      in_flight_bytes += s.bytes;
+      st.peak_admitted = std::max(st.peak_admitted, in_flight_bytes);
+      st.batches += 1;
+      st.largest_batch = std::max<uint32_t>(st.largest_batch, (uint32_t)s.idx.size());
+      progress = true;
+    }
+    if (!progress) std::this_thread::yield();
+  }
+  if (fail != EO_OK) {
+    for (auto &s : slots)
+      if (s.busy) {
+        s.res.resize(s.idx.size());
+        (void)eo_batch_wait(s.b, s.res.data(), (uint32_t)s.idx.size());
+      }
+    cleanup();
+    return fail;
+  }
+  st.wall_ms = ms_since(t0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t nR = reqs[i].table ? reqs[i].table->n : reqs[i].nR;
+    st.tuples += nR + reqs[i].nS;
+  }
+  percentiles(lat, &st);
+  if (latency_ms)
+    for (uint32_t i = 0; i < n; ++i) latency_ms[i] = lat[i];
+  if (stats) *stats = st;
+  cleanup();
+  return EO_OK;
+}

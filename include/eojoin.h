@@ -158,6 +158,19 @@ eo_status eo_copy_d2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
  * HBM ceiling reported beside every roofline fraction. */
 eo_status eo_bandwidth_probe(eo_ctx *ctx, uint64_t bytes, int iters, double *gbs);
 
+/* Seeded synthetic relations written straight into device memory (R, S are
+ * device pointers), for request streams too large to build on the host.
+ * Counterpart of createTuple (joins/hashJoin_gpu/main_street/hj.cc:62-160),
+ * which fills host arrays with clock-seeded rand().
+ *   eo_gen_build: key = a seeded permutation of [0, n), payload = row;
+ *   eo_gen_probe: with probability `selectivity` a key uniform in [0, nR)
+ *     (exactly one match in a build side from eo_gen_build(nR)), otherwise
+ *     uniform in [nR, 2^31 - 1); payload = row. */
+eo_status eo_gen_build(eo_ctx *ctx, eo_tuple *R, uint64_t n, uint64_t seed,
+                       const eo_join_opts *opts);
+eo_status eo_gen_probe(eo_ctx *ctx, eo_tuple *S, uint64_t n, uint64_t nR,
+                       double selectivity, uint64_t seed, const eo_join_opts *opts);
+
 /* ---- primitives ---------------------------------------------------------- */
 
 /* Exclusive prefix sum of n uint32 values; *total (optional) = sum.
@@ -320,6 +333,40 @@ eo_status eo_batch_wait(eo_batch *batch, eo_result *results, uint32_t n);
  * may have changed) from a hipGraph captured on first use; complete it with
  * eo_batch_wait.  A new eo_batch_submit discards the graph. */
 eo_status eo_batch_replay(eo_batch *batch);
+
+/* ---- request loop ------------------------------------------------------------ */
+
+enum { EO_SERVE_BATCHED = 0, EO_SERVE_PER_JOIN = 1 };
+
+typedef struct eo_serve_opts {
+  uint32_t struct_size;  /* sizeof(eo_serve_opts) */
+  int32_t mode;          /* EO_SERVE_BATCHED, or EO_SERVE_PER_JOIN: one eo_hash_join per request */
+  uint32_t max_batch;    /* requests per batch (0 = 512) */
+  uint32_t slots;        /* batches in flight at once (0 = 2, at most 8) */
+  uint64_t hbm_budget;   /* bytes admitted at once (0 = the ctx budget, 200 GB by default) */
+} eo_serve_opts;
+
+typedef struct eo_serve_stats {
+  double wall_ms;          /* loop start to last completion */
+  double p50_ms, p99_ms;   /* latency = completion - arrival, nearest rank */
+  double max_ms, mean_ms;
+  uint64_t tuples;         /* sum of |R| + |S| over the requests */
+  uint64_t n_out;          /* sum of match counts */
+  uint32_t batches;
+  uint32_t largest_batch;
+  uint64_t peak_admitted;  /* largest admitted bytes in flight */
+} eo_serve_stats;
+
+/* Serves a stream of requests: request i arrives arrival_ms[i] after the call
+ * starts (non-decreasing; NULL = all at once).  Admission by HBM budget (a
+ * request's inputs, partition scratch and output buffer count while it is in
+ * flight), continuous batching into eo_batch objects (one per slot, each on
+ * its own stream).  results[i] and latency_ms[i] (optional) per request; a
+ * request larger than the budget gets EO_EBUDGET.  No reference counterpart:
+ * the reference runs one join per process (hj.cc:1189-1269). */
+eo_status eo_serve_run(eo_ctx *ctx, const eo_request *reqs, const double *arrival_ms,
+                       uint32_t n, const eo_serve_opts *opts, eo_result *results,
+                       double *latency_ms, eo_serve_stats *stats);
 
 #ifdef __cplusplus
 }

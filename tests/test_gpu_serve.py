@@ -1,0 +1,121 @@
+"""GPU tests of the device generator and the request loop (continuous batching).
+
+Each served result must equal the oracle's result for that request alone, in
+both serving modes, whatever batch it landed in.
+"""
+import numpy as np
+import pytest
+
+import eating_one_amd as eo
+import oracle
+from eating_one_amd import DeviceArray, gen
+from common import same_multiset
+
+pytestmark = pytest.mark.gpu
+
+
+def test_gen_build_is_a_seeded_permutation(ctx):
+    for n in (1, 2, 1000, 100003, 1 << 20):
+        d = DeviceArray(ctx, 8 * n)
+        ctx.gen_build(d, n, seed=12345)
+        t = d.to_host(np.int32, (n, 2))
+        assert np.array_equal(np.sort(t[:, 0]), np.arange(n, dtype=np.int32)), n
+        assert np.array_equal(t[:, 1], np.arange(n, dtype=np.int32))
+        if n >= 1000:
+            assert not np.array_equal(t[:, 0], np.arange(n, dtype=np.int32))
+            d2 = DeviceArray(ctx, 8 * n)
+            ctx.gen_build(d2, n, seed=12345)
+            assert np.array_equal(d2.to_host(np.int32, (n, 2)), t)
+            ctx.gen_build(d2, n, seed=777)
+            assert not np.array_equal(d2.to_host(np.int32, (n, 2)), t)
+
+
+@pytest.mark.parametrize("sel", [0.0, 0.1, 0.55, 1.0])
+def test_gen_probe_selectivity_and_join_count(ctx, sel):
+    nR, nS = 50000, 200000
+    dR, dS = DeviceArray(ctx, 8 * nR), DeviceArray(ctx, 8 * nS)
+    ctx.gen_build(dR, nR, seed=1)
+    ctx.gen_probe(dS, nS, nR, sel, seed=2)
+    R = dR.to_host(np.int32, (nR, 2))
+    S = dS.to_host(np.int32, (nS, 2))
+    hits = int((S[:, 0] < nR).sum())
+    assert (S[:, 0] >= 0).all() and (S[:, 0] < (1 << 31) - 1).all()
+    assert np.array_equal(S[:, 1], np.arange(nS, dtype=np.int32))
+    assert abs(hits / nS - sel) < 0.01
+    # every hit matches exactly one build tuple
+    n, st = ctx.hash_join_device(dR, nR, dS, nS, None, 0)
+    assert n == hits
+    assert len(oracle.hash_join(R, S, nparts=1024)) == hits
+
+
+def _stream(ctx, seed, n_req, lo, hi, n_tables=3, reuse=0.25):
+    rng = np.random.default_rng(seed)
+    tables = []
+    for k in range(n_tables):
+        nR = int(2 ** rng.uniform(lo, hi))
+        R, _ = gen.c2(seed=seed * 10 + k, nR=nR, nS=1)
+        tables.append((R, ctx.table_create(R)))
+    items = []
+    for i in range(n_req):
+        if rng.random() < reuse:
+            R, t = tables[int(rng.integers(0, n_tables))]
+            dR = None
+        else:
+            nR = int(2 ** rng.uniform(lo, hi))
+            R, _ = gen.c2(seed=seed * 1000 + i, nR=nR, nS=1)
+            t = None
+            dR = DeviceArray.from_host(ctx, R)
+        nS = int(2 ** rng.uniform(lo, hi))
+        sel = float(rng.uniform(0.1, 1.0))
+        keys = np.where(rng.random(nS) < sel, rng.integers(0, len(R), nS), rng.integers(len(R), 1 << 30, nS))
+        S = np.stack([keys.astype(np.int32), np.arange(nS, dtype=np.int32)], 1)
+        want = oracle.hash_join(R, S, nparts=max(1, len(R) // 64))
+        dS = DeviceArray.from_host(ctx, S)
+        dout = DeviceArray(ctx, max(len(want), 1) * 8)
+        items.append(dict(R=R, S=S, want=want, dR=dR, dS=dS, dout=dout, table=t))
+    reqs = [eo.request(R=it["dR"], nR=len(it["R"]), S=it["dS"], nS=len(it["S"]), out=it["dout"],
+                       out_cap=len(it["want"]), table=it["table"], tag=i) for i, it in enumerate(items)]
+    return tables, items, reqs
+
+
+@pytest.mark.parametrize("mode", [eo._abi.EO_SERVE_BATCHED, eo._abi.EO_SERVE_PER_JOIN])
+def test_serve_results_equal_oracle(ctx, mode):
+    tables, items, reqs = _stream(ctx, 31, 40, 8, 18)
+    rng = np.random.default_rng(5)
+    arrival = np.cumsum(rng.exponential(0.2, size=len(reqs)))  # 5000 requests/s
+    res, lat, st = ctx.serve(reqs, arrival, mode=mode)
+    for i, ((tag, n, status), it) in enumerate(zip(res, items)):
+        assert tag == i and status == 0 and n == len(it["want"]), i
+        got = it["dout"].to_host(np.int32, (n, 2)) if n else np.empty((0, 2), np.int32)
+        assert same_multiset(got, it["want"]), i
+    assert (lat > 0).all()
+    assert st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert st["n_out"] == sum(len(it["want"]) for it in items)
+    assert st["tuples"] == sum(len(it["R"]) + len(it["S"]) for it in items)
+    if mode == eo._abi.EO_SERVE_BATCHED:
+        assert st["batches"] <= len(reqs)
+    for _, t in tables:
+        t.free()
+
+
+def test_serve_budget_admission(ctx):
+    """A small budget forces many batches, exclusive runs and a refusal."""
+    tables, items, reqs = _stream(ctx, 41, 24, 10, 17, reuse=0.0)
+    sizes = [8 * (len(it["R"]) + 2 * len(it["S"])) for it in items]
+    budget = int(np.median(sizes) * 6)
+    huge = int(np.argmax(sizes))
+    res, lat, st = ctx.serve(reqs, None, hbm_budget=budget, max_batch=4)
+    assert st["peak_admitted"] <= budget
+    assert st["batches"] >= len(reqs) // 4
+    for i, ((tag, n, status), it) in enumerate(zip(res, items)):
+        if status == eo._abi.EO_EBUDGET:
+            continue
+        assert status == 0 and n == len(it["want"]), i
+        got = it["dout"].to_host(np.int32, (n, 2)) if n else np.empty((0, 2), np.int32)
+        assert same_multiset(got, it["want"]), i
+    # a budget below the largest request refuses exactly the requests above it
+    res2, _, _ = ctx.serve(reqs, None, hbm_budget=sizes[huge] // 3)
+    assert res2[huge][2] == eo._abi.EO_EBUDGET
+    assert any(r[2] == 0 for r in res2)
+    for _, t in tables:
+        t.free()
