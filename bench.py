@@ -11,7 +11,7 @@ when the timed region starts; pairs written to a device buffer).
   cpu_baseline the CPU_JOIN hash join restated in oracle/ on a bounded sample
 
 Other configurations (parity cases; reported in the design notes, not the
-driver's line): --config c3-nlj | c3-hidx | c4.
+driver's line): --config c3-nlj | c3-hidx | c4 | c5.
 """
 from __future__ import annotations
 
@@ -307,12 +307,115 @@ def run_c4(a):
     return res
 
 
+def run_c5(a):
+    """C5: a stream of joins of log-uniform sizes, served three ways on the same
+    device-resident requests: per-join eo_hash_join calls (closed loop), batched
+    (closed loop), and batched under Poisson arrivals (latency)."""
+    import numpy as np
+
+    import eating_one_amd as eo
+    from eating_one_amd import _abi
+
+    rng = np.random.default_rng(a.seed if a.seed is not None else 0xE01C5)
+    n_req = a.c5_requests
+    ctx = eo.Context(0, hbm_budget=int(a.c5_budget))
+    # 64 resident build tables; 25 % of the requests reuse one of them
+    t_sizes = (2 ** rng.uniform(a.c5_lo, a.c5_hi, size=a.c5_tables)).astype(np.int64)
+    tables = []
+    for k, nR in enumerate(t_sizes):
+        d = eo.DeviceArray(ctx, 8 * int(nR))
+        ctx.gen_build(d, int(nR), seed=0x7AB1E000 + k)
+        tables.append((int(nR), ctx.table_create(d, n=int(nR)), d))
+    use_table = rng.random(n_req) < a.c5_reuse
+    table_id = rng.integers(0, a.c5_tables, size=n_req)
+    nR = (2 ** rng.uniform(a.c5_lo, a.c5_hi, size=n_req)).astype(np.int64)
+    nR = np.where(use_table, t_sizes[table_id], nR)
+    nS = (2 ** rng.uniform(a.c5_lo, a.c5_hi, size=n_req)).astype(np.int64)
+    sel = rng.uniform(0.1, 1.0, size=n_req)
+    pad = lambda n: (n + 1) // 2 * 2  # 16-byte aligned starts
+    r_off = np.concatenate([[0], np.cumsum(np.where(use_table, 0, pad(nR)))])
+    s_off = np.concatenate([[0], np.cumsum(pad(nS))])
+    log(f"C5: {n_req} requests, |R|,|S| = 2^U, U~U[{a.c5_lo},{a.c5_hi}]; R {8 * r_off[-1] / 1e9:.1f} GB, "
+        f"S {8 * s_off[-1] / 1e9:.1f} GB, out {8 * s_off[-1] / 1e9:.1f} GB; {int(use_table.sum())} on tables")
+    dR = eo.DeviceArray(ctx, 8 * max(int(r_off[-1]), 1))
+    dS = eo.DeviceArray(ctx, 8 * max(int(s_off[-1]), 1))
+    dO = eo.DeviceArray(ctx, 8 * max(int(s_off[-1]), 1))
+    t0 = time.time()
+    reqs = []
+    for i in range(n_req):
+        if use_table[i]:
+            t = tables[int(table_id[i])][1]
+            Rp = None
+        else:
+            t = None
+            Rp = dR.ptr + 8 * int(r_off[i])
+            ctx.gen_build(dR, int(nR[i]), seed=0xB0000000 + i, offset=8 * int(r_off[i]))
+        ctx.gen_probe(dS, int(nS[i]), int(nR[i]), float(sel[i]), seed=0x50000000 + i, offset=8 * int(s_off[i]))
+        reqs.append(eo.request(R=Rp, nR=int(nR[i]), S=dS.ptr + 8 * int(s_off[i]), nS=int(nS[i]),
+                               out=dO.ptr + 8 * int(s_off[i]), out_cap=int(nS[i]), table=t, tag=i))
+    log(f"generated in {time.time() - t0:.1f} s")
+    tuples = int(nR.sum() + nS.sum())
+    kw = dict(max_batch=a.c5_max_batch, slots=a.c5_slots)
+    out = {}
+    counts = None
+    for name, mode in (("per_join", _abi.EO_SERVE_PER_JOIN), ("batched", _abi.EO_SERVE_BATCHED)):
+        ctx.serve(reqs, None, mode=mode, **kw)  # warm-up: arenas grow to their steady size
+        walls = []
+        for _ in range(max(1, a.c5_reps)):
+            res, lat, st = ctx.serve(reqs, None, mode=mode, **kw)
+            walls.append(st["wall_ms"])
+        got = [r[1] for r in res]
+        assert all(r[2] == 0 for r in res), name
+        if counts is None:
+            counts = got
+        assert got == counts, "per-join and batched counts differ"
+        out[name] = {"wall_ms": float(np.median(walls)), "tuples_per_s": tuples / (np.median(walls) * 1e-3),
+                     "batches": st["batches"], "largest_batch": st["largest_batch"]}
+        log(f"{name}: {np.median(walls):.1f} ms for {n_req} requests ({tuples / np.median(walls) / 1e6:.2f} G tuples/s)")
+    capacity_qps = n_req / (out["batched"]["wall_ms"] * 1e-3)
+    qps = a.c5_qps if a.c5_qps else round(0.7 * capacity_qps, -2)
+    arrival = np.cumsum(rng.exponential(1000.0 / qps, size=n_req))
+    lat_res = {}
+    for name, mode in (("batched", _abi.EO_SERVE_BATCHED), ("per_join", _abi.EO_SERVE_PER_JOIN)):
+        res, lat, st = ctx.serve(reqs, arrival, mode=mode, **kw)
+        assert [r[1] for r in res] == counts
+        lat_res[name] = {"p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+                         "mean_ms": st["mean_ms"], "wall_ms": st["wall_ms"], "batches": st["batches"],
+                         "peak_admitted_gb": st["peak_admitted"] / 1e9}
+        log(f"{name} at {qps:.0f} req/s: p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
+    res = {"metric": "C5 join stream throughput, batched persistent kernel", "value": out["batched"]["tuples_per_s"],
+           "unit": "join tuples/s", "higher_is_better": True, "dtype": "int32", "data": "synthetic (device generator)",
+           "config": {"workload": f"C5 stream: {n_req} joins, |R|,|S|=2^U U~U[{a.c5_lo},{a.c5_hi}], uniform keys, "
+                                  f"selectivity U[0.1,1], {a.c5_tables} resident tables reused by "
+                                  f"{a.c5_reuse:.0%} of requests, HBM budget {a.c5_budget / 1e9:.0f} GB",
+                      "requests": n_req, "tuples": tuples, "n_out": int(sum(counts)), "max_batch": a.c5_max_batch,
+                      "slots": a.c5_slots},
+           "closed_loop": out, "speedup_batched_vs_per_join": out["per_join"]["wall_ms"] / out["batched"]["wall_ms"],
+           "poisson": {"qps": qps, **{k: v for k, v in lat_res.items()}}}
+    for _, t, d in tables:
+        t.free()
+        d.close()
+    ctx.close()
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4", "c5"])
+    ap.add_argument("--seed", type=int, default=None)
+    ap.add_argument("--c5-requests", type=int, default=4096)
+    ap.add_argument("--c5-lo", type=float, default=10)
+    ap.add_argument("--c5-hi", type=float, default=24)
+    ap.add_argument("--c5-tables", type=int, default=64)
+    ap.add_argument("--c5-reuse", type=float, default=0.25)
+    ap.add_argument("--c5-budget", type=float, default=200e9)
+    ap.add_argument("--c5-max-batch", type=int, default=512)
+    ap.add_argument("--c5-slots", type=int, default=2)
+    ap.add_argument("--c5-reps", type=int, default=3)
+    ap.add_argument("--c5-qps", type=float, default=0)
     ap.add_argument("--log2-r", type=int, default=27)
     ap.add_argument("--theta", type=float, default=0.0)
     ap.add_argument("--selectivity", type=float, default=1.0)
@@ -329,7 +432,7 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): this build has no multi-GPU path; "
               "the sharding design is described in DESIGN.md", file=sys.stderr)
         return 2
-    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4}[a.config]
+    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5}[a.config]
     print(json.dumps(fn(a)), flush=True)
     return 0
 

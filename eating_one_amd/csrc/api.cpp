@@ -215,6 +215,8 @@ eo_status eo_ctx_destroy(eo_ctx *c) {
   if (!c) return EO_EINVAL;
   DeviceGuard g(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (eo_batch *b : c->serve_pool) eo_batch_free(b);
+  c->serve_pool.clear();
   c->arena.release();
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);

@@ -51,6 +51,7 @@ struct eo_ctx {
   uint64_t hbm_budget = 0;
   hipEvent_t ev[8] = {};
   eo::LaunchTimer timer;  // per-launch events, installed while a call collects stats
+  std::vector<eo_batch *> serve_pool;  // batch objects reused by eo_serve_run
 };
 
 namespace eo {

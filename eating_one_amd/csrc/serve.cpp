@@ -115,21 +115,20 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
   eo_serve_stats st{};
   auto arrival = [&](uint32_t i) { return arrival_ms ? arrival_ms[i] : 0.0; };
 
+  // batch objects (each with its stream and grow-only arena) persist in the
+  // ctx between calls, so a warmed-up loop allocates nothing
   std::vector<Slot> slots(mode == EO_SERVE_BATCHED ? nslots : 0);
-  auto cleanup = [&]() {
-    for (auto &s : slots)
-      if (s.b) {
-        eo_batch_free(s.b);
-        s.b = nullptr;
-      }
-  };
-  for (auto &s : slots) {
-    eo_status e = eo_batch_create(c, &s.b);
-    if (e != EO_OK) {
-      cleanup();
-      return e;
-    }
+  while (c->serve_pool.size() < slots.size()) {
+    eo_batch *b = nullptr;
+    eo_status e = eo_batch_create(c, &b);
+    if (e != EO_OK) return e;
+    c->serve_pool.push_back(b);
   }
+  for (size_t k = 0; k < slots.size(); ++k) slots[k].b = c->serve_pool[k];
+  auto cleanup = [&]() {
+    for (size_t k = 0; k < slots.size(); ++k) c->serve_pool[k] = slots[k].b;
+    c->serve_pool.erase(std::remove(c->serve_pool.begin(), c->serve_pool.end(), nullptr), c->serve_pool.end());
+  };
   std::vector<uint64_t> rbytes(n);
   for (uint32_t i = 0; i < n; ++i) rbytes[i] = request_bytes(reqs[i], c->props.cus);
 
@@ -194,9 +193,11 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
       }
       done += (uint32_t)s.idx.size();
       if (s.exclusive) {  // give back an arena grown past the per-batch share
+        eo_batch *nb = nullptr;
         eo_batch_free(s.b);
         s.b = nullptr;
-        fail = eo_batch_create(c, &s.b);
+        fail = eo_batch_create(c, &nb);
+        s.b = nb;
         s.exclusive = false;
       }
       progress = true;
