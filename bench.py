@@ -89,6 +89,21 @@ def cpu_baseline_hash(sample_log2_r: int, theta: float, sel: float, threads: int
     }
 
 
+def read_traffic(kernel):
+    """HBM bytes per launch of a C2 kernel family, from the committed rocprofv3
+    PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel
+    calibration recorded in the same file); None if absent."""
+    path = os.path.join(ROOT, "profiles", "hash_join_c2_traffic.txt")
+    try:
+        for line in open(path):
+            f = line.split()
+            if len(f) >= 6 and f[0] == "traffic" and f[1] == kernel:
+                return float(f[5])
+    except OSError:
+        pass
+    return None
+
+
 def copy_ceiling(ctx):
     g = ctypes.c_double()
     st = ctx.lib.eo_bandwidth_probe(ctx.handle, 1 << 30, 10, ctypes.byref(g))
@@ -147,14 +162,7 @@ def run_c2(a):
     dom_mean_ms = dom_ms / dom_launches
     dom_bytes = kernel_alg_bytes(dom_name, nR, nS, n_out)
     achieved = dom_bytes / (dom_mean_ms * 1e-3) / 1e9 if dom_bytes else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
-    if os.path.exists(tfile):
-        try:
-            tj = json.load(open(tfile))
-            traffic = tj.get(dom_name, {}).get("bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    traffic = read_traffic(dom_name) if a.log2_r == 27 else None
     alg_bytes = 24 * (nR + nS) + 8 * n_out
     pipe = alg_bytes / dt / 1e9
     ceiling = copy_ceiling(ctx)
