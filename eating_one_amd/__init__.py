@@ -14,6 +14,7 @@ TUPLE.val), column 1 the payload / row id (reference TUPLE.key).
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -263,6 +264,32 @@ class Context:
                "eo_serve_run")
         stats = {f: getattr(st, f) for f, _ in _abi.ServeStats._fields_}
         return [(r.tag, r.n_out, r.status) for r in res[:n]], lat[:n], stats
+
+    # ---- autotuner ----------------------------------------------------------------
+    def tune_hash_join(self, dR: DeviceArray, nR: int, dS: DeviceArray, nS: int, dout: DeviceArray | None = None,
+                       out_cap: int = 0, reps: int = 3) -> dict:
+        """Time candidate plans for this size class and store the fastest (eo_tune_hash_join)."""
+        r = _abi.TuneResult()
+        _check(self.lib.eo_tune_hash_join(self.handle, dR.ptr, nR, dS.ptr, nS, dout.ptr if dout else None, out_cap,
+                                          reps, ctypes.byref(r)), "eo_tune_hash_join")
+        return {f: getattr(r, f) for f, _ in _abi.TuneResult._fields_ if f != "reserved"}
+
+    def tune_lookup(self, nR: int, nS: int):
+        r = _abi.TuneResult()
+        st = self.lib.eo_tune_lookup(self.handle, nR, nS, ctypes.byref(r))
+        if st == _abi.EO_EAGAIN:
+            return None
+        _check(st, "eo_tune_lookup")
+        return {f: getattr(r, f) for f, _ in _abi.TuneResult._fields_ if f != "reserved"}
+
+    def tune_save(self, path: str):
+        _check(self.lib.eo_tune_save(self.handle, os.fsencode(path)), "eo_tune_save")
+
+    def tune_load(self, path: str):
+        _check(self.lib.eo_tune_load(self.handle, os.fsencode(path)), "eo_tune_load")
+
+    def tune_clear(self):
+        _check(self.lib.eo_tune_clear(self.handle), "eo_tune_clear")
 
     def gen_build(self, dR: DeviceArray, n: int, seed: int, offset: int = 0):
         """Seeded build relation on the device: key = permutation of [0, n), payload = row."""

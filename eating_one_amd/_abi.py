@@ -148,6 +148,19 @@ class ServeStats(ctypes.Structure):
     ]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("radix_bits", I32),
+        ("passes", I32),
+        ("lds_tuples", I32),
+        ("s_tile", I32),
+        ("ms", ctypes.c_double),
+        ("default_ms", ctypes.c_double),
+        ("candidates", U32),
+        ("reserved", U32),
+    ]
+
+
 EO_SERVE_BATCHED, EO_SERVE_PER_JOIN = 0, 1
 EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
 EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
@@ -193,6 +206,11 @@ SIGNATURES = {
     "eo_serve_run": (INT, [P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), U32,
                            ctypes.POINTER(ServeOpts), ctypes.POINTER(Result), ctypes.POINTER(ctypes.c_double),
                            ctypes.POINTER(ServeStats)]),
+    "eo_tune_hash_join": (INT, [P, P, U64, P, U64, P, U64, INT, ctypes.POINTER(TuneResult)]),
+    "eo_tune_lookup": (INT, [P, U64, U64, ctypes.POINTER(TuneResult)]),
+    "eo_tune_clear": (INT, [P]),
+    "eo_tune_save": (INT, [P, ctypes.c_char_p]),
+    "eo_tune_load": (INT, [P, ctypes.c_char_p]),
     "eo_gen_build": (INT, [P, P, U64, U64, ctypes.POINTER(JoinOpts)]),
     "eo_gen_probe": (INT, [P, P, U64, U64, ctypes.c_double, U64, ctypes.POINTER(JoinOpts)]),
 }

@@ -17,6 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libeojoin.so")
+CLI = os.path.join(PKG, "eo_join")
 
 SOURCES = [
     "scan.hip",
@@ -82,6 +83,15 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the command-line driver (plain C over the C ABI)
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc and os.path.exists(os.path.join(CSRC, "eo_join_cli.c")):
+        cmd = [cc, "-O2", "-Wall", "-I", INCLUDE, os.path.join(CSRC, "eo_join_cli.c"), "-o", CLI + ".tmp",
+               "-L", os.path.dirname(LIB), "-leojoin", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(CLI + ".tmp", CLI)
     if verbose:
         print(f"built {LIB} from {len(objs)} objects")
     return LIB

@@ -437,7 +437,10 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   if ((nR && !R) || (nS && !S) || (out_cap && !out)) return EO_EINVAL;
   *n_out = 0;
   HashPlan hp;
-  eo_status st = plan_hash_join(nR, nS, o, &hp, c->props.cus);
+  eo_join_opts eff{};
+  if (o) eff = *o;
+  apply_tuned(c, nR, nS, &eff);
+  eo_status st = plan_hash_join(nR, nS, &eff, &hp, c->props.cus);
   if (st != EO_OK) return st;
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));

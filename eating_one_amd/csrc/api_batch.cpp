@@ -120,7 +120,9 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
       eo_status st = plan_hash_join(nR, q.nS, &to, &J.hp, cus);
       if (st != EO_OK) return st;
     } else {
-      eo_status st = plan_hash_join(nR, q.nS, &po, &J.hp, cus);
+      eo_join_opts to = po;
+      apply_tuned(c, nR, q.nS, &to);
+      eo_status st = plan_hash_join(nR, q.nS, &to, &J.hp, cus);
       if (st != EO_OK) return st;
     }
     const uint32_t P = 1u << J.hp.bits;

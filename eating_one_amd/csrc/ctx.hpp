@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "eojoin.h"
@@ -32,6 +34,13 @@ struct Layout {
   }
 };
 
+// Autotuner table: size class (ceil log2 |R|, ceil log2 |S|) -> plan.
+using TuneKey = std::pair<int, int>;
+struct TunedPlan {
+  int radix_bits, passes, lds_tuples, s_tile;
+  double ms;
+};
+
 struct DeviceProps {
   std::string name, arch;
   int cus = 0;
@@ -52,6 +61,7 @@ struct eo_ctx {
   hipEvent_t ev[8] = {};
   eo::LaunchTimer timer;  // per-launch events, installed while a call collects stats
   std::vector<eo_batch *> serve_pool;  // batch objects reused by eo_serve_run
+  std::map<eo::TuneKey, eo::TunedPlan> tuned;  // autotuner results by size class
 };
 
 namespace eo {
@@ -86,6 +96,10 @@ inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
                              eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s);
 size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus);
+TuneKey tune_key(uint64_t nR, uint64_t nS);
+// Fills radix_bits / passes / s_tile of *o from the tuning table when the
+// caller left all plan fields at 0 and the size class has an entry.
+void apply_tuned(const eo_ctx *c, uint64_t nR, uint64_t nS, eo_join_opts *o);
 // Folds a launch timer's events into stats->kernels[] (stream synchronised).
 void fold_timer(const LaunchTimer &t, eo_join_stats *stats);
 

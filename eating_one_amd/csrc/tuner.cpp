@@ -1,0 +1,204 @@
+// Autotuner for the hash join plan.
+//
+// The reference fixes its plan with compile-time tables (PART_STANDARD,
+// per-size `#define`s; joins/hashJoin_gpu/main_street/hj.cc:378-436 derives
+// p_num from |R| / 64 and 6 bits per pass).  Here the planner's choice
+// (plan.cpp: partitions sized to one LDS table) is a starting point: for one
+// size class (ceil log2 |R|, ceil log2 |S|) eo_tune_hash_join times every
+// candidate on the device — radix bits around the planner's, one or two
+// passes, three S spans per probe tile — and keeps the fastest in the
+// context's table.  Joins and batches that leave radix_bits, passes,
+// lds_tuples and s_tile at 0 then use the table's entry for their class.
+// Every candidate computes the same result (tested), so tuning only changes
+// speed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+#include "eojoin.h"
+#include "plan.hpp"
+
+namespace eo {
+
+namespace {
+int ceil_log2(uint64_t x) {
+  int b = 0;
+  while ((1ull << b) < x) ++b;
+  return b;
+}
+}  // namespace
+
+TuneKey tune_key(uint64_t nR, uint64_t nS) { return {ceil_log2(std::max<uint64_t>(nR, 1)), ceil_log2(std::max<uint64_t>(nS, 1))}; }
+
+void apply_tuned(const eo_ctx *c, uint64_t nR, uint64_t nS, eo_join_opts *o) {
+  if (o->radix_bits || o->passes || o->lds_tuples || o->s_tile) return;  // caller's explicit plan wins
+  auto it = c->tuned.find(tune_key(nR, nS));
+  if (it == c->tuned.end()) return;
+  o->radix_bits = it->second.radix_bits ? it->second.radix_bits : -1;
+  o->passes = it->second.passes;
+  o->s_tile = it->second.s_tile;
+}
+
+}  // namespace eo
+
+using namespace eo;
+
+extern "C" {
+
+eo_status eo_tune_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
+                            uint64_t out_cap, int reps, eo_tune_result *res) {
+  if (!c || !res || nR == 0 || nS == 0 || !R || !S || (out_cap && !out) || nR > kMaxRows || nS > kMaxRows)
+    return EO_EINVAL;
+  reps = std::max(1, std::min(reps, 16));
+  std::memset(res, 0, sizeof(*res));
+  HashPlan base;
+  eo_status st = plan_hash_join(nR, nS, nullptr, &base, c->props.cus);
+  if (st != EO_OK) return st;
+  // candidates: bits b0-1 .. b0+2, passes 1 (bits <= 11) or 2, span auto / x2 / x0.5
+  struct Cand {
+    int bits, passes, s_tile;
+  };
+  std::vector<Cand> cands;
+  for (int b = std::max(0, base.bits - 1); b <= std::min(kMaxJoinBits, base.bits + 2); ++b) {
+    for (int p = 1; p <= 2; ++p) {
+      if (b == 0 && p == 2) continue;
+      if (b > 0 && p == 1 && b > kMaxPassBits) continue;
+      if (p == 2 && (b < 2 || b > 2 * kMaxPassBits)) continue;
+      HashPlan hp;
+      eo_join_opts o{};
+      o.radix_bits = b ? b : -1;
+      o.passes = b ? p : 0;
+      if (plan_hash_join(nR, nS, &o, &hp, c->props.cus) != EO_OK) continue;
+      for (int k = 0; k < 3; ++k) {
+        uint32_t span = hp.s_span;
+        if (k == 1) span = std::min<uint32_t>(span * 2, 1u << 18);
+        if (k == 2) span = std::max<uint32_t>(span / 2, 8192u);
+        if (k > 0 && span == hp.s_span) continue;
+        cands.push_back({b, b ? p : 0, k == 0 ? 0 : (int)span});
+      }
+      if (b == 0) break;
+    }
+  }
+  c->tuned.erase(tune_key(nR, nS));  // the baseline below is the untuned planner
+  eo_join_stats stats;
+  auto time_plan = [&](const eo_join_opts &o, double *best) -> eo_status {
+    *best = 1e30;
+    for (int r = 0; r <= reps; ++r) {  // first run warms the arena
+      uint64_t n = 0;
+      eo_status s2 = eo_hash_join(c, R, nR, S, nS, out, out_cap, &n, &o);
+      if (s2 != EO_OK && s2 != EO_ECAPACITY) return s2;
+      if (r > 0) *best = std::min(*best, stats.ms_total);
+    }
+    return EO_OK;
+  };
+  eo_join_opts o{};
+  o.struct_size = sizeof(o);
+  o.where = EO_DEVICE;
+  o.stats = &stats;
+  // the untuned planner
+  double def_ms = 0;
+  st = time_plan(o, &def_ms);
+  if (st != EO_OK) return st;
+  TunedPlan best{0, 0, 0, 0, 1e30};
+  for (const Cand &cd : cands) {
+    o.radix_bits = cd.bits ? cd.bits : -1;
+    o.passes = cd.passes;
+    o.s_tile = cd.s_tile;
+    double ms = 0;
+    st = time_plan(o, &ms);
+    if (st != EO_OK) return st;
+    if (ms < best.ms) best = {cd.bits, cd.passes, 0, cd.s_tile, ms};
+  }
+  c->tuned[tune_key(nR, nS)] = best;
+  res->radix_bits = best.radix_bits;
+  res->passes = best.passes;
+  res->lds_tuples = 0;
+  res->s_tile = best.s_tile;
+  res->ms = best.ms;
+  res->default_ms = def_ms;
+  res->candidates = (uint32_t)cands.size();
+  return EO_OK;
+}
+
+eo_status eo_tune_lookup(eo_ctx *c, uint64_t nR, uint64_t nS, eo_tune_result *res) {
+  if (!c || !res) return EO_EINVAL;
+  std::memset(res, 0, sizeof(*res));
+  auto it = c->tuned.find(tune_key(nR, nS));
+  if (it == c->tuned.end()) return EO_EAGAIN;
+  res->radix_bits = it->second.radix_bits;
+  res->passes = it->second.passes;
+  res->s_tile = it->second.s_tile;
+  res->ms = it->second.ms;
+  return EO_OK;
+}
+
+eo_status eo_tune_clear(eo_ctx *c) {
+  if (!c) return EO_EINVAL;
+  c->tuned.clear();
+  return EO_OK;
+}
+
+// Text format, one size class per line:
+//   eojoin-tune 1
+//   <log2 |R|> <log2 |S|> <radix_bits> <passes> <s_tile> <ms>
+eo_status eo_tune_save(eo_ctx *c, const char *path) {
+  if (!c || !path) return EO_EINVAL;
+  FILE *f = std::fopen(path, "w");
+  if (!f) {
+    set_last_error(std::string("cannot write ") + path);
+    return EO_EINVAL;
+  }
+  std::fprintf(f, "eojoin-tune 1\n");
+  for (const auto &kv : c->tuned)
+    std::fprintf(f, "%d %d %d %d %d %.6f\n", kv.first.first, kv.first.second, kv.second.radix_bits,
+                 kv.second.passes, kv.second.s_tile, kv.second.ms);
+  const bool ok = std::fclose(f) == 0;
+  return ok ? EO_OK : EO_EINVAL;
+}
+
+eo_status eo_tune_load(eo_ctx *c, const char *path) {
+  if (!c || !path) return EO_EINVAL;
+  FILE *f = std::fopen(path, "r");
+  if (!f) {
+    set_last_error(std::string("cannot read ") + path);
+    return EO_EINVAL;
+  }
+  int ver = 0;
+  if (std::fscanf(f, "eojoin-tune %d", &ver) != 1 || ver != 1) {
+    std::fclose(f);
+    set_last_error("not an eojoin tuning table");
+    return EO_EINVAL;
+  }
+  std::vector<std::pair<TuneKey, TunedPlan>> rows;
+  int kr, ks, b, p, st;
+  double ms;
+  int got;
+  while ((got = std::fscanf(f, "%d %d %d %d %d %lf", &kr, &ks, &b, &p, &st, &ms)) == 6) {
+    eo_join_opts o{};
+    o.radix_bits = b ? b : -1;
+    o.passes = p;
+    o.s_tile = st;
+    HashPlan hp;
+    if (kr < 0 || kr > 31 || ks < 0 || ks > 31 || st < 0 ||
+        plan_hash_join(1ull << kr, 1ull << ks, &o, &hp) != EO_OK) {
+      std::fclose(f);
+      set_last_error("invalid tuning table row");
+      return EO_EINVAL;
+    }
+    rows.push_back({{kr, ks}, {b, p, 0, st, ms}});
+  }
+  const bool clean_end = got == EOF;
+  std::fclose(f);
+  if (!clean_end) {
+    set_last_error("malformed tuning table");
+    return EO_EINVAL;
+  }
+  for (auto &r : rows) c->tuned[r.first] = r.second;
+  return EO_OK;
+}
+
+}  // extern "C"

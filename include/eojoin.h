@@ -334,6 +334,36 @@ eo_status eo_batch_wait(eo_batch *batch, eo_result *results, uint32_t n);
  * eo_batch_wait.  A new eo_batch_submit discards the graph. */
 eo_status eo_batch_replay(eo_batch *batch);
 
+/* ---- autotuner ---------------------------------------------------------------- */
+
+typedef struct eo_tune_result {
+  int32_t radix_bits;   /* chosen plan (0 bits = no partitioning) */
+  int32_t passes;
+  int32_t lds_tuples;   /* 0: the planner's LDS table size */
+  int32_t s_tile;       /* 0: the planner's S span */
+  double ms;            /* best device time of the chosen plan */
+  double default_ms;    /* device time of the untuned planner's plan */
+  uint32_t candidates;  /* plans timed */
+  uint32_t reserved;
+} eo_tune_result;
+
+/* Times candidate plans (radix bits around the planner's choice, one or two
+ * passes, three S spans) for the size class of (|R|, |S|) = (ceil log2 nR,
+ * ceil log2 nS) on the given device inputs, best of `reps` runs each, and
+ * stores the fastest in the ctx's table.  eo_hash_join and eo_batch_submit
+ * use the table for requests whose opts leave radix_bits, passes,
+ * lds_tuples and s_tile at 0.  Replaces the reference's compile-time plan
+ * constants (joins/hashJoin_gpu/main_street/hj.cc:378-436, tuple.h). */
+eo_status eo_tune_hash_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
+                            const eo_tuple *S, uint64_t nS, eo_pair *out,
+                            uint64_t out_cap, int reps, eo_tune_result *result);
+/* The table entry for a size class: EO_OK, or EO_EAGAIN when untuned. */
+eo_status eo_tune_lookup(eo_ctx *ctx, uint64_t nR, uint64_t nS, eo_tune_result *result);
+eo_status eo_tune_clear(eo_ctx *ctx);
+/* Text file, one size class per line (validated on load). */
+eo_status eo_tune_save(eo_ctx *ctx, const char *path);
+eo_status eo_tune_load(eo_ctx *ctx, const char *path);
+
 /* ---- request loop ------------------------------------------------------------ */
 
 enum { EO_SERVE_BATCHED = 0, EO_SERVE_PER_JOIN = 1 };

@@ -97,3 +97,15 @@ int main(void) {
     want = [ctypes.sizeof(t) for t in (_abi.JoinOpts, _abi.JoinStats, _abi.KernelStat, _abi.CtxOpts,
                                        _abi.DeviceInfo, _abi.IndexInfo, _abi.Request, _abi.Result)] + [8]
     assert got == want
+
+
+def test_cli_usage_and_no_gpu_error():
+    """The eo_join driver keeps the reference argv order (<op> <|S|> <|R|>)."""
+    import subprocess
+    exe = os.path.join(ROOT, "eating_one_amd", "eo_join")
+    if not os.path.exists(exe):
+        pytest.skip("eo_join not built")
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "<|S|> <|R|>" in r.stderr
+    r = subprocess.run([exe, "bogus", "10", "10"], capture_output=True, text=True)
+    assert r.returncode == 2

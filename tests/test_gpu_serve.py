@@ -119,3 +119,20 @@ def test_serve_budget_admission(ctx):
     assert any(r[2] == 0 for r in res2)
     for _, t in tables:
         t.free()
+
+
+@pytest.mark.parametrize("op", ["hash", "nlj", "hidx", "sidx", "learned"])
+def test_cli_counts(op):
+    """eo_join <op> <|S|> <|R|>: with selectivity 1 every S key matches exactly one R key."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "eating_one_amd", "eo_join")
+    nS, nR = 300000, 70000
+    r = subprocess.run([exe, op, str(nS), str(nR), "7", "1.0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    count = int([l for l in lines if l.startswith("count:")][0].split()[1])
+    assert count == nS
+    assert sum(l.startswith("pair ") for l in lines) == 3
+    r = subprocess.run([exe, op, str(nS), str(nR), "7", "0.0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "count: 0" in r.stdout
