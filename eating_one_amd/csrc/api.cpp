@@ -109,6 +109,11 @@ eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const 
     return EO_OK;
   }
   const int b_lo = hp.bits_pass1, b_hi = hp.bits_pass2;
+  if (aligned16(in) && aligned16(tmp) && msd2_supported(b_hi, b_lo)) {
+    EO_TRY_HIP(partition_msd2(n, b_hi, b_lo, in, tmp, out, part_start, scratch, ctx->d_err, cus, s),
+               "partition (two passes, one histogram)");
+    return EO_OK;
+  }
   if (aligned16(in) && aligned16(tmp)) {
     const uint32_t F1 = 1u << b_hi;
     uint32_t *start1 = (uint32_t *)scratch;
@@ -139,6 +144,7 @@ size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus) {
                  partition_pass_scratch_bytes(plan_partition_pass(n, b_hi, b_lo, 0u, cus)));
     b = std::max(b, head + partition_pass_scratch_bytes(plan_partition_pass(n, b_hi, b_lo, 0u, cus, true)));
     b = std::max(b, head + seg_pass_scratch_bytes(plan_seg_pass(n, 1u << b_hi, b_lo, 0, cus)));
+    if (msd2_supported(b_hi, b_lo)) b = std::max(b, msd2_scratch_bytes(n, b_hi, b_lo, cus));
   }
   return b;
 }

@@ -146,6 +146,16 @@ hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint3
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
                        uint64_t *tile_cnt, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream);
 
+// Two-pass MSD partition (high digit bits_hi over the whole array, then the
+// low digit bits_lo inside each first-pass partition) from a single
+// histogram pass; needs 16-byte aligned in and tmp and bits_hi + bits_lo <=
+// kMsd2MaxBits (the combined-digit histogram lives in LDS).
+constexpr int kMsd2MaxBits = 14;
+bool msd2_supported(int bits_hi, int bits_lo);
+size_t msd2_scratch_bytes(uint64_t n, int bits_hi, int bits_lo, int cus);
+hipError_t partition_msd2(uint64_t n, int bits_hi, int bits_lo, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
+                          uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream);
+
 // ---- batch.hip --------------------------------------------------------------
 // One relation of one join to partition in a batch group (all of a group's
 // relations use the same number of radix bits, digit = key & (2^bits - 1)).

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kPartNT) void k_part_hist(const uint2 *__restrict__
   __shared__ uint32_t hist[C::F];
   uint32_t g, k;
   if (geo.seg_start) {
-    g = xcd_local_index(blockIdx.x, gridDim.x);
+    g = blockIdx.x;  // grid is an upper bound: spread the real stripes over all XCDs
     k = 0;
   } else {
     const uint32_t gk = xcd_local_index(blockIdx.x, geo.G * geo.K);
@@ -245,7 +245,8 @@ __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__re
                                                                uint32_t xm, const uint32_t *__restrict__ offsets,
                                                                uint2 *__restrict__ out, uint32_t *err) {
   __shared__ tile::ScatterShared<BITS> sh;
-  const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
+  // segmented: the grid is an upper bound, so spread the real stripes over all XCDs
+  const uint32_t g = geo.seg_start ? blockIdx.x : xcd_local_index(blockIdx.x, gridDim.x);
   StripeRef sr;
   if (!stripe_of<(1 << BITS)>(geo, g, 0, &sr, true)) return;
   for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
@@ -359,6 +360,182 @@ hipError_t launch_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple *o
   return hipGetLastError();
 }
 
+// ---- two-pass MSD partition from one histogram ------------------------------------
+// The first pass's histogram counts the combined digit bin = key & (2^(bh+bl) - 1)
+// = dh * F2 + dl of every sub-stripe.  Summed over dl it gives the first pass's
+// counts; summed over the sub-stripes of stripe g (k_msd_reduce, a transpose,
+// instead of device-scope atomics, which bypass the per-XCD L2) it gives
+// counts2[bin * G + g]: how many tuples of each (high, low) digit pair the
+// first-pass stripe g holds.  The first
+// pass writes stripe g's tuples of high digit dh as one contiguous chunk
+// (dh, g), chunks ordered by g inside segment dh, so after one exclusive scan
+// of counts2 (order dh, dl, g) the tuples of digit pair (dh, dl) that come
+// from chunks g0..g1-1 have one contiguous output range starting at
+// off2[(dh * F2 + dl) * G + g0].  The second pass therefore needs no
+// histogram of its own: its stripes are groups of consecutive chunks of one
+// segment, sized so that the groups fill the GPU in about one wave.
+template <int TB>
+__global__ __launch_bounds__(kPartNT) void k_msd_hist(const uint2 *__restrict__ in, StripeGeo geo, int bl,
+                                                      uint32_t *__restrict__ counts1,
+                                                      uint32_t *__restrict__ partial) {
+  constexpr uint32_t FT = 1u << TB;
+  __shared__ uint32_t hist[FT];
+  const uint32_t gk = xcd_local_index(blockIdx.x, geo.G * geo.K);
+  const uint32_t g = gk / geo.K, k = gk % geo.K;
+  StripeRef sr;
+  stripe_of<1>(geo, g, k, &sr);
+  for (uint32_t b = threadIdx.x; b < FT; b += kPartNT) hist[b] = 0;
+  __syncthreads();
+  tile::hist_stripe<TB, kPartNT, true>(hist, in, sr.beg, sr.end, 0, 0u);
+  __syncthreads();
+  const uint32_t F2 = 1u << bl, F1 = FT >> bl;
+  for (uint32_t dh = threadIdx.x; dh < F1; dh += kPartNT) {
+    uint32_t sum = 0;
+    for (uint32_t dl = 0; dl < F2; ++dl) sum += hist[dh * F2 + ((dl + dh) & (F2 - 1))];  // skewed: fewer bank conflicts
+    counts1[((uint64_t)dh * geo.G + g) * geo.K + k] = sum;
+  }
+  // the whole combined-digit histogram of this sub-stripe, one coalesced row
+  uint32_t *row = partial + (uint64_t)(g * geo.K + k) * FT;
+  for (uint32_t b = threadIdx.x; b < FT; b += kPartNT) row[b] = hist[b];
+}
+
+// counts2[b * G + g] = sum over k of partial[(g * K + k) * FT + b]: a tiled
+// transpose (64 bins x 64 stripes per workgroup) so that reads and writes are
+// both coalesced.
+__global__ __launch_bounds__(256) void k_msd_reduce(const uint32_t *__restrict__ partial, uint32_t G, uint32_t K,
+                                                    uint32_t FT, uint32_t *__restrict__ counts2) {
+  __shared__ uint32_t t[64][65];
+  const uint32_t b0 = blockIdx.x * 64, g0 = blockIdx.y * 64;
+  const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+  for (uint32_t gi = ty; gi < 64; gi += 4) {
+    const uint32_t g = g0 + gi, b = b0 + tx;
+    uint32_t sum = 0;
+    if (g < G && b < FT)
+      for (uint32_t k = 0; k < K; ++k) sum += partial[(uint64_t)(g * K + k) * FT + b];
+    t[gi][tx] = sum;
+  }
+  __syncthreads();
+  for (uint32_t bi = ty; bi < 64; bi += 4) {
+    const uint32_t b = b0 + bi, g = g0 + tx;
+    if (b < FT && g < G) counts2[(uint64_t)b * G + g] = t[tx][bi];
+  }
+}
+
+// grp[dh] = number of second-pass stripes of segment dh (>= 1, <= G); grp[F1] = 0.
+__global__ void k_msd_groups(const uint32_t *__restrict__ off1, uint32_t G, uint32_t K, uint32_t F1, uint64_t n,
+                             uint64_t L, uint32_t *__restrict__ grp) {
+  const uint32_t dh = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dh < F1) {
+    const uint64_t beg = off1[(uint64_t)dh * G * K];
+    const uint64_t end = dh + 1 < F1 ? off1[(uint64_t)(dh + 1) * G * K] : n;
+    uint64_t J = (end - beg + L / 2) / L;
+    grp[dh] = (uint32_t)(J < 1 ? 1 : (J > G ? G : J));
+  } else if (dh == F1) {
+    grp[F1] = 0;
+  }
+}
+
+template <int BL>
+__global__ __launch_bounds__(kFastNT) void k_msd_scatter(const uint2 *__restrict__ in, uint64_t n,
+                                                         const uint32_t *__restrict__ off1, uint32_t G, uint32_t K,
+                                                         uint32_t F1, const uint32_t *__restrict__ grp_base,
+                                                         const uint32_t *__restrict__ off2, uint2 *__restrict__ out,
+                                                         uint32_t *err) {
+  constexpr uint32_t F2 = 1u << BL;
+  __shared__ tile::ScatterShared<BL> sh;
+  // the grid is an upper bound on the group count: index groups round-robin
+  // over the XCDs (block b runs on XCD b % 8) so that the real ones spread
+  // over all of them
+  const uint32_t w = blockIdx.x;
+  if (w >= grp_base[F1]) return;
+  uint32_t lo = 0, hi = F1;  // grp_base[lo] <= w < grp_base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (grp_base[mid] <= w) lo = mid; else hi = mid;
+  }
+  const uint32_t dh = lo;
+  const uint32_t J = grp_base[dh + 1] - grp_base[dh], j = w - grp_base[dh];
+  const uint32_t g0 = (uint32_t)((uint64_t)j * G / J), g1 = (uint32_t)((uint64_t)(j + 1) * G / J);
+  const uint64_t c0 = (uint64_t)dh * G + g0, c1 = (uint64_t)dh * G + g1;
+  const uint64_t beg = off1[c0 * K];
+  const uint64_t end = c1 == (uint64_t)F1 * G ? n : off1[c1 * K];
+  for (uint32_t d = threadIdx.x; d < F2; d += kFastNT) {
+    sh.cursor[d] = off2[((uint64_t)dh * F2 + d) * G + g0];
+    sh.cnt[d] = 0;
+  }
+  tile::scatter_stripe<BL>(sh, in, beg, end, n, 0, 0u, out, err);
+}
+
+__global__ void k_msd_starts(const uint32_t *__restrict__ off2, uint32_t G, uint32_t FT, uint64_t n,
+                             uint32_t *__restrict__ part_start) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < FT) part_start[i] = off2[(uint64_t)i * G];
+  if (i == FT) part_start[FT] = (uint32_t)n;
+}
+
+struct Msd2Layout {
+  size_t o_c1, o_c2, o_part, o_grp, o_scan, total;
+  uint64_t m1, m2, mp;
+};
+
+Msd2Layout msd2_layout(const PartPassPlan &p1, int bl) {
+  Msd2Layout L{};
+  const uint64_t F1 = 1ull << p1.bits;
+  L.m1 = (uint64_t)p1.stripes * p1.hist_split * F1;
+  L.m2 = (F1 << bl) * p1.stripes;
+  L.mp = L.m2 * p1.hist_split;
+  auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+  L.o_c1 = 0;
+  L.o_c2 = L.o_c1 + al(L.m1 * 4);
+  L.o_part = L.o_c2 + al(L.m2 * 4);
+  L.o_grp = L.o_part + al(L.mp * 4);
+  L.o_scan = L.o_grp + al((F1 + 1) * 4);
+  L.total = L.o_scan + std::max({scan_scratch_bytes_u32(L.m1), scan_scratch_bytes_u32(L.m2),
+                                 scan_scratch_bytes_u32(F1 + 1)});
+  return L;
+}
+
+template <int BH, int BL>
+hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
+                       uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
+  constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL);
+  const Msd2Layout L = msd2_layout(p1, BL);
+  char *sc = (char *)scratch;
+  uint32_t *c1 = (uint32_t *)(sc + L.o_c1), *c2 = (uint32_t *)(sc + L.o_c2), *grp = (uint32_t *)(sc + L.o_grp);
+  uint32_t *part = (uint32_t *)(sc + L.o_part);
+  void *scan_scratch = sc + L.o_scan;
+  const uint2 *vin = reinterpret_cast<const uint2 *>(in);
+  const StripeGeo geo{p1.n, p1.stripe_len, p1.stripes, p1.hist_split, nullptr, nullptr, 0};
+  hipError_t e;
+  EO_TIMED("part_hist", stream,
+           hipLaunchKernelGGL(k_msd_hist<BH + BL>, dim3(p1.stripes * p1.hist_split), dim3(kPartNT), 0, stream, vin,
+                              geo, BL, c1, part));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_msd_reduce, dim3((FT + 63) / 64, (p1.stripes + 63) / 64), dim3(256), 0, stream, part,
+                     p1.stripes, p1.hist_split, FT, c2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32(c1, c1, L.m1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
+  if ((e = scan_u32(c2, c2, L.m2, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
+  EO_TIMED("part_scatter", stream,
+           hipLaunchKernelGGL(k_part_scatter_fast<BH>, dim3(p1.stripes), dim3(kFastNT), 0, stream, vin, geo, BL, 0u,
+                              c1, reinterpret_cast<uint2 *>(tmp), err));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_msd_groups, dim3((F1 + 1 + 255) / 256), dim3(256), 0, stream, c1, p1.stripes, p1.hist_split, F1,
+                     p1.n, group_len, grp);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = scan_u32(grp, grp, F1 + 1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
+  const uint64_t max_groups = F1 + p1.n / group_len + 1;
+  const uint32_t grid = (uint32_t)((max_groups + kXcds - 1) / kXcds * kXcds);
+  EO_TIMED("seg_scatter", stream,
+           hipLaunchKernelGGL(k_msd_scatter<BL>, dim3(grid), dim3(kFastNT), 0, stream,
+                              reinterpret_cast<const uint2 *>(tmp), p1.n, c1, p1.stripes, p1.hist_split, F1, grp, c2,
+                              reinterpret_cast<uint2 *>(out), err));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_msd_starts, dim3((FT + 1 + 255) / 256), dim3(256), 0, stream, c2, p1.stripes, FT, p1.n,
+                     part_start);
+  return hipGetLastError();
+}
+
 uint32_t tile_of_bits(int bits) {
   switch (bits) {
 #define EO_TILE_CASE(B) \
@@ -466,6 +643,46 @@ hipError_t partition_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple
     EO_SEG_CASE(1) EO_SEG_CASE(2) EO_SEG_CASE(3) EO_SEG_CASE(4) EO_SEG_CASE(5) EO_SEG_CASE(6)
     EO_SEG_CASE(7) EO_SEG_CASE(8) EO_SEG_CASE(9) EO_SEG_CASE(10) EO_SEG_CASE(11)
 #undef EO_SEG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+
+bool msd2_supported(int bits_hi, int bits_lo) {
+  return bits_hi >= 1 && bits_lo >= 1 && bits_hi <= 7 && bits_lo <= 7 && bits_hi + bits_lo <= kMsd2MaxBits;
+}
+
+static PartPassPlan msd2_pass1(uint64_t n, int bh, int bl, int cus) {
+  return plan_partition_pass(n, bh, bl, 0u, cus, true);
+}
+
+uint64_t msd2_group_len(uint64_t n, int cus) {
+  // about one wave of second-pass workgroups (one per CU)
+  const uint64_t len = n / (uint64_t)std::max(cus, 8);
+  return std::max<uint64_t>(len, (uint64_t)kFastTile);
+}
+
+size_t msd2_scratch_bytes(uint64_t n, int bh, int bl, int cus) {
+  return msd2_layout(msd2_pass1(n, bh, bl, cus), bl).total;
+}
+
+hipError_t partition_msd2(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
+                          uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream) {
+  if (!msd2_supported(bh, bl)) return hipErrorInvalidValue;
+  if (((uintptr_t)in & 15) || ((uintptr_t)tmp & 15) || ((uintptr_t)out & 7)) return hipErrorInvalidValue;
+  const PartPassPlan p1 = msd2_pass1(n, bh, bl, cus);
+  const uint64_t L = msd2_group_len(n, cus);
+  switch (bh * 16 + bl) {
+#define EO_MSD_CASE(BH, BL) \
+  case BH * 16 + BL:        \
+    return launch_msd2<BH, BL>(p1, L, in, tmp, out, part_start, scratch, err, stream);
+#define EO_MSD_ROW(BH) \
+  EO_MSD_CASE(BH, 1) EO_MSD_CASE(BH, 2) EO_MSD_CASE(BH, 3) EO_MSD_CASE(BH, 4) EO_MSD_CASE(BH, 5) EO_MSD_CASE(BH, 6) \
+  EO_MSD_CASE(BH, 7)
+    EO_MSD_ROW(1) EO_MSD_ROW(2) EO_MSD_ROW(3) EO_MSD_ROW(4) EO_MSD_ROW(5) EO_MSD_ROW(6) EO_MSD_ROW(7)
+#undef EO_MSD_ROW
+#undef EO_MSD_CASE
     default:
       return hipErrorInvalidValue;
   }

@@ -7,6 +7,7 @@ identical count for joins.
 import numpy as np
 import pytest
 
+import eating_one_amd as eo
 import oracle
 from eating_one_amd import CapacityError, gen
 from common import (brute_join, eo_to_ref, join_count_closed_form, join_ori_doc, libc_rand_table, ref_to_eo,
@@ -159,6 +160,19 @@ def test_hash_join_c2_shape_small(ctx, theta, sel):
     R, S = gen.c2(seed=17, nR=1 << 18, nS=1 << 19, theta=theta, selectivity=sel)
     got = ctx.hash_join(R, S)
     want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert same_multiset(got, want)
+
+
+@pytest.mark.parametrize("bits,theta,sel", [(13, 0.0, 1.0), (13, 0.9, 1.0), (14, 0.5, 0.1), (9, 0.0, 1.0), (5, 0.9, 1.0)])
+def test_hash_join_two_pass_shapes(ctx, bits, theta, sel):
+    """Two MSD passes from one combined-digit histogram, at C2's shape scaled down."""
+    R, S = gen.c2(seed=bits * 7 + 3, nR=1 << 21, nS=1 << 22, theta=theta, selectivity=sel)
+    st = eo.JoinStats()
+    got = ctx.hash_join(R, S, radix_bits=bits, passes=2, stats=st)
+    assert st.passes == 2 and st.radix_bits == bits
+    assert "seg_hist" not in st.kernel_table()
+    want = oracle.hash_join(R, S, nparts=1 << 15)
+    assert len(got) == len(want)
     assert same_multiset(got, want)
 
 
