@@ -13,6 +13,29 @@ constexpr int kXcds = 8;  // MI355X: 8 accelerator complex dies, WGs dealt round
 
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// Streaming (nontemporal) vector stores and loads for data written or read
+// once: outputs of the scatters and the probe, the copy kernel.
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_stream(uint2 *p, uint2 v) {
+  u32x2_t x;
+  x.x = v.x;
+  x.y = v.y;
+  __builtin_nontemporal_store(x, reinterpret_cast<u32x2_t *>(p));
+}
+__device__ __forceinline__ void st_stream(uint4 *p, uint4 v) {
+  u32x4_t x;
+  x.x = v.x;
+  x.y = v.y;
+  x.z = v.z;
+  x.w = v.w;
+  __builtin_nontemporal_store(x, reinterpret_cast<u32x4_t *>(p));
+}
+__device__ __forceinline__ uint4 ld_stream(const uint4 *p) {
+  const u32x4_t x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(p));
+  return make_uint4(x.x, x.y, x.z, x.w);
+}
+
 // ---- DPP wave scans ---------------------------------------------------------
 // GFX9 DPP controls: row_shr:n = 0x110+n (within 16-lane rows),
 // row_bcast:15 = 0x142 (lane 15 of row r -> all lanes of row r+1),

@@ -561,8 +561,8 @@ PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_m
   cus = std::max(cus, kXcds);
   const uint64_t tile = fast ? (uint64_t)kFastTile : tile_of_bits(bits);
   const uint64_t tiles = std::max<uint64_t>(1, (n + tile - 1) / tile);
-  // fast: one 152 KB-LDS workgroup per CU, one stripe each; stable: two per CU.
-  uint64_t g = std::min<uint64_t>(tiles, (uint64_t)cus * (fast ? 1 : 2));
+  // fast: kFastWgPerCu 64 KB-LDS workgroups per CU, one stripe each; stable: two per CU.
+  uint64_t g = std::min<uint64_t>(tiles, (uint64_t)cus * (fast ? tile::kFastWgPerCu : 2));
   g = (g + kXcds - 1) / kXcds * kXcds;
   uint64_t len = (n + g - 1) / g;
   len = std::max<uint64_t>(tile, (len + 1) & ~1ull);
@@ -658,8 +658,8 @@ static PartPassPlan msd2_pass1(uint64_t n, int bh, int bl, int cus) {
 }
 
 uint64_t msd2_group_len(uint64_t n, int cus) {
-  // about one wave of second-pass workgroups (one per CU)
-  const uint64_t len = n / (uint64_t)std::max(cus, 8);
+  // about one wave of second-pass workgroups (kFastWgPerCu per CU)
+  const uint64_t len = n / ((uint64_t)std::max(cus, 8) * tile::kFastWgPerCu);
   return std::max<uint64_t>(len, (uint64_t)kFastTile);
 }
 

@@ -141,13 +141,13 @@ __device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restr
       if (m[u] == 0) continue;
       uint64_t pos = base + sh.wtot[u * NW + w] + wex[u];
       if (m[u] == 1) {
-        if (pos < out_cap) out[pos] = make_uint2(fp[u], cur[u].y);
+        if (pos < out_cap) st_stream(out + pos, make_uint2(fp[u], cur[u].y));
       } else {
         const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
         const uint32_t e1 = sh.off[h + 1];
         for (uint32_t e = sh.off[h]; e < e1; ++e) {
           if (sh.keys[e] == cur[u].x) {
-            if (pos < out_cap) out[pos] = make_uint2(sh.pays[e], cur[u].y);
+            if (pos < out_cap) st_stream(out + pos, make_uint2(sh.pays[e], cur[u].y));
             ++pos;
           }
         }
@@ -167,7 +167,8 @@ __device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restr
 // next tile's loads are issued before the flush so reads overlap writes.
 constexpr int kFastNT = 1024;
 constexpr int kFastPairs = 8;                         // uint4 loads (2 tuples) per thread per tile
-constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples
+constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples: a 128 KB staging tile
+constexpr int kFastWgPerCu = 1;                       // one workgroup per CU (LDS)
 
 __device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shift, uint32_t mask) {
   return ((key ^ xm) >> shift) & mask;
@@ -201,19 +202,21 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
   const uint64_t nfull = (end - beg) / 2;              // complete 16-byte pairs
   const uint64_t npairs = (end - beg + 1) / 2;         // incl. a trailing single tuple
 
+  // Tiles are read with 16-byte loads only: a load of another width into the
+  // same registers would make the compiler wait for every earlier load (and
+  // store) before each prefetch load.  The stripe's odd last tuple, if any, is
+  // read once here and substituted when its tile is consumed.
+  const bool odd_tail = nfull < npairs;
+  uint2 last = make_uint2(0u, 0u);
+  if (odd_tail) last = in[beg + 2 * nfull];
   uint4 cur[kFastPairs];
   auto load_tile = [&](uint64_t t0pair, uint4 *dst) {
 #pragma unroll
     for (int j = 0; j < kFastPairs; ++j) {
       const uint64_t pi = t0pair + (uint64_t)j * kFastNT + threadIdx.x;
-      if (pi < nfull) {
-        dst[j] = in4[pi];
-      } else if (pi < npairs) {
-        const uint2 last = in[beg + 2 * pi];
-        dst[j] = make_uint4(last.x, last.y, 0u, 0u);
-      } else {
-        dst[j] = make_uint4(0u, 0u, 0u, 0u);
-      }
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (pi < nfull) v = in4[pi];
+      dst[j] = v;
     }
   };
   load_tile(0, cur);
@@ -222,6 +225,11 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
   for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
     const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples read in this tile
     const uint32_t tskip = t0 == 0 ? (uint32_t)skip : 0u;                            // lead-in tuple, not ours
+    if (odd_tail) {
+#pragma unroll
+      for (int j = 0; j < kFastPairs; ++j)
+        if (t0 + (uint64_t)j * kFastNT + threadIdx.x == nfull) cur[j] = make_uint4(last.x, last.y, 0u, 0u);
+    }
     uint32_t rk[2 * kFastPairs];
 #pragma unroll
     for (int j = 0; j < kFastPairs; ++j) {

@@ -13,11 +13,12 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, u
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
-    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-    dst[i] = a;
-    dst[i + stride] = b;
-    dst[i + 2 * stride] = c;
-    dst[i + 3 * stride] = d;
+    const uint4 a = ld_stream(src + i), b = ld_stream(src + i + stride), c = ld_stream(src + i + 2 * stride),
+                d = ld_stream(src + i + 3 * stride);
+    st_stream(dst + i, a);
+    st_stream(dst + i + stride, b);
+    st_stream(dst + i + 2 * stride, c);
+    st_stream(dst + i + 3 * stride, d);
   }
   for (; i < n; i += stride) dst[i] = src[i];
 }
@@ -50,7 +51,7 @@ void timer_end(hipStream_t s) {
 
 hipError_t copy_bytes(const void *src, void *dst, uint64_t bytes, int cus, hipStream_t stream) {
   const uint64_t n = bytes / 16;
-  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)cus * 8));
+  const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, (uint64_t)cus * 32));
   hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, stream, (const uint4 *)src, (uint4 *)dst, n);
   return hipGetLastError();
 }
