@@ -303,7 +303,7 @@ __device__ __forceinline__ void hist_stripe(uint32_t *hist, const uint2 *__restr
       for (; i + 3 * NT < npairs; i += 4 * NT) {
         uint4 q[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = in4[i + u * NT];
+        for (int u = 0; u < 4; ++u) q[u] = ld_stream(in4 + i + u * NT);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           atomicAdd(&hist[digit_of(q[u].x, xm, shift, F - 1)], 1u);
