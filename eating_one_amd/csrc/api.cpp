@@ -668,6 +668,42 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   PhaseClock clk(c, s, stats != nullptr);
   ScopedTimer tmr(c, stats);
   clk.mark();
+  if (!(o && (o->flags & EO_F_DETERMINISTIC))) {
+    // one fused pass; with out == NULL (or a too small buffer) it counts every
+    // match and writes only the pairs that fit
+    eo_pair *dout = out;
+    const uint64_t cap_dev = out ? out_cap : 0;
+    bool staged = false;
+    if (host && out && out_cap) {
+      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+      staged = true;
+    }
+    EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
+    hipError_t e = nlj_fused(np, dR, dS, small, out ? dout : nullptr, cap_dev, s);
+    clk.mark();
+    uint64_t total = 0;
+    eo_status st = EO_OK;
+    if (e == hipSuccess) {
+      st = read_u64(c, small, &total, 1, s);
+      if (st == EO_OK && staged && total && total <= out_cap)
+        e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    }
+    if (staged) (void)hipFreeAsync(dout, s);
+    if (e != hipSuccess) return hip_fail(e, "nlj");
+    if (st != EO_OK) return st;
+    EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+    st = check_device_errors(c, s);
+    *n_out = total;
+    if (stats) {
+      stats->n_out = total;
+      stats->ms_count = out ? 0.0 : clk.ms(0, 1);
+      stats->ms_write = out ? clk.ms(0, 1) : 0.0;
+      stats->ms_total = clk.ms(0, 1);
+      tmr.finish();
+    }
+    if (st != EO_OK) return st;
+    return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
+  }
   EO_TRY_HIP(nlj_count(np, dR, dS, cnt, s), "nlj count");
   EO_TRY_HIP(scan_u64(cnt, cnt, slots, small, A + o_scr, c->d_err, s), "nlj scan");
   clk.mark();

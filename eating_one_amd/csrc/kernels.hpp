@@ -212,6 +212,11 @@ NljPlan plan_nlj(uint64_t nR, uint64_t nS);
 uint64_t nlj_count_slots(const NljPlan &p);  // per-wave count entries
 hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *wave_count,
                      hipStream_t stream);
+// One pass: every match counted into *cursor (zero on entry); with out != NULL
+// the pairs are written at positions reserved per wave (order unspecified,
+// positions >= out_cap counted only).
+hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
+                     uint64_t out_cap, hipStream_t stream);
 hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off,
                      eo_pair *out, uint64_t out_cap, hipStream_t stream);
 

@@ -123,6 +123,98 @@ __global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uin
     nlj_tile<false>(rkey, rpay, rn, sr, wave_count + slot, 0, nullptr, 0);
 }
 
+// Fused single pass (default order): every (R key group, S register) compare
+// happens once.  A wave records the R groups whose OR-accumulated ballot shows
+// a hit in an LDS list (at most 512 groups per R tile), counts their matches
+// exactly, reserves its output range with one 64-bit atomic, and then
+// re-scans only the recorded groups to write the pairs.  WRITE = false only
+// counts (one atomic per wave).
+template <bool WRITE>
+__global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ R, uint64_t nR,
+                                                      const uint2 *__restrict__ S, uint64_t nS, uint32_t s_tiles,
+                                                      uint64_t *cursor, uint2 *__restrict__ out, uint64_t out_cap) {
+  // payloads are not staged: only the recorded hit groups need them (read from
+  // global memory in the write phase), which keeps LDS at 20 KB per workgroup
+  __shared__ __attribute__((aligned(16))) int32_t rkey[kNljRTile];
+  __shared__ uint16_t hits[WRITE ? kNljNW : 1][WRITE ? kNljRTile / kNljGroup : 1];
+  const uint32_t b = blockIdx.x;
+  const uint32_t st = b % s_tiles, rt = b / s_tiles;
+  const uint64_t r0 = (uint64_t)rt * kNljRTile;
+  const uint32_t rn = (uint32_t)std::min<uint64_t>(kNljRTile, nR - r0);
+  for (uint32_t i = threadIdx.x; i < rn; i += kNljNT) rkey[i] = (int32_t)R[r0 + i].x;
+  SRegs sr;
+  const uint64_t s0 = (uint64_t)st * kNljSTile;
+  load_s(S, nS, s0, sr, false);
+  __syncthreads();
+  const int w = threadIdx.x / kWave;
+  const uint64_t lt = lanemask_lt();
+  const uint32_t ngroups = (rn + kNljGroup - 1) / kNljGroup;
+  uint64_t total = 0;
+  uint32_t nh = 0;
+  for (uint32_t gi = 0; gi < ngroups; ++gi) {
+    const uint32_t g = gi * kNljGroup;
+    const uint32_t gn = rn - g < (uint32_t)kNljGroup ? rn - g : (uint32_t)kNljGroup;  // wave-uniform
+    int32_t rk[kNljGroup];
+    if (gn == kNljGroup) {
+      const int4 *q = reinterpret_cast<const int4 *>(rkey + g);
+      const int4 a = q[0], c = q[1];
+      rk[0] = a.x; rk[1] = a.y; rk[2] = a.z; rk[3] = a.w;
+      rk[4] = c.x; rk[5] = c.y; rk[6] = c.z; rk[7] = c.w;
+    } else {
+#pragma unroll
+      for (int q2 = 0; q2 < kNljGroup; ++q2) rk[q2] = rkey[g + (q2 < (int)gn ? q2 : 0)];
+    }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int q2 = 0; q2 < kNljGroup; ++q2)
+#pragma unroll
+      for (int k = 0; k < kNljK; ++k) acc |= __ballot(sr.key[k] == rk[q2]);
+    if (acc) {
+      uint64_t cnt = 0;
+#pragma unroll
+      for (int q2 = 0; q2 < kNljGroup; ++q2) {
+        if (q2 >= (int)gn) break;
+#pragma unroll
+        for (int k = 0; k < kNljK; ++k) cnt += __popcll(__ballot(sr.key[k] == rk[q2]) & sr.valid[k]);
+      }
+      if (cnt) {
+        total += cnt;
+        if (WRITE) {
+          if (lane_id() == 0) hits[w][nh] = (uint16_t)gi;
+          ++nh;
+        }
+      }
+    }
+  }
+  uint64_t base = 0;
+  if (total) {
+    if (lane_id() == 0) base = atomic_add_agent(cursor, total);
+    base = read_lane(base, 0);
+  }
+  if (!WRITE || total == 0) return;
+  uint64_t running = 0;
+  for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t g = (uint32_t)hits[w][h] * kNljGroup;
+    const uint32_t gn = rn - g < (uint32_t)kNljGroup ? rn - g : (uint32_t)kNljGroup;
+    for (uint32_t q2 = 0; q2 < gn; ++q2) {
+      const int32_t r = rkey[g + q2];
+#pragma unroll
+      for (int k = 0; k < kNljK; ++k) {
+        const bool eq = sr.key[k] == r && ((sr.valid[k] >> lane_id()) & 1);
+        const uint64_t m = __ballot(eq);
+        if (m) {
+          if (eq) {
+            const uint64_t pos = base + running + __popcll(m & lt);
+            const uint64_t si = s0 + (uint64_t)w * kWave * kNljK + (uint64_t)k * kWave + lane_id();
+            if (pos < out_cap) out[pos] = make_uint2(R[r0 + g + q2].y, S[si].y);
+          }
+          running += __popcll(m);
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 NljPlan plan_nlj(uint64_t nR, uint64_t nS) {
@@ -145,6 +237,24 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   EO_TIMED("nlj_count", stream, hipLaunchKernelGGL(k_nlj<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
                      p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0));
+  return hipGetLastError();
+}
+
+hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
+                     uint64_t out_cap, hipStream_t stream) {
+  const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
+  if (grid == 0) return hipSuccess;
+  if (grid >= (1ull << 31)) return hipErrorInvalidValue;
+  if (out)
+    EO_TIMED("nlj_fused", stream,
+             hipLaunchKernelGGL(k_nlj_fused<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream,
+                                reinterpret_cast<const uint2 *>(R), p.nR, reinterpret_cast<const uint2 *>(S), p.nS,
+                                p.s_tiles, cursor, reinterpret_cast<uint2 *>(out), out_cap));
+  else
+    EO_TIMED("nlj_count", stream,
+             hipLaunchKernelGGL(k_nlj_fused<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream,
+                                reinterpret_cast<const uint2 *>(R), p.nR, reinterpret_cast<const uint2 *>(S), p.nS,
+                                p.s_tiles, cursor, nullptr, 0));
   return hipGetLastError();
 }
 

@@ -186,6 +186,32 @@ def test_nlj_small_and_edges(ctx):
     assert same_multiset(ctx.nlj_join(R, S), oracle.nlj_join(R, S))
 
 
+def test_nlj_fused_modes_and_dense_hits(ctx):
+    """Fused pass vs deterministic count/scan/write; every R group hitting; capacity."""
+    R, S = gen.c3(seed=12, nR=9000, nS=7000, key_range=300)
+    want = oracle.nlj_join(R, S)
+    fused = ctx.nlj_join(R, S)
+    det1 = ctx.nlj_join(R, S, deterministic=True)
+    det2 = ctx.nlj_join(R, S, deterministic=True)
+    assert same_multiset(fused, want) and same_multiset(det1, want)
+    assert np.array_equal(det1, det2)
+    assert ctx.nlj_join(R, S, count_only=True) == len(want)
+    # one key everywhere: every group of every wave records a hit
+    R1 = np.stack([np.full(4100, 7), np.arange(4100)], 1).astype(np.int32)
+    S1 = np.stack([np.full(700, 7), np.arange(700)], 1).astype(np.int32)
+    got = ctx.nlj_join(R1, S1)
+    assert len(got) == 4100 * 700
+    assert same_multiset(got, brute_join(R1, S1))
+    import ctypes
+    from eating_one_amd import _abi
+    out = np.empty((len(want) - 1, 2), dtype=np.int32)
+    cnt = ctypes.c_uint64()
+    o = ctx.opts()
+    st = ctx.lib.eo_nlj_join(ctx.handle, R.ctypes.data, len(R), S.ctypes.data, len(S), out.ctypes.data,
+                             len(want) - 1, ctypes.byref(cnt), ctypes.byref(o))
+    assert st == _abi.EO_ECAPACITY and cnt.value == len(want)
+
+
 def test_nlj_c3_full(ctx):
     R, S = gen.c3()
     got = ctx.nlj_join(R, S)
