@@ -285,8 +285,43 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   PhaseClock clk(c, s, stats != nullptr);
   ScopedTimer tmr(c, stats);
   clk.mark();
+  if (!(o && (o->flags & EO_F_DETERMINISTIC))) {
+    // one lookup pass; each tile reserves its output range with one atomic
+    // (with out == NULL, or a too small buffer, it counts every match)
+    eo_pair *dout = out;
+    bool staged = false;
+    if (host && out && out_cap) {
+      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+      staged = true;
+    }
+    EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
+    hipError_t e = index_probe(kind, out != nullptr, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS,
+                               nS, nullptr, small, out ? dout : nullptr, out ? out_cap : 0, nullptr, s);
+    clk.mark();
+    uint64_t total = 0;
+    if (e == hipSuccess) {
+      st = read_u64(c, small, &total, 1, s);
+      if (st == EO_OK && staged && total && total <= out_cap)
+        e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    }
+    if (staged) (void)hipFreeAsync(dout, s);
+    if (e != hipSuccess) return hip_fail(e, "index join");
+    if (st != EO_OK) return st;
+    EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+    st = check_device_errors(c, s);
+    *n_out = total;
+    if (stats) {
+      stats->n_out = total;
+      stats->ms_count = out ? 0.0 : clk.ms(0, 1);
+      stats->ms_write = out ? clk.ms(0, 1) : 0.0;
+      stats->ms_total = clk.ms(0, 1);
+      tmr.finish();
+    }
+    if (st != EO_OK) return st;
+    return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
+  }
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt, nullptr,
-                         0, nullptr, s),
+                         nullptr, 0, nullptr, s),
              "index count");
   EO_TRY_HIP(scan_u64(cnt, cnt, tiles, small, A + o_scr, c->d_err, s), "index scan");
   clk.mark();
@@ -302,7 +337,7 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     eo_pair *dout = out;
     if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * sizeof(eo_pair), s), "output staging");
     hipError_t e = index_probe(kind, true, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt,
-                               dout, total, nullptr, s);
+                               nullptr, dout, total, nullptr, s);
     clk.mark();
     if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
     if (host) (void)hipFreeAsync(dout, s);
@@ -345,7 +380,7 @@ eo_status eo_index_lookup(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint
   PhaseClock clk(c, s, o && o->stats);
   ScopedTimer tmr(c, o ? o->stats : nullptr);
   clk.mark();
-  EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, nullptr,
+  EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, nullptr, nullptr,
                          nullptr, 0, dpos, s),
              "index lookup");
   clk.mark();

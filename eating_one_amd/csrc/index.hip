@@ -264,11 +264,13 @@ __device__ __forceinline__ void locate(const IndexView &ix, const int32_t *kk, c
 
 template <int KIND, bool WRITE>
 __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint2 *__restrict__ S, uint64_t nS,
-                                                        uint64_t *__restrict__ tile_cnt, uint2 *__restrict__ out,
-                                                        uint64_t out_cap, uint64_t *__restrict__ lb_out) {
+                                                        uint64_t *__restrict__ tile_cnt, uint64_t *cursor,
+                                                        uint2 *__restrict__ out, uint64_t out_cap,
+                                                        uint64_t *__restrict__ lb_out) {
   __shared__ int32_t kk[KIND == 2 ? kKnots + 1 : 1];
   __shared__ float kp[KIND == 2 ? kKnots + 1 : 1];
   __shared__ uint64_t red[kIdxNT / kWave + 1];
+  __shared__ uint64_t base_sh;
   if (KIND == 2) {
     for (int i = threadIdx.x; i <= kKnots; i += kIdxNT) {
       kk[i] = ix.knot_key[i];
@@ -305,11 +307,23 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
   }
   if (!WRITE) {
     const uint64_t tot = block_sum<kIdxNT, uint64_t>((uint64_t)total, red);
-    if (threadIdx.x == 0 && tile_cnt) tile_cnt[t] = tot;
+    if (threadIdx.x == 0) {
+      if (tile_cnt) tile_cnt[t] = tot;
+      else if (cursor && tot) atomic_add_agent(cursor, tot);
+    }
     return;
   }
+  // output offsets: scanned tile counts, or a range reserved with one atomic
   uint64_t tile_total;
-  uint64_t pos = tile_cnt[t] + block_excl_scan<kIdxNT, uint64_t>((uint64_t)total, red, &tile_total);
+  const uint64_t excl = block_excl_scan<kIdxNT, uint64_t>((uint64_t)total, red, &tile_total);
+  uint64_t pos;
+  if (tile_cnt) {
+    pos = tile_cnt[t] + excl;
+  } else {
+    if (threadIdx.x == 0) base_sh = tile_total ? atomic_add_agent(cursor, tile_total) : 0ull;
+    __syncthreads();
+    pos = base_sh + excl;
+  }
 #pragma unroll
   for (int u = 0; u < kIdxU; ++u) {
     if (m[u] == 0) continue;
@@ -325,19 +339,19 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
 
 template <int KIND>
 hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint64_t nS, uint64_t *tile_cnt,
-                        eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream) {
+                        uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream) {
   const uint64_t tiles = (nS + kIdxTile - 1) / kIdxTile;
   if (tiles == 0) return hipSuccess;
   if (tiles >= (1ull << 31)) return hipErrorInvalidValue;
   if (write)
     EO_TIMED("index_write", stream,
              hipLaunchKernelGGL((k_index_probe<KIND, true>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
-                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, reinterpret_cast<uint2 *>(out),
-                                out_cap, lb_out));
+                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor,
+                                reinterpret_cast<uint2 *>(out), out_cap, lb_out));
   else
     EO_TIMED("index_probe", stream,
              hipLaunchKernelGGL((k_index_probe<KIND, false>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
-                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, nullptr, 0, lb_out));
+                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor, nullptr, 0, lb_out));
   return hipGetLastError();
 }
 
@@ -389,7 +403,8 @@ hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *
 
 hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
-                       uint64_t *tile_cnt, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream) {
+                       uint64_t *tile_cnt, uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out,
+                       hipStream_t stream) {
   IndexView ix{};
   ix.kind = kind;
   ix.n = n;
@@ -404,9 +419,9 @@ hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint3
     ix.leaves = (const LeafModel *)(mp + ((kKnots + 1) * (sizeof(int32_t) + sizeof(float)) + 63) / 64 * 64);
   }
   switch (kind) {
-    case 0: return launch_probe<0>(ix, write, S, nS, tile_cnt, out, out_cap, lb_out, stream);
-    case 1: return launch_probe<1>(ix, write, S, nS, tile_cnt, out, out_cap, lb_out, stream);
-    case 2: return launch_probe<2>(ix, write, S, nS, tile_cnt, out, out_cap, lb_out, stream);
+    case 0: return launch_probe<0>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, stream);
+    case 1: return launch_probe<1>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, stream);
+    case 2: return launch_probe<2>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, stream);
     default: return hipErrorInvalidValue;
   }
 }

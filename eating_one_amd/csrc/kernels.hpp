@@ -139,12 +139,16 @@ hipError_t hidx_build(const eo_tuple *R, uint64_t n, int log2nb, uint32_t *keys,
 hipError_t sidx_split(const eo_tuple *sorted, uint64_t n, uint32_t *keys, uint32_t *pays, hipStream_t stream);
 // Fits the learned model over sorted keys; leaf_scratch: n u16.
 hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *leaf_scratch, hipStream_t stream);
-// Count (write = false: per-tile counts into tile_cnt) or write (write = true:
-// tile_cnt holds exclusive-scanned tile offsets).  lb_out (optional, nS u64):
-// the index position each probe starts from (lower bound for sorted kinds).
+// Count (write = false: per-tile counts into tile_cnt, or, with tile_cnt ==
+// NULL, the total added into *cursor) or write (write = true: tile_cnt holds
+// exclusive-scanned tile offsets, or, with tile_cnt == NULL, every tile
+// reserves its range from *cursor and positions >= out_cap are counted only).
+// lb_out (optional, nS u64): the index position each probe starts from
+// (lower bound for sorted kinds).
 hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
-                       uint64_t *tile_cnt, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream);
+                       uint64_t *tile_cnt, uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out,
+                       hipStream_t stream);
 
 // Two-pass MSD partition (high digit bits_hi over the whole array, then the
 // low digit bits_lo inside each first-pass partition) from a single

@@ -104,3 +104,26 @@ def test_learned_index_c4_shape(ctx):
 def test_learned_index_duplicates_and_uniform(ctx):
     R, S = gen.uniform(5, 1 << 20, 1 << 20, 1 << 18)
     _judge_learned(ctx, R, S)
+
+
+@pytest.mark.parametrize("kind", ["hash", "sorted", "learned"])
+def test_index_join_fused_and_deterministic(ctx, kind):
+    """Default one-pass index join (atomic output reservation) vs count/scan/write."""
+    import ctypes
+    from eating_one_amd import _abi
+    R, S = gen.c3(seed=77, nR=50000, nS=80000, key_range=20000)
+    ix = ctx.hidx_build(R) if kind == "hash" else ctx.sidx_build(R, learned=(kind == "learned"))
+    want = oracle.hash_join(R, S, nparts=1 << 10)
+    fused = ix.join(S)
+    d1 = ix.join(S, deterministic=True)
+    d2 = ix.join(S, deterministic=True)
+    assert same_multiset(fused, want) and same_multiset(d1, want)
+    assert np.array_equal(d1, d2)
+    assert ix.join(S, count_only=True) == len(want)
+    out = np.empty((len(want) - 5, 2), dtype=np.int32)
+    n = ctypes.c_uint64()
+    o = ctx.opts()
+    st = ctx.lib.eo_index_join(ctx.handle, ix.handle, S.ctypes.data, len(S), out.ctypes.data, len(want) - 5,
+                               ctypes.byref(n), ctypes.byref(o))
+    assert st == _abi.EO_ECAPACITY and n.value == len(want)
+    ix.free()
