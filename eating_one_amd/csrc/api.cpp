@@ -510,7 +510,8 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   if (deterministic)
     EO_TRY_HIP(join_plan_count(jp, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s), "tile plan");
   else
-    EO_TRY_HIP(join_fused_plan(P, hp.s_span, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s),
+    EO_TRY_HIP(join_fused_plan(P, hp.s_span, join_fused_cap(hp.lds_cap), r_start, s_start, tile_start, small,
+                               A + o_jscr, c->d_err, s),
                "tile plan");
   uint64_t n_tiles = 0;
   st = read_u64(c, small, &n_tiles, 1, s);
@@ -539,7 +540,8 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     }
     EO_TRY_HIP(fill_u64(small + 1, 0, 1, s), "cursor reset");
     clk.mark();  // 2
-    hipError_t e = join_fused(P, hp.s_span, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, dout, cap_dev,
+    hipError_t e = join_fused(P, hp.s_span, join_fused_cap(hp.lds_cap), (uint32_t)n_tiles, Rp, r_start, Sp, s_start,
+                              tile_start, dout, cap_dev,
                               small + 1, s);
     clk.mark();  // 3
     if (e == hipSuccess) {

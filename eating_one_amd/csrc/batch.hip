@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
                                                  const uint32_t *__restrict__ pair_join,
                                                  const uint32_t *__restrict__ tile_start, uint32_t npairs,
                                                  uint32_t *queue_head) {
-  __shared__ tile::ProbeShared sh;
+  __shared__ tile::ProbeShared<tile::ProbeBig> sh;
   __shared__ uint32_t item[2];
   const uint32_t total = tile_start[npairs];
   for (;;) {
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
     ref.r_n = min((uint32_t)kFCap, rn - c * kFCap);
     ref.s_beg = sb + j * J.s_span;
     ref.s_n = min(J.s_span, sn - j * J.s_span);
-    tile::probe_tile(sh, J.R, J.S, ref, J.out, J.out_cap, J.cursor);
+    tile::probe_tile<tile::ProbeBig>(sh, J.R, J.S, ref, J.out, J.out_cap, J.cursor);
   }
 }
 

@@ -223,7 +223,7 @@ using namespace tile;
 
 __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_start, uint32_t P,
                                              const uint32_t *r_start, const uint32_t *s_start, uint32_t s_span,
-                                             SpanRef *ref) {
+                                             uint32_t cap, SpanRef *ref) {
   uint32_t lo = 0, hi = P;
   if (t >= tile_start[P]) return false;
   while (hi - lo > 1) {
@@ -234,29 +234,33 @@ __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_st
   const uint32_t local = t - tile_start[p];
   const uint32_t rb = r_start[p], rn = r_start[p + 1] - rb;
   const uint32_t sb = s_start[p], sn = s_start[p + 1] - sb;
-  const uint32_t nc = (rn + kFCap - 1) / kFCap;
+  const uint32_t nc = (rn + cap - 1) / cap;
   const uint32_t c = local % nc, j = local / nc;
-  ref->r_beg = rb + c * kFCap;
-  ref->r_n = min((uint32_t)kFCap, rn - c * kFCap);
+  ref->r_beg = rb + c * cap;
+  ref->r_n = min(cap, rn - c * cap);
   ref->s_beg = sb + j * s_span;
   ref->s_n = min(s_span, sn - j * s_span);
   return true;
 }
 
-__global__ __launch_bounds__(kFNT) void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
+// The half-size table variant is limited to 64 VGPRs so that two 16-wave
+// workgroups fit on a CU (8 waves per SIMD).
+template <typename C>
+__global__ __launch_bounds__(kFNT) __attribute__((amdgpu_waves_per_eu(C::Cap <= 8704 ? 8 : 1)))
+void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
                                                      const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
                                                      uint64_t *out_cursor) {
-  __shared__ tile::ProbeShared sh;
+  __shared__ tile::ProbeShared<C> sh;
   const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
   SpanRef ref;
-  if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, &ref)) return;
-  tile::probe_tile(sh, R, S, ref, out, out_cap, out_cursor);
+  if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+  tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor);
 }
 
 __global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,
-                             uint32_t *tiles) {
+                             uint32_t cap, uint32_t *tiles) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > P) return;
   if (p == P) {
@@ -265,7 +269,7 @@ __global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, u
   }
   const uint32_t rn = r_start[p + 1] - r_start[p];
   const uint32_t sn = s_start[p + 1] - s_start[p];
-  tiles[p] = (rn && sn) ? ((rn + kFCap - 1) / kFCap) * ((sn + s_span - 1) / s_span) : 0u;
+  tiles[p] = (rn && sn) ? ((rn + cap - 1) / cap) * ((sn + s_span - 1) / s_span) : 0u;
 }
 
 __global__ void k_plan_tiles(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t cap,
@@ -353,27 +357,38 @@ hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *
 
 namespace eo {
 
-uint32_t join_fused_cap() { return kFCap; }
+uint32_t join_fused_cap(uint32_t lds_cap) {
+  return lds_cap <= 8192 ? (uint32_t)tile::ProbeSmall::Cap : (uint32_t)tile::ProbeBig::Cap;
+}
 
-hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, const uint32_t *r_start, const uint32_t *s_start,
-                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
-                           hipStream_t stream) {
+hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
+                           const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
+                           uint32_t *err, hipStream_t stream) {
   const uint32_t n = parts + 1;
-  hipLaunchKernelGGL(k_plan_spans, dim3((n + 255) / 256), dim3(256), 0, stream, r_start, s_start, parts, s_span,
+  hipLaunchKernelGGL(k_plan_spans, dim3((n + 255) / 256), dim3(256), 0, stream, r_start, s_start, parts, s_span, cap,
                      tile_start);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return scan_u32(tile_start, tile_start, n, n_tiles_dev, scan_scratch, err, stream);
 }
 
-hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
-                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, eo_pair *out,
-                      uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream) {
+hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
+                      const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
-  EO_TIMED("probe", stream, hipLaunchKernelGGL(k_join_fused, dim3(grid), dim3(kFNT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
-                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, parts, s_span,
-                     reinterpret_cast<uint2 *>(out), out_cap, out_cursor));
+  const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
+  uint2 *vout = reinterpret_cast<uint2 *>(out);
+  if (cap == (uint32_t)tile::ProbeSmall::Cap)
+    EO_TIMED("probe", stream,
+             hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
+                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor));
+  else if (cap == (uint32_t)tile::ProbeBig::Cap)
+    EO_TIMED("probe", stream,
+             hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
+                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor));
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

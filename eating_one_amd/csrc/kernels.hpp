@@ -119,13 +119,15 @@ uint32_t join_max_lds_cap();
 // (tile_start: P+1 u32, *n_tiles_dev = total).  *out_cursor must be zero on
 // entry and holds the match count afterwards; pairs beyond out_cap are
 // counted, not written.
-uint32_t join_fused_cap();
-hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, const uint32_t *r_start, const uint32_t *s_start,
-                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
-                           hipStream_t stream);
-hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
-                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, eo_pair *out,
-                      uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream);
+// The probe's LDS table capacity for a planned lds_cap (8192 or less: the
+// half-size table, two workgroups per CU; otherwise the full-size table).
+uint32_t join_fused_cap(uint32_t lds_cap);
+hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
+                           const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
+                           uint32_t *err, hipStream_t stream);
+hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
+                      const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream);
 
 // ---- index.hip --------------------------------------------------------------
 // Resident indexes.  kind 0: hash (keys/pays grouped by bucket, off[NB+1]);

@@ -11,87 +11,100 @@ namespace eo {
 namespace tile {
 
 // ---- fused probe tile ----------------------------------------------------------
+// ProbeCfg<CAP, LOG2NB>: an LDS table of up to CAP R tuples in 2^LOG2NB hash
+// buckets, built and probed by a 1024-thread workgroup (8 S tuples per thread
+// per batch).  ProbeBig (17408 tuples, ~156 KB) runs one workgroup per CU;
+// ProbeSmall (8704 tuples, ~78 KB) lets two share a CU.
+template <int CAP, int LOG2NB>
+struct ProbeCfg {
+  static constexpr int NT = 1024;
+  static constexpr int Cap = CAP;
+  static constexpr int Log2NB = LOG2NB;
+  static constexpr int NB = 1 << LOG2NB;
+  static constexpr int SPT = 8;
+  static constexpr int Batch = NT * SPT;
+  static constexpr int RPT = (CAP + NT - 1) / NT;
+  static constexpr int BPT = NB / NT;
+};
+using ProbeBig = ProbeCfg<17408, 13>;   // a 16K-tuple partition plus binomial spread
+using ProbeSmall = ProbeCfg<8704, 12>;  // an 8K-tuple partition plus spread
 constexpr int kFNT = 1024;
-constexpr int kFCap = 17408;  // a 16K-tuple partition plus binomial spread
-constexpr int kFLog2NB = 13;
-constexpr int kFNB = 1 << kFLog2NB;
-constexpr int kFSPT = 8;
-constexpr int kFBatch = kFNT * kFSPT;
-constexpr int kFRPT = (kFCap + kFNT - 1) / kFNT;
-constexpr int kFBPT = kFNB / kFNT;
+constexpr int kFCap = ProbeBig::Cap;
 
 struct SpanRef {
   uint32_t r_beg, r_n, s_beg, s_n;
 };
 
+template <typename C>
 struct ProbeShared {
-  uint32_t keys[kFCap];
-  uint32_t pays[kFCap];
-  uint16_t off[kFNB + 2];
-  uint32_t scan_tmp[kFNT / kWave + 1];
-  uint32_t wtot[kFSPT * (kFNT / kWave)];
+  uint32_t keys[C::Cap];
+  uint32_t pays[C::Cap];
+  uint16_t off[C::NB + 2];
+  uint32_t scan_tmp[C::NT / kWave + 1];
+  uint32_t wtot[C::SPT * (C::NT / kWave)];
   uint64_t base_sh;
 };
 
-// Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= kFCap) and probes the S
+// Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= C::Cap) and probes the S
 // span S[ref.s_beg, +s_n) against it, writing (R payload, S payload) pairs at
 // positions reserved from *cursor; positions >= out_cap are counted only.
 // All threads of the workgroup call it; it leaves LDS free for reuse.
-__device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restrict__ R, const uint2 *__restrict__ S,
+template <typename C>
+__device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__restrict__ R, const uint2 *__restrict__ S,
                                            const SpanRef &ref, uint2 *__restrict__ out, uint64_t out_cap,
                                            uint64_t *cursor) {
-  constexpr int NW = kFNT / kWave;
+  constexpr int NW = C::NT / kWave;
   auto load_batch = [&](uint32_t b0, uint2 *dst) {
 #pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
-      const uint32_t i = b0 + u * kFNT + threadIdx.x;
+    for (int u = 0; u < C::SPT; ++u) {
+      const uint32_t i = b0 + u * C::NT + threadIdx.x;
       dst[u] = i < ref.s_n ? S[ref.s_beg + i] : make_uint2(0u, 0u);
     }
   };
-  uint2 cur[kFSPT];
+  uint2 cur[C::SPT];
   load_batch(0, cur);
 
   // ---- build: rank by LDS atomics on counters aliased onto sh.keys[] ----
-  uint32_t *cnt = sh.keys;  // kFNB u32 counters, dead once the offsets exist
-  for (int h = threadIdx.x; h < kFNB; h += kFNT) cnt[h] = 0;
+  uint32_t *cnt = sh.keys;  // C::NB u32 counters, dead once the offsets exist
+  for (int h = threadIdx.x; h < C::NB; h += C::NT) cnt[h] = 0;
   __syncthreads();
-  uint2 r[kFRPT];
-  uint32_t hh[kFRPT], rk[kFRPT];
+  uint2 r[C::RPT];
+  uint32_t hh[C::RPT], rk[C::RPT];
 #pragma unroll
-  for (int k = 0; k < kFRPT; ++k) {
-    const uint32_t i = k * kFNT + threadIdx.x;
+  for (int k = 0; k < C::RPT; ++k) {
+    const uint32_t i = k * C::NT + threadIdx.x;
     if (i < ref.r_n) r[k] = R[ref.r_beg + i];
   }
 #pragma unroll
-  for (int k = 0; k < kFRPT; ++k) {
-    const uint32_t i = k * kFNT + threadIdx.x;
+  for (int k = 0; k < C::RPT; ++k) {
+    const uint32_t i = k * C::NT + threadIdx.x;
     if (i < ref.r_n) {
-      hh[k] = bucket_hash((int32_t)r[k].x, kFLog2NB);
+      hh[k] = bucket_hash((int32_t)r[k].x, C::Log2NB);
       rk[k] = atomicAdd(&cnt[hh[k]], 1u);
     }
   }
   __syncthreads();
   {
-    uint32_t loc[kFBPT];
+    uint32_t loc[C::BPT];
     uint32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kFBPT; ++k) {
-      loc[k] = cnt[threadIdx.x * kFBPT + k];
+    for (int k = 0; k < C::BPT; ++k) {
+      loc[k] = cnt[threadIdx.x * C::BPT + k];
       sum += loc[k];
     }
     uint32_t tot;
-    uint32_t ex = block_excl_scan<kFNT, uint32_t>(sum, sh.scan_tmp, &tot);
+    uint32_t ex = block_excl_scan<C::NT, uint32_t>(sum, sh.scan_tmp, &tot);
 #pragma unroll
-    for (int k = 0; k < kFBPT; ++k) {
-      sh.off[threadIdx.x * kFBPT + k] = (uint16_t)ex;
+    for (int k = 0; k < C::BPT; ++k) {
+      sh.off[threadIdx.x * C::BPT + k] = (uint16_t)ex;
       ex += loc[k];
     }
-    if (threadIdx.x == 0) sh.off[kFNB] = (uint16_t)ref.r_n;  // r_n <= 16384 fits u16
+    if (threadIdx.x == 0) sh.off[C::NB] = (uint16_t)ref.r_n;  // r_n <= Cap fits u16
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < kFRPT; ++k) {
-    const uint32_t i = k * kFNT + threadIdx.x;
+  for (int k = 0; k < C::RPT; ++k) {
+    const uint32_t i = k * C::NT + threadIdx.x;
     if (i < ref.r_n) {
       const uint32_t pos = sh.off[hh[k]] + rk[k];
       sh.keys[pos] = r[k].x;
@@ -102,16 +115,16 @@ __device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restr
 
   // ---- probe the S span batch by batch ----
   const int w = threadIdx.x / kWave, lane = lane_id();
-  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += kFBatch) {
-    uint2 nxt[kFSPT];
-    if (b0 + kFBatch < ref.s_n) load_batch(b0 + kFBatch, nxt);
-    uint32_t m[kFSPT], fp[kFSPT];
+  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += C::Batch) {
+    uint2 nxt[C::SPT];
+    if (b0 + C::Batch < ref.s_n) load_batch(b0 + C::Batch, nxt);
+    uint32_t m[C::SPT], fp[C::SPT];
 #pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
+    for (int u = 0; u < C::SPT; ++u) {
       m[u] = 0;
       fp[u] = 0;
-      if (b0 + u * kFNT + threadIdx.x < ref.s_n) {
-        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
+      if (b0 + u * C::NT + threadIdx.x < ref.s_n) {
+        const uint32_t h = bucket_hash((int32_t)cur[u].x, C::Log2NB);
         const uint32_t e1 = sh.off[h + 1];
         for (uint32_t e = sh.off[h]; e < e1; ++e) {
           if (sh.keys[e] == cur[u].x) {
@@ -121,29 +134,29 @@ __device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restr
         }
       }
     }
-    uint32_t wex[kFSPT];
+    uint32_t wex[C::SPT];
 #pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
+    for (int u = 0; u < C::SPT; ++u) {
       const uint32_t inc = wave_incl_scan(m[u]);
       wex[u] = inc - m[u];
       if (lane == kWave - 1) sh.wtot[u * NW + w] = inc;
     }
     __syncthreads();
     uint32_t batch_total;
-    const uint32_t v = threadIdx.x < kFSPT * NW ? sh.wtot[threadIdx.x] : 0u;
-    const uint32_t vex = block_excl_scan<kFNT, uint32_t>(v, sh.scan_tmp, &batch_total);
-    if (threadIdx.x < kFSPT * NW) sh.wtot[threadIdx.x] = vex;
+    const uint32_t v = threadIdx.x < C::SPT * NW ? sh.wtot[threadIdx.x] : 0u;
+    const uint32_t vex = block_excl_scan<C::NT, uint32_t>(v, sh.scan_tmp, &batch_total);
+    if (threadIdx.x < C::SPT * NW) sh.wtot[threadIdx.x] = vex;
     if (threadIdx.x == 0) sh.base_sh = batch_total ? atomic_add_agent(cursor, (uint64_t)batch_total) : 0ull;
     __syncthreads();
     const uint64_t base = sh.base_sh;
 #pragma unroll
-    for (int u = 0; u < kFSPT; ++u) {
+    for (int u = 0; u < C::SPT; ++u) {
       if (m[u] == 0) continue;
       uint64_t pos = base + sh.wtot[u * NW + w] + wex[u];
       if (m[u] == 1) {
         if (pos < out_cap) st_stream(out + pos, make_uint2(fp[u], cur[u].y));
       } else {
-        const uint32_t h = bucket_hash((int32_t)cur[u].x, kFLog2NB);
+        const uint32_t h = bucket_hash((int32_t)cur[u].x, C::Log2NB);
         const uint32_t e1 = sh.off[h + 1];
         for (uint32_t e = sh.off[h]; e < e1; ++e) {
           if (sh.keys[e] == cur[u].x) {
@@ -155,7 +168,7 @@ __device__ __forceinline__ void probe_tile(ProbeShared &sh, const uint2 *__restr
     }
     __syncthreads();  // sh.wtot / sh.base_sh reuse
 #pragma unroll
-    for (int u = 0; u < kFSPT; ++u) cur[u] = nxt[u];
+    for (int u = 0; u < C::SPT; ++u) cur[u] = nxt[u];
   }
   __syncthreads();
 }
