@@ -218,6 +218,15 @@ class Context:
         _check(self.lib.eo_hidx_build(self.handle, _vp(R), len(R), ctypes.byref(o), ctypes.byref(h)), "eo_hidx_build")
         return Index(self, h.value)
 
+    def sidx_import(self, sorted_pairs, learned: bool = False, **kw) -> "Index":
+        """Sorted index from (key, payload) pairs already sorted by key (eo_sidx_import)."""
+        P = _tuples(sorted_pairs)
+        h = ctypes.c_void_p()
+        o = self.opts(**kw)
+        _check(self.lib.eo_sidx_import(self.handle, _vp(P), len(P), int(learned), ctypes.byref(o), ctypes.byref(h)),
+               "eo_sidx_import")
+        return Index(self, h.value)
+
     def sidx_build(self, R, learned: bool = False, **kw) -> "Index":
         """Sorted index on R (reference sort_index createIndex), optionally with the learned model."""
         R = _tuples(R)
@@ -392,6 +401,40 @@ class Batch:
             self.free()
         except Exception:
             pass
+
+
+# ---- the reference's table / index files (host only, no device needed) ------------
+def read_ref_table(path: str, raw: bool = False) -> np.ndarray:
+    """Reads a reference table file (int32 count + {row id, val} tuples; raw: no count)
+    as (n, 2) int32 (key = val, payload = row id)."""
+    L = _abi.load()
+    n = ctypes.c_uint64()
+    flags = _abi.EO_REF_FILE_RAW if raw else 0
+    _check(L.eo_ref_table_read(os.fsencode(path), flags, None, 0, ctypes.byref(n)), "eo_ref_table_read")
+    out = np.empty((n.value, 2), dtype=np.int32)
+    _check(L.eo_ref_table_read(os.fsencode(path), flags, _vp(out), n.value, ctypes.byref(n)), "eo_ref_table_read")
+    return out
+
+
+def write_ref_table(path: str, T, raw: bool = False):
+    T = _tuples(T)
+    flags = _abi.EO_REF_FILE_RAW if raw else 0
+    _check(_abi.load().eo_ref_table_write(os.fsencode(path), flags, _vp(T), len(T)), "eo_ref_table_write")
+
+
+def read_ref_index(path: str) -> np.ndarray:
+    """Reads a reference BUCKET index file as (n, 2) int32 (key = val, payload = adr), sorted by key."""
+    L = _abi.load()
+    n = ctypes.c_uint64()
+    _check(L.eo_ref_index_read(os.fsencode(path), None, 0, ctypes.byref(n)), "eo_ref_index_read")
+    out = np.empty((n.value, 2), dtype=np.int32)
+    _check(L.eo_ref_index_read(os.fsencode(path), _vp(out), n.value, ctypes.byref(n)), "eo_ref_index_read")
+    return out
+
+
+def write_ref_index(path: str, sorted_pairs):
+    P = _tuples(sorted_pairs)
+    _check(_abi.load().eo_ref_index_write(os.fsencode(path), _vp(P), len(P)), "eo_ref_index_write")
 
 
 class Index:

@@ -165,6 +165,7 @@ EO_SERVE_BATCHED, EO_SERVE_PER_JOIN = 0, 1
 EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
 EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
 EO_F_BINARY_SEARCH = 4
+EO_REF_FILE_RAW = 1
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -193,6 +194,11 @@ SIGNATURES = {
     "eo_index_export": (INT, [P, P, P, P]),
     "eo_index_query": (INT, [P, ctypes.POINTER(IndexInfo)]),
     "eo_index_free": (INT, [P]),
+    "eo_sidx_import": (INT, [P, P, U64, INT, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
+    "eo_ref_table_read": (INT, [ctypes.c_char_p, INT, P, U64, PU64]),
+    "eo_ref_table_write": (INT, [ctypes.c_char_p, INT, P, U64]),
+    "eo_ref_index_read": (INT, [ctypes.c_char_p, P, U64, PU64]),
+    "eo_ref_index_write": (INT, [ctypes.c_char_p, P, U64]),
     "eo_table_create": (INT, [P, P, U64, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
     "eo_table_query": (INT, [P, PU64, ctypes.POINTER(I32), PU64]),
     "eo_table_free": (INT, [P]),

@@ -32,6 +32,7 @@ SOURCES = [
     "api.cpp",
     "api_index.cpp",
     "api_batch.cpp",
+    "api_io.cpp",
     "plan.cpp",
     "tuner.cpp",
     "serve.cpp",

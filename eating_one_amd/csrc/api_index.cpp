@@ -129,8 +129,10 @@ eo_status eo_hidx_build(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_join
   return EO_OK;
 }
 
-eo_status eo_sidx_build(eo_ctx *c, const eo_tuple *R, uint64_t nR, int learned, const eo_join_opts *o,
-                        eo_index **out) {
+// presorted: R is already sorted by key (an imported index): checked on the
+// device, no radix sort.
+static eo_status sidx_build_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, int learned, const eo_join_opts *o,
+                                 eo_index **out, bool presorted) {
   if (!c || !out || !opts_ok(o) || nR > kMaxRows || (nR && !R)) return EO_EINVAL;
   *out = nullptr;
   DeviceGuard g(c->device);
@@ -161,6 +163,7 @@ eo_status eo_sidx_build(eo_ctx *c, const eo_tuple *R, uint64_t nR, int learned, 
   const size_t o_b = L.add(nR * sizeof(eo_tuple));
   const size_t o_leaf = L.add(learned ? nR * sizeof(uint16_t) : 0);
   const size_t o_scr = L.add(pscr);
+  const size_t o_flag = L.add(sizeof(uint64_t));
   hipError_t e = hipStreamSynchronize(s);
   if (e == hipSuccess) e = c->arena.reserve(L.total);
   if (e != hipSuccess) {
@@ -178,8 +181,25 @@ eo_status eo_sidx_build(eo_ctx *c, const eo_tuple *R, uint64_t nR, int learned, 
   eo_tuple *bufa = (eo_tuple *)(A + o_a), *bufb = (eo_tuple *)(A + o_b);
   const eo_tuple *src = dR;
   e = hipSuccess;
+  if (nR && presorted) {
+    uint32_t *flag = (uint32_t *)(A + o_flag);
+    e = check_sorted(dR, nR, flag, s);
+    uint64_t unsorted = 0;
+    if (e == hipSuccess) {
+      st = read_u64(c, (const uint64_t *)flag, &unsorted, 1, s);
+      if (st != EO_OK) {
+        free_index(ix);
+        return st;
+      }
+      if ((uint32_t)unsorted) {
+        free_index(ix);
+        set_last_error("eo_sidx_import: input is not sorted by key");
+        return EO_EINVAL;
+      }
+    }
+  }
   if (nR) {
-    for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+    for (int k = 0; k < 4 && e == hipSuccess && !presorted; ++k) {
       eo_tuple *dst = (k % 2 == 0) ? bufa : bufb;
       e = partition_pass(pp[k], src, dst, nullptr, A + o_scr, c->d_err, s);
       src = dst;
@@ -224,6 +244,16 @@ eo_status eo_sidx_build(eo_ctx *c, const eo_tuple *R, uint64_t nR, int learned, 
   }
   *out = ix;
   return EO_OK;
+}
+
+eo_status eo_sidx_build(eo_ctx *c, const eo_tuple *R, uint64_t nR, int learned, const eo_join_opts *o,
+                        eo_index **out) {
+  return sidx_build_impl(c, R, nR, learned, o, out, false);
+}
+
+eo_status eo_sidx_import(eo_ctx *c, const eo_tuple *sorted, uint64_t n, int learned, const eo_join_opts *o,
+                         eo_index **out) {
+  return sidx_build_impl(c, sorted, n, learned, o, out, true);
 }
 
 eo_status eo_index_query(const eo_index *ix, eo_index_info *info) {

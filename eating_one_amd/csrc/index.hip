@@ -355,6 +355,13 @@ hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint
   return hipGetLastError();
 }
 
+// *flag = 1 if some adjacent pair is out of (signed) key order.
+__global__ void k_check_sorted(const uint2 *__restrict__ in, uint64_t n, uint32_t *flag) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    if ((int32_t)in[i].x > (int32_t)in[i + 1].x) *flag = 1u;
+}
+
 uint32_t grid_for(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 16384)); }
 
 }  // namespace
@@ -377,6 +384,14 @@ hipError_t hidx_build(const eo_tuple *R, uint64_t n, int log2nb, uint32_t *keys,
   if ((e = scan_u32(off, off, nb + 1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
   if ((e = hipMemcpyAsync(cursor, off, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_hidx_place, dim3(grid_for(n)), dim3(256), 0, stream, vR, n, log2nb, cursor, keys, pays);
+  return hipGetLastError();
+}
+
+hipError_t check_sorted(const eo_tuple *in, uint64_t n, uint32_t *flag, hipStream_t stream) {
+  hipError_t e = fill_u32(flag, 0u, 1, stream);
+  if (e != hipSuccess || n < 2) return e;
+  hipLaunchKernelGGL(k_check_sorted, dim3(grid_for(n)), dim3(256), 0, stream, reinterpret_cast<const uint2 *>(in), n,
+                     flag);
   return hipGetLastError();
 }
 

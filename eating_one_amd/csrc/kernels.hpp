@@ -139,6 +139,8 @@ size_t learned_model_bytes();
 hipError_t hidx_build(const eo_tuple *R, uint64_t n, int log2nb, uint32_t *keys, uint32_t *pays, uint32_t *off,
                       uint32_t *cursor, void *scan_scratch, uint32_t *err, hipStream_t stream);
 hipError_t sidx_split(const eo_tuple *sorted, uint64_t n, uint32_t *keys, uint32_t *pays, hipStream_t stream);
+// *flag (device) = 1 if in[] is not sorted by signed key, else 0.
+hipError_t check_sorted(const eo_tuple *in, uint64_t n, uint32_t *flag, hipStream_t stream);
 // Fits the learned model over sorted keys; leaf_scratch: n u16.
 hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *leaf_scratch, hipStream_t stream);
 // Count (write = false: per-tile counts into tile_cnt, or, with tile_cnt ==

@@ -264,11 +264,36 @@ eo_status eo_index_lookup(eo_ctx *ctx, const eo_index *index, const eo_tuple *S,
                           uint64_t nS, int method, uint64_t *pos,
                           const eo_join_opts *opts);
 
+/* Sorted (optionally learned) index from pairs already sorted by key, e.g.
+ * the reference's BUCKET index file read by eo_ref_index_read: no sort, the
+ * order is verified on the device (EO_EINVAL if the input is not sorted by
+ * signed key).  Same index as eo_sidx_build of the same pairs. */
+eo_status eo_sidx_import(eo_ctx *ctx, const eo_tuple *sorted, uint64_t n, int learned,
+                         const eo_join_opts *opts, eo_index **index);
+
 /* Copies the index arrays (keys, payloads; index order) to host memory. */
 eo_status eo_index_export(eo_ctx *ctx, const eo_index *index, uint32_t *keys,
                           uint32_t *pays);
 eo_status eo_index_query(const eo_index *index, eo_index_info *info);
 eo_status eo_index_free(eo_index *index);
+
+/* ---- the reference's on-disk table and index files (host only) ----------- */
+
+/* Table file of joins/hashJoin_gpu/main_street/TableBuild.cpp:113-141: an
+ * int32 row count, then that many TUPLEs {int32 key (row id), int32 val}.
+ * EO_REF_FILE_RAW: the same tuples without the count (joins/join_ori/R, S).
+ * Read: with out == NULL only *n is returned; cap < n gives EO_ECAPACITY.
+ * Tuples come back as eo_tuple {key = val, payload = row id}; write converts
+ * back.  Neither needs a device. */
+#define EO_REF_FILE_RAW 1
+eo_status eo_ref_table_read(const char *path, int flags, eo_tuple *out, uint64_t cap,
+                            uint64_t *n);
+eo_status eo_ref_table_write(const char *path, int flags, const eo_tuple *tuples, uint64_t n);
+/* Sorted-index file of joins/innerJoin_gpu/sort_index/main_street/
+ * TableBuild.cpp:180-218: BUCKETs {int32 val, int32 adr} sorted by val, no
+ * header (n = file size / 8); read as eo_tuple {key = val, payload = adr}. */
+eo_status eo_ref_index_read(const char *path, eo_tuple *out, uint64_t cap, uint64_t *n);
+eo_status eo_ref_index_write(const char *path, const eo_tuple *sorted, uint64_t n);
 
 /* ---- resident build tables and batched joins ------------------------------ */
 

@@ -127,3 +127,25 @@ def test_index_join_fused_and_deterministic(ctx, kind):
                                ctypes.byref(n), ctypes.byref(o))
     assert st == _abi.EO_ECAPACITY and n.value == len(want)
     ix.free()
+
+
+def test_sidx_import_from_reference_index_file(ctx, tmp_path):
+    """A BUCKET index file read from disk gives the same index as building it on the device."""
+    import eating_one_amd as eo
+    R, S = gen.c3(seed=81, nR=70000, nS=90000, key_range=30000)
+    bucket = R[np.argsort(R[:, 0], kind="stable")]
+    path = tmp_path / "index"
+    eo.write_ref_index(str(path), bucket)
+    for learned in (False, True):
+        imp = ctx.sidx_import(eo.read_ref_index(str(path)), learned=learned)
+        built = ctx.sidx_build(R, learned=learned)
+        k1, p1 = imp.export()
+        k2, p2 = built.export()
+        assert np.array_equal(k1, k2) and np.array_equal(k1, bucket[:, 0])
+        assert same_multiset(imp.join(S), oracle.hash_join(R, S, nparts=1 << 10))
+        if learned:
+            assert imp.info()["max_err"] == built.info()["max_err"]
+        imp.free()
+        built.free()
+    with pytest.raises(eo.EOError):
+        ctx.sidx_import(R)  # not sorted
