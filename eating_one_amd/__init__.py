@@ -138,7 +138,7 @@ class Context:
     # ---- options -----------------------------------------------------------
     @staticmethod
     def opts(where=EO_HOST, radix_bits=0, passes=0, lds_tuples=0, s_tile=0, stats: JoinStats | None = None,
-             deterministic: bool = False, flags: int = 0):
+             deterministic: bool = False, flags: int = 0, result16: bool = False):
         o = JoinOpts()
         o.struct_size = ctypes.sizeof(JoinOpts)
         o.where = where
@@ -146,7 +146,7 @@ class Context:
         o.passes = passes
         o.lds_tuples = lds_tuples
         o.s_tile = s_tile
-        o.flags = flags | (EO_F_DETERMINISTIC if deterministic else 0)
+        o.flags = flags | (EO_F_DETERMINISTIC if deterministic else 0) | (_abi.EO_F_RESULT16 if result16 else 0)
         o.stats = ctypes.pointer(stats) if stats is not None else None
         return o
 
@@ -181,7 +181,8 @@ class Context:
         _check(fn(self.handle, _vp(R), len(R), _vp(S), len(S), None, 0, ctypes.byref(n), ctypes.byref(o)), name)
         if count_only:
             return n.value
-        out = np.empty((max(n.value, 1), 2), dtype=np.int32)
+        cols = 4 if (o.flags & _abi.EO_F_RESULT16) else 2  # 16-byte RESULT rows or 8-byte pairs
+        out = np.empty((max(n.value, 1), cols), dtype=np.int32)
         n2 = ctypes.c_uint64()
         st = fn(self.handle, _vp(R), len(R), _vp(S), len(S), _vp(out), n.value, ctypes.byref(n2), ctypes.byref(o))
         if st == _abi.EO_ECAPACITY:
@@ -454,6 +455,8 @@ class Index:
         """Index join of S (reference inner-idx-bnlj.cc join())."""
         S = _tuples(S)
         flags = kw.pop("flags", 0) | (_abi.EO_F_BINARY_SEARCH if binary_search else 0)
+        if kw.pop("result16", False):
+            flags |= _abi.EO_F_RESULT16
         o = self.ctx.opts(flags=flags, **kw)
         L = self.ctx.lib
         n = ctypes.c_uint64()
@@ -461,7 +464,8 @@ class Index:
                                ctypes.byref(o)), "eo_index_join")
         if count_only:
             return n.value
-        out = np.empty((max(n.value, 1), 2), dtype=np.int32)
+        cols = 4 if (o.flags & _abi.EO_F_RESULT16) else 2  # 16-byte RESULT rows or 8-byte pairs
+        out = np.empty((max(n.value, 1), cols), dtype=np.int32)
         n2 = ctypes.c_uint64()
         st = L.eo_index_join(self.ctx.handle, self.handle, _vp(S), len(S), _vp(out), n.value, ctypes.byref(n2),
                              ctypes.byref(o))

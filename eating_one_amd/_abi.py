@@ -18,6 +18,7 @@ EO_OK, EO_EINVAL, EO_ENOMEM, EO_ECAPACITY, EO_EHIP, EO_EBUDGET = 0, 1, 2, 3, 4, 
 EO_ENODEV, EO_EAGAIN, EO_ECANCELED, EO_ETIMEDOUT, EO_EINTERNAL = 6, 7, 8, 9, 10
 EO_HOST, EO_DEVICE = 0, 1
 EO_F_NO_SYNC_STATS, EO_F_DETERMINISTIC = 1, 2
+EO_F_RESULT16 = 8
 
 
 EO_MAX_KERNEL_STATS = 12

@@ -451,6 +451,8 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (nR == 0 || nS == 0) return EO_OK;
+  const bool wide = o && (o->flags & EO_F_RESULT16);
+  const size_t row = wide ? 16 : sizeof(eo_pair);
 
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
@@ -506,7 +508,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   clk.mark();  // 1
 
   const bool deterministic = o && (o->flags & EO_F_DETERMINISTIC);
-  JoinTilePlan jp{P, hp.lds_cap, hp.s_tile, 0};
+  JoinTilePlan jp{P, hp.lds_cap, hp.s_tile, 0, wide};
   if (deterministic)
     EO_TRY_HIP(join_plan_count(jp, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s), "tile plan");
   else
@@ -535,19 +537,18 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     // every match and writes only the pairs that fit.
     uint64_t cap_dev = out ? out_cap : 0;
     if (host && out && out_cap) {
-      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * row, s), "output staging");
       staged = true;
     }
     EO_TRY_HIP(fill_u64(small + 1, 0, 1, s), "cursor reset");
     clk.mark();  // 2
     hipError_t e = join_fused(P, hp.s_span, join_fused_cap(hp.lds_cap), (uint32_t)n_tiles, Rp, r_start, Sp, s_start,
-                              tile_start, dout, cap_dev,
-                              small + 1, s);
+                              tile_start, dout, cap_dev, small + 1, wide, s);
     clk.mark();  // 3
     if (e == hipSuccess) {
       st = read_u64(c, small + 1, &total, 1, s);
       if (st == EO_OK && staged && total <= out_cap && total)
-        e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "fused probe");
@@ -602,7 +603,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     return EO_ECAPACITY;
   }
   if (host && total) {
-    hipError_t e = hipMallocAsync((void **)&dout, total * sizeof(eo_pair), s);
+    hipError_t e = hipMallocAsync((void **)&dout, total * row, s);
     if (e != hipSuccess) {
       free_side();
       return hip_fail(e, "output staging");
@@ -613,7 +614,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
                               total, s);
     clk.mark();  // 3
     if (e == hipSuccess && host)
-      e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+      e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     if (host) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) {
       free_side();
@@ -645,6 +646,8 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (nR == 0 || nS == 0) return EO_OK;
+  const bool wide = o && (o->flags & EO_F_RESULT16);
+  const size_t row = wide ? 16 : sizeof(eo_pair);
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
   const bool host = where_of(o) == EO_HOST;
@@ -677,18 +680,18 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
     const uint64_t cap_dev = out ? out_cap : 0;
     bool staged = false;
     if (host && out && out_cap) {
-      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * row, s), "output staging");
       staged = true;
     }
     EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
-    hipError_t e = nlj_fused(np, dR, dS, small, out ? dout : nullptr, cap_dev, s);
+    hipError_t e = nlj_fused(np, dR, dS, small, out ? dout : nullptr, cap_dev, wide, s);
     clk.mark();
     uint64_t total = 0;
     eo_status st = EO_OK;
     if (e == hipSuccess) {
       st = read_u64(c, small, &total, 1, s);
       if (st == EO_OK && staged && total && total <= out_cap)
-        e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "nlj");
@@ -719,10 +722,10 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   if (total > out_cap) return EO_ECAPACITY;
   if (total) {
     eo_pair *dout = out;
-    if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * sizeof(eo_pair), s), "output staging");
-    hipError_t e = nlj_write(np, dR, dS, cnt, dout, total, s);
+    if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * row, s), "output staging");
+    hipError_t e = nlj_write(np, dR, dS, cnt, dout, total, wide, s);
     clk.mark();
-    if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     if (host) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "nlj write");
   } else {

@@ -295,6 +295,8 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
   if (nS == 0 || ix->n == 0) return EO_OK;
+  const bool wide = o && (o->flags & EO_F_RESULT16);
+  const size_t row = wide ? 16 : sizeof(eo_pair);
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
   const bool host = where_of(o) == EO_HOST;
@@ -321,18 +323,18 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     eo_pair *dout = out;
     bool staged = false;
     if (host && out && out_cap) {
-      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+      EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * row, s), "output staging");
       staged = true;
     }
     EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
     hipError_t e = index_probe(kind, out != nullptr, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS,
-                               nS, nullptr, small, out ? dout : nullptr, out ? out_cap : 0, nullptr, s);
+                               nS, nullptr, small, out ? dout : nullptr, out ? out_cap : 0, nullptr, wide, s);
     clk.mark();
     uint64_t total = 0;
     if (e == hipSuccess) {
       st = read_u64(c, small, &total, 1, s);
       if (st == EO_OK && staged && total && total <= out_cap)
-        e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+        e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "index join");
@@ -351,7 +353,7 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
   }
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt, nullptr,
-                         nullptr, 0, nullptr, s),
+                         nullptr, 0, nullptr, false, s),
              "index count");
   EO_TRY_HIP(scan_u64(cnt, cnt, tiles, small, A + o_scr, c->d_err, s), "index scan");
   clk.mark();
@@ -365,11 +367,11 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   if (total > out_cap) return EO_ECAPACITY;
   if (total) {
     eo_pair *dout = out;
-    if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * sizeof(eo_pair), s), "output staging");
+    if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * row, s), "output staging");
     hipError_t e = index_probe(kind, true, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt,
-                               nullptr, dout, total, nullptr, s);
+                               nullptr, dout, total, nullptr, wide, s);
     clk.mark();
-    if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     if (host) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "index write");
   } else {
@@ -411,7 +413,7 @@ eo_status eo_index_lookup(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint
   ScopedTimer tmr(c, o ? o->stats : nullptr);
   clk.mark();
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, nullptr, nullptr,
-                         nullptr, 0, dpos, s),
+                         nullptr, 0, dpos, false, s),
              "index lookup");
   clk.mark();
   if (host) EO_TRY_HIP(hipMemcpyAsync(pos, dpos, nS * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "d2h");

@@ -31,6 +31,21 @@ __device__ __forceinline__ void st_stream(uint4 *p, uint4 v) {
   x.w = v.w;
   __builtin_nontemporal_store(x, reinterpret_cast<u32x4_t *>(p));
 }
+// Result row pos: an 8-byte (R payload, S payload) pair, or with wide the
+// reference's 16-byte RESULT {rkey, rval, lkey, lval} = (R payload, key,
+// S payload, key).  stream: nontemporal store.
+template <bool STREAM = false>
+__device__ __forceinline__ void emit_pair(uint2 *out, uint64_t pos, uint32_t rp, uint32_t sp, uint32_t key,
+                                          bool wide) {
+  if (wide) {
+    uint4 *o4 = reinterpret_cast<uint4 *>(out) + pos;
+    const uint4 v = make_uint4(rp, key, sp, key);
+    if (STREAM) st_stream(o4, v); else *o4 = v;
+  } else {
+    if (STREAM) st_stream(out + pos, make_uint2(rp, sp)); else out[pos] = make_uint2(rp, sp);
+  }
+}
+
 __device__ __forceinline__ uint4 ld_stream(const uint4 *p) {
   const u32x4_t x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(p));
   return make_uint4(x.x, x.y, x.z, x.w);

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(C::NT) void k_join_write(const uint2 *__restrict__ 
                                                       const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
                                                       const uint32_t *__restrict__ tile_start, uint32_t P,
                                                       uint32_t s_tile, const uint64_t *__restrict__ tile_off,
-                                                      uint2 *__restrict__ out, uint64_t out_cap) {
+                                                      uint2 *__restrict__ out, uint64_t out_cap, int wide) {
   __shared__ uint2 ent[C::CAP];
   __shared__ uint32_t off[C::NB + 1];
   __shared__ uint32_t scan_tmp[C::NT / kWave + 1];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(C::NT) void k_join_write(const uint2 *__restrict__ 
       for (uint32_t e = b0[u]; e < b1[u]; ++e) {
         const uint2 r = ent[e];
         if (r.x == s[u].x) {
-          if (pos < out_cap) out[pos] = make_uint2(r.y, s[u].y);
+          if (pos < out_cap) emit_pair(out, pos, r.y, s[u].y, s[u].x, wide != 0);
           ++pos;
         }
       }
@@ -251,12 +251,12 @@ void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
                                                      const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
-                                                     uint64_t *out_cursor) {
+                                                     uint64_t *out_cursor, int wide) {
   __shared__ tile::ProbeShared<C> sh;
   const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
   SpanRef ref;
   if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
-  tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor);
+  tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0);
 }
 
 __global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,
@@ -302,7 +302,7 @@ hipError_t launch_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   EO_TIMED("probe_write", stream, hipLaunchKernelGGL(k_join_write<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
                      reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_off,
-                     reinterpret_cast<uint2 *>(out), out_cap));
+                     reinterpret_cast<uint2 *>(out), out_cap, (int)jp.wide));
   return hipGetLastError();
 }
 
@@ -374,7 +374,7 @@ hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const 
 
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream) {
+                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
@@ -382,11 +382,11 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
   if (cap == (uint32_t)tile::ProbeSmall::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
-                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor));
+                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide));
   else if (cap == (uint32_t)tile::ProbeBig::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
-                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor));
+                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide));
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

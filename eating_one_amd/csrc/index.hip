@@ -266,7 +266,7 @@ template <int KIND, bool WRITE>
 __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint2 *__restrict__ S, uint64_t nS,
                                                         uint64_t *__restrict__ tile_cnt, uint64_t *cursor,
                                                         uint2 *__restrict__ out, uint64_t out_cap,
-                                                        uint64_t *__restrict__ lb_out) {
+                                                        uint64_t *__restrict__ lb_out, int wide) {
   __shared__ int32_t kk[KIND == 2 ? kKnots + 1 : 1];
   __shared__ float kp[KIND == 2 ? kKnots + 1 : 1];
   __shared__ uint64_t red[kIdxNT / kWave + 1];
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
     uint64_t j = b[u];
     for (uint32_t c = 0; c < m[u]; ++j) {
       if (ix.keys[j] != s[u].x) continue;
-      if (pos < out_cap) out[pos] = make_uint2(ix.pays[j], s[u].y);
+      if (pos < out_cap) emit_pair(out, pos, ix.pays[j], s[u].y, s[u].x, wide != 0);
       ++pos;
       ++c;
     }
@@ -339,7 +339,8 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
 
 template <int KIND>
 hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint64_t nS, uint64_t *tile_cnt,
-                        uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, hipStream_t stream) {
+                        uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, bool wide,
+                        hipStream_t stream) {
   const uint64_t tiles = (nS + kIdxTile - 1) / kIdxTile;
   if (tiles == 0) return hipSuccess;
   if (tiles >= (1ull << 31)) return hipErrorInvalidValue;
@@ -347,11 +348,11 @@ hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint
     EO_TIMED("index_write", stream,
              hipLaunchKernelGGL((k_index_probe<KIND, true>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
                                 reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor,
-                                reinterpret_cast<uint2 *>(out), out_cap, lb_out));
+                                reinterpret_cast<uint2 *>(out), out_cap, lb_out, (int)wide));
   else
     EO_TIMED("index_probe", stream,
              hipLaunchKernelGGL((k_index_probe<KIND, false>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
-                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor, nullptr, 0, lb_out));
+                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor, nullptr, 0, lb_out, 0));
   return hipGetLastError();
 }
 
@@ -419,7 +420,7 @@ hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *
 hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
                        uint64_t *tile_cnt, uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out,
-                       hipStream_t stream) {
+                       bool wide, hipStream_t stream) {
   IndexView ix{};
   ix.kind = kind;
   ix.n = n;
@@ -434,9 +435,9 @@ hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint3
     ix.leaves = (const LeafModel *)(mp + ((kKnots + 1) * (sizeof(int32_t) + sizeof(float)) + 63) / 64 * 64);
   }
   switch (kind) {
-    case 0: return launch_probe<0>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, stream);
-    case 1: return launch_probe<1>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, stream);
-    case 2: return launch_probe<2>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, stream);
+    case 0: return launch_probe<0>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, stream);
+    case 1: return launch_probe<1>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, stream);
+    case 2: return launch_probe<2>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, stream);
     default: return hipErrorInvalidValue;
   }
 }

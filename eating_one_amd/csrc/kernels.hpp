@@ -98,6 +98,7 @@ struct JoinTilePlan {
   uint32_t lds_cap;    // R tuples per LDS table (C)
   uint32_t s_tile;     // S tuples per tile
   int log2_buckets;    // LDS bucket count = 2^log2_buckets
+  bool wide = false;   // write 16-byte RESULT rows (see emit_pair)
 };
 // Tile planning: tiles_per_part -> exclusive scan -> tile_part map.
 size_t join_plan_scratch_bytes(uint32_t parts);
@@ -125,9 +126,10 @@ uint32_t join_fused_cap(uint32_t lds_cap);
 hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
                            const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
                            uint32_t *err, hipStream_t stream);
+// wide: out holds 16-byte RESULT rows {R payload, key, S payload, key}.
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, hipStream_t stream);
+                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, hipStream_t stream);
 
 // ---- index.hip --------------------------------------------------------------
 // Resident indexes.  kind 0: hash (keys/pays grouped by bucket, off[NB+1]);
@@ -152,7 +154,7 @@ hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *
 hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
                        uint64_t *tile_cnt, uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out,
-                       hipStream_t stream);
+                       bool wide, hipStream_t stream);
 
 // Two-pass MSD partition (high digit bits_hi over the whole array, then the
 // low digit bits_lo inside each first-pass partition) from a single
@@ -224,8 +226,8 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
 // the pairs are written at positions reserved per wave (order unspecified,
 // positions >= out_cap counted only).
 hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
-                     uint64_t out_cap, hipStream_t stream);
+                     uint64_t out_cap, bool wide, hipStream_t stream);
 hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off,
-                     eo_pair *out, uint64_t out_cap, hipStream_t stream);
+                     eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream);
 
 }  // namespace eo

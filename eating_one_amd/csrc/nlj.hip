@@ -49,7 +49,7 @@ __device__ __forceinline__ void load_s(const uint2 *__restrict__ S, uint64_t nS,
 
 template <bool WRITE>
 __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpay, uint32_t rn, const SRegs &sr,
-                                         uint64_t *count, uint64_t base, uint2 *out, uint64_t out_cap) {
+                                         uint64_t *count, uint64_t base, uint2 *out, uint64_t out_cap, bool wide) {
   const uint64_t lt = lanemask_lt();
   uint64_t running = 0;
   uint32_t g = 0;
@@ -73,7 +73,7 @@ __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpa
           if (m) {
             if (WRITE && sr.key[k] == rk[q2] && ((sr.valid[k] >> lane_id()) & 1)) {
               const uint64_t pos = base + running + __popcll(m & lt);
-              if (pos < out_cap) out[pos] = make_uint2((uint32_t)rpay[g + q2], (uint32_t)sr.pay[k]);
+              if (pos < out_cap) emit_pair(out, pos, (uint32_t)rpay[g + q2], (uint32_t)sr.pay[k], (uint32_t)rk[q2], wide);
             }
             running += __popcll(m);
           }
@@ -88,7 +88,7 @@ __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpa
       if (m) {
         if (WRITE && sr.key[k] == r && ((sr.valid[k] >> lane_id()) & 1)) {
           const uint64_t pos = base + running + __popcll(m & lt);
-          if (pos < out_cap) out[pos] = make_uint2((uint32_t)rpay[g], (uint32_t)sr.pay[k]);
+          if (pos < out_cap) emit_pair(out, pos, (uint32_t)rpay[g], (uint32_t)sr.pay[k], (uint32_t)r, wide);
         }
         running += __popcll(m);
       }
@@ -101,7 +101,7 @@ template <bool WRITE>
 __global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uint64_t nR, const uint2 *__restrict__ S,
                                                 uint64_t nS, uint32_t s_tiles, uint64_t *wave_count,
                                                 const uint64_t *__restrict__ wave_off, uint2 *__restrict__ out,
-                                                uint64_t out_cap) {
+                                                uint64_t out_cap, int wide) {
   __shared__ __attribute__((aligned(16))) int32_t rkey[kNljRTile];
   __shared__ int32_t rpay[WRITE ? kNljRTile : 1];
   const uint32_t b = blockIdx.x;
@@ -118,9 +118,9 @@ __global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uin
   __syncthreads();
   const uint64_t slot = (uint64_t)b * kNljNW + threadIdx.x / kWave;
   if (WRITE)
-    nlj_tile<true>(rkey, rpay, rn, sr, nullptr, wave_off[slot], out, out_cap);
+    nlj_tile<true>(rkey, rpay, rn, sr, nullptr, wave_off[slot], out, out_cap, wide != 0);
   else
-    nlj_tile<false>(rkey, rpay, rn, sr, wave_count + slot, 0, nullptr, 0);
+    nlj_tile<false>(rkey, rpay, rn, sr, wave_count + slot, 0, nullptr, 0, false);
 }
 
 // Fused single pass (default order): every (R key group, S register) compare
@@ -132,7 +132,8 @@ __global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uin
 template <bool WRITE>
 __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ R, uint64_t nR,
                                                       const uint2 *__restrict__ S, uint64_t nS, uint32_t s_tiles,
-                                                      uint64_t *cursor, uint2 *__restrict__ out, uint64_t out_cap) {
+                                                      uint64_t *cursor, uint2 *__restrict__ out, uint64_t out_cap,
+                                                      int wide) {
   // payloads are not staged: only the recorded hit groups need them (read from
   // global memory in the write phase), which keeps LDS at 20 KB per workgroup
   __shared__ __attribute__((aligned(16))) int32_t rkey[kNljRTile];
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ 
           if (eq) {
             const uint64_t pos = base + running + __popcll(m & lt);
             const uint64_t si = s0 + (uint64_t)w * kWave * kNljK + (uint64_t)k * kWave + lane_id();
-            if (pos < out_cap) out[pos] = make_uint2(R[r0 + g + q2].y, S[si].y);
+            if (pos < out_cap) emit_pair(out, pos, R[r0 + g + q2].y, S[si].y, (uint32_t)r, wide != 0);
           }
           running += __popcll(m);
         }
@@ -236,12 +237,12 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   EO_TIMED("nlj_count", stream, hipLaunchKernelGGL(k_nlj<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
-                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0));
+                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0, 0));
   return hipGetLastError();
 }
 
 hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
-                     uint64_t out_cap, hipStream_t stream) {
+                     uint64_t out_cap, bool wide, hipStream_t stream) {
   const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
@@ -249,23 +250,23 @@ hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
     EO_TIMED("nlj_fused", stream,
              hipLaunchKernelGGL(k_nlj_fused<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream,
                                 reinterpret_cast<const uint2 *>(R), p.nR, reinterpret_cast<const uint2 *>(S), p.nS,
-                                p.s_tiles, cursor, reinterpret_cast<uint2 *>(out), out_cap));
+                                p.s_tiles, cursor, reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
   else
     EO_TIMED("nlj_count", stream,
              hipLaunchKernelGGL(k_nlj_fused<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream,
                                 reinterpret_cast<const uint2 *>(R), p.nR, reinterpret_cast<const uint2 *>(S), p.nS,
-                                p.s_tiles, cursor, nullptr, 0));
+                                p.s_tiles, cursor, nullptr, 0, 0));
   return hipGetLastError();
 }
 
 hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off, eo_pair *out,
-                     uint64_t out_cap, hipStream_t stream) {
+                     uint64_t out_cap, bool wide, hipStream_t stream) {
   const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   EO_TIMED("nlj_write", stream, hipLaunchKernelGGL(k_nlj<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
                      p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, nullptr, wave_off,
-                     reinterpret_cast<uint2 *>(out), out_cap));
+                     reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
   return hipGetLastError();
 }
 

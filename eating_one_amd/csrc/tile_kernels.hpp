@@ -48,11 +48,12 @@ struct ProbeShared {
 // Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= C::Cap) and probes the S
 // span S[ref.s_beg, +s_n) against it, writing (R payload, S payload) pairs at
 // positions reserved from *cursor; positions >= out_cap are counted only.
+// wide: 16-byte RESULT rows instead of pairs (see emit_pair).
 // All threads of the workgroup call it; it leaves LDS free for reuse.
 template <typename C>
 __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__restrict__ R, const uint2 *__restrict__ S,
                                            const SpanRef &ref, uint2 *__restrict__ out, uint64_t out_cap,
-                                           uint64_t *cursor) {
+                                           uint64_t *cursor, bool wide = false) {
   constexpr int NW = C::NT / kWave;
   auto load_batch = [&](uint32_t b0, uint2 *dst) {
 #pragma unroll
@@ -154,13 +155,13 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
       if (m[u] == 0) continue;
       uint64_t pos = base + sh.wtot[u * NW + w] + wex[u];
       if (m[u] == 1) {
-        if (pos < out_cap) st_stream(out + pos, make_uint2(fp[u], cur[u].y));
+        if (pos < out_cap) emit_pair<true>(out, pos, fp[u], cur[u].y, cur[u].x, wide);
       } else {
         const uint32_t h = bucket_hash((int32_t)cur[u].x, C::Log2NB);
         const uint32_t e1 = sh.off[h + 1];
         for (uint32_t e = sh.off[h]; e < e1; ++e) {
           if (sh.keys[e] == cur[u].x) {
-            if (pos < out_cap) st_stream(out + pos, make_uint2(sh.pays[e], cur[u].y));
+            if (pos < out_cap) emit_pair<true>(out, pos, sh.pays[e], cur[u].y, cur[u].x, wide);
             ++pos;
           }
         }

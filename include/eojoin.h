@@ -117,6 +117,11 @@ typedef struct eo_join_opts {
  * join sequence, hj.cc:1011-1091).  Default: one fused pass per tile that
  * reserves its output range with an atomic (order varies run to run). */
 #define EO_F_DETERMINISTIC 2
+/* Output rows in the reference's 16-byte RESULT layout {rkey, rval, lkey,
+ * lval} = (R payload, key, S payload, key) (joins/hashJoin_gpu/main_street/
+ * tuple.h:48-53) instead of 8-byte eo_pairs: `out` then points to 16-byte
+ * rows and out_cap counts rows.  eo_hash_join, eo_nlj_join, eo_index_join. */
+#define EO_F_RESULT16 8
 
 typedef struct eo_device_info {
   char name[64];

@@ -217,3 +217,22 @@ def test_nlj_c3_full(ctx):
     got = ctx.nlj_join(R, S)
     assert len(got) == join_count_closed_form(R, S)
     assert same_multiset(got, oracle.hash_join(R, S, nparts=1 << 12))
+
+
+@pytest.mark.parametrize("det", [False, True])
+def test_result16_rows_all_joins(ctx, det):
+    """EO_F_RESULT16: rows {R payload, key, S payload, key} as the reference's RESULT."""
+    R, S = gen.c3(seed=91, nR=40000, nS=60000, key_range=15000)
+    want = oracle.hash_join(R, S, nparts=1 << 10)
+    rkey = dict(zip(R[:, 1].tolist(), R[:, 0].tolist()))  # payload -> key
+    for name, fn in (("hash", lambda **k: ctx.hash_join(R, S, **k)), ("nlj", lambda **k: ctx.nlj_join(R, S, **k))):
+        rows = fn(result16=True, deterministic=det)
+        assert rows.shape == (len(want), 4), name
+        assert np.array_equal(rows[:, 1], rows[:, 3]), name
+        assert same_multiset(rows[:, [0, 2]], want), name
+        assert all(rkey[int(p)] == int(k) for p, k in zip(rows[:2000, 0], rows[:2000, 1])), name
+    ix = ctx.sidx_build(R, learned=True)
+    rows = ix.join(S, result16=True, deterministic=det)
+    assert rows.shape == (len(want), 4) and np.array_equal(rows[:, 1], rows[:, 3])
+    assert same_multiset(rows[:, [0, 2]], want)
+    ix.free()
