@@ -96,6 +96,34 @@ class DeviceArray:
             pass
 
 
+class HostArray:
+    """Pinned, device-mapped host memory (eo_host_alloc): kernels can write into it directly."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(ctx.lib.eo_host_alloc(ctx.handle, self.nbytes, ctypes.byref(p)), "eo_host_alloc")
+        self.ptr = p.value
+
+    def view(self, dtype, shape) -> np.ndarray:
+        """A numpy view of the memory (valid until close)."""
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        buf = (ctypes.c_char * n).from_address(self.ptr)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def close(self):
+        if self.ptr is not None and self.ctx.handle:
+            self.ctx.lib.eo_host_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU context (eo_ctx): stream, scratch arena, device queries."""
 
@@ -138,7 +166,7 @@ class Context:
     # ---- options -----------------------------------------------------------
     @staticmethod
     def opts(where=EO_HOST, radix_bits=0, passes=0, lds_tuples=0, s_tile=0, stats: JoinStats | None = None,
-             deterministic: bool = False, flags: int = 0, result16: bool = False):
+             deterministic: bool = False, flags: int = 0, result16: bool = False, left_outer: bool = False):
         o = JoinOpts()
         o.struct_size = ctypes.sizeof(JoinOpts)
         o.where = where
@@ -146,7 +174,8 @@ class Context:
         o.passes = passes
         o.lds_tuples = lds_tuples
         o.s_tile = s_tile
-        o.flags = flags | (EO_F_DETERMINISTIC if deterministic else 0) | (_abi.EO_F_RESULT16 if result16 else 0)
+        o.flags = (flags | (EO_F_DETERMINISTIC if deterministic else 0) | (_abi.EO_F_RESULT16 if result16 else 0)
+                   | (_abi.EO_F_LEFT_OUTER if left_outer else 0))
         o.stats = ctypes.pointer(stats) if stats is not None else None
         return o
 
@@ -457,6 +486,8 @@ class Index:
         flags = kw.pop("flags", 0) | (_abi.EO_F_BINARY_SEARCH if binary_search else 0)
         if kw.pop("result16", False):
             flags |= _abi.EO_F_RESULT16
+        if kw.pop("left_outer", False):
+            flags |= _abi.EO_F_LEFT_OUTER
         o = self.ctx.opts(flags=flags, **kw)
         L = self.ctx.lib
         n = ctypes.c_uint64()

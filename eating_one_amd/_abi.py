@@ -19,6 +19,8 @@ EO_ENODEV, EO_EAGAIN, EO_ECANCELED, EO_ETIMEDOUT, EO_EINTERNAL = 6, 7, 8, 9, 10
 EO_HOST, EO_DEVICE = 0, 1
 EO_F_NO_SYNC_STATS, EO_F_DETERMINISTIC = 1, 2
 EO_F_RESULT16 = 8
+EO_F_LEFT_OUTER = 16
+EO_NULL_PAYLOAD = -(1 << 31)
 
 
 EO_MAX_KERNEL_STATS = 12
@@ -179,6 +181,8 @@ SIGNATURES = {
     "eo_device_query": (INT, [P, ctypes.POINTER(DeviceInfo)]),
     "eo_dev_alloc": (INT, [P, U64, ctypes.POINTER(P)]),
     "eo_dev_free": (INT, [P, P]),
+    "eo_host_alloc": (INT, [P, U64, ctypes.POINTER(P)]),
+    "eo_host_free": (INT, [P, P]),
     "eo_copy_h2d": (INT, [P, P, P, U64]),
     "eo_copy_d2h": (INT, [P, P, P, U64]),
     "eo_copy_d2d": (INT, [P, P, P, U64]),

@@ -265,6 +265,22 @@ eo_status eo_dev_alloc(eo_ctx *c, uint64_t bytes, void **dptr) {
   return EO_OK;
 }
 
+eo_status eo_host_alloc(eo_ctx *c, uint64_t bytes, void **hptr) {
+  if (!c || !hptr) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  *hptr = nullptr;
+  EO_TRY_HIP(hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocMapped | hipHostMallocPortable), "eo_host_alloc");
+  return EO_OK;
+}
+
+eo_status eo_host_free(eo_ctx *c, void *hptr) {
+  if (!c) return EO_EINVAL;
+  if (!hptr) return EO_OK;
+  DeviceGuard g(c->device);
+  EO_TRY_HIP(hipHostFree(hptr), "eo_host_free");
+  return EO_OK;
+}
+
 eo_status eo_dev_free(eo_ctx *c, void *dptr) {
   if (!c) return EO_EINVAL;
   DeviceGuard g(c->device);
@@ -450,9 +466,14 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   if (st != EO_OK) return st;
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (nR == 0 || nS == 0) return EO_OK;
+  if (nS == 0 || (nR == 0 && !(o && (o->flags & EO_F_LEFT_OUTER)))) return EO_OK;
   const bool wide = o && (o->flags & EO_F_RESULT16);
   const size_t row = wide ? 16 : sizeof(eo_pair);
+  const bool outer = o && (o->flags & EO_F_LEFT_OUTER);
+  if (outer && (o->flags & EO_F_DETERMINISTIC)) {
+    set_last_error("EO_F_LEFT_OUTER: the deterministic count/write path has no outer mode");
+    return EO_EINVAL;
+  }
 
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
@@ -469,6 +490,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const size_t o_ss = L.add((P + 1) * sizeof(uint32_t));
   const size_t o_ts = L.add((P + 1) * sizeof(uint32_t));
   const size_t o_small = L.add(8 * sizeof(uint64_t));
+  const size_t o_bits = L.add(outer ? matched_words(nS) * sizeof(uint32_t) : 0);
   const size_t o_pscr = L.add(std::max(partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus)));
   const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
   const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_tile + 1024;
@@ -541,9 +563,12 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
       staged = true;
     }
     EO_TRY_HIP(fill_u64(small + 1, 0, 1, s), "cursor reset");
+    uint32_t *matched = outer ? (uint32_t *)(A + o_bits) : nullptr;
+    if (outer) EO_TRY_HIP(fill_u32(matched, 0u, matched_words(nS), s), "match bitmap reset");
     clk.mark();  // 2
     hipError_t e = join_fused(P, hp.s_span, join_fused_cap(hp.lds_cap), (uint32_t)n_tiles, Rp, r_start, Sp, s_start,
-                              tile_start, dout, cap_dev, small + 1, wide, s);
+                              tile_start, dout, cap_dev, small + 1, wide, matched, s);
+    if (e == hipSuccess && outer) e = emit_unmatched(Sp, nS, matched, small + 1, dout, cap_dev, wide, s);
     clk.mark();  // 3
     if (e == hipSuccess) {
       st = read_u64(c, small + 1, &total, 1, s);
@@ -645,7 +670,12 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   *n_out = 0;
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (nR == 0 || nS == 0) return EO_OK;
+  const bool outer = o && (o->flags & EO_F_LEFT_OUTER);
+  if (nS == 0 || (nR == 0 && !outer)) return EO_OK;
+  if (outer && (o->flags & EO_F_DETERMINISTIC)) {
+    set_last_error("EO_F_LEFT_OUTER: the deterministic count/write path has no outer mode");
+    return EO_EINVAL;
+  }
   const bool wide = o && (o->flags & EO_F_RESULT16);
   const size_t row = wide ? 16 : sizeof(eo_pair);
   DeviceGuard g(c->device);
@@ -658,6 +688,7 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   const size_t o_sin = L.add(host ? nS * sizeof(eo_tuple) : 0);
   const size_t o_cnt = L.add(slots * sizeof(uint64_t));
   const size_t o_small = L.add(8 * sizeof(uint64_t));
+  const size_t o_bits = L.add(outer ? matched_words(nS) * sizeof(uint32_t) : 0);
   const size_t o_scr = L.add(scan_scratch_bytes_u64(slots));
   EO_TRY_HIP(hipStreamSynchronize(s), "sync");
   EO_TRY_HIP(c->arena.reserve(L.total), "arena");
@@ -684,7 +715,10 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
       staged = true;
     }
     EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
-    hipError_t e = nlj_fused(np, dR, dS, small, out ? dout : nullptr, cap_dev, wide, s);
+    uint32_t *matched = outer ? (uint32_t *)(A + o_bits) : nullptr;
+    if (outer) EO_TRY_HIP(fill_u32(matched, 0u, matched_words(nS), s), "match bitmap reset");
+    hipError_t e = nlj_fused(np, dR, dS, small, out ? dout : nullptr, cap_dev, wide, matched, s);
+    if (e == hipSuccess && outer) e = emit_unmatched(dS, nS, matched, small, out ? dout : nullptr, cap_dev, wide, s);
     clk.mark();
     uint64_t total = 0;
     eo_status st = EO_OK;

@@ -294,7 +294,8 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   if (kind < 0) return EO_EINVAL;
   eo_join_stats *stats = o ? o->stats : nullptr;
   if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (nS == 0 || ix->n == 0) return EO_OK;
+  const bool outer = o && (o->flags & EO_F_LEFT_OUTER);
+  if (nS == 0 || (ix->n == 0 && !outer)) return EO_OK;
   const bool wide = o && (o->flags & EO_F_RESULT16);
   const size_t row = wide ? 16 : sizeof(eo_pair);
   DeviceGuard g(c->device);
@@ -306,6 +307,8 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   const size_t o_cnt = L.add(tiles * sizeof(uint64_t));
   const size_t o_small = L.add(64);
   const size_t o_scr = L.add(scan_scratch_bytes_u64(tiles));
+  const bool empty_outer = outer && ix->n == 0;  // every probe unmatched
+  const size_t o_bits = L.add(empty_outer ? matched_words(nS) * sizeof(uint32_t) : 0);
   EO_TRY_HIP(hipStreamSynchronize(s), "sync");
   EO_TRY_HIP(c->arena.reserve(L.total), "arena");
   char *A = c->arena.base;
@@ -317,6 +320,10 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
   PhaseClock clk(c, s, stats != nullptr);
   ScopedTimer tmr(c, stats);
   clk.mark();
+  if (empty_outer && (o->flags & EO_F_DETERMINISTIC)) {
+    set_last_error("EO_F_LEFT_OUTER on an empty index: default order only");
+    return EO_EINVAL;
+  }
   if (!(o && (o->flags & EO_F_DETERMINISTIC))) {
     // one lookup pass; each tile reserves its output range with one atomic
     // (with out == NULL, or a too small buffer, it counts every match)
@@ -327,8 +334,15 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
       staged = true;
     }
     EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
-    hipError_t e = index_probe(kind, out != nullptr, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS,
-                               nS, nullptr, small, out ? dout : nullptr, out ? out_cap : 0, nullptr, wide, s);
+    hipError_t e = hipSuccess;
+    if (empty_outer) {
+      uint32_t *none = (uint32_t *)(A + o_bits);
+      e = fill_u32(none, 0u, matched_words(nS), s);
+      if (e == hipSuccess) e = emit_unmatched(dS, nS, none, small, out ? dout : nullptr, out ? out_cap : 0, wide, s);
+    } else {
+      e = index_probe(kind, out != nullptr, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS,
+                      nullptr, small, out ? dout : nullptr, out ? out_cap : 0, nullptr, wide, outer, s);
+    }
     clk.mark();
     uint64_t total = 0;
     if (e == hipSuccess) {
@@ -353,7 +367,7 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
   }
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt, nullptr,
-                         nullptr, 0, nullptr, false, s),
+                         nullptr, 0, nullptr, false, outer, s),
              "index count");
   EO_TRY_HIP(scan_u64(cnt, cnt, tiles, small, A + o_scr, c->d_err, s), "index scan");
   clk.mark();
@@ -369,7 +383,7 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     eo_pair *dout = out;
     if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * row, s), "output staging");
     hipError_t e = index_probe(kind, true, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, cnt,
-                               nullptr, dout, total, nullptr, wide, s);
+                               nullptr, dout, total, nullptr, wide, outer, s);
     clk.mark();
     if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     if (host) (void)hipFreeAsync(dout, s);
@@ -413,7 +427,7 @@ eo_status eo_index_lookup(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint
   ScopedTimer tmr(c, o ? o->stats : nullptr);
   clk.mark();
   EO_TRY_HIP(index_probe(kind, false, ix->n, ix->log2nb, ix->keys, ix->pays, ix->off, ix->model, dS, nS, nullptr, nullptr,
-                         nullptr, 0, dpos, false, s),
+                         nullptr, 0, dpos, false, false, s),
              "index lookup");
   clk.mark();
   if (host) EO_TRY_HIP(hipMemcpyAsync(pos, dpos, nS * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "d2h");

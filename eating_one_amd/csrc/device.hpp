@@ -46,6 +46,19 @@ __device__ __forceinline__ void emit_pair(uint2 *out, uint64_t pos, uint32_t rp,
   }
 }
 
+// Row of an unmatched probe tuple in a left outer join: R side null.
+constexpr uint32_t kNullPayload = 0x80000000u;  // INT32_MIN
+template <bool STREAM = false>
+__device__ __forceinline__ void emit_null(uint2 *out, uint64_t pos, uint32_t sp, uint32_t key, bool wide) {
+  if (wide) {
+    uint4 *o4 = reinterpret_cast<uint4 *>(out) + pos;
+    const uint4 v = make_uint4(kNullPayload, kNullPayload, sp, key);
+    if (STREAM) st_stream(o4, v); else *o4 = v;
+  } else {
+    if (STREAM) st_stream(out + pos, make_uint2(kNullPayload, sp)); else out[pos] = make_uint2(kNullPayload, sp);
+  }
+}
+
 __device__ __forceinline__ uint4 ld_stream(const uint4 *p) {
   const u32x4_t x = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(p));
   return make_uint4(x.x, x.y, x.z, x.w);
@@ -185,6 +198,19 @@ __device__ __forceinline__ uint64_t atomic_add_agent(uint64_t *p, uint64_t v) {
 }
 __device__ __forceinline__ uint32_t atomic_or_agent(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Sets bit (base + lane) of the bitmap for every lane set in mask (the wave's
+// 64 positions are consecutive): at most three word atomics per wave.
+__device__ __forceinline__ void mark_bits(uint32_t *bits, uint64_t base, uint64_t mask) {
+  if (!mask) return;
+  const uint32_t s = (uint32_t)(base & 31);
+  const int lane = lane_id();
+  uint32_t v = 0;
+  if (lane == 0) v = (uint32_t)(mask << s);
+  else if (lane == 1) v = s ? (uint32_t)(mask >> (32 - s)) : (uint32_t)(mask >> 32);
+  else if (lane == 2) v = s ? (uint32_t)(mask >> (64 - s)) : 0u;
+  if (v) atomic_or_agent(bits + (base >> 5) + lane, v);
 }
 
 // Workgroup -> work-item mapping that keeps consecutive work items on one XCD:

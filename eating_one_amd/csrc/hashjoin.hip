@@ -251,12 +251,12 @@ void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
                                                      const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
-                                                     uint64_t *out_cursor, int wide) {
+                                                     uint64_t *out_cursor, int wide, uint32_t *matched) {
   __shared__ tile::ProbeShared<C> sh;
   const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
   SpanRef ref;
   if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
-  tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0);
+  tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched);
 }
 
 __global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,
@@ -374,7 +374,8 @@ hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const 
 
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, hipStream_t stream) {
+                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched,
+                      hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
@@ -382,13 +383,64 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
   if (cap == (uint32_t)tile::ProbeSmall::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
-                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide));
+                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
+                                matched));
   else if (cap == (uint32_t)tile::ProbeBig::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
-                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide));
+                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
+                                matched));
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+namespace {
+
+// Left outer join: one row (null R side) per S tuple whose bit in `matched`
+// is clear; each workgroup reserves its rows from *cursor with one atomic.
+// out == NULL: only counts.
+__global__ __launch_bounds__(256) void k_emit_unmatched(const uint2 *__restrict__ S, uint64_t n,
+                                                        const uint32_t *__restrict__ matched, uint64_t *cursor,
+                                                        uint2 *__restrict__ out, uint64_t out_cap, int wide) {
+  __shared__ uint64_t red[256 / kWave + 1];
+  __shared__ uint64_t base_sh;
+  const uint64_t wd = (uint64_t)blockIdx.x * 256 + threadIdx.x;  // one 32-position word per thread
+  uint32_t unm = 0;
+  if (wd * 32 < n) {
+    const uint64_t left = n - wd * 32;
+    const uint32_t valid = left >= 32 ? ~0u : ((1u << left) - 1u);
+    unm = ~matched[wd] & valid;
+  }
+  uint64_t tot;
+  const uint64_t excl = block_excl_scan<256, uint64_t>((uint64_t)__popc(unm), red, &tot);
+  if (threadIdx.x == 0) base_sh = tot ? atomic_add_agent(cursor, tot) : 0ull;
+  __syncthreads();
+  if (!out) return;
+  uint64_t pos = base_sh + excl;
+  while (unm) {
+    const uint32_t b = (uint32_t)__ffs(unm) - 1u;
+    unm &= unm - 1u;
+    const uint2 t = S[wd * 32 + b];
+    if (pos < out_cap) emit_null(out, pos, t.y, t.x, wide != 0);
+    ++pos;
+  }
+}
+
+}  // namespace
+
+size_t matched_words(uint64_t n) { return (n + 31) / 32; }
+
+hipError_t emit_unmatched(const eo_tuple *S, uint64_t n, const uint32_t *matched, uint64_t *cursor, eo_pair *out,
+                          uint64_t out_cap, bool wide, hipStream_t stream) {
+  const uint64_t words = matched_words(n);
+  if (words == 0) return hipSuccess;
+  const uint64_t blocks = (words + 255) / 256;
+  if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
+  EO_TIMED("outer_rows", stream,
+           hipLaunchKernelGGL(k_emit_unmatched, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                              reinterpret_cast<const uint2 *>(S), n, matched, cursor, reinterpret_cast<uint2 *>(out),
+                              out_cap, (int)wide));
   return hipGetLastError();
 }
 

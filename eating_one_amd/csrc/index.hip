@@ -266,7 +266,7 @@ template <int KIND, bool WRITE>
 __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint2 *__restrict__ S, uint64_t nS,
                                                         uint64_t *__restrict__ tile_cnt, uint64_t *cursor,
                                                         uint2 *__restrict__ out, uint64_t out_cap,
-                                                        uint64_t *__restrict__ lb_out, int wide) {
+                                                        uint64_t *__restrict__ lb_out, int wide, int outer) {
   __shared__ int32_t kk[KIND == 2 ? kKnots + 1 : 1];
   __shared__ float kp[KIND == 2 ? kKnots + 1 : 1];
   __shared__ uint64_t red[kIdxNT / kWave + 1];
@@ -284,6 +284,7 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
   uint64_t b[kIdxU];
   uint32_t m[kIdxU];
   uint32_t total = 0;
+  uint32_t nul = 0;
 #pragma unroll
   for (int u = 0; u < kIdxU; ++u) {
     const uint64_t i = s0 + (uint64_t)u * kIdxNT + threadIdx.x;
@@ -303,7 +304,8 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
     } else {
       for (uint64_t j = b[u]; j < e && ix.keys[j] == s[u].x; ++j) ++m[u];
     }
-    total += m[u];
+    if (outer && m[u] == 0) nul |= 1u << u;  // left outer join: one null-padded row
+    total += m[u] + ((nul >> u) & 1u);
   }
   if (!WRITE) {
     const uint64_t tot = block_sum<kIdxNT, uint64_t>((uint64_t)total, red);
@@ -326,6 +328,11 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
   }
 #pragma unroll
   for (int u = 0; u < kIdxU; ++u) {
+    if ((nul >> u) & 1u) {
+      if (pos < out_cap) emit_null(out, pos, s[u].y, s[u].x, wide != 0);
+      ++pos;
+      continue;
+    }
     if (m[u] == 0) continue;
     uint64_t j = b[u];
     for (uint32_t c = 0; c < m[u]; ++j) {
@@ -340,7 +347,7 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
 template <int KIND>
 hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint64_t nS, uint64_t *tile_cnt,
                         uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out, bool wide,
-                        hipStream_t stream) {
+                        bool outer, hipStream_t stream) {
   const uint64_t tiles = (nS + kIdxTile - 1) / kIdxTile;
   if (tiles == 0) return hipSuccess;
   if (tiles >= (1ull << 31)) return hipErrorInvalidValue;
@@ -348,11 +355,12 @@ hipError_t launch_probe(const IndexView &ix, bool write, const eo_tuple *S, uint
     EO_TIMED("index_write", stream,
              hipLaunchKernelGGL((k_index_probe<KIND, true>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
                                 reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor,
-                                reinterpret_cast<uint2 *>(out), out_cap, lb_out, (int)wide));
+                                reinterpret_cast<uint2 *>(out), out_cap, lb_out, (int)wide, (int)outer));
   else
     EO_TIMED("index_probe", stream,
              hipLaunchKernelGGL((k_index_probe<KIND, false>), dim3((uint32_t)tiles), dim3(kIdxNT), 0, stream, ix,
-                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor, nullptr, 0, lb_out, 0));
+                                reinterpret_cast<const uint2 *>(S), nS, tile_cnt, cursor, nullptr, 0, lb_out, 0,
+                                (int)outer));
   return hipGetLastError();
 }
 
@@ -420,7 +428,7 @@ hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *
 hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
                        uint64_t *tile_cnt, uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out,
-                       bool wide, hipStream_t stream) {
+                       bool wide, bool outer, hipStream_t stream) {
   IndexView ix{};
   ix.kind = kind;
   ix.n = n;
@@ -435,9 +443,9 @@ hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint3
     ix.leaves = (const LeafModel *)(mp + ((kKnots + 1) * (sizeof(int32_t) + sizeof(float)) + 63) / 64 * 64);
   }
   switch (kind) {
-    case 0: return launch_probe<0>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, stream);
-    case 1: return launch_probe<1>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, stream);
-    case 2: return launch_probe<2>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, stream);
+    case 0: return launch_probe<0>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, outer, stream);
+    case 1: return launch_probe<1>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, outer, stream);
+    case 2: return launch_probe<2>(ix, write, S, nS, tile_cnt, cursor, out, out_cap, lb_out, wide, outer, stream);
     default: return hipErrorInvalidValue;
   }
 }

@@ -126,10 +126,18 @@ uint32_t join_fused_cap(uint32_t lds_cap);
 hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
                            const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
                            uint32_t *err, hipStream_t stream);
-// wide: out holds 16-byte RESULT rows {R payload, key, S payload, key}.
+// wide: out holds 16-byte RESULT rows {R payload, key, S payload, key};
+// matched (optional): bitmap over S positions of the tuples that matched.
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, hipStream_t stream);
+                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched,
+                      hipStream_t stream);
+// Left outer join: bitmap words over n S positions, and the pass that emits
+// one null-padded row per S tuple whose bit is clear (rows reserved from
+// *cursor; out == NULL counts only).
+size_t matched_words(uint64_t n);
+hipError_t emit_unmatched(const eo_tuple *S, uint64_t n, const uint32_t *matched, uint64_t *cursor, eo_pair *out,
+                          uint64_t out_cap, bool wide, hipStream_t stream);
 
 // ---- index.hip --------------------------------------------------------------
 // Resident indexes.  kind 0: hash (keys/pays grouped by bucket, off[NB+1]);
@@ -150,11 +158,12 @@ hipError_t learned_fit(const uint32_t *keys, uint64_t n, void *model, uint16_t *
 // exclusive-scanned tile offsets, or, with tile_cnt == NULL, every tile
 // reserves its range from *cursor and positions >= out_cap are counted only).
 // lb_out (optional, nS u64): the index position each probe starts from
-// (lower bound for sorted kinds).
+// (lower bound for sorted kinds).  outer: a probe without a match counts
+// (and writes) one null-padded row (left outer join).
 hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint32_t *keys, const uint32_t *pays,
                        const uint32_t *off, const void *model, const eo_tuple *S, uint64_t nS,
                        uint64_t *tile_cnt, uint64_t *cursor, eo_pair *out, uint64_t out_cap, uint64_t *lb_out,
-                       bool wide, hipStream_t stream);
+                       bool wide, bool outer, hipStream_t stream);
 
 // Two-pass MSD partition (high digit bits_hi over the whole array, then the
 // low digit bits_lo inside each first-pass partition) from a single
@@ -226,7 +235,7 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
 // the pairs are written at positions reserved per wave (order unspecified,
 // positions >= out_cap counted only).
 hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
-                     uint64_t out_cap, bool wide, hipStream_t stream);
+                     uint64_t out_cap, bool wide, uint32_t *matched, hipStream_t stream);
 hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off,
                      eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream);
 

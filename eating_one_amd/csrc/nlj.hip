@@ -133,7 +133,7 @@ template <bool WRITE>
 __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ R, uint64_t nR,
                                                       const uint2 *__restrict__ S, uint64_t nS, uint32_t s_tiles,
                                                       uint64_t *cursor, uint2 *__restrict__ out, uint64_t out_cap,
-                                                      int wide) {
+                                                      int wide, uint32_t *matched) {
   // payloads are not staged: only the recorded hit groups need them (read from
   // global memory in the write phase), which keeps LDS at 20 KB per workgroup
   __shared__ __attribute__((aligned(16))) int32_t rkey[kNljRTile];
@@ -152,6 +152,9 @@ __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ 
   const uint32_t ngroups = (rn + kNljGroup - 1) / kNljGroup;
   uint64_t total = 0;
   uint32_t nh = 0;
+  uint64_t hitk[kNljK];  // lanes of each S register that matched (left outer join)
+#pragma unroll
+  for (int k = 0; k < kNljK; ++k) hitk[k] = 0;
   for (uint32_t gi = 0; gi < ngroups; ++gi) {
     const uint32_t g = gi * kNljGroup;
     const uint32_t gn = rn - g < (uint32_t)kNljGroup ? rn - g : (uint32_t)kNljGroup;  // wave-uniform
@@ -176,7 +179,11 @@ __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ 
       for (int q2 = 0; q2 < kNljGroup; ++q2) {
         if (q2 >= (int)gn) break;
 #pragma unroll
-        for (int k = 0; k < kNljK; ++k) cnt += __popcll(__ballot(sr.key[k] == rk[q2]) & sr.valid[k]);
+        for (int k = 0; k < kNljK; ++k) {
+          const uint64_t mm = __ballot(sr.key[k] == rk[q2]) & sr.valid[k];
+          cnt += __popcll(mm);
+          hitk[k] |= mm;
+        }
       }
       if (cnt) {
         total += cnt;
@@ -186,6 +193,11 @@ __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ 
         }
       }
     }
+  }
+  if (matched) {
+#pragma unroll
+    for (int k = 0; k < kNljK; ++k)
+      mark_bits(matched, s0 + (uint64_t)w * kWave * kNljK + (uint64_t)k * kWave, hitk[k]);
   }
   uint64_t base = 0;
   if (total) {
@@ -242,7 +254,7 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
 }
 
 hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
-                     uint64_t out_cap, bool wide, hipStream_t stream) {
+                     uint64_t out_cap, bool wide, uint32_t *matched, hipStream_t stream) {
   const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
@@ -250,12 +262,12 @@ hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
     EO_TIMED("nlj_fused", stream,
              hipLaunchKernelGGL(k_nlj_fused<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream,
                                 reinterpret_cast<const uint2 *>(R), p.nR, reinterpret_cast<const uint2 *>(S), p.nS,
-                                p.s_tiles, cursor, reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
+                                p.s_tiles, cursor, reinterpret_cast<uint2 *>(out), out_cap, (int)wide, matched));
   else
     EO_TIMED("nlj_count", stream,
              hipLaunchKernelGGL(k_nlj_fused<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream,
                                 reinterpret_cast<const uint2 *>(R), p.nR, reinterpret_cast<const uint2 *>(S), p.nS,
-                                p.s_tiles, cursor, nullptr, 0, 0));
+                                p.s_tiles, cursor, nullptr, 0, 0, matched));
   return hipGetLastError();
 }
 

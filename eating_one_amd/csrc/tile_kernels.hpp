@@ -48,12 +48,14 @@ struct ProbeShared {
 // Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= C::Cap) and probes the S
 // span S[ref.s_beg, +s_n) against it, writing (R payload, S payload) pairs at
 // positions reserved from *cursor; positions >= out_cap are counted only.
-// wide: 16-byte RESULT rows instead of pairs (see emit_pair).
+// wide: 16-byte RESULT rows instead of pairs (see emit_pair).  matched
+// (optional bitmap over S positions): sets the bit of every S tuple that
+// matched, for a left outer join.
 // All threads of the workgroup call it; it leaves LDS free for reuse.
 template <typename C>
 __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__restrict__ R, const uint2 *__restrict__ S,
                                            const SpanRef &ref, uint2 *__restrict__ out, uint64_t out_cap,
-                                           uint64_t *cursor, bool wide = false) {
+                                           uint64_t *cursor, bool wide = false, uint32_t *matched = nullptr) {
   constexpr int NW = C::NT / kWave;
   auto load_batch = [&](uint32_t b0, uint2 *dst) {
 #pragma unroll
@@ -134,6 +136,11 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
           }
         }
       }
+    }
+    if (matched) {
+#pragma unroll
+      for (int u = 0; u < C::SPT; ++u)
+        mark_bits(matched, (uint64_t)ref.s_beg + b0 + (uint64_t)u * C::NT + (uint64_t)w * kWave, __ballot(m[u] != 0));
     }
     uint32_t wex[C::SPT];
 #pragma unroll

@@ -122,6 +122,13 @@ typedef struct eo_join_opts {
  * tuple.h:48-53) instead of 8-byte eo_pairs: `out` then points to 16-byte
  * rows and out_cap counts rows.  eo_hash_join, eo_nlj_join, eo_index_join. */
 #define EO_F_RESULT16 8
+/* Left outer join: every probe-side (S, the reference's "left") tuple with
+ * no match also yields one row whose R side is EO_NULL_PAYLOAD (RESULT16:
+ * rkey = rval = EO_NULL_PAYLOAD).  The reference names such a join
+ * (joins/outerJoin_gpu) but implements none.  eo_hash_join and eo_nlj_join
+ * (default order only) and eo_index_join. */
+#define EO_F_LEFT_OUTER 16
+#define EO_NULL_PAYLOAD ((int32_t)0x80000000)
 
 typedef struct eo_device_info {
   char name[64];
@@ -153,6 +160,13 @@ eo_status eo_device_query(eo_ctx *ctx, eo_device_info *info);
 
 eo_status eo_dev_alloc(eo_ctx *ctx, uint64_t bytes, void **dptr);
 eo_status eo_dev_free(eo_ctx *ctx, void *dptr);
+/* Pinned, device-mapped host memory (zero copy).  Passed as `out` with
+ * where = EO_DEVICE, the join kernels write their rows straight into host
+ * memory over PCIe, as the reference's cuMemHostAlloc(DEVICEMAP) result
+ * buffers (joins/hashJoin_gpu/main_street/hj.cc:152); inputs may live there
+ * too (read over PCIe on every pass). */
+eo_status eo_host_alloc(eo_ctx *ctx, uint64_t bytes, void **hptr);
+eo_status eo_host_free(eo_ctx *ctx, void *hptr);
 eo_status eo_copy_h2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 eo_status eo_copy_d2h(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 eo_status eo_copy_d2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);

@@ -236,3 +236,68 @@ def test_result16_rows_all_joins(ctx, det):
     assert rows.shape == (len(want), 4) and np.array_equal(rows[:, 1], rows[:, 3])
     assert same_multiset(rows[:, [0, 2]], want)
     ix.free()
+
+
+def test_zero_copy_host_output(ctx):
+    """Rows written by the kernels straight into pinned, device-mapped host memory."""
+    from eating_one_amd import DeviceArray, HostArray
+    R, S = gen.c2(seed=95, nR=1 << 16, nS=1 << 18, selectivity=0.5)
+    want = oracle.hash_join(R, S, nparts=1 << 10)
+    dR, dS = DeviceArray.from_host(ctx, R), DeviceArray.from_host(ctx, S)
+    h = HostArray(ctx, len(want) * 8)
+    n, st = ctx.hash_join_device(dR, len(R), dS, len(S), h, len(want))
+    assert st == 0 and n == len(want)
+    assert same_multiset(h.view(np.int32, (n, 2)).copy(), want)
+    h.close()
+
+
+def _outer_want(R, S, inner):
+    """Left outer join = the oracle's inner join plus (NULL, S payload) for every
+    S tuple whose key is absent from R (set membership, numpy.isin)."""
+    from eating_one_amd import _abi
+    miss = S[~np.isin(S[:, 0], R[:, 0])]
+    nul = np.stack([np.full(len(miss), _abi.EO_NULL_PAYLOAD, np.int32), miss[:, 1]], 1)
+    return np.concatenate([inner, nul]).astype(np.int32)
+
+
+@pytest.mark.parametrize("bits", [-1, 4, 13])
+def test_left_outer_hash_join(ctx, bits):
+    R, S = gen.c2(seed=97 + max(bits, 0), nR=1 << 17, nS=1 << 18, theta=0.5, selectivity=0.4)
+    want = _outer_want(R, S, oracle.hash_join(R, S, nparts=1 << 10))
+    kw = dict(radix_bits=bits, passes=2 if bits > 11 else 0)
+    got = ctx.hash_join(R, S, left_outer=True, **kw)
+    assert same_multiset(got, want)
+    assert ctx.hash_join(R, S, left_outer=True, count_only=True, **kw) == len(want)
+    rows = ctx.hash_join(R, S, left_outer=True, result16=True, **kw)
+    assert same_multiset(rows[:, [0, 2]], want)
+    nul = rows[:, 0] == -(1 << 31)
+    assert (rows[nul, 1] == -(1 << 31)).all() and np.array_equal(rows[:, 2] >= 0, np.ones(len(rows), bool))
+
+
+def test_left_outer_multi_chunk_partitions_and_empty_build(ctx):
+    # one key fills partitions beyond one LDS table: S tuples probe several R chunks
+    R = np.stack([np.full(40000, 3), np.arange(40000)], 1).astype(np.int32)
+    S = np.stack([np.array([3, 4, 5, 3] * 50), np.arange(200)], 1).astype(np.int32)
+    want = _outer_want(R, S, brute_join(R, S))
+    assert same_multiset(ctx.hash_join(R, S, left_outer=True, radix_bits=2), want)
+    assert same_multiset(ctx.nlj_join(R, S, left_outer=True), want)
+    e = np.empty((0, 2), np.int32)
+    assert same_multiset(ctx.hash_join(e, S, left_outer=True), _outer_want(e, S, e))
+    assert same_multiset(ctx.nlj_join(e, S, left_outer=True), _outer_want(e, S, e))
+    with pytest.raises(eo.EOError):
+        ctx.hash_join(R, S, left_outer=True, deterministic=True)
+
+
+def test_left_outer_nlj_and_index_joins(ctx):
+    R, S = gen.c3(seed=98, nR=30000, nS=50000, key_range=60000)
+    want = _outer_want(R, S, oracle.hash_join(R, S, nparts=1 << 10))
+    assert same_multiset(ctx.nlj_join(R, S, left_outer=True), want)
+    for ix in (ctx.hidx_build(R), ctx.sidx_build(R), ctx.sidx_build(R, learned=True)):
+        assert same_multiset(ix.join(S, left_outer=True), want)
+        assert same_multiset(ix.join(S, left_outer=True, deterministic=True), want)
+        assert ix.join(S, left_outer=True, count_only=True) == len(want)
+        ix.free()
+    ix = ctx.sidx_build(np.empty((0, 2), np.int32))
+    assert same_multiset(ix.join(S, left_outer=True), _outer_want(np.empty((0, 2), np.int32), S,
+                                                                  np.empty((0, 2), np.int32)))
+    ix.free()
