@@ -11,7 +11,7 @@ when the timed region starts; pairs written to a device buffer).
   cpu_baseline the CPU_JOIN hash join restated in oracle/ on a bounded sample
 
 Other configurations (parity cases; reported in the design notes, not the
-driver's line): --config c3-nlj | c3-hidx | c4 | c5.
+driver's line): --config c3-nlj | c3-hidx | c4 | c5 | nlj-dist.
 """
 from __future__ import annotations
 
@@ -248,6 +248,48 @@ def run_c3_nlj(a):
     return res
 
 
+def run_nlj_dist(a):
+    """Distance-predicate NLJ (the reference's CPU_JOIN/3d_cpu predicate) on
+    2^17 x 2^17 random 3-D points; FP64 roofline: 8 flops per compared pair
+    (3 subtractions, 3 products, 2 sums) against MI355X's 78.6 TFLOP/s."""
+    import numpy as np
+
+    import eating_one_amd as eo
+    from eating_one_amd import _abi
+
+    n = 1 << 17
+    rng = np.random.default_rng(0xE01D3)
+    pt = eo.Context.POINT3
+    R, S = np.zeros(n, dtype=pt), np.zeros(n, dtype=pt)
+    R["id"], S["id"] = np.arange(n), np.arange(n)
+    R["v"] = rng.uniform(0, 64, size=(n, 3))
+    S["v"] = rng.uniform(0, 64, size=(n, 3))
+    ctx = eo.Context(0)
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    L = ctx.lib
+    cnt = ctypes.c_uint64()
+    o = ctx.opts(where=_abi.EO_DEVICE)
+    assert L.eo_nlj_dist_join(ctx.handle, dR.ptr, n, dS.ptr, n, 1.0, None, 0, ctypes.byref(cnt), ctypes.byref(o)) == 0
+    dout = eo.DeviceArray(ctx, max(cnt.value, 1) * 8)
+    stats = _abi.JoinStats()
+    o = ctx.opts(where=_abi.EO_DEVICE, stats=stats)
+    t = []
+    for it in range(a.warmup + a.steps):
+        assert L.eo_nlj_dist_join(ctx.handle, dR.ptr, n, dS.ptr, n, 1.0, dout.ptr, cnt.value, ctypes.byref(cnt),
+                                  ctypes.byref(o)) == 0
+        if it >= a.warmup:
+            t.append(stats.ms_total)
+    ms = float(np.median(t))
+    pairs = float(n) * n
+    tflops = 8 * pairs / (ms * 1e-3) / 1e12
+    res = {"metric": "distance NLJ compared pairs/s, 2^17 x 2^17 3-D points", "value": pairs / (ms * 1e-3),
+           "unit": "pairs/s", "ms_per_step": ms, "n_out": cnt.value,
+           "roofline": {"bound": "fp64", "achieved": tflops, "peak": 78.6, "unit": "TFLOP/s", "frac": tflops / 78.6,
+                        "kernel": "nlj_dist"}}
+    ctx.close()
+    return res
+
+
 def run_c3_hidx(a):
     import eating_one_amd as eo
     from eating_one_amd import _abi, gen
@@ -426,7 +468,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist"])
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--c5-requests", type=int, default=4096)
     ap.add_argument("--c5-lo", type=float, default=10)
@@ -454,7 +496,7 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): this build has no multi-GPU path; "
               "the sharding design is described in DESIGN.md", file=sys.stderr)
         return 2
-    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5}[a.config]
+    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5, "nlj-dist": run_nlj_dist}[a.config]
     print(json.dumps(fn(a)), flush=True)
     return 0
 

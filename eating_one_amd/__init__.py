@@ -228,6 +228,26 @@ class Context:
         """Block nested-loop join (reference blockJoin_gpu join())."""
         return self._join(self.lib.eo_nlj_join, "eo_nlj_join", R, S, count_only, self.opts(**kw))
 
+    POINT3 = np.dtype([("id", np.int32), ("pad", np.int32), ("v", np.float64, (3,))])
+
+    def nlj_dist_join(self, R, S, distance: float = 1.0, count_only: bool = False, **kw):
+        """Distance-predicate NLJ over 3-D points (reference CPU_JOIN/3d_cpu join()):
+        R, S structured arrays of POINT3; returns (R id, S id) pairs."""
+        R = np.ascontiguousarray(R, dtype=self.POINT3)
+        S = np.ascontiguousarray(S, dtype=self.POINT3)
+        o = self.opts(**kw)
+        n = ctypes.c_uint64()
+        fn = self.lib.eo_nlj_dist_join
+        _check(fn(self.handle, _vp(R), len(R), _vp(S), len(S), float(distance), None, 0, ctypes.byref(n),
+                  ctypes.byref(o)), "eo_nlj_dist_join")
+        if count_only:
+            return n.value
+        out = np.empty((max(n.value, 1), 2), dtype=np.int32)
+        n2 = ctypes.c_uint64()
+        _check(fn(self.handle, _vp(R), len(R), _vp(S), len(S), float(distance), _vp(out), n.value, ctypes.byref(n2),
+                  ctypes.byref(o)), "eo_nlj_dist_join")
+        return out[: n.value]
+
     def hash_join_device(self, dR: DeviceArray, nR: int, dS: DeviceArray, nS: int, dout: DeviceArray | None,
                          out_cap: int, **kw):
         """Device-resident hash join; returns (n_out, status)."""

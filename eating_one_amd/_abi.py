@@ -192,6 +192,7 @@ SIGNATURES = {
     "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),
     "eo_hash_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_nlj_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_nlj_dist_join": (INT, [P, P, U64, P, U64, ctypes.c_double, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_hidx_build": (INT, [P, P, U64, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
     "eo_sidx_build": (INT, [P, P, U64, INT, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
     "eo_index_join": (INT, [P, P, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),

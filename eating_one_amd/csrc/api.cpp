@@ -775,4 +775,69 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   return check_device_errors(c, s);
 }
 
+eo_status eo_nlj_dist_join(eo_ctx *c, const eo_point3 *R, uint64_t nR, const eo_point3 *S, uint64_t nS,
+                           double distance, eo_pair *out, uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o) {
+  if (!c || !n_out || !opts_ok(o) || nR > kMaxRows || nS > kMaxRows) return EO_EINVAL;
+  if ((nR && !R) || (nS && !S) || (out_cap && !out) || !(distance >= 0.0)) return EO_EINVAL;
+  if (o && (o->flags & (EO_F_DETERMINISTIC | EO_F_RESULT16 | EO_F_LEFT_OUTER))) {
+    set_last_error("eo_nlj_dist_join: only the default output mode");
+    return EO_EINVAL;
+  }
+  *n_out = 0;
+  eo_join_stats *stats = o ? o->stats : nullptr;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (nR == 0 || nS == 0) return EO_OK;
+  DeviceGuard g(c->device);
+  hipStream_t s = pick_stream(c, o);
+  const bool host = where_of(o) == EO_HOST;
+  Layout L;
+  const size_t o_rin = L.add(host ? nR * sizeof(eo_point3) : 0);
+  const size_t o_sin = L.add(host ? nS * sizeof(eo_point3) : 0);
+  const size_t o_small = L.add(8 * sizeof(uint64_t));
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+  char *A = c->arena.base;
+  const eo_point3 *dR = host ? (eo_point3 *)(A + o_rin) : R;
+  const eo_point3 *dS = host ? (eo_point3 *)(A + o_sin) : S;
+  uint64_t *small = (uint64_t *)(A + o_small);
+  if (host) {
+    EO_TRY_HIP(hipMemcpyAsync((void *)dR, R, nR * sizeof(eo_point3), hipMemcpyHostToDevice, s), "h2d R");
+    EO_TRY_HIP(hipMemcpyAsync((void *)dS, S, nS * sizeof(eo_point3), hipMemcpyHostToDevice, s), "h2d S");
+  }
+  PhaseClock clk(c, s, stats != nullptr);
+  ScopedTimer tmr(c, stats);
+  eo_pair *dout = out;
+  bool staged = false;
+  if (host && out && out_cap) {
+    EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * sizeof(eo_pair), s), "output staging");
+    staged = true;
+  }
+  EO_TRY_HIP(fill_u64(small, 0, 1, s), "cursor reset");
+  clk.mark();
+  hipError_t e = nlj_dist(dR, nR, dS, nS, distance * distance, small, out ? dout : nullptr, out ? out_cap : 0, s);
+  clk.mark();
+  uint64_t total = 0;
+  eo_status st = EO_OK;
+  if (e == hipSuccess) {
+    st = read_u64(c, small, &total, 1, s);
+    if (st == EO_OK && staged && total && total <= out_cap)
+      e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+  }
+  if (staged) (void)hipFreeAsync(dout, s);
+  if (e != hipSuccess) return hip_fail(e, "distance join");
+  if (st != EO_OK) return st;
+  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+  st = check_device_errors(c, s);
+  *n_out = total;
+  if (stats) {
+    stats->n_out = total;
+    stats->ms_total = clk.ms(0, 1);
+    stats->ms_write = out ? clk.ms(0, 1) : 0.0;
+    stats->ms_count = out ? 0.0 : clk.ms(0, 1);
+    tmr.finish();
+  }
+  if (st != EO_OK) return st;
+  return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
+}
+
 }  // extern "C"

@@ -236,6 +236,11 @@ hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uin
 // positions >= out_cap counted only).
 hipError_t nlj_fused(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *cursor, eo_pair *out,
                      uint64_t out_cap, bool wide, uint32_t *matched, hipStream_t stream);
+// Distance-predicate NLJ over reference 3-D points: pairs (R id, S id) with
+// squared distance < d2, counted into *cursor (zero on entry), written when
+// out != NULL (positions >= out_cap counted only).
+hipError_t nlj_dist(const eo_point3 *R, uint64_t nR, const eo_point3 *S, uint64_t nS, double d2, uint64_t *cursor,
+                    eo_pair *out, uint64_t out_cap, hipStream_t stream);
 hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, const uint64_t *wave_off,
                      eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream);
 

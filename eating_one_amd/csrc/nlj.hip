@@ -228,6 +228,126 @@ __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ 
   }
 }
 
+// ---- distance predicate (FP64) ---------------------------------------------------
+// The multi-attribute predicate of joins/CPU_JOIN/3d_cpu/nestLoopJoin.cc:73-81:
+// a pair matches when (r0-s0)^2 + (r1-s1)^2 + (r2-s2)^2 < d2, accumulated in
+// that order with every subtraction, product and sum rounded on its own
+// (__dsub_rn / __dmul_rn / __dadd_rn: no fused multiply-add), which makes each
+// decision bit-identical to the reference's unfused x86-64 arithmetic.  Same
+// fused single-pass structure as k_nlj_fused: R tile (1024 points, SoA
+// doubles) in LDS, 4 S points per lane, OR-accumulated ballots per group of 8
+// R points, per-wave hit-group list, one output reservation per wave.
+constexpr int kDistNT = 256;
+constexpr int kDistNW = kDistNT / kWave;
+constexpr int kDistK = 4;
+constexpr int kDistSTile = kDistNT * kDistK;  // 1024
+constexpr int kDistRTile = 1024;
+constexpr int kDistGroup = 8;
+
+struct Point3 {
+  int32_t id, pad;
+  double v[3];
+};
+
+__device__ __forceinline__ bool dist_lt(double sx, double sy, double sz, double rx, double ry, double rz, double d2) {
+  const double t0 = __dsub_rn(rx, sx), t1 = __dsub_rn(ry, sy), t2 = __dsub_rn(rz, sz);
+  double acc = __dmul_rn(t0, t0);
+  acc = __dadd_rn(acc, __dmul_rn(t1, t1));
+  acc = __dadd_rn(acc, __dmul_rn(t2, t2));
+  return acc < d2;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(kDistNT) void k_nlj_dist(const Point3 *__restrict__ R, uint64_t nR,
+                                                      const Point3 *__restrict__ S, uint64_t nS, uint32_t s_tiles,
+                                                      double d2, uint64_t *cursor, uint2 *__restrict__ out,
+                                                      uint64_t out_cap) {
+  __shared__ double rx[kDistRTile], ry[kDistRTile], rz[kDistRTile];
+  __shared__ uint16_t hits[WRITE ? kDistNW : 1][WRITE ? kDistRTile / kDistGroup : 1];
+  const uint32_t b = blockIdx.x;
+  const uint32_t st = b % s_tiles, rt = b / s_tiles;
+  const uint64_t r0 = (uint64_t)rt * kDistRTile;
+  const uint32_t rn = (uint32_t)std::min<uint64_t>(kDistRTile, nR - r0);
+  for (uint32_t i = threadIdx.x; i < rn; i += kDistNT) {
+    const Point3 &p = R[r0 + i];
+    rx[i] = p.v[0];
+    ry[i] = p.v[1];
+    rz[i] = p.v[2];
+  }
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const uint64_t s0 = (uint64_t)st * kDistSTile + (uint64_t)w * kWave * kDistK;
+  double sx[kDistK], sy[kDistK], sz[kDistK];
+  uint64_t valid[kDistK];
+#pragma unroll
+  for (int k = 0; k < kDistK; ++k) {
+    const uint64_t i = s0 + (uint64_t)k * kWave + lane;
+    const bool ok = i < nS;
+    sx[k] = ok ? S[i].v[0] : 0.0;
+    sy[k] = ok ? S[i].v[1] : 0.0;
+    sz[k] = ok ? S[i].v[2] : 0.0;
+    valid[k] = __ballot(ok);
+  }
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  const uint32_t ngroups = (rn + kDistGroup - 1) / kDistGroup;
+  uint64_t total = 0;
+  uint32_t nh = 0;
+  for (uint32_t gi = 0; gi < ngroups; ++gi) {
+    const uint32_t g = gi * kDistGroup;
+    const uint32_t gn = rn - g < (uint32_t)kDistGroup ? rn - g : (uint32_t)kDistGroup;  // wave-uniform
+    uint64_t acc = 0;
+#pragma unroll
+    for (int q = 0; q < kDistGroup; ++q) {
+      const uint32_t j = g + (q < (int)gn ? q : 0);
+      const double x = rx[j], y = ry[j], z = rz[j];
+#pragma unroll
+      for (int k = 0; k < kDistK; ++k) acc |= __ballot(dist_lt(sx[k], sy[k], sz[k], x, y, z, d2));
+    }
+    if (!acc) continue;
+    uint64_t cnt = 0;
+    for (uint32_t q = 0; q < gn; ++q) {
+      const double x = rx[g + q], y = ry[g + q], z = rz[g + q];
+#pragma unroll
+      for (int k = 0; k < kDistK; ++k) cnt += __popcll(__ballot(dist_lt(sx[k], sy[k], sz[k], x, y, z, d2)) & valid[k]);
+    }
+    if (cnt) {
+      total += cnt;
+      if (WRITE) {
+        if (lane == 0) hits[w][nh] = (uint16_t)gi;
+        ++nh;
+      }
+    }
+  }
+  uint64_t base = 0;
+  if (total) {
+    if (lane == 0) base = atomic_add_agent(cursor, total);
+    base = read_lane(base, 0);
+  }
+  if (!WRITE || total == 0) return;
+  uint64_t running = 0;
+  for (uint32_t h = 0; h < nh; ++h) {
+    const uint32_t g = (uint32_t)hits[w][h] * kDistGroup;
+    const uint32_t gn = rn - g < (uint32_t)kDistGroup ? rn - g : (uint32_t)kDistGroup;
+    for (uint32_t q = 0; q < gn; ++q) {
+      const double x = rx[g + q], y = ry[g + q], z = rz[g + q];
+#pragma unroll
+      for (int k = 0; k < kDistK; ++k) {
+        const bool eq = dist_lt(sx[k], sy[k], sz[k], x, y, z, d2) && ((valid[k] >> lane) & 1);
+        const uint64_t m = __ballot(eq);
+        if (m) {
+          if (eq) {
+            const uint64_t pos = base + running + __popcll(m & lt);
+            const uint64_t si = s0 + (uint64_t)k * kWave + lane;
+            if (pos < out_cap) out[pos] = make_uint2((uint32_t)R[r0 + g + q].id, (uint32_t)S[si].id);
+          }
+          running += __popcll(m);
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 NljPlan plan_nlj(uint64_t nR, uint64_t nS) {
@@ -279,6 +399,24 @@ hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, con
   EO_TIMED("nlj_write", stream, hipLaunchKernelGGL(k_nlj<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
                      p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, nullptr, wave_off,
                      reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
+  return hipGetLastError();
+}
+
+hipError_t nlj_dist(const eo_point3 *R, uint64_t nR, const eo_point3 *S, uint64_t nS, double d2, uint64_t *cursor,
+                    eo_pair *out, uint64_t out_cap, hipStream_t stream) {
+  const uint64_t r_tiles = (nR + kDistRTile - 1) / kDistRTile, s_tiles = (nS + kDistSTile - 1) / kDistSTile;
+  const uint64_t grid = r_tiles * s_tiles;
+  if (grid == 0) return hipSuccess;
+  if (grid >= (1ull << 31)) return hipErrorInvalidValue;
+  const Point3 *vR = reinterpret_cast<const Point3 *>(R), *vS = reinterpret_cast<const Point3 *>(S);
+  if (out)
+    EO_TIMED("nlj_dist", stream,
+             hipLaunchKernelGGL(k_nlj_dist<true>, dim3((uint32_t)grid), dim3(kDistNT), 0, stream, vR, nR, vS, nS,
+                                (uint32_t)s_tiles, d2, cursor, reinterpret_cast<uint2 *>(out), out_cap));
+  else
+    EO_TIMED("nlj_dist_count", stream,
+             hipLaunchKernelGGL(k_nlj_dist<false>, dim3((uint32_t)grid), dim3(kDistNT), 0, stream, vR, nR, vS, nS,
+                                (uint32_t)s_tiles, d2, cursor, nullptr, 0));
   return hipGetLastError();
 }
 

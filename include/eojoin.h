@@ -231,6 +231,24 @@ eo_status eo_nlj_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
                       uint64_t out_cap, uint64_t *n_out,
                       const eo_join_opts *opts);
 
+/* A 3-D point of the reference's distance join (joins/CPU_JOIN/3d_cpu/
+ * nestLoopJoin.cc, TUPLE {int id; double val[VAL_NUM = 3]}, 32 bytes). */
+typedef struct eo_point3 {
+  int32_t id;
+  int32_t pad;
+  double v[3];
+} eo_point3;
+
+/* Nested-loop join with the reference's multi-attribute distance predicate
+ * (nestLoopJoin.cc:73-81): a pair matches when
+ * (r.v0-s.v0)^2 + (r.v1-s.v1)^2 + (r.v2-s.v2)^2 < distance^2, evaluated in FP64
+ * in that order without fused multiply-add (decisions identical to the
+ * reference's).  Rows are eo_pairs (R id, S id).  Same sizing and error
+ * conventions as eo_hash_join; the output order is unspecified. */
+eo_status eo_nlj_dist_join(eo_ctx *ctx, const eo_point3 *R, uint64_t nR, const eo_point3 *S,
+                           uint64_t nS, double distance, eo_pair *out, uint64_t out_cap,
+                           uint64_t *n_out, const eo_join_opts *opts);
+
 /* ---- resident indexes ------------------------------------------------------ */
 
 typedef enum eo_index_kind { EO_INDEX_HASH = 0, EO_INDEX_SORTED = 1 } eo_index_kind;

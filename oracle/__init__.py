@@ -51,6 +51,8 @@ def lib():
             getattr(L, name).restype = U64
         L.oracle_sidx_build.argtypes = [P, U64, P]
         L.oracle_sidx_build.restype = I
+        L.oracle_nlj_dist_join.argtypes = [P, U64, P, U64, ctypes.c_double, P, U64]
+        L.oracle_nlj_dist_join.restype = U64
         L.oracle_hash_join_mt.argtypes = [P, U64, P, U64, U32, I, P, U64]
         L.oracle_hash_join_mt.restype = U64
         _lib = L
@@ -102,6 +104,24 @@ def sidx_join(R, S, count_only: bool = False):
 def nlj_join(R, S, count_only: bool = False):
     """joins/CPU_JOIN/block_cpu/nestLoopJoin.cc:61-77."""
     return _run_join(lib().oracle_nlj_join, (), R, S, (), count_only)
+
+
+POINT3 = np.dtype([("id", np.int32), ("pad", np.int32), ("v", np.float64, (3,))])  # reference 3d TUPLE, 32 bytes
+
+
+def nlj_dist_join(R, S, distance: float = 1.0, count_only: bool = False):
+    """joins/CPU_JOIN/3d_cpu/nestLoopJoin.cc:73-101: pairs (R id, S id) with
+    squared distance < distance^2.  R, S: arrays of POINT3."""
+    R = np.ascontiguousarray(R, dtype=POINT3)
+    S = np.ascontiguousarray(S, dtype=POINT3)
+    fn = lib().oracle_nlj_dist_join
+    n = fn(_ptr(R), len(R), _ptr(S), len(S), float(distance), None, 0)
+    if count_only:
+        return n
+    out = np.empty((max(n, 1), 2), dtype=np.int32)
+    n2 = fn(_ptr(R), len(R), _ptr(S), len(S), float(distance), _ptr(out), n)
+    assert n2 == n
+    return out[:n]
 
 
 def hash_join_mt(R, S, nparts: int = 1 << 20, threads: int = 1, count_only: bool = False):

@@ -310,6 +310,40 @@ uint64_t oracle_nlj_join(const otuple *R, uint64_t nR, const otuple *S, uint64_t
   return n;
 }
 
+/* ---- distance-predicate nested loop (FP64) -----------------------------------
+ * joins/CPU_JOIN/3d_cpu/nestLoopJoin.cc: TUPLE {int id; double val[3]}
+ * (:17-21 of its tuple section), predicate eval() (:73-81): with
+ * d = r.val[i] - l.val[i], the sum d0*d0 + d1*d1 + d2*d2 accumulated left to
+ * right, each product and sum rounded separately (this file is built with
+ * -ffp-contract=off, as the reference's x86-64 build has no fused
+ * multiply-add), compared with DISTANCE^2 by `<`.  join() (:84-101) loops
+ * left (S) outside, right (R) inside and emits (lid, rid); here the pair is
+ * (R id, S id) like every other join of this file. */
+typedef struct {
+  int32_t id;
+  int32_t pad;
+  double v[3];
+} opoint3;
+
+static int dist_match(const opoint3 *l, const opoint3 *r, double d2) {
+  double acc = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double t = r->v[i] - l->v[i];
+    acc += t * t;
+  }
+  return acc < d2;
+}
+
+uint64_t oracle_nlj_dist_join(const opoint3 *R, uint64_t nR, const opoint3 *S, uint64_t nS, double distance,
+                              opair *out, uint64_t cap) {
+  const double d2 = distance * distance;
+  uint64_t n = 0;
+  for (uint64_t i = 0; i < nS; ++i)
+    for (uint64_t j = 0; j < nR; ++j)
+      if (dist_match(&S[i], &R[j], d2)) emit(out, cap, &n, R[j].id, S[i].id);
+  return n;
+}
+
 /* ---- N-core hash join ------------------------------------------------------
  * The pthread hash join of joins/multi_joins/hash/thread-hash-bnlj.cc:146-501:
  * per-thread histograms over static slices (lcount_partitioning, :146), a

@@ -301,3 +301,22 @@ def test_left_outer_nlj_and_index_joins(ctx):
     assert same_multiset(ix.join(S, left_outer=True), _outer_want(np.empty((0, 2), np.int32), S,
                                                                   np.empty((0, 2), np.int32)))
     ix.free()
+
+
+@pytest.mark.parametrize("nR,nS,dist", [(3000, 5000, 1.0), (1025, 1023, 0.5), (4100, 2100, 2.0)])
+def test_nlj_dist_join_equals_oracle(ctx, nR, nS, dist):
+    rng = np.random.default_rng(nR + nS)
+    R = np.zeros(nR, dtype=oracle.POINT3)
+    S = np.zeros(nS, dtype=oracle.POINT3)
+    R["id"], S["id"] = np.arange(nR), np.arange(nS) + 10 ** 6
+    R["v"] = rng.uniform(0, 12, size=(nR, 3))
+    S["v"] = rng.uniform(0, 12, size=(nS, 3))
+    # boundary pairs: S points at distance exactly dist, and one ulp inside, of R points
+    k = min(200, nR, nS // 2)
+    S["v"][:k] = R["v"][:k] + np.array([dist, 0.0, 0.0])
+    S["v"][k:2 * k] = R["v"][:k] + np.array([0.0, np.nextafter(dist, 0.0), 0.0])
+    want = oracle.nlj_dist_join(R, S, dist)
+    got = ctx.nlj_dist_join(R, S, dist)
+    assert len(got) == len(want) > 0
+    assert same_multiset(got, want)
+    assert ctx.nlj_dist_join(R, S, dist, count_only=True) == len(want)

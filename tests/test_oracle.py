@@ -190,3 +190,38 @@ def test_c2_generator_shape():
     assert 0.07 < inside.mean() < 0.13
     assert np.all(S[~inside, 0] >= (1 << 14))
     assert len(oracle.hash_join(R, S, nparts=256)) == int(inside.sum())
+
+
+def _points(rng, n, lo=0.0, hi=6.0):
+    P = np.zeros(n, dtype=oracle.POINT3)
+    P["id"] = np.arange(n)
+    P["v"] = rng.uniform(lo, hi, size=(n, 3))
+    return P
+
+
+def test_dist_join_oracle_against_numpy_and_boundaries():
+    """The distance predicate: numpy's elementwise float64 arithmetic in the same
+    order (t0*t0 + t1*t1 + t2*t2, each op rounded) decides every pair alike."""
+    rng = np.random.default_rng(3)
+    R, S = _points(rng, 300), _points(rng, 400)
+    got = oracle.nlj_dist_join(R, S, 1.0)
+    t = R["v"][None, :, :] - S["v"][:, None, :]  # r - s
+    d = t[..., 0] * t[..., 0] + t[..., 1] * t[..., 1] + t[..., 2] * t[..., 2]
+    si, ri = np.nonzero(d < 1.0)
+    want = np.stack([R["id"][ri], S["id"][si]], 1)
+    assert same_multiset(got, want) and len(got) > 0
+    # exact distance 1 is excluded (strict <), one ulp inside is included
+    P = np.zeros(3, dtype=oracle.POINT3)
+    P["id"] = [0, 1, 2]
+    P["v"] = [[0, 0, 0], [1, 0, 0], [np.nextafter(1.0, 0.0), 0, 0]]
+    pairs = {tuple(p) for p in oracle.nlj_dist_join(P, P, 1.0).tolist()}
+    assert (1, 0) not in pairs and (0, 1) not in pairs
+    assert (2, 0) in pairs and (0, 2) in pairs
+    # the reference's own data shape: integer values repeated in all 3 coordinates
+    # match only when equal
+    Q = np.zeros(50, dtype=oracle.POINT3)
+    Q["id"] = np.arange(50)
+    Q["v"] = np.repeat(rng.integers(0, 20, 50)[:, None], 3, 1)
+    n = oracle.nlj_dist_join(Q, Q, 1.0, count_only=True)
+    vals = Q["v"][:, 0]
+    assert n == int((vals[:, None] == vals[None, :]).sum())
