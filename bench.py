@@ -125,7 +125,7 @@ def run_c2(a):
     info = ctx.device_info()
     dR = eo.DeviceArray.from_host(ctx, R)
     dS = eo.DeviceArray.from_host(ctx, S)
-    keep = (R, S) if (not a.no_cpu and a.cpu_log2_r == a.log2_r) else None
+    keep = (R, S) if ((not a.no_cpu and a.cpu_log2_r == a.log2_r) or a.host_leg) else None
     del R, S
     kw = dict(radix_bits=a.radix_bits, passes=a.passes, lds_tuples=a.lds, s_tile=a.s_tile)
     n_out, _ = ctx.hash_join_device(dR, nR, dS, nS, None, 0, **kw)
@@ -195,6 +195,22 @@ def run_c2(a):
                      "kernels_ms_per_join": {k: v[0] / a.steps for k, v in kern.items()}},
         "copy_ceiling_gbs": ceiling,
     }
+    if a.host_leg:
+        # the PCIe-inclusive rate: host R and S in, pairs back to host memory (not `value`)
+        hR, hS = keep
+        hout = np.empty((n_out, 2), dtype=np.int32)
+        nn = ctypes.c_uint64()
+        oh = ctx.opts(where=_abi.EO_HOST, **kw)
+        tl = []
+        for it in range(3):
+            t1 = time.perf_counter()
+            st = ctx.lib.eo_hash_join(ctx.handle, hR.ctypes.data, nR, hS.ctypes.data, nS, hout.ctypes.data, n_out,
+                                      ctypes.byref(nn), ctypes.byref(oh))
+            tl.append(time.perf_counter() - t1)
+            assert st == 0 and nn.value == n_out
+        res["pcie_inclusive"] = {"ms_per_join": min(tl) * 1e3, "tuples_per_s": (nR + nS) / min(tl),
+                                 "note": "host R, S in and pairs out through pageable host memory"}
+        log(f"host path (PCIe-inclusive): {min(tl) * 1e3:.1f} ms per join")
     ctx.close()
     if not a.no_cpu:
         log("cpu baseline ...")
@@ -491,6 +507,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-log2-r", type=int, default=27)
     ap.add_argument("--cpu-threads", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): this build has no multi-GPU path; "
