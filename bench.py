@@ -83,7 +83,8 @@ def cpu_baseline_hash(sample_log2_r: int, theta: float, sel: float, threads: int
         "cores": threads,
         "kind": "port",
         "sample": f"oracle/cpu_join.c restatement of CPU_JOIN hash_cpu (partition + per-partition nested loop, "
-                  f"timer scope hj.cc:209-230), |R|=2^{sample_log2_r}, |S|=2^{sample_log2_r + 1}, "
+                  f"timer scope hj.cc:209-230{'; pthread version of multi_joins/hash' if threads > 1 else ''}), "
+                  f"|R|=2^{sample_log2_r}, |S|=2^{sample_log2_r + 1}, "
                   f"theta={theta}, selectivity={sel}, {nparts} partitions, {n} matches, {dt:.2f} s on "
                   f"{threads} of {ncpu} host threads ({model})",
     }
@@ -215,6 +216,10 @@ def run_c2(a):
     if not a.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline_hash(a.cpu_log2_r, a.theta, a.selectivity, a.cpu_threads, keep)
+        if a.cpu_mt_threads > 1:
+            # the multi_joins-style pthread restatement on the box's host share, reported beside it
+            mt = cpu_baseline_hash(a.cpu_log2_r, a.theta, a.selectivity, a.cpu_mt_threads, keep)
+            res["cpu_baseline"]["multi_thread"] = {k: mt[k] for k in ("value", "unit", "cores", "sample")}
     return res
 
 
@@ -506,6 +511,8 @@ def main(argv=None) -> int:
     ap.add_argument("--c4-log2-r", type=int, default=26)
     ap.add_argument("--cpu-log2-r", type=int, default=27)
     ap.add_argument("--cpu-threads", type=int, default=1)
+    ap.add_argument("--cpu-mt-threads", type=int, default=min(16, os.cpu_count() or 1),
+                    help="threads of the additional N-core CPU baseline (0/1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
     a = ap.parse_args(argv)
