@@ -378,9 +378,11 @@ class Table:
         return {"n": n.value, "radix_bits": bits.value, "bytes": nbytes.value}
 
     def free(self):
-        if self.handle:
+        # once its context is closed a table can no longer be freed through it;
+        # its device memory then goes back when the process exits
+        if self.handle and self.ctx.handle:
             self.ctx.lib.eo_table_free(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -442,9 +444,9 @@ class Batch:
         return self.wait()
 
     def free(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.ctx.lib.eo_batch_free(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -540,9 +542,9 @@ class Index:
         return keys, pays
 
     def free(self):
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.ctx.lib.eo_index_free(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -20,6 +20,8 @@
  *     out are then unspecified (with EO_F_DETERMINISTIC nothing is written);
  *   - output order is unspecified: compare results as multisets;
  *   - one eo_ctx per host thread; calls on one ctx are serialised by the caller;
+ *   - indexes, tables and batches belong to the ctx they were made on: free
+ *     them before eo_ctx_destroy of that ctx;
  *   - scratch memory comes from a ctx-owned device arena that only grows.
  */
 #ifndef EOJOIN_H

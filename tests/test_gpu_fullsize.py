@@ -1,0 +1,52 @@
+"""Parity at the full benchmark sizes through size-independent properties.
+
+The oracle cannot run 2^27 x 2^28 in seconds; at that size the checks are
+properties every correct result has and a plausible mistake breaks:
+  C2 (R a permutation of [0, 2^27), every S key drawn from R): the count is
+  |S|, every S payload appears exactly once, and the R and S keys of every
+  pair are equal (a swapped field, a dropped or duplicated tuple, or a wrong
+  partition would each fail one of them);
+  C4: the learned model's lower bound equals binary search for every one of
+  the 2^27 probes whose key is in R (and finds no equal key for the others).
+"""
+import numpy as np
+import pytest
+
+import eating_one_amd as eo
+from eating_one_amd import _abi, gen
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c2_full_size_properties(ctx):
+    nR, nS = 1 << 27, 1 << 28
+    R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS)
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    dout = eo.DeviceArray(ctx, nS * 8)
+    n, st = ctx.hash_join_device(dR, nR, dS, nS, dout, nS)
+    assert st == 0 and n == nS
+    pairs = dout.to_host(np.int32, (nS, 2))
+    dout.close()
+    assert np.array_equal(np.bincount(pairs[:, 1], minlength=nS), np.ones(nS, dtype=np.int64))
+    assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])
+
+
+def test_c4_full_size_learned_equals_binary(ctx):
+    """For every probe whose key is in R the learned lower bound equals binary
+    search; for an absent key the learned window holds no equal key."""
+    R, S = gen.c4(nR=1 << 26, nS=1 << 27)
+    ix = ctx.sidx_build(R, learned=True)
+    try:
+        lb = ix.lookup(S, method=_abi.EO_LOOKUP_BINARY)
+        ll = ix.lookup(S, method=_abi.EO_LOOKUP_LEARNED)
+        keys, _ = ix.export()
+    finally:
+        ix.free()
+    n = len(keys)
+    hit = lb < n
+    present = np.zeros(len(S), bool)
+    present[hit] = keys[lb[hit]] == S[hit, 0]
+    assert int(present.sum()) == int(round(0.8 * len(S)))        # 80 % of the probes are R keys
+    assert np.array_equal(ll[present], lb[present])
+    pos = np.minimum(ll[~present], n - 1).astype(np.int64)
+    assert not np.any(keys[pos] == S[~present, 0])
