@@ -253,7 +253,9 @@ void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
                                                      uint64_t *out_cursor, int wide, uint32_t *matched) {
   __shared__ tile::ProbeShared<C> sh;
-  const uint32_t t = xcd_local_index(blockIdx.x, gridDim.x);
+  // round-robin over the XCDs: the span tiles of one hot (skewed) partition
+  // are consecutive and must not all land on one XCD
+  const uint32_t t = blockIdx.x;
   SpanRef ref;
   if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
   tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched);

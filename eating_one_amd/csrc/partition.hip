@@ -29,6 +29,7 @@ using tile::kFastNT;
 using tile::kFastTile;
 
 constexpr int kPartNT = 512;
+constexpr int kMsdGroupsPerWg = 4;  // second-pass groups per persistent workgroup
 
 template <int BITS>
 struct PartCfg {
@@ -421,7 +422,9 @@ __global__ __launch_bounds__(256) void k_msd_reduce(const uint32_t *__restrict__
   }
 }
 
-// grp[dh] = number of second-pass stripes of segment dh (>= 1, <= G); grp[F1] = 0.
+// grp[dh] = number of second-pass groups of segment dh (>= 1, <= G); grp[F1] = 0
+// (the scan turns grp into group bases); grp[F1 + 1] is the work ticket of
+// k_msd_scatter, cleared here because this kernel runs just before it.
 __global__ void k_msd_groups(const uint32_t *__restrict__ off1, uint32_t G, uint32_t K, uint32_t F1, uint64_t n,
                              uint64_t L, uint32_t *__restrict__ grp) {
   const uint32_t dh = blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,38 +435,49 @@ __global__ void k_msd_groups(const uint32_t *__restrict__ off1, uint32_t G, uint
     grp[dh] = (uint32_t)(J < 1 ? 1 : (J > G ? G : J));
   } else if (dh == F1) {
     grp[F1] = 0;
+    grp[F1 + 1] = 0;
   }
 }
 
+// Second pass, persistent: each workgroup takes groups from the ticket until
+// none is left, so that segments of very different sizes (skewed keys) still
+// keep every CU busy to the end.  A group is a run of consecutive first-pass
+// chunks of one segment dh; it scatters by the low digit into the final
+// partitions, whose starts off2 the combined histogram already holds.
 template <int BL>
 __global__ __launch_bounds__(kFastNT) void k_msd_scatter(const uint2 *__restrict__ in, uint64_t n,
                                                          const uint32_t *__restrict__ off1, uint32_t G, uint32_t K,
                                                          uint32_t F1, const uint32_t *__restrict__ grp_base,
-                                                         const uint32_t *__restrict__ off2, uint2 *__restrict__ out,
-                                                         uint32_t *err) {
+                                                         uint32_t *ticket, const uint32_t *__restrict__ off2,
+                                                         uint2 *__restrict__ out, uint32_t *err) {
   constexpr uint32_t F2 = 1u << BL;
   __shared__ tile::ScatterShared<BL> sh;
-  // the grid is an upper bound on the group count: index groups round-robin
-  // over the XCDs (block b runs on XCD b % 8) so that the real ones spread
-  // over all of them
-  const uint32_t w = blockIdx.x;
-  if (w >= grp_base[F1]) return;
-  uint32_t lo = 0, hi = F1;  // grp_base[lo] <= w < grp_base[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (grp_base[mid] <= w) lo = mid; else hi = mid;
+  __shared__ uint32_t s_w;
+  const uint32_t total = grp_base[F1];
+  for (;;) {
+    if (threadIdx.x == 0) s_w = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t w = s_w;
+    if (w >= total) break;  // uniform: every workgroup leaves once the tickets pass total
+    uint32_t lo = 0, hi = F1;  // grp_base[lo] <= w < grp_base[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (grp_base[mid] <= w) lo = mid; else hi = mid;
+    }
+    const uint32_t dh = lo;
+    const uint32_t J = grp_base[dh + 1] - grp_base[dh], j = w - grp_base[dh];
+    const uint32_t g0 = (uint32_t)((uint64_t)j * G / J), g1 = (uint32_t)((uint64_t)(j + 1) * G / J);
+    const uint64_t c0 = (uint64_t)dh * G + g0, c1 = (uint64_t)dh * G + g1;
+    const uint64_t beg = off1[c0 * K];
+    const uint64_t end = c1 == (uint64_t)F1 * G ? n : off1[c1 * K];
+    for (uint32_t d = threadIdx.x; d < F2; d += kFastNT) {
+      sh.cursor[d] = off2[((uint64_t)dh * F2 + d) * G + g0];
+      sh.cnt[d] = 0;
+    }
+    __syncthreads();
+    tile::scatter_stripe<BL>(sh, in, beg, end, n, 0, 0u, out, err);
+    __syncthreads();  // sh and s_w are reused by the next group
   }
-  const uint32_t dh = lo;
-  const uint32_t J = grp_base[dh + 1] - grp_base[dh], j = w - grp_base[dh];
-  const uint32_t g0 = (uint32_t)((uint64_t)j * G / J), g1 = (uint32_t)((uint64_t)(j + 1) * G / J);
-  const uint64_t c0 = (uint64_t)dh * G + g0, c1 = (uint64_t)dh * G + g1;
-  const uint64_t beg = off1[c0 * K];
-  const uint64_t end = c1 == (uint64_t)F1 * G ? n : off1[c1 * K];
-  for (uint32_t d = threadIdx.x; d < F2; d += kFastNT) {
-    sh.cursor[d] = off2[((uint64_t)dh * F2 + d) * G + g0];
-    sh.cnt[d] = 0;
-  }
-  tile::scatter_stripe<BL>(sh, in, beg, end, n, 0, 0u, out, err);
 }
 
 __global__ void k_msd_starts(const uint32_t *__restrict__ off2, uint32_t G, uint32_t FT, uint64_t n,
@@ -489,15 +503,15 @@ Msd2Layout msd2_layout(const PartPassPlan &p1, int bl) {
   L.o_c2 = L.o_c1 + al(L.m1 * 4);
   L.o_part = L.o_c2 + al(L.m2 * 4);
   L.o_grp = L.o_part + al(L.mp * 4);
-  L.o_scan = L.o_grp + al((F1 + 1) * 4);
+  L.o_scan = L.o_grp + al((F1 + 2) * 4);
   L.total = L.o_scan + std::max({scan_scratch_bytes_u32(L.m1), scan_scratch_bytes_u32(L.m2),
                                  scan_scratch_bytes_u32(F1 + 1)});
   return L;
 }
 
 template <int BH, int BL>
-hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
-                       uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
+hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, uint32_t workers, const eo_tuple *in, eo_tuple *tmp,
+                       eo_tuple *out, uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
   constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL);
   const Msd2Layout L = msd2_layout(p1, BL);
   char *sc = (char *)scratch;
@@ -525,11 +539,11 @@ hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, const eo_tupl
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = scan_u32(grp, grp, F1 + 1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
   const uint64_t max_groups = F1 + p1.n / group_len + 1;
-  const uint32_t grid = (uint32_t)((max_groups + kXcds - 1) / kXcds * kXcds);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(max_groups, std::max<uint32_t>(workers, 1));
   EO_TIMED("seg_scatter", stream,
            hipLaunchKernelGGL(k_msd_scatter<BL>, dim3(grid), dim3(kFastNT), 0, stream,
-                              reinterpret_cast<const uint2 *>(tmp), p1.n, c1, p1.stripes, p1.hist_split, F1, grp, c2,
-                              reinterpret_cast<uint2 *>(out), err));
+                              reinterpret_cast<const uint2 *>(tmp), p1.n, c1, p1.stripes, p1.hist_split, F1, grp,
+                              grp + F1 + 1, c2, reinterpret_cast<uint2 *>(out), err));
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_msd_starts, dim3((FT + 1 + 255) / 256), dim3(256), 0, stream, c2, p1.stripes, FT, p1.n,
                      part_start);
@@ -658,8 +672,10 @@ static PartPassPlan msd2_pass1(uint64_t n, int bh, int bl, int cus) {
 }
 
 uint64_t msd2_group_len(uint64_t n, int cus) {
-  // about one wave of second-pass workgroups (kFastWgPerCu per CU)
-  const uint64_t len = n / ((uint64_t)std::max(cus, 8) * tile::kFastWgPerCu);
+  // kMsdGroupsPerWg groups per persistent second-pass workgroup (kFastWgPerCu
+  // per CU): small enough that the dynamic hand-out evens out segments of
+  // unequal size, large enough that a group spans several tiles
+  const uint64_t len = n / ((uint64_t)std::max(cus, 8) * tile::kFastWgPerCu * kMsdGroupsPerWg);
   return std::max<uint64_t>(len, (uint64_t)kFastTile);
 }
 
@@ -673,10 +689,11 @@ hipError_t partition_msd2(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tup
   if (((uintptr_t)in & 15) || ((uintptr_t)tmp & 15) || ((uintptr_t)out & 7)) return hipErrorInvalidValue;
   const PartPassPlan p1 = msd2_pass1(n, bh, bl, cus);
   const uint64_t L = msd2_group_len(n, cus);
+  const uint32_t workers = (uint32_t)std::max(cus, 8) * tile::kFastWgPerCu;
   switch (bh * 16 + bl) {
 #define EO_MSD_CASE(BH, BL) \
   case BH * 16 + BL:        \
-    return launch_msd2<BH, BL>(p1, L, in, tmp, out, part_start, scratch, err, stream);
+    return launch_msd2<BH, BL>(p1, L, workers, in, tmp, out, part_start, scratch, err, stream);
 #define EO_MSD_ROW(BH) \
   EO_MSD_CASE(BH, 1) EO_MSD_CASE(BH, 2) EO_MSD_CASE(BH, 3) EO_MSD_CASE(BH, 4) EO_MSD_CASE(BH, 5) EO_MSD_CASE(BH, 6) \
   EO_MSD_CASE(BH, 7)

@@ -320,3 +320,20 @@ def test_nlj_dist_join_equals_oracle(ctx, nR, nS, dist):
     assert len(got) == len(want) > 0
     assert same_multiset(got, want)
     assert ctx.nlj_dist_join(R, S, dist, count_only=True) == len(want)
+
+
+@pytest.mark.parametrize("hot", [0.3, 0.7, 1.0])
+def test_two_pass_hot_key(ctx, hot):
+    """One key holding up to all of S: one first-pass segment far larger than the
+    rest, handed out to the persistent second pass in many groups."""
+    rng = np.random.default_rng(int(hot * 10))
+    nR, nS = 1 << 20, 1 << 21
+    R = np.stack([rng.permutation(nR), np.arange(nR)], 1).astype(np.int32)
+    keys = rng.integers(0, nR, nS)
+    keys[rng.random(nS) < hot] = 777
+    S = np.stack([keys, np.arange(nS)], 1).astype(np.int32)
+    got = ctx.hash_join(R, S, radix_bits=13, passes=2)
+    assert len(got) == nS
+    # every S tuple matches the single R tuple with its key (R payload = row)
+    assert np.array_equal(np.sort(got[:, 1]), np.arange(nS))
+    assert np.array_equal(R[got[:, 0], 0], S[got[:, 1], 0])
