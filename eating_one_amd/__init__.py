@@ -248,6 +248,41 @@ class Context:
                   ctypes.byref(o)), "eo_nlj_dist_join")
         return out[: n.value]
 
+    def stream_join(self, R, S, op: str = "hash", r_tile: int = 0, s_chunk: int = 0, out_chunk: int = 0,
+                    result16: bool = False, left_outer: bool = False, count_only: bool = False, out=None,
+                    out_cap: int | None = None, stats: "_abi.StreamStats | None" = None, join_opts=None):
+        """Out-of-core streaming join over host-resident R and S (eo_stream_join): R in
+        resident tiles of r_tile tuples, S streamed in chunks of s_chunk tuples.  R, S may be
+        numpy arrays or HostArray views (pinned: copied by DMA directly).  `out` (optional): a
+        preallocated (rows, 2|4) int32 array or HostArray view; then returns (n_out, status).
+        Otherwise returns the rows (count_only: the count)."""
+        R, S = _tuples(R), _tuples(S)
+        o = _abi.StreamOpts()
+        o.struct_size = ctypes.sizeof(_abi.StreamOpts)
+        o.op = {"hash": _abi.EO_STREAM_HASH, "nlj": _abi.EO_STREAM_NLJ}[op]
+        o.r_tile, o.s_chunk, o.out_chunk = r_tile, s_chunk, out_chunk
+        o.flags = (_abi.EO_F_RESULT16 if result16 else 0) | (_abi.EO_F_LEFT_OUTER if left_outer else 0)
+        o.join = ctypes.pointer(join_opts) if join_opts is not None else None
+        o.stats = ctypes.pointer(stats) if stats is not None else None
+        n = ctypes.c_uint64()
+        fn = self.lib.eo_stream_join
+        if out is not None:
+            cap = len(out) if out_cap is None else out_cap
+            st = fn(self.handle, _vp(R), len(R), _vp(S), len(S), _vp(out), cap, ctypes.byref(n), ctypes.byref(o))
+            if st not in (_abi.EO_OK, _abi.EO_ECAPACITY):
+                _check(st, "eo_stream_join")
+            return n.value, st
+        _check(fn(self.handle, _vp(R), len(R), _vp(S), len(S), None, 0, ctypes.byref(n), ctypes.byref(o)),
+               "eo_stream_join")
+        if count_only:
+            return n.value
+        rows = np.empty((max(n.value, 1), 4 if result16 else 2), dtype=np.int32)
+        n2 = ctypes.c_uint64()
+        _check(fn(self.handle, _vp(R), len(R), _vp(S), len(S), _vp(rows), n.value, ctypes.byref(n2),
+                  ctypes.byref(o)), "eo_stream_join")
+        assert n2.value == n.value
+        return rows[: n.value]
+
     def hash_join_device(self, dR: DeviceArray, nR: int, dS: DeviceArray, nS: int, dout: DeviceArray | None,
                          out_cap: int, **kw):
         """Device-resident hash join; returns (n_out, status)."""

@@ -164,6 +164,34 @@ class TuneResult(ctypes.Structure):
     ]
 
 
+class StreamStats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double),
+        ("h2d_bytes", U64),
+        ("d2h_bytes", U64),
+        ("n_out", U64),
+        ("r_tiles", U32),
+        ("s_chunks", U32),
+        ("regrown", U32),
+        ("pinned", U32),
+    ]
+
+
+class StreamOpts(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", U32),
+        ("op", I32),
+        ("r_tile", U64),
+        ("s_chunk", U64),
+        ("out_chunk", U64),
+        ("flags", I32),
+        ("reserved", I32),
+        ("join", ctypes.POINTER(JoinOpts)),
+        ("stats", ctypes.POINTER(StreamStats)),
+    ]
+
+
+EO_STREAM_HASH, EO_STREAM_NLJ = 0, 1
 EO_SERVE_BATCHED, EO_SERVE_PER_JOIN = 0, 1
 EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
 EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
@@ -193,6 +221,7 @@ SIGNATURES = {
     "eo_hash_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_nlj_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_nlj_dist_join": (INT, [P, P, U64, P, U64, ctypes.c_double, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_stream_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(StreamOpts)]),
     "eo_hidx_build": (INT, [P, P, U64, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
     "eo_sidx_build": (INT, [P, P, U64, INT, ctypes.POINTER(JoinOpts), ctypes.POINTER(P)]),
     "eo_index_join": (INT, [P, P, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),

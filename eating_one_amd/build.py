@@ -36,6 +36,7 @@ SOURCES = [
     "plan.cpp",
     "tuner.cpp",
     "serve.cpp",
+    "stream.cpp",
 ]
 
 ARCH = os.environ.get("EO_OFFLOAD_ARCH", "gfx950")

@@ -50,6 +50,34 @@ void Arena::release() {
   cap = 0;
 }
 
+hipError_t PinnedBuf::reserve(size_t bytes) {
+  if (bytes <= cap) return hipSuccess;
+  release();
+  const size_t want = (std::max<size_t>(bytes, (size_t)16 << 20) + (1u << 21) - 1) & ~(size_t)((1u << 21) - 1);
+  hipError_t e = hipHostMalloc((void **)&base, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    base = nullptr;
+    cap = 0;
+    return e;
+  }
+  cap = want;
+  return hipSuccess;
+}
+
+void PinnedBuf::release() {
+  if (base) (void)hipHostFree(base);
+  base = nullptr;
+  cap = 0;
+}
+
+bool is_pinned(const eo_ctx *c, const void *p, size_t bytes) {
+  const uintptr_t a = (uintptr_t)p;
+  auto it = c->pinned.upper_bound(a);
+  if (it == c->pinned.begin()) return false;
+  --it;
+  return a >= it->first && a + bytes <= it->first + it->second;
+}
+
 eo_status read_u64(eo_ctx *ctx, const uint64_t *dev, uint64_t *host, int n, hipStream_t stream) {
   EO_TRY_HIP(hipMemcpyAsync(ctx->h_pinned, dev, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, stream), "readback");
   EO_TRY_HIP(hipStreamSynchronize(stream), "readback sync");
@@ -224,6 +252,12 @@ eo_status eo_ctx_destroy(eo_ctx *c) {
   for (eo_batch *b : c->serve_pool) eo_batch_free(b);
   c->serve_pool.clear();
   c->arena.release();
+  for (hipStream_t &x : c->xfer)
+    if (x) {
+      (void)hipStreamSynchronize(x);
+      (void)hipStreamDestroy(x);
+    }
+  c->stage.release();
   for (auto &ev : c->ev)
     if (ev) (void)hipEventDestroy(ev);
   for (auto &ev : c->timer.ev)
@@ -270,6 +304,7 @@ eo_status eo_host_alloc(eo_ctx *c, uint64_t bytes, void **hptr) {
   DeviceGuard g(c->device);
   *hptr = nullptr;
   EO_TRY_HIP(hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocMapped | hipHostMallocPortable), "eo_host_alloc");
+  c->pinned[(uintptr_t)*hptr] = bytes ? bytes : 1;
   return EO_OK;
 }
 
@@ -277,6 +312,7 @@ eo_status eo_host_free(eo_ctx *c, void *hptr) {
   if (!c) return EO_EINVAL;
   if (!hptr) return EO_OK;
   DeviceGuard g(c->device);
+  c->pinned.erase((uintptr_t)hptr);
   EO_TRY_HIP(hipHostFree(hptr), "eo_host_free");
   return EO_OK;
 }
@@ -572,8 +608,8 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     clk.mark();  // 3
     if (e == hipSuccess) {
       st = read_u64(c, small + 1, &total, 1, s);
-      if (st == EO_OK && staged && total <= out_cap && total)
-        e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
+      if (st == EO_OK && staged && total)  // on overflow: the out_cap rows that were written
+        e = hipMemcpyAsync(out, dout, std::min(total, out_cap) * row, hipMemcpyDeviceToHost, s);
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "fused probe");
@@ -724,8 +760,8 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
     eo_status st = EO_OK;
     if (e == hipSuccess) {
       st = read_u64(c, small, &total, 1, s);
-      if (st == EO_OK && staged && total && total <= out_cap)
-        e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
+      if (st == EO_OK && staged && total)  // on overflow: the out_cap rows that were written
+        e = hipMemcpyAsync(out, dout, std::min(total, out_cap) * row, hipMemcpyDeviceToHost, s);
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "nlj");
@@ -820,8 +856,8 @@ eo_status eo_nlj_dist_join(eo_ctx *c, const eo_point3 *R, uint64_t nR, const eo_
   eo_status st = EO_OK;
   if (e == hipSuccess) {
     st = read_u64(c, small, &total, 1, s);
-    if (st == EO_OK && staged && total && total <= out_cap)
-      e = hipMemcpyAsync(out, dout, total * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
+    if (st == EO_OK && staged && total)  // on overflow: the out_cap rows that were written
+      e = hipMemcpyAsync(out, dout, std::min(total, out_cap) * sizeof(eo_pair), hipMemcpyDeviceToHost, s);
   }
   if (staged) (void)hipFreeAsync(dout, s);
   if (e != hipSuccess) return hip_fail(e, "distance join");

@@ -347,8 +347,8 @@ eo_status eo_index_join(eo_ctx *c, const eo_index *ix, const eo_tuple *S, uint64
     uint64_t total = 0;
     if (e == hipSuccess) {
       st = read_u64(c, small, &total, 1, s);
-      if (st == EO_OK && staged && total && total <= out_cap)
-        e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
+      if (st == EO_OK && staged && total)  // on overflow: the out_cap rows that were written
+        e = hipMemcpyAsync(out, dout, std::min(total, out_cap) * row, hipMemcpyDeviceToHost, s);
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "index join");

@@ -41,6 +41,15 @@ struct TunedPlan {
   double ms;
 };
 
+// Pinned host buffer that only grows between calls (staging of the
+// streaming join when the caller's buffers are pageable).
+struct PinnedBuf {
+  char *base = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes);  // may free and reallocate (nothing in flight)
+  void release();
+};
+
 struct DeviceProps {
   std::string name, arch;
   int cus = 0;
@@ -62,6 +71,9 @@ struct eo_ctx {
   eo::LaunchTimer timer;  // per-launch events, installed while a call collects stats
   std::vector<eo_batch *> serve_pool;  // batch objects reused by eo_serve_run
   std::map<eo::TuneKey, eo::TunedPlan> tuned;  // autotuner results by size class
+  std::map<uintptr_t, size_t> pinned;           // eo_host_alloc blocks: start -> bytes
+  hipStream_t xfer[2] = {};                     // copy streams of eo_stream_join (H2D, D2H)
+  eo::PinnedBuf stage;                          // its pinned staging
 };
 
 namespace eo {
@@ -102,6 +114,9 @@ TuneKey tune_key(uint64_t nR, uint64_t nS);
 void apply_tuned(const eo_ctx *c, uint64_t nR, uint64_t nS, eo_join_opts *o);
 // Folds a launch timer's events into stats->kernels[] (stream synchronised).
 void fold_timer(const LaunchTimer &t, eo_join_stats *stats);
+
+// True when [p, p + bytes) lies inside one eo_host_alloc block of ctx.
+bool is_pinned(const eo_ctx *ctx, const void *p, size_t bytes);
 
 inline hipStream_t pick_stream(eo_ctx *ctx, const eo_join_opts *o) {
   return (o && o->stream) ? (hipStream_t)o->stream : ctx->stream;

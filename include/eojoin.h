@@ -251,6 +251,50 @@ eo_status eo_nlj_dist_join(eo_ctx *ctx, const eo_point3 *R, uint64_t nR, const e
                            uint64_t nS, double distance, eo_pair *out, uint64_t out_cap,
                            uint64_t *n_out, const eo_join_opts *opts);
 
+/* ---- out-of-core streaming join ------------------------------------------- */
+
+/* Host-resident R and S of any size, also beyond device memory: R is taken
+ * r_tile tuples at a time (uploaded, and partitioned once per tile for the
+ * hash join); S streams through the device s_chunk tuples at a time, chunk
+ * i + 1 crossing PCIe while chunk i is joined and the rows of chunk i - 1
+ * travel back.  Replaces the host tile loop of
+ * joins/blockJoin_gpu/main_street/nestLoopJoin.cc:323-519 (HtoD of a tile
+ * pair, count, presum, join, DtoH, append; every step synchronous). */
+typedef enum eo_stream_op { EO_STREAM_HASH = 0, EO_STREAM_NLJ = 1 } eo_stream_op;
+
+typedef struct eo_stream_stats {
+  double ms_total;      /* wall time of the call */
+  uint64_t h2d_bytes;   /* bytes copied host -> device (R tiles and S chunks) */
+  uint64_t d2h_bytes;   /* bytes of rows copied device -> host */
+  uint64_t n_out;       /* match count */
+  uint32_t r_tiles;     /* R tiles */
+  uint32_t s_chunks;    /* S chunks joined, over all R tiles */
+  uint32_t regrown;     /* chunks joined again after their rows outgrew the staging */
+  uint32_t pinned;      /* bit 0: R, bit 1: S, bit 2: out were eo_host_alloc memory */
+} eo_stream_stats;
+
+typedef struct eo_stream_opts {
+  uint32_t struct_size;    /* sizeof(eo_stream_opts) */
+  int32_t op;              /* eo_stream_op */
+  uint64_t r_tile;         /* R tuples resident at once (0: all of R; at most 2^31 - 1) */
+  uint64_t s_chunk;        /* S tuples per chunk (0: 2^24; at most 2^31 - 1) */
+  uint64_t out_chunk;      /* device rows staged per chunk (0: s_chunk); grows when outgrown */
+  int32_t flags;           /* EO_F_RESULT16; EO_F_LEFT_OUTER when R fits one tile */
+  int32_t reserved;
+  eo_join_opts *join;      /* optional plan overrides of the hash join per R tile (where ignored) */
+  eo_stream_stats *stats;  /* optional out */
+} eo_stream_opts;
+
+/* R, S and out are host memory; blocks from eo_host_alloc are read and
+ * written by DMA directly, other memory through pinned staging buffers that
+ * host threads fill and drain.  nR and nS may exceed 2^31 (payloads stay the
+ * caller's int32 values).  out == NULL counts only; out_cap < count returns
+ * EO_ECAPACITY with *n_out = count and out holding the first out_cap rows
+ * produced.  Rows are R-tile by R-tile, chunk by chunk; the order inside a
+ * chunk is unspecified. */
+eo_status eo_stream_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS,
+                         eo_pair *out, uint64_t out_cap, uint64_t *n_out, const eo_stream_opts *opts);
+
 /* ---- resident indexes ------------------------------------------------------ */
 
 typedef enum eo_index_kind { EO_INDEX_HASH = 0, EO_INDEX_SORTED = 1 } eo_index_kind;

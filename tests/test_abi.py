@@ -73,6 +73,7 @@ def test_invalid_arguments_rejected_before_device_use():
     n = ctypes.c_uint64()
     assert L.eo_hash_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL
     assert L.eo_nlj_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL
+    assert L.eo_stream_join(None, None, 0, None, 0, None, 0, ctypes.byref(n), None) == _abi.EO_EINVAL
 
 
 def test_struct_layouts_match_the_header(tmp_path):
@@ -85,9 +86,10 @@ def test_struct_layouts_match_the_header(tmp_path):
     src.write_text('''#include <stdio.h>
 #include "eojoin.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(eo_join_opts), sizeof(eo_join_stats),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(eo_join_opts), sizeof(eo_join_stats),
          sizeof(eo_kernel_stat), sizeof(eo_ctx_opts), sizeof(eo_device_info), sizeof(eo_index_info),
-         sizeof(eo_request), sizeof(eo_result), sizeof(eo_tuple));
+         sizeof(eo_request), sizeof(eo_result), sizeof(eo_stream_opts), sizeof(eo_stream_stats),
+         sizeof(eo_tuple));
   return 0;
 }
 ''')
@@ -95,7 +97,8 @@ int main(void) {
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(t) for t in (_abi.JoinOpts, _abi.JoinStats, _abi.KernelStat, _abi.CtxOpts,
-                                       _abi.DeviceInfo, _abi.IndexInfo, _abi.Request, _abi.Result)] + [8]
+                                       _abi.DeviceInfo, _abi.IndexInfo, _abi.Request, _abi.Result,
+                                       _abi.StreamOpts, _abi.StreamStats)] + [8]
     assert got == want
 
 
