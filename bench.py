@@ -223,6 +223,82 @@ def run_c2(a):
     return res
 
 
+def run_stream(a):
+    """C2 sizes held in host memory, joined by eo_stream_join (R resident, S streamed
+    in chunks, rows copied back): the PCIe-bound out-of-core path."""
+    import numpy as np
+
+    import eating_one_amd as eo
+    from eating_one_amd import _abi, gen
+
+    nR, nS = 1 << a.log2_r, 1 << (a.log2_r + 1)
+    log(f"generating C2 |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1} in host memory")
+    R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS, theta=a.theta, selectivity=a.selectivity)
+    ctx = eo.Context(0)
+    n_out = ctx.stream_join(R, S, s_chunk=a.s_chunk, count_only=True)
+    hR, hS, hO = eo.HostArray(ctx, R.nbytes), eo.HostArray(ctx, S.nbytes), eo.HostArray(ctx, 8 * max(n_out, 1))
+    vR, vS, vO = hR.view(np.int32, R.shape), hS.view(np.int32, S.shape), hO.view(np.int32, (max(n_out, 1), 2))
+    vR[:] = R
+    vS[:] = S
+    # PCIe rates of plain pinned copies (1 GiB each way) for the roofline
+    cb = min(1 << 30, S.nbytes, 8 * max(n_out, 1))
+    dbuf = eo.DeviceArray(ctx, cb)
+    rates = {}
+    for name, fn, dst, src in (("h2d", ctx.lib.eo_copy_h2d, dbuf.ptr, hS.ptr), ("d2h", ctx.lib.eo_copy_d2h, hO.ptr, dbuf.ptr)):
+        fn(ctx.handle, dst, src, cb)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            assert fn(ctx.handle, dst, src, cb) == 0
+        rates[name] = 3 * cb / (time.perf_counter() - t0) / 1e9
+    dbuf.close()
+    legs = {}
+    for leg, (r, s_, o) in (("pinned", (vR, vS, vO)), ("pageable", (R, S, np.empty((max(n_out, 1), 2), np.int32)))):
+        times = []
+        st = _abi.StreamStats()
+        for it in range(a.warmup + a.steps):
+            t0 = time.perf_counter()
+            n, status = ctx.stream_join(r, s_, s_chunk=a.s_chunk, out=o, stats=st)
+            dt = time.perf_counter() - t0
+            assert status == _abi.EO_OK and n == n_out
+            if it >= a.warmup:
+                times.append(dt)
+        ms = 1e3 * sum(times) / len(times)
+        legs[leg] = {"ms": ms, "tuples_per_s": (nR + nS) / (ms * 1e-3),
+                     "pcie_gbs": (st.h2d_bytes + st.d2h_bytes) / (ms * 1e-3) / 1e9,
+                     "h2d_bytes": st.h2d_bytes, "d2h_bytes": st.d2h_bytes, "s_chunks": st.s_chunks,
+                     "pinned_flags": st.pinned}
+        log(f"{leg}: {ms:.1f} ms per join, {legs[leg]['pcie_gbs']:.1f} GB/s over PCIe")
+    # the whole-relation host path for comparison (copies in, joins, copies out)
+    o = ctx.opts()
+    out = np.empty((max(n_out, 1), 2), np.int32)
+    n = ctypes.c_uint64()
+    times = []
+    for it in range(1 + min(a.steps, 3)):
+        t0 = time.perf_counter()
+        assert ctx.lib.eo_hash_join(ctx.handle, R.ctypes.data, nR, S.ctypes.data, nS, out.ctypes.data, n_out,
+                                    ctypes.byref(n), ctypes.byref(o)) == 0
+        if it:
+            times.append(time.perf_counter() - t0)
+    whole_ms = 1e3 * sum(times) / len(times)
+    pin = legs["pinned"]
+    moved = pin["h2d_bytes"] + pin["d2h_bytes"]
+    # the bound: each direction at its plain-copy rate, both directions at once
+    floor_ms = 1e3 * max(pin["h2d_bytes"] / (rates["h2d"] * 1e9), pin["d2h_bytes"] / (rates["d2h"] * 1e9))
+    res = {"metric": "streamed join tuples/s, C2 2^27 x 2^28 in host memory", "value": pin["tuples_per_s"],
+           "unit": "tuples/s", "ms_per_step": pin["ms"], "steps": a.steps, "warmup": a.warmup,
+           "config": {"workload": f"C2 hash join |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1} from host memory, "
+                                  f"s_chunk={a.s_chunk or 1 << 24}", "n_out": n_out},
+           "legs": legs, "whole_relation_host_path_ms": whole_ms, "pcie_copy_gbs": rates,
+           "roofline": {"bound": "pcie", "achieved": moved / (pin["ms"] * 1e-3) / 1e9,
+                        "peak": rates["h2d"] + rates["d2h"], "unit": "GB/s",
+                        "frac_of_duplex_floor": floor_ms / pin["ms"]}}
+    hO.close()
+    hS.close()
+    hR.close()
+    ctx.close()
+    return res
+
+
 def run_c3_nlj(a):
     import eating_one_amd as eo
     from eating_one_amd import _abi, gen
@@ -489,7 +565,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist", "stream"])
+    ap.add_argument("--s-chunk", type=int, default=0, help="stream: S tuples per chunk (0: library default)")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--c5-requests", type=int, default=4096)
     ap.add_argument("--c5-lo", type=float, default=10)
@@ -517,10 +594,11 @@ def main(argv=None) -> int:
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): this build has no multi-GPU path; "
-              "the sharding design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): the library has no multi-GPU path",
+              file=sys.stderr)
         return 2
-    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5, "nlj-dist": run_nlj_dist}[a.config]
+    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,
+          "nlj-dist": run_nlj_dist, "stream": run_stream}[a.config]
     print(json.dumps(fn(a)), flush=True)
     return 0
 

@@ -511,9 +511,20 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     return EO_EINVAL;
   }
 
+  const bool host = where_of(o) == EO_HOST;
+  if (host && !stats && !(o && o->stream) && !(o && (o->flags & EO_F_DETERMINISTIC)) && nS >= kStreamMinS) {
+    // Large host-resident S: stream it in chunks past R (copies in both
+    // directions overlapped with the join) instead of copying both relations
+    // whole, joining, and copying the rows back.
+    eo_stream_opts so{};
+    so.struct_size = sizeof(so);
+    so.op = EO_STREAM_HASH;
+    so.flags = o ? (o->flags & (EO_F_RESULT16 | EO_F_LEFT_OUTER)) : 0;
+    so.join = &eff;
+    return eo_stream_join(c, R, nR, S, nS, out, out_cap, n_out, &so);
+  }
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
-  const bool host = where_of(o) == EO_HOST;
   const uint32_t P = 1u << hp.bits;
 
   Layout L;

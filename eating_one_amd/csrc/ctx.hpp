@@ -96,6 +96,8 @@ eo_status check_device_errors(eo_ctx *ctx, hipStream_t stream);
 eo_status read_u64(eo_ctx *ctx, const uint64_t *dev, uint64_t *host, int n, hipStream_t stream);
 
 constexpr uint64_t kMaxRows = (1ull << 31) - 1;
+// Host-mode hash joins with at least this many S tuples run as streaming joins.
+constexpr uint64_t kStreamMinS = 1ull << 22;
 
 struct HashPlan;
 bool opts_ok(const eo_join_opts *o);

@@ -281,7 +281,7 @@ typedef struct eo_stream_opts {
   uint64_t out_chunk;      /* device rows staged per chunk (0: s_chunk); grows when outgrown */
   int32_t flags;           /* EO_F_RESULT16; EO_F_LEFT_OUTER when R fits one tile */
   int32_t reserved;
-  eo_join_opts *join;      /* optional plan overrides of the hash join per R tile (where ignored) */
+  const eo_join_opts *join; /* optional plan overrides of the hash join per R tile (where ignored) */
   eo_stream_stats *stats;  /* optional out */
 } eo_stream_opts;
 

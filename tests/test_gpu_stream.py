@@ -129,3 +129,25 @@ def test_stream_join_edge_cases(ctx):
         assert same_multiset(one, oracle.nlj_join(r, s))
         odd = ctx.stream_join(R[:300], S[:500], op=op, r_tile=7, s_chunk=33)
         assert same_multiset(odd, oracle.nlj_join(R[:300], S[:500]))
+
+
+def test_host_mode_hash_join_streams_large_probe_sides(ctx):
+    """eo_hash_join on host buffers with |S| >= 2^22 runs as a streaming join; the
+    rows equal the device-resident join's (and the whole-copy path's, taken when
+    stats are requested)."""
+    R, S = gen.c2(seed=21, nR=1 << 21, nS=(1 << 22) + 3, theta=0.5, selectivity=0.5)
+    from eating_one_amd import DeviceArray
+    dR, dS = DeviceArray.from_host(ctx, R), DeviceArray.from_host(ctx, S)
+    n, _ = ctx.hash_join_device(dR, len(R), dS, len(S), None, 0)
+    dout = DeviceArray(ctx, 8 * n)
+    n2, st = ctx.hash_join_device(dR, len(R), dS, len(S), dout, n)
+    assert st == _abi.EO_OK and n2 == n
+    want = dout.to_host(np.int32, (n, 2))
+    assert same_multiset(ctx.hash_join(R, S), want)
+    assert same_multiset(ctx.hash_join(R, S, stats=_abi.JoinStats()), want)
+    rows = ctx.hash_join(R, S, result16=True)
+    assert same_multiset(rows[:, [0, 2]], want)
+    outer = ctx.hash_join(R, S, left_outer=True)
+    assert same_multiset(outer, _outer_want(R, S, want))
+    for a in (dout, dS, dR):
+        a.close()
