@@ -113,11 +113,13 @@ __global__ void k_fill(T *p, T v, uint64_t n) {
 }
 
 constexpr int kScanNT = 256;
+constexpr int kScanNTBig = 1024;  // long inputs: 4x longer tiles, 4x shorter look-back chain
+constexpr uint64_t kScanBigMin = 1ull << 20;
 constexpr int kScanIPT32 = 16;
 constexpr int kScanIPT64 = 8;
 
 template <typename T, int IPT>
-size_t scan_scratch(uint64_t n) {
+size_t scan_scratch(uint64_t n) {  // sized for the short tiles: covers both launch shapes
   const uint64_t tiles = (n + (uint64_t)kScanNT * IPT - 1) / ((uint64_t)kScanNT * IPT);
   return (tiles + 1) * sizeof(uint64_t);
 }
@@ -129,15 +131,20 @@ hipError_t scan_impl(const T *in, T *out, uint64_t n, uint64_t *total_dev, void 
     if (total_dev) return fill_u64(total_dev, 0, 1, stream);
     return hipSuccess;
   }
-  const uint64_t tile = (uint64_t)kScanNT * IPT;
+  const bool big = n >= kScanBigMin;
+  const uint64_t tile = (uint64_t)(big ? kScanNTBig : kScanNT) * IPT;
   const uint64_t tiles = (n + tile - 1) / tile;
   if (tiles >= (1ull << 31)) return hipErrorInvalidValue;
   uint64_t *status = (uint64_t *)scratch;
   uint32_t *ticket = (uint32_t *)(status + tiles);
   hipError_t e = fill_u64(status, 0, tiles + 1, stream);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_scan<T, kScanNT, IPT>), dim3((uint32_t)tiles), dim3(kScanNT), 0, stream, in, out, n,
-                     status, ticket, total_dev, err);
+  if (big)
+    hipLaunchKernelGGL((k_scan<T, kScanNTBig, IPT>), dim3((uint32_t)tiles), dim3(kScanNTBig), 0, stream, in, out, n,
+                       status, ticket, total_dev, err);
+  else
+    hipLaunchKernelGGL((k_scan<T, kScanNT, IPT>), dim3((uint32_t)tiles), dim3(kScanNT), 0, stream, in, out, n,
+                       status, ticket, total_dev, err);
   return hipGetLastError();
 }
 

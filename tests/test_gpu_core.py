@@ -16,7 +16,8 @@ from common import (brute_join, eo_to_ref, join_count_closed_form, join_ori_doc,
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 4095, 4096, 4097, 100003, 1 << 20, (1 << 22) + 17])
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 4095, 4096, 4097, 100003, (1 << 20) - 1, 1 << 20, (1 << 20) + 16383,
+                               (1 << 22) + 17])
 def test_scan_u32(ctx, n):
     rng = np.random.default_rng(n)
     a = rng.integers(0, 1 << 10, size=n).astype(np.uint32)
@@ -26,7 +27,7 @@ def test_scan_u32(ctx, n):
     assert tot == wtot
 
 
-@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 300001, 1 << 21])
+@pytest.mark.parametrize("n", [0, 1, 2047, 2048, 2049, 300001, (1 << 20) + 8191, 1 << 21])
 def test_scan_u64(ctx, n):
     rng = np.random.default_rng(n + 1)
     a = rng.integers(0, 1 << 40, size=n).astype(np.uint64)
