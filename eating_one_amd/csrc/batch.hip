@@ -118,23 +118,6 @@ __global__ void k_bplan(const BatchJoin *__restrict__ joins, const uint32_t *__r
   tiles[q] = (rn && sn) ? ((rn + kFCap - 1) / kFCap) * ((sn + J.s_span - 1) / J.s_span) : 0u;
 }
 
-// Largest q in [0, n) with ts[q] <= t, for a non-decreasing ts with ts[0] <= t:
-// a 64-ary search by one wave (one load per lane per level, 4 levels for 2^22).
-__device__ __forceinline__ uint32_t wave_find(const uint32_t *__restrict__ ts, uint32_t n, uint32_t t) {
-  uint32_t lo = 0;
-  while (n > 1) {
-    const uint32_t step = (n + kWave - 1) / kWave;
-    const uint32_t idx = lo + (uint32_t)lane_id() * step;
-    const bool le = idx < lo + n && ts[idx] <= t;
-    const uint64_t m = __ballot(le);  // a prefix of the lanes; lane 0 always set
-    const uint32_t k = 63u - (uint32_t)__clzll(m);
-    const uint32_t nlo = lo + k * step;
-    n = min(step, lo + n - nlo);
-    lo = nlo;
-  }
-  return lo;
-}
-
 // Persistent probe: every workgroup takes tiles from the queue until it is
 // empty; every workgroup leaves the loop once its ticket passes the total.
 __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ joins,

@@ -179,6 +179,24 @@ __device__ __forceinline__ T block_sum(T x, T *lds) {
   return total;
 }
 
+// Largest q in [0, n) with ts[q] <= t, for a non-decreasing ts with ts[0] <= t:
+// a 64-ary search by one wave (one load per lane per level, 4 levels for 2^22).
+// Every lane of the wave must take part (the ballot assumes a full wave).
+__device__ __forceinline__ uint32_t wave_find(const uint32_t *__restrict__ ts, uint32_t n, uint32_t t) {
+  uint32_t lo = 0;
+  while (n > 1) {
+    const uint32_t step = (n + kWave - 1) / kWave;
+    const uint32_t idx = lo + (uint32_t)lane_id() * step;
+    const bool le = idx < lo + n && ts[idx] <= t;
+    const uint64_t m = __ballot(le);  // a prefix of the lanes; lane 0 always set
+    const uint32_t k = 63u - (uint32_t)__clzll(m);
+    const uint32_t nlo = lo + k * step;
+    n = min(step, lo + n - nlo);
+    lo = nlo;
+  }
+  return lo;
+}
+
 // ---- coherent atomics -------------------------------------------------------
 // Agent scope makes loads/stores coherent across the XCDs' separate L2s.
 __device__ __forceinline__ uint64_t ld_acquire(const uint64_t *p) {

@@ -44,18 +44,14 @@ struct TileRef {
   uint32_t r_beg, r_n, s_beg, s_n;
 };
 
-// Locates tile t: p = last partition with tile_start[p] <= t, then (j, c).
+// Locates tile t: p = last partition with tile_start[p] <= t (a wave-wide
+// search: all threads of the workgroup call it), then (j, c).
 template <int CAP>
 __device__ __forceinline__ bool resolve_tile(uint32_t t, const uint32_t *tile_start, uint32_t P,
                                              const uint32_t *r_start, const uint32_t *s_start, uint32_t s_tile,
                                              TileRef *ref) {
-  uint32_t lo = 0, hi = P;  // invariant: tile_start[lo] <= t < tile_start[hi]
   if (t >= tile_start[P]) return false;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (tile_start[mid] <= t) lo = mid; else hi = mid;
-  }
-  const uint32_t p = lo;
+  const uint32_t p = wave_find(tile_start, P, t);
   const uint32_t local = t - tile_start[p];
   const uint32_t rb = r_start[p], rn = r_start[p + 1] - rb;
   const uint32_t sb = s_start[p], sn = s_start[p + 1] - sb;
@@ -224,13 +220,8 @@ using namespace tile;
 __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_start, uint32_t P,
                                              const uint32_t *r_start, const uint32_t *s_start, uint32_t s_span,
                                              uint32_t cap, SpanRef *ref) {
-  uint32_t lo = 0, hi = P;
   if (t >= tile_start[P]) return false;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (tile_start[mid] <= t) lo = mid; else hi = mid;
-  }
-  const uint32_t p = lo;
+  const uint32_t p = wave_find(tile_start, P, t);
   const uint32_t local = t - tile_start[p];
   const uint32_t rb = r_start[p], rn = r_start[p + 1] - rb;
   const uint32_t sb = s_start[p], sn = s_start[p + 1] - sb;

@@ -459,12 +459,7 @@ __global__ __launch_bounds__(kFastNT) void k_msd_scatter(const uint2 *__restrict
     __syncthreads();
     const uint32_t w = s_w;
     if (w >= total) break;  // uniform: every workgroup leaves once the tickets pass total
-    uint32_t lo = 0, hi = F1;  // grp_base[lo] <= w < grp_base[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (grp_base[mid] <= w) lo = mid; else hi = mid;
-    }
-    const uint32_t dh = lo;
+    const uint32_t dh = wave_find(grp_base, F1, w);  // grp_base[dh] <= w < grp_base[dh + 1]
     const uint32_t J = grp_base[dh + 1] - grp_base[dh], j = w - grp_base[dh];
     const uint32_t g0 = (uint32_t)((uint64_t)j * G / J), g1 = (uint32_t)((uint64_t)(j + 1) * G / J);
     const uint64_t c0 = (uint64_t)dh * G + g0, c1 = (uint64_t)dh * G + g1;
