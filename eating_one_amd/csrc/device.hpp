@@ -197,6 +197,15 @@ __device__ __forceinline__ uint32_t wave_find(const uint32_t *__restrict__ ts, u
   return lo;
 }
 
+// Returns v unchanged through an empty asm statement (no instruction): the
+// optimiser cannot merge expressions computed from the result with earlier
+// ones.  Used where a rare branch recomputes compare ballots: merged, all of
+// them would have to stay live in scalar registers (spilling) on the hot path.
+__device__ __forceinline__ int32_t opaque(int32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // ---- coherent atomics -------------------------------------------------------
 // Agent scope makes loads/stores coherent across the XCDs' separate L2s.
 __device__ __forceinline__ uint64_t ld_acquire(const uint64_t *p) {

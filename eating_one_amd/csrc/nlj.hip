@@ -66,6 +66,8 @@ __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpa
       for (int k = 0; k < kNljK; ++k) acc |= __ballot(sr.key[k] == rk[q2]);
     if (acc) {
 #pragma unroll
+      for (int q2 = 0; q2 < kNljGroup; ++q2) rk[q2] = opaque(rk[q2]);  // recompute, do not keep the ballots
+#pragma unroll
       for (int q2 = 0; q2 < kNljGroup; ++q2)
 #pragma unroll
         for (int k = 0; k < kNljK; ++k) {
@@ -175,6 +177,8 @@ __global__ __launch_bounds__(kNljNT) void k_nlj_fused(const uint2 *__restrict__ 
       for (int k = 0; k < kNljK; ++k) acc |= __ballot(sr.key[k] == rk[q2]);
     if (acc) {
       uint64_t cnt = 0;
+#pragma unroll
+      for (int q2 = 0; q2 < kNljGroup; ++q2) rk[q2] = opaque(rk[q2]);  // recompute, do not keep the ballots
 #pragma unroll
       for (int q2 = 0; q2 < kNljGroup; ++q2) {
         if (q2 >= (int)gn) break;
