@@ -90,6 +90,29 @@ def cpu_baseline_hash(sample_log2_r: int, theta: float, sel: float, threads: int
     }
 
 
+def cpu_port(value, unit, sample, dt, threads=1):
+    """A cpu_baseline record for an oracle restatement timed on the host."""
+    model, ncpu = host_info()
+    return {"value": value, "unit": unit, "cores": threads, "kind": "port",
+            "sample": f"{sample}, {dt:.2f} s on {threads} of {ncpu} host threads ({model})"}
+
+
+def cpu_baseline_nlj(S, log2_r_sample: int):
+    """CPU_JOIN block_cpu NLJ (timer scope nestLoopJoin.cc:171-173) on the first
+    2^log2_r_sample R tuples of C3 against all of S: compares/s."""
+    import oracle
+    from eating_one_amd import gen
+
+    R, _ = gen.c3()
+    r = R[: 1 << log2_r_sample]
+    t0 = time.perf_counter()
+    n = oracle.nlj_join(r, S, count_only=True)
+    dt = time.perf_counter() - t0
+    return cpu_port(len(r) * len(S) / dt, "compares/s",
+                    f"oracle/cpu_join.c restatement of CPU_JOIN block_cpu, |R| = 2^{log2_r_sample} of C3's R, "
+                    f"|S| = {len(S)}, {n} matches", dt)
+
+
 def read_traffic(kernel):
     """HBM bytes per launch of a C2 kernel family, from the committed rocprofv3
     PMC summary (FETCH_SIZE and WRITE_SIZE runs scaled by the copy-kernel
@@ -342,6 +365,8 @@ def run_c3_nlj(a):
            "roofline": {"bound": "valu", "achieved": rate / 1e9, "peak": peak / 1e9, "unit": "Gcompares/s",
                         "frac": rate / peak, "kernel": dom[0]}}
     ctx.close()
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_nlj(S, 16)
     return res
 
 
@@ -384,6 +409,15 @@ def run_nlj_dist(a):
            "roofline": {"bound": "fp64", "achieved": tflops, "peak": 78.6, "unit": "TFLOP/s", "frac": tflops / 78.6,
                         "kernel": "nlj_dist"}}
     ctx.close()
+    if not a.no_cpu:
+        import oracle
+        r = R[: 1 << 14]
+        t0 = time.perf_counter()
+        m = oracle.nlj_dist_join(r, S, 1.0, count_only=True)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = cpu_port(len(r) * float(n) / dt, "pairs/s",
+                                       f"oracle/cpu_join.c restatement of CPU_JOIN 3d_cpu, |R| = 2^14 of the "
+                                       f"2^17 points, |S| = 2^17, {m} matches", dt)
     return res
 
 
@@ -416,6 +450,15 @@ def run_c3_hidx(a):
            "pipeline": {"alg_bytes": alg, "achieved_gbs": alg / (ms + build_ms) / 1e6}}
     ix.free()
     ctx.close()
+    if not a.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        m = oracle.hidx_join(R, S, count_only=True)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = cpu_port(len(S) / dt, "probes/s",
+                                       f"oracle/cpu_join.c restatement of CPU_JOIN inner_cpu (2^27-bucket index "
+                                       f"build + probe, timer scope inner-idx-bnlj.cc:142-186), full C3, {m} matches",
+                                       dt)
     return res
 
 
@@ -465,6 +508,60 @@ def run_c4(a):
     res["speedup_join"] = res["binary_join_ms"] / res["learned_join_ms"]
     ix.free()
     ctx.close()
+    if not a.no_cpu:
+        import oracle
+        r, s_ = gen.c4(nR=1 << 24, nS=1 << 25)  # 1/4 of the workload
+        t0 = time.perf_counter()
+        oracle.sidx_build(r)
+        build_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        m = oracle.sidx_join(r, s_, count_only=True)
+        dt = time.perf_counter() - t0
+        probe_s = max(dt - build_s, 1e-9)
+        res["cpu_baseline"] = cpu_port(len(s_) / probe_s, "probes/s",
+                                       f"oracle/cpu_join.c restatement of CPU_JOIN sortIndex_cpu (quicksort build "
+                                       f"{build_s:.2f} s, then binary-search probes; timer scopes "
+                                       f"inner-idx-bnlj.cc:204-238), C4 shape at |R| = 2^24, |S| = 2^25, "
+                                       f"{m} matches", dt)
+    return res
+
+
+def run_c1(a):
+    """C1: the CPU_JOIN path's own shape (2^16 x 2^20, every S tuple matches
+    once), launch-bound on the GPU; the oracle on the full inputs beside it."""
+    import numpy as np
+
+    import eating_one_amd as eo
+    from eating_one_amd import _abi, gen
+
+    R, S = gen.c1()
+    ctx = eo.Context(0)
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    n, _ = ctx.hash_join_device(dR, len(R), dS, len(S), None, 0)
+    dout = eo.DeviceArray(ctx, 8 * n)
+    t = []
+    for it in range(a.warmup + a.steps):
+        t0 = time.perf_counter()
+        n2, st = ctx.hash_join_device(dR, len(R), dS, len(S), dout, n)
+        if it >= a.warmup:
+            t.append(time.perf_counter() - t0)
+        assert st == _abi.EO_OK and n2 == n
+    ms = 1e3 * float(np.median(t))
+    alg = 24 * (len(R) + len(S)) + 8 * n
+    res = {"metric": "join tuples/s, C1 2^16 x 2^20", "value": (len(R) + len(S)) / (ms * 1e-3),
+           "unit": "tuples/s", "ms_per_step": ms, "n_out": n, "steps": a.steps, "warmup": a.warmup,
+           "pipeline": {"alg_bytes": alg, "achieved_gbs": alg / (ms * 1e-3) / 1e9}}
+    for d in (dout, dS, dR):
+        d.close()
+    ctx.close()
+    if not a.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        m = oracle.hash_join(R, S, count_only=True)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = cpu_port((len(R) + len(S)) / dt, "tuples/s",
+                                       f"oracle/cpu_join.c restatement of CPU_JOIN hash_cpu (2^20 partitions, "
+                                       f"timer scope hj.cc:209-230), full C1, {m} matches", dt)
     return res
 
 
@@ -565,7 +662,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist", "stream"])
+    ap.add_argument("--config", default="c2",
+                    choices=["c1", "c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist", "stream"])
     ap.add_argument("--s-chunk", type=int, default=0, help="stream: S tuples per chunk (0: library default)")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--c5-requests", type=int, default=4096)
@@ -597,7 +695,7 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): the library has no multi-GPU path",
               file=sys.stderr)
         return 2
-    fn = {"c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,
+    fn = {"c1": run_c1, "c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,
           "nlj-dist": run_nlj_dist, "stream": run_stream}[a.config]
     print(json.dumps(fn(a)), flush=True)
     return 0
