@@ -101,12 +101,35 @@ __device__ __forceinline__ void build_table(const uint2 *__restrict__ R, uint32_
     if (threadIdx.x == 0) off[C::NB] = rn;
   }
   __syncthreads();
+  // Place every tuple at a slot fixed by the input alone, so that the count
+  // and write passes, and every run, see one bucket order: lanes of a wave
+  // that share a bucket are ranked by lane (match over the bucket bits), one
+  // leader per group takes the group's slots with one atomic, and the waves
+  // take turns.  Packed per tuple: rank in group | leader lane << 8 | group size << 16.
+  const int w = threadIdx.x / kWave;
+  const uint64_t lt = lanemask_lt();
+  uint32_t grp[C::RPT];
 #pragma unroll
   for (int k = 0; k < C::RPT; ++k) {
-    const uint32_t i = k * C::NT + threadIdx.x;
-    if (i < rn) ent[atomicSub(&off[hh[k]], 1u) - 1] = r[k];
+    const bool valid = k * C::NT + threadIdx.x < rn;
+    const uint64_t peers = match_peers<C::LOG2NB>(valid ? hh[k] : 0u, __ballot(valid));
+    const uint32_t lead = valid ? (uint32_t)(__ffsll((unsigned long long)peers) - 1) : (uint32_t)lane_id();
+    grp[k] = (uint32_t)__popcll(peers & lt) | lead << 8 | (uint32_t)__popcll(peers) << 16;
   }
-  __syncthreads();
+  for (int turn = 0; turn < C::NT / kWave; ++turn) {
+    if (w == turn) {
+#pragma unroll
+      for (int k = 0; k < C::RPT; ++k) {
+        const bool valid = k * C::NT + threadIdx.x < rn;
+        const uint32_t before = grp[k] & 0xffu;
+        uint32_t top = 0;
+        if (valid && before == 0) top = atomicSub(&off[hh[k]], grp[k] >> 16);
+        top = __shfl(top, (int)((grp[k] >> 8) & 0xffu));
+        if (valid) ent[top - 1 - before] = r[k];
+      }
+    }
+    __syncthreads();
+  }
 }
 
 template <typename C>

@@ -106,6 +106,19 @@ def test_hash_join_plans(ctx, bits, passes, cap, det):
     assert same_multiset(got, want)
 
 
+@pytest.mark.parametrize("bits,passes,cap", [(-1, 0, 16384), (8, 1, 8192), (11, 1, 4096), (12, 2, 4096),
+                                             (13, 2, 16384)])
+def test_hash_join_deterministic_order_repeats(ctx, bits, passes, cap):
+    """EO_F_DETERMINISTIC: the same rows in the same order on every run, with
+    duplicate keys on both sides (stable partitions, fixed bucket order)."""
+    R, S = gen.c3(seed=bits + 7, nR=70000, nS=110000, key_range=1 << 14)
+    R = np.concatenate([R, R[:5000]])  # more duplicates, some several LDS chunks deep
+    runs = [ctx.hash_join(R, S, radix_bits=bits, passes=passes, lds_tuples=cap, deterministic=True)
+            for _ in range(3)]
+    assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
+    assert same_multiset(runs[0], oracle.hash_join(R, S, nparts=1 << 12))
+
+
 def test_hash_join_edge_cases(ctx):
     e = np.empty((0, 2), dtype=np.int32)
     one = np.array([[5, 9]], dtype=np.int32)
