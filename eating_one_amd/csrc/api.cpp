@@ -508,10 +508,6 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const bool wide = o && (o->flags & EO_F_RESULT16);
   const size_t row = wide ? 16 : sizeof(eo_pair);
   const bool outer = o && (o->flags & EO_F_LEFT_OUTER);
-  if (outer && (o->flags & EO_F_DETERMINISTIC)) {
-    set_last_error("EO_F_LEFT_OUTER: the deterministic count/write path has no outer mode");
-    return EO_EINVAL;
-  }
 
   const bool host = where_of(o) == EO_HOST;
   if (host && !stats && !(o && o->stream) && !(o && (o->flags & EO_F_DETERMINISTIC)) && nS >= kStreamMinS) {
@@ -540,6 +536,9 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const size_t o_ts = L.add((P + 1) * sizeof(uint32_t));
   const size_t o_small = L.add(8 * sizeof(uint64_t));
   const size_t o_bits = L.add(outer ? matched_words(nS) * sizeof(uint32_t) : 0);
+  const uint64_t ublocks = outer ? unmatched_blocks(nS) : 0;  // deterministic outer rows: block offsets
+  const size_t o_ub = L.add(ublocks * sizeof(uint64_t));
+  const size_t o_ubs = L.add(ublocks ? scan_scratch_bytes_u64(ublocks) : 0);
   const size_t o_pscr = L.add(std::max(partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus)));
   const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
   const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_tile + 1024;
@@ -659,15 +658,29 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   auto free_side = [&]() {
     if (side) (void)hipFreeAsync(B, s);
   };
-  EO_TRY_HIP(join_count(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt, s), "count");
+  // Left outer: the count pass marks matched S tuples; the null rows follow
+  // the inner rows, in S order, at scanned per-block offsets.
+  uint32_t *bits = outer ? (uint32_t *)(A + o_bits) : nullptr;
+  uint64_t *ub = (uint64_t *)(A + o_ub);
+  if (outer) EO_TRY_HIP(fill_u32(bits, 0u, matched_words(nS), s), "match bitmap reset");
+  EO_TRY_HIP(join_count(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, bits, tile_cnt, s), "count");
   EO_TRY_HIP(scan_u64(tile_cnt, tile_cnt, n_tiles, small + 1, B + o_cscr, c->d_err, s), "count scan");
+  if (outer) {
+    EO_TRY_HIP(count_unmatched(nS, bits, ub, s), "unmatched count");
+    EO_TRY_HIP(scan_u64(ub, ub, ublocks, small + 2, A + o_ubs, c->d_err, s), "unmatched scan");
+  } else {
+    EO_TRY_HIP(fill_u64(small + 2, 0, 1, s), "fill");
+  }
   clk.mark();  // 2
-  st = read_u64(c, small + 1, &total, 1, s);
+  uint64_t totals[2] = {0, 0};
+  st = read_u64(c, small + 1, totals, 2, s);
   if (st == EO_OK) st = check_device_errors(c, s);
   if (st != EO_OK) {
     free_side();
     return st;
   }
+  const uint64_t inner = totals[0];
+  total = totals[0] + totals[1];
   *n_out = total;
   if (stats) stats->n_out = total;
   if (!out) {
@@ -686,8 +699,10 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     }
   }
   if (total) {
-    hipError_t e = join_write(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt, dout,
-                              total, s);
+    hipError_t e = inner ? join_write(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt,
+                                      dout, total, s)
+                         : hipSuccess;
+    if (e == hipSuccess && outer) e = emit_unmatched_at(Sp, nS, bits, ub, inner, dout, total, wide, s);
     clk.mark();  // 3
     if (e == hipSuccess && host)
       e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
@@ -723,10 +738,6 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   if (stats) std::memset(stats, 0, sizeof(*stats));
   const bool outer = o && (o->flags & EO_F_LEFT_OUTER);
   if (nS == 0 || (nR == 0 && !outer)) return EO_OK;
-  if (outer && (o->flags & EO_F_DETERMINISTIC)) {
-    set_last_error("EO_F_LEFT_OUTER: the deterministic count/write path has no outer mode");
-    return EO_EINVAL;
-  }
   const bool wide = o && (o->flags & EO_F_RESULT16);
   const size_t row = wide ? 16 : sizeof(eo_pair);
   DeviceGuard g(c->device);
@@ -740,6 +751,9 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   const size_t o_cnt = L.add(slots * sizeof(uint64_t));
   const size_t o_small = L.add(8 * sizeof(uint64_t));
   const size_t o_bits = L.add(outer ? matched_words(nS) * sizeof(uint32_t) : 0);
+  const uint64_t ublocks = outer ? unmatched_blocks(nS) : 0;  // deterministic outer rows: block offsets
+  const size_t o_ub = L.add(ublocks * sizeof(uint64_t));
+  const size_t o_ubs = L.add(ublocks ? scan_scratch_bytes_u64(ublocks) : 0);
   const size_t o_scr = L.add(scan_scratch_bytes_u64(slots));
   EO_TRY_HIP(hipStreamSynchronize(s), "sync");
   EO_TRY_HIP(c->arena.reserve(L.total), "arena");
@@ -794,13 +808,25 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
     if (st != EO_OK) return st;
     return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
   }
-  EO_TRY_HIP(nlj_count(np, dR, dS, cnt, s), "nlj count");
+  // Left outer: the count pass marks matched S tuples; the null rows follow
+  // the inner rows, in S order, at scanned per-block offsets.
+  uint32_t *bits = outer ? (uint32_t *)(A + o_bits) : nullptr;
+  uint64_t *ub = (uint64_t *)(A + o_ub);
+  if (outer) EO_TRY_HIP(fill_u32(bits, 0u, matched_words(nS), s), "match bitmap reset");
+  EO_TRY_HIP(nlj_count(np, dR, dS, cnt, s, bits), "nlj count");
   EO_TRY_HIP(scan_u64(cnt, cnt, slots, small, A + o_scr, c->d_err, s), "nlj scan");
+  if (outer) {
+    EO_TRY_HIP(count_unmatched(nS, bits, ub, s), "unmatched count");
+    EO_TRY_HIP(scan_u64(ub, ub, ublocks, small + 1, A + o_ubs, c->d_err, s), "unmatched scan");
+  } else {
+    EO_TRY_HIP(fill_u64(small + 1, 0, 1, s), "fill");
+  }
   clk.mark();
-  uint64_t total = 0;
-  eo_status st = read_u64(c, small, &total, 1, s);
+  uint64_t totals[2] = {0, 0};
+  eo_status st = read_u64(c, small, totals, 2, s);
   if (st == EO_OK) st = check_device_errors(c, s);
   if (st != EO_OK) return st;
+  const uint64_t inner = totals[0], total = totals[0] + totals[1];
   *n_out = total;
   if (stats) stats->n_out = total;
   if (!out) return EO_OK;
@@ -808,7 +834,8 @@ eo_status eo_nlj_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple 
   if (total) {
     eo_pair *dout = out;
     if (host) EO_TRY_HIP(hipMallocAsync((void **)&dout, total * row, s), "output staging");
-    hipError_t e = nlj_write(np, dR, dS, cnt, dout, total, wide, s);
+    hipError_t e = inner ? nlj_write(np, dR, dS, cnt, dout, total, wide, s) : hipSuccess;
+    if (e == hipSuccess && outer) e = emit_unmatched_at(dS, nS, bits, ub, inner, dout, total, wide, s);
     clk.mark();
     if (e == hipSuccess && host) e = hipMemcpyAsync(out, dout, total * row, hipMemcpyDeviceToHost, s);
     if (host) (void)hipFreeAsync(dout, s);

@@ -136,7 +136,8 @@ template <typename C>
 __global__ __launch_bounds__(C::NT) void k_join_count(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
                                                       const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
                                                       const uint32_t *__restrict__ tile_start, uint32_t P,
-                                                      uint32_t s_tile, uint64_t *__restrict__ tile_count) {
+                                                      uint32_t s_tile, uint64_t *__restrict__ tile_count,
+                                                      uint32_t *matched) {
   __shared__ uint2 ent[C::CAP];
   __shared__ uint32_t off[C::NB + 1];
   __shared__ uint32_t scan_tmp[C::NT / kWave + 1];
@@ -157,11 +158,15 @@ __global__ __launch_bounds__(C::NT) void k_join_count(const uint2 *__restrict__ 
 #pragma unroll
     for (int u = 0; u < C::U; ++u) {
       const uint32_t i = i0 + u * C::NT + threadIdx.x;
+      uint32_t m = 0;
       if (i < ref.s_n) {
         const uint32_t h = bucket_hash(key[u], C::LOG2NB);
         const uint32_t e1 = off[h + 1];
-        for (uint32_t e = off[h]; e < e1; ++e) cnt += (int32_t)ent[e].x == key[u];
+        for (uint32_t e = off[h]; e < e1; ++e) m += (int32_t)ent[e].x == key[u];
       }
+      cnt += m;
+      if (matched)  // left outer join: S positions with a match in this tile
+        mark_bits(matched, (uint64_t)ref.s_beg + i0 + u * C::NT + (threadIdx.x / kWave) * kWave, __ballot(m != 0));
     }
   }
   const uint64_t tot = block_sum<C::NT, uint64_t>((uint64_t)cnt, red);
@@ -304,10 +309,11 @@ __global__ void k_plan_tiles(const uint32_t *r_start, const uint32_t *s_start, u
 template <typename C>
 hipError_t launch_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
                         const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, uint64_t *tile_count,
-                        hipStream_t stream) {
+                        uint32_t *matched, hipStream_t stream) {
   const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
   EO_TIMED("probe_count", stream, hipLaunchKernelGGL(k_join_count<C>, dim3(grid), dim3(C::NT), 0, stream, reinterpret_cast<const uint2 *>(R), r_start,
-                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_count));
+                     reinterpret_cast<const uint2 *>(S), s_start, tile_start, jp.parts, jp.s_tile, tile_count,
+                     matched));
   return hipGetLastError();
 }
 
@@ -343,13 +349,16 @@ hipError_t join_plan_count(const JoinTilePlan &jp, const uint32_t *r_start, cons
 hipError_t join_plan_map(const JoinTilePlan &, const uint32_t *, uint32_t *, hipStream_t) { return hipSuccess; }
 
 hipError_t join_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
-                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, const uint32_t *,
+                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start, uint32_t *matched,
                       uint64_t *tile_count, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
   switch (jp.lds_cap) {
-    case Cfg4K::CAP: return launch_count<Cfg4K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, stream);
-    case Cfg8K::CAP: return launch_count<Cfg8K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, stream);
-    case Cfg16K::CAP: return launch_count<Cfg16K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, stream);
+    case Cfg4K::CAP:
+      return launch_count<Cfg4K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, matched, stream);
+    case Cfg8K::CAP:
+      return launch_count<Cfg8K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, matched, stream);
+    case Cfg16K::CAP:
+      return launch_count<Cfg16K>(jp, n_tiles, R, r_start, S, s_start, tile_start, tile_count, matched, stream);
     default: return hipErrorInvalidValue;
   }
 }
@@ -414,10 +423,13 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
 namespace {
 
 // Left outer join: one row (null R side) per S tuple whose bit in `matched`
-// is clear; each workgroup reserves its rows from *cursor with one atomic.
-// out == NULL: only counts.
+// is clear.  Each workgroup (256 words = 8192 positions) reserves its rows
+// from *cursor with one atomic, or, with block_off, writes them at
+// base + block_off[workgroup] (S order: deterministic).  out == NULL: only
+// counts.
 __global__ __launch_bounds__(256) void k_emit_unmatched(const uint2 *__restrict__ S, uint64_t n,
                                                         const uint32_t *__restrict__ matched, uint64_t *cursor,
+                                                        const uint64_t *__restrict__ block_off, uint64_t base,
                                                         uint2 *__restrict__ out, uint64_t out_cap, int wide) {
   __shared__ uint64_t red[256 / kWave + 1];
   __shared__ uint64_t base_sh;
@@ -430,7 +442,10 @@ __global__ __launch_bounds__(256) void k_emit_unmatched(const uint2 *__restrict_
   }
   uint64_t tot;
   const uint64_t excl = block_excl_scan<256, uint64_t>((uint64_t)__popc(unm), red, &tot);
-  if (threadIdx.x == 0) base_sh = tot ? atomic_add_agent(cursor, tot) : 0ull;
+  if (threadIdx.x == 0) {
+    if (block_off) base_sh = base + block_off[blockIdx.x];
+    else base_sh = tot ? atomic_add_agent(cursor, tot) : 0ull;
+  }
   __syncthreads();
   if (!out) return;
   uint64_t pos = base_sh + excl;
@@ -443,9 +458,47 @@ __global__ __launch_bounds__(256) void k_emit_unmatched(const uint2 *__restrict_
   }
 }
 
+// Unmatched S positions per 256-word block (the counts k_emit_unmatched's
+// deterministic mode scans into block offsets).
+__global__ __launch_bounds__(256) void k_count_unmatched(uint64_t n, const uint32_t *__restrict__ matched,
+                                                         uint64_t *__restrict__ block_cnt) {
+  __shared__ uint64_t red[256 / kWave + 1];
+  const uint64_t wd = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  uint32_t unm = 0;
+  if (wd * 32 < n) {
+    const uint64_t left = n - wd * 32;
+    const uint32_t valid = left >= 32 ? ~0u : ((1u << left) - 1u);
+    unm = ~matched[wd] & valid;
+  }
+  const uint64_t tot = block_sum<256, uint64_t>((uint64_t)__popc(unm), red);
+  if (threadIdx.x == 0) block_cnt[blockIdx.x] = tot;
+}
+
 }  // namespace
 
 size_t matched_words(uint64_t n) { return (n + 31) / 32; }
+
+uint64_t unmatched_blocks(uint64_t n) { return (matched_words(n) + 255) / 256; }
+
+hipError_t count_unmatched(uint64_t n, const uint32_t *matched, uint64_t *block_cnt, hipStream_t stream) {
+  const uint64_t blocks = unmatched_blocks(n);
+  if (blocks == 0) return hipSuccess;
+  if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_count_unmatched, dim3((uint32_t)blocks), dim3(256), 0, stream, n, matched, block_cnt);
+  return hipGetLastError();
+}
+
+hipError_t emit_unmatched_at(const eo_tuple *S, uint64_t n, const uint32_t *matched, const uint64_t *block_off,
+                             uint64_t base, eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream) {
+  const uint64_t blocks = unmatched_blocks(n);
+  if (blocks == 0) return hipSuccess;
+  if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
+  EO_TIMED("outer_rows", stream,
+           hipLaunchKernelGGL(k_emit_unmatched, dim3((uint32_t)blocks), dim3(256), 0, stream,
+                              reinterpret_cast<const uint2 *>(S), n, matched, nullptr, block_off, base,
+                              reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
+  return hipGetLastError();
+}
 
 hipError_t emit_unmatched(const eo_tuple *S, uint64_t n, const uint32_t *matched, uint64_t *cursor, eo_pair *out,
                           uint64_t out_cap, bool wide, hipStream_t stream) {
@@ -455,8 +508,8 @@ hipError_t emit_unmatched(const eo_tuple *S, uint64_t n, const uint32_t *matched
   if (blocks >= (1ull << 31)) return hipErrorInvalidValue;
   EO_TIMED("outer_rows", stream,
            hipLaunchKernelGGL(k_emit_unmatched, dim3((uint32_t)blocks), dim3(256), 0, stream,
-                              reinterpret_cast<const uint2 *>(S), n, matched, cursor, reinterpret_cast<uint2 *>(out),
-                              out_cap, (int)wide));
+                              reinterpret_cast<const uint2 *>(S), n, matched, cursor, nullptr, 0ull,
+                              reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
   return hipGetLastError();
 }
 

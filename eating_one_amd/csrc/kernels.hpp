@@ -107,9 +107,10 @@ hipError_t join_plan_count(const JoinTilePlan &jp, const uint32_t *r_start, cons
                            uint32_t *err, hipStream_t stream);
 hipError_t join_plan_map(const JoinTilePlan &jp, const uint32_t *tile_start, uint32_t *tile_part,
                          hipStream_t stream);
+// matched (optional): bitmap over S positions, bits set for S tuples with a match.
 hipError_t join_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
                       const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      const uint32_t *tile_part, uint64_t *tile_count, hipStream_t stream);
+                      uint32_t *matched, uint64_t *tile_count, hipStream_t stream);
 hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
                       const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
                       const uint32_t *tile_part, const uint64_t *tile_off, eo_pair *out, uint64_t out_cap,
@@ -136,6 +137,13 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
 // one null-padded row per S tuple whose bit is clear (rows reserved from
 // *cursor; out == NULL counts only).
 size_t matched_words(uint64_t n);
+// Deterministic variant: count_unmatched writes unmatched_blocks(n) block
+// counts; after their exclusive scan, emit_unmatched_at writes each block's
+// rows at base + block_off[block], in S order.
+uint64_t unmatched_blocks(uint64_t n);
+hipError_t count_unmatched(uint64_t n, const uint32_t *matched, uint64_t *block_cnt, hipStream_t stream);
+hipError_t emit_unmatched_at(const eo_tuple *S, uint64_t n, const uint32_t *matched, const uint64_t *block_off,
+                             uint64_t base, eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream);
 hipError_t emit_unmatched(const eo_tuple *S, uint64_t n, const uint32_t *matched, uint64_t *cursor, eo_pair *out,
                           uint64_t out_cap, bool wide, hipStream_t stream);
 
@@ -229,8 +237,9 @@ struct NljPlan {
 };
 NljPlan plan_nlj(uint64_t nR, uint64_t nS);
 uint64_t nlj_count_slots(const NljPlan &p);  // per-wave count entries
+// matched (optional): bitmap over S positions, bits set for S tuples with a match.
 hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *wave_count,
-                     hipStream_t stream);
+                     hipStream_t stream, uint32_t *matched = nullptr);
 // One pass: every match counted into *cursor (zero on entry); with out != NULL
 // the pairs are written at positions reserved per wave (order unspecified,
 // positions >= out_cap counted only).

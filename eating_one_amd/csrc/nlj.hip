@@ -47,11 +47,17 @@ __device__ __forceinline__ void load_s(const uint2 *__restrict__ S, uint64_t nS,
   }
 }
 
+// matched (count pass, optional): bitmap over S positions; this wave's S
+// register k covers positions s_wave + k*64 + lane.
 template <bool WRITE>
 __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpay, uint32_t rn, const SRegs &sr,
-                                         uint64_t *count, uint64_t base, uint2 *out, uint64_t out_cap, bool wide) {
+                                         uint64_t *count, uint64_t base, uint2 *out, uint64_t out_cap, bool wide,
+                                         uint32_t *matched = nullptr, uint64_t s_wave = 0) {
   const uint64_t lt = lanemask_lt();
   uint64_t running = 0;
+  uint64_t hitk[kNljK];
+#pragma unroll
+  for (int k = 0; k < kNljK; ++k) hitk[k] = 0;
   uint32_t g = 0;
   for (; g + kNljGroup <= rn; g += kNljGroup) {
     int32_t rk[kNljGroup];
@@ -72,6 +78,7 @@ __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpa
 #pragma unroll
         for (int k = 0; k < kNljK; ++k) {
           const uint64_t m = __ballot(sr.key[k] == rk[q2]) & sr.valid[k];
+          hitk[k] |= m;
           if (m) {
             if (WRITE && sr.key[k] == rk[q2] && ((sr.valid[k] >> lane_id()) & 1)) {
               const uint64_t pos = base + running + __popcll(m & lt);
@@ -87,6 +94,7 @@ __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpa
 #pragma unroll
     for (int k = 0; k < kNljK; ++k) {
       const uint64_t m = __ballot(sr.key[k] == r) & sr.valid[k];
+      hitk[k] |= m;
       if (m) {
         if (WRITE && sr.key[k] == r && ((sr.valid[k] >> lane_id()) & 1)) {
           const uint64_t pos = base + running + __popcll(m & lt);
@@ -97,13 +105,17 @@ __device__ __forceinline__ void nlj_tile(const int32_t *rkey, const int32_t *rpa
     }
   }
   if (!WRITE && lane_id() == 0) *count = running;
+  if (!WRITE && matched) {
+#pragma unroll
+    for (int k = 0; k < kNljK; ++k) mark_bits(matched, s_wave + (uint64_t)k * kWave, hitk[k]);
+  }
 }
 
 template <bool WRITE>
 __global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uint64_t nR, const uint2 *__restrict__ S,
                                                 uint64_t nS, uint32_t s_tiles, uint64_t *wave_count,
                                                 const uint64_t *__restrict__ wave_off, uint2 *__restrict__ out,
-                                                uint64_t out_cap, int wide) {
+                                                uint64_t out_cap, int wide, uint32_t *matched) {
   __shared__ __attribute__((aligned(16))) int32_t rkey[kNljRTile];
   __shared__ int32_t rpay[WRITE ? kNljRTile : 1];
   const uint32_t b = blockIdx.x;
@@ -122,7 +134,8 @@ __global__ __launch_bounds__(kNljNT) void k_nlj(const uint2 *__restrict__ R, uin
   if (WRITE)
     nlj_tile<true>(rkey, rpay, rn, sr, nullptr, wave_off[slot], out, out_cap, wide != 0);
   else
-    nlj_tile<false>(rkey, rpay, rn, sr, wave_count + slot, 0, nullptr, 0, false);
+    nlj_tile<false>(rkey, rpay, rn, sr, wave_count + slot, 0, nullptr, 0, false, matched,
+                    (uint64_t)st * kNljSTile + (uint64_t)(threadIdx.x / kWave) * kWave * kNljK);
 }
 
 // Fused single pass (default order): every (R key group, S register) compare
@@ -368,12 +381,13 @@ NljPlan plan_nlj(uint64_t nR, uint64_t nS) {
 uint64_t nlj_count_slots(const NljPlan &p) { return (uint64_t)p.r_tiles * p.s_tiles * kNljNW; }
 
 hipError_t nlj_count(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, uint64_t *wave_count,
-                     hipStream_t stream) {
+                     hipStream_t stream, uint32_t *matched) {
   const uint64_t grid = (uint64_t)p.r_tiles * p.s_tiles;
   if (grid == 0) return hipSuccess;
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   EO_TIMED("nlj_count", stream, hipLaunchKernelGGL(k_nlj<false>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
-                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0, 0));
+                     p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, wave_count, nullptr, nullptr, 0, 0,
+                     matched));
   return hipGetLastError();
 }
 
@@ -402,7 +416,7 @@ hipError_t nlj_write(const NljPlan &p, const eo_tuple *R, const eo_tuple *S, con
   if (grid >= (1ull << 31)) return hipErrorInvalidValue;
   EO_TIMED("nlj_write", stream, hipLaunchKernelGGL(k_nlj<true>, dim3((uint32_t)grid), dim3(kNljNT), 0, stream, reinterpret_cast<const uint2 *>(R),
                      p.nR, reinterpret_cast<const uint2 *>(S), p.nS, p.s_tiles, nullptr, wave_off,
-                     reinterpret_cast<uint2 *>(out), out_cap, (int)wide));
+                     reinterpret_cast<uint2 *>(out), out_cap, (int)wide, nullptr));
   return hipGetLastError();
 }
 

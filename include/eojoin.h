@@ -116,8 +116,9 @@ typedef struct eo_join_opts {
 #define EO_F_NO_SYNC_STATS 1  /* do not record per-phase events */
 /* Deterministic output order: separate count and write passes with a device
  * scan of per-tile counts between them (the reference's count -> presum ->
- * join sequence, hj.cc:1011-1091).  Default: one fused pass per tile that
- * reserves its output range with an atomic (order varies run to run). */
+ * join sequence, hj.cc:1011-1091): the same rows in the same order on every
+ * run.  Default: one fused pass per tile that reserves its output range with
+ * an atomic (order varies run to run). */
 #define EO_F_DETERMINISTIC 2
 /* Output rows in the reference's 16-byte RESULT layout {rkey, rval, lkey,
  * lval} = (R payload, key, S payload, key) (joins/hashJoin_gpu/main_street/
@@ -127,8 +128,9 @@ typedef struct eo_join_opts {
 /* Left outer join: every probe-side (S, the reference's "left") tuple with
  * no match also yields one row whose R side is EO_NULL_PAYLOAD (RESULT16:
  * rkey = rval = EO_NULL_PAYLOAD).  The reference names such a join
- * (joins/outerJoin_gpu) but implements none.  eo_hash_join and eo_nlj_join
- * (default order only) and eo_index_join. */
+ * (joins/outerJoin_gpu) but implements none.  eo_hash_join, eo_nlj_join and
+ * eo_index_join; with EO_F_DETERMINISTIC the null rows follow the inner rows
+ * in S order (the hash join's S order is its stable partition's). */
 #define EO_F_LEFT_OUTER 16
 #define EO_NULL_PAYLOAD ((int32_t)0x80000000)
 

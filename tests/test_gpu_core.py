@@ -9,7 +9,7 @@ import pytest
 
 import eating_one_amd as eo
 import oracle
-from eating_one_amd import CapacityError, gen
+from eating_one_amd import CapacityError, _abi, gen
 from common import (brute_join, eo_to_ref, join_count_closed_form, join_ori_doc, libc_rand_table, ref_to_eo,
                     same_multiset, sha256)
 
@@ -298,8 +298,41 @@ def test_left_outer_multi_chunk_partitions_and_empty_build(ctx):
     e = np.empty((0, 2), np.int32)
     assert same_multiset(ctx.hash_join(e, S, left_outer=True), _outer_want(e, S, e))
     assert same_multiset(ctx.nlj_join(e, S, left_outer=True), _outer_want(e, S, e))
-    with pytest.raises(eo.EOError):
-        ctx.hash_join(R, S, left_outer=True, deterministic=True)
+    for fn in (lambda r, **k: ctx.hash_join(r, S, radix_bits=2, **k), lambda r, **k: ctx.nlj_join(r, S, **k)):
+        d1, d2 = fn(R, left_outer=True, deterministic=True), fn(R, left_outer=True, deterministic=True)
+        assert np.array_equal(d1, d2) and same_multiset(d1, want)
+        assert same_multiset(fn(e, left_outer=True, deterministic=True), _outer_want(e, S, e))
+
+
+@pytest.mark.parametrize("kind", ["hash", "hash2", "nlj"])
+def test_left_outer_deterministic_order(ctx, kind):
+    """Deterministic left outer join: inner rows in count/write order, then one
+    null row per unmatched S tuple in S order; identical on every run; with
+    RESULT16 rows, count-only and a too small buffer (nothing written)."""
+    R, S = gen.c3(seed=61, nR=40000, nS=70000, key_range=90000)
+    kw = {"hash": {"radix_bits": 7}, "hash2": {"radix_bits": 12, "passes": 2}, "nlj": {}}[kind]
+    fn = ctx.nlj_join if kind == "nlj" else ctx.hash_join
+    want = _outer_want(R, S, oracle.hash_join(R, S, nparts=1 << 10))
+    d1 = fn(R, S, left_outer=True, deterministic=True, **kw)
+    d2 = fn(R, S, left_outer=True, deterministic=True, **kw)
+    assert np.array_equal(d1, d2) and same_multiset(d1, want)
+    nul = d1[:, 0] == _abi.EO_NULL_PAYLOAD
+    first_null = int(np.argmax(nul))
+    assert nul[first_null:].all() and not nul[:first_null].any()  # null rows last
+    # their S payloads in S order (hash: the stable partition's order, by key mod 2^bits)
+    Sp = S if kind == "nlj" else S[np.argsort(S[:, 0] & ((1 << kw["radix_bits"]) - 1), kind="stable")]
+    assert np.array_equal(d1[nul, 1], Sp[~np.isin(Sp[:, 0], R[:, 0]), 1])
+    assert fn(R, S, left_outer=True, deterministic=True, count_only=True, **kw) == len(want)
+    w16 = fn(R, S, left_outer=True, deterministic=True, result16=True, **kw)
+    assert np.array_equal(w16[:, [0, 2]], d1)
+    out = np.full((len(want) - 1, 2), 7, np.int32)
+    import ctypes
+    n = ctypes.c_uint64()
+    o = ctx.opts(left_outer=True, deterministic=True, **kw)
+    lib_fn = ctx.lib.eo_nlj_join if kind == "nlj" else ctx.lib.eo_hash_join
+    st = lib_fn(ctx.handle, R.ctypes.data, len(R), S.ctypes.data, len(S), out.ctypes.data, len(out),
+                ctypes.byref(n), ctypes.byref(o))
+    assert st == _abi.EO_ECAPACITY and n.value == len(want) and (out == 7).all()
 
 
 def test_left_outer_nlj_and_index_joins(ctx):
