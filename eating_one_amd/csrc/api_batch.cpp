@@ -43,6 +43,8 @@ struct JoinInfo {
   uint32_t req = 0;  // request index
   HashPlan hp;
   SideInfo r, s;
+  uint64_t bits_word = 0;  // left outer: first word of this join's bitmap in the batch's bitmap region
+  uint32_t eblocks = 0;    // left outer: blocks of the null-row kernel
 };
 
 }  // namespace
@@ -76,6 +78,10 @@ struct eo_batch {
          o_pair = 0, o_tiles = 0, o_tscr = 0;
   size_t o_joins_blob = 0;
   uint32_t npairs = 0;
+  bool wide = false, outer = false;  // batch flags: EO_F_RESULT16, EO_F_LEFT_OUTER
+  size_t o_bits = 0, o_emit = 0;     // bitmap region (arena), emit_start (blob)
+  uint64_t bits_words = 0;
+  uint32_t emit_blocks = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -94,6 +100,10 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   const int cus = c->props.cus;
   eo_join_opts po{};
   if (o) po = *o;
+  b->wide = o && (o->flags & EO_F_RESULT16);
+  b->outer = o && (o->flags & EO_F_LEFT_OUTER);
+  b->bits_words = 0;
+  b->emit_blocks = 0;
   po.stats = nullptr;
   po.stream = nullptr;
   po.flags = 0;
@@ -110,9 +120,16 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
     if (nR > kMaxRows || q.nS > kMaxRows) return EO_EINVAL;
     if ((!q.table && nR && !q.R) || (q.nS && !q.S) || (q.out_cap && !q.out)) return EO_EINVAL;
     if (q.table && q.table->ctx != c) return EO_EINVAL;
-    if (nR == 0 || q.nS == 0) continue;
+    if (q.nS == 0 || (nR == 0 && !b->outer)) continue;  // no rows (an outer join has a null row per S tuple)
     JoinInfo J;
     J.req = i;
+    if (b->outer) {
+      J.bits_word = b->bits_words;
+      b->bits_words += (matched_words(q.nS) + 63) / 64 * 64;  // 256-byte aligned per join
+      J.eblocks = (uint32_t)unmatched_blocks(q.nS);
+      if ((uint64_t)b->emit_blocks + J.eblocks >= (1ull << 31)) return EO_EINVAL;
+      b->emit_blocks += J.eblocks;
+    }
     if (q.table) {
       eo_join_opts to = po;
       to.radix_bits = q.table->hp.bits ? q.table->hp.bits : -1;
@@ -195,8 +212,13 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
       blob += 2 * sizeof(uint32_t);
     }
   }
+  if (b->outer) {
+    b->o_emit = blob;
+    blob += ((size_t)nj + 1) * sizeof(uint32_t);
+  }
   blob_bytes = (blob + 15) / 16 * 16;
   b->o_blob = L.add(blob_bytes);
+  b->o_bits = L.add(b->bits_words * sizeof(uint32_t));
   b->o_cursor = L.add((size_t)std::max<uint32_t>(nj, 1) * sizeof(uint64_t));
   b->o_head = L.add(sizeof(uint64_t));
   b->o_counts = L.add(max_counts * sizeof(uint64_t));
@@ -245,8 +267,20 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
     bj.out_cap = q.out ? q.out_cap : 0;
     bj.cursor = cursor + j;
     bj.pair_base = pair_base;
+    bj.wide = b->wide ? 1u : 0u;
+    bj.matched = b->outer ? reinterpret_cast<uint32_t *>(A + b->o_bits) + J.bits_word : nullptr;
+    bj.nS = J.s.n;
     pair_base += bj.P;
     hj[j] = bj;
+  }
+  if (b->outer) {
+    uint32_t *es = reinterpret_cast<uint32_t *>(H + b->o_emit);
+    uint32_t acc = 0;
+    for (size_t j = 0; j < b->joins.size(); ++j) {
+      es[j] = acc;
+      acc += b->joins[j].eblocks;
+    }
+    es[b->joins.size()] = acc;
   }
   const uint64_t len = batch_stripe_len();
   size_t gi = 0;
@@ -287,6 +321,8 @@ eo_status enqueue_batch(eo_batch *b, hipStream_t s) {
   if (nj == 0) return EO_OK;
   EO_TRY_HIP(fill_u64(cursor, 0, nj, s), "cursor reset");
   EO_TRY_HIP(fill_u32(head, 0u, 1, s), "queue reset");
+  if (b->outer)
+    EO_TRY_HIP(fill_u32(reinterpret_cast<uint32_t *>(A + b->o_bits), 0u, b->bits_words, s), "match bitmap reset");
   uint64_t *counts = reinterpret_cast<uint64_t *>(A + b->o_counts);
   for (auto &G : b->groups)
     EO_TRY_HIP(batch_partition_group(G.g, counts, A + b->o_cscr, c->d_err, s), "batch partition");
@@ -304,6 +340,10 @@ eo_status enqueue_batch(eo_batch *b, hipStream_t s) {
   uint32_t *tile_start = reinterpret_cast<uint32_t *>(A + b->o_tiles);
   EO_TRY_HIP(batch_plan(dj, nj, pair_join, b->npairs, tile_start, A + b->o_tscr, c->d_err, s), "batch plan");
   EO_TRY_HIP(batch_probe(dj, pair_join, tile_start, b->npairs, head, c->props.cus, s), "batch probe");
+  if (b->outer)
+    EO_TRY_HIP(batch_emit_unmatched(dj, nj, reinterpret_cast<const uint32_t *>(A + b->o_blob + b->o_emit),
+                                    b->emit_blocks, s),
+               "outer rows");
   EO_TRY_HIP(hipMemcpyAsync(b->h_counts, cursor, nj * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "count readback");
   return EO_OK;
 }

@@ -155,7 +155,38 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
     ref.r_n = min((uint32_t)kFCap, rn - c * kFCap);
     ref.s_beg = sb + j * J.s_span;
     ref.s_n = min(J.s_span, sn - j * J.s_span);
-    tile::probe_tile<tile::ProbeBig>(sh, J.R, J.S, ref, J.out, J.out_cap, J.cursor);
+    tile::probe_tile<tile::ProbeBig>(sh, J.R, J.S, ref, J.out, J.out_cap, J.cursor, J.wide != 0, J.matched);
+  }
+}
+
+// Null rows of the batch's left outer joins: block b belongs to the join j
+// with emit_start[j] <= b < emit_start[j + 1] and covers 256 bitmap words
+// (8192 S positions) of it; one atomic per block on the join's cursor.
+__global__ __launch_bounds__(256) void k_bemit(const BatchJoin *__restrict__ joins, uint32_t njoins,
+                                               const uint32_t *__restrict__ emit_start) {
+  __shared__ uint64_t red[256 / kWave + 1];
+  __shared__ uint64_t base_sh;
+  const uint32_t j = wave_find(emit_start, njoins, blockIdx.x);
+  const BatchJoin J = joins[j];
+  const uint64_t wd = (uint64_t)(blockIdx.x - emit_start[j]) * 256 + threadIdx.x;
+  uint32_t unm = 0;
+  if (wd * 32 < J.nS) {
+    const uint64_t left = J.nS - wd * 32;
+    const uint32_t valid = left >= 32 ? ~0u : ((1u << left) - 1u);
+    unm = ~J.matched[wd] & valid;
+  }
+  uint64_t tot;
+  const uint64_t excl = block_excl_scan<256, uint64_t>((uint64_t)__popc(unm), red, &tot);
+  if (threadIdx.x == 0) base_sh = tot ? atomic_add_agent(J.cursor, tot) : 0ull;
+  __syncthreads();
+  if (!J.out) return;
+  uint64_t pos = base_sh + excl;
+  while (unm) {
+    const uint32_t b = (uint32_t)__ffs(unm) - 1u;
+    unm &= unm - 1u;
+    const uint2 t = J.S[wd * 32 + b];
+    if (pos < J.out_cap) emit_null(J.out, pos, t.y, t.x, J.wide != 0);
+    ++pos;
   }
 }
 
@@ -206,6 +237,14 @@ hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_jo
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return scan_u32(tile_start, tile_start, (uint64_t)npairs + 1, nullptr, scan_scratch, err, stream);
+}
+
+hipError_t batch_emit_unmatched(const BatchJoin *joins, uint32_t njoins, const uint32_t *emit_start,
+                                uint32_t blocks, hipStream_t stream) {
+  if (njoins == 0 || blocks == 0) return hipSuccess;
+  EO_TIMED("outer_rows", stream,
+           hipLaunchKernelGGL(k_bemit, dim3(blocks), dim3(256), 0, stream, joins, njoins, emit_start));
+  return hipGetLastError();
 }
 
 hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,

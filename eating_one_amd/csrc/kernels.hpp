@@ -212,7 +212,9 @@ struct BatchJoin {
   uint64_t out_cap;
   uint64_t *cursor;          // match count / output reservation (zero on entry)
   uint32_t pair_base;        // first (join, partition) pair
-  uint32_t pad;
+  uint32_t wide;             // 16-byte RESULT rows
+  uint32_t *matched;         // left outer join: bitmap over S positions (zero on entry), or NULL
+  uint64_t nS;
 };
 uint64_t batch_stripe_len();
 hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
@@ -221,6 +223,11 @@ hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *sc
 // tile_start (npairs + 1 u32: scanned probe tile numbers of each pair).
 hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_join, uint32_t npairs,
                       uint32_t *tile_start, void *scan_scratch, uint32_t *err, hipStream_t stream);
+// Left outer joins of a batch: one null row per unmatched S tuple of every
+// join with a bitmap; emit_start (njoins + 1 u32) is the prefix of each
+// join's unmatched_blocks(nS), emit_start[njoins] the grid.
+hipError_t batch_emit_unmatched(const BatchJoin *joins, uint32_t njoins, const uint32_t *emit_start,
+                                uint32_t blocks, hipStream_t stream);
 // *queue_head must be zero on entry; grid = one workgroup per CU
 hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,
                        uint32_t *queue_head, int cus, hipStream_t stream);

@@ -431,7 +431,9 @@ eo_status eo_batch_footprint(eo_ctx *ctx, const eo_request *reqs, uint32_t n,
                              uint64_t *bytes);
 /* Enqueues the batch and returns without waiting.  opts (optional): stream,
  * stats (filled by eo_batch_wait), radix_bits / passes / lds_tuples / s_tile
- * overrides for requests without a table. */
+ * overrides for requests without a table, and flags for every request:
+ * EO_F_RESULT16 (16-byte rows), EO_F_LEFT_OUTER (a null row per unmatched S
+ * tuple; |S|/8 more scratch bytes per request, not in eo_batch_footprint). */
 eo_status eo_batch_submit(eo_batch *batch, const eo_request *reqs, uint32_t n,
                           const eo_join_opts *opts);
 /* EO_OK when the batch in flight has finished, EO_EAGAIN while it runs. */

@@ -198,3 +198,49 @@ def test_batch_stats_and_footprint(ctx):
     assert "batch_probe" in names
     assert st.ms_total > 0
     b.free()
+
+
+def test_batch_result16_and_left_outer(ctx):
+    """Batch flags: EO_F_RESULT16 rows and EO_F_LEFT_OUTER null rows for every
+    request (resident table, empty build side, misaligned S included); graph
+    replays repeat the eager rows."""
+    from eating_one_amd import _abi
+    inputs = _mixed_inputs(71, 6, lo=10, hi=17)
+    R0, S0 = gen.c2(seed=72, nR=5000, nS=9000, selectivity=0.4)
+    table = ctx.table_create(R0)
+    empty = np.empty((0, 2), np.int32)
+    cases = [(R, S, None, 0) for R, S in inputs] + [(R0, S0, table, 0), (empty, S0, None, 0), (R0, S0, None, 1)]
+
+    def outer_want(R, S):
+        inner = _oracle(R, S) if len(R) else np.empty((0, 2), np.int32)
+        miss = S[~np.isin(S[:, 0], R[:, 0])]
+        return np.concatenate([inner, np.stack([np.full(len(miss), _abi.EO_NULL_PAYLOAD, np.int32), miss[:, 1]], 1)])
+
+    b = ctx.batch()
+    try:
+        for wide in (False, True):
+            reqs, bufs = [], []
+            for i, (R, S, t, off) in enumerate(cases):
+                want = outer_want(R, S)
+                row = 16 if wide else 8
+                dR = DeviceArray.from_host(ctx, R) if (t is None and len(R)) else None
+                pad = np.zeros((off, 2), np.int32)
+                dS = DeviceArray.from_host(ctx, np.concatenate([pad, S]) if off else S)
+                dout = DeviceArray(ctx, max(len(want), 1) * row)
+                bufs.append((dR, dS, dout, want))
+                reqs.append(eo.request(R=dR, nR=len(R), S=dS.ptr + 8 * off, nS=len(S), out=dout, out_cap=len(want),
+                                       table=t, tag=i))
+            for rep in range(2):
+                res = b.run(reqs, left_outer=True, result16=wide) if rep == 0 else (b.replay() or b.wait())
+                for (tag, n, st), (dR, dS, dout, want) in zip(res, bufs):
+                    assert st == _abi.EO_OK and n == len(want), (tag, n, len(want))
+                    rows = dout.to_host(np.int32, (n, 4 if wide else 2))
+                    got = rows[:, [0, 2]] if wide else rows
+                    assert same_multiset(got, want), tag
+            for dR, dS, dout, _ in bufs:
+                for a in (dout, dS, dR):
+                    if a is not None:
+                        a.close()
+    finally:
+        b.free()
+        table.free()
