@@ -523,6 +523,14 @@ def run_c4(a):
                                        f"{build_s:.2f} s, then binary-search probes; timer scopes "
                                        f"inner-idx-bnlj.cc:204-238), C4 shape at |R| = 2^24, |S| = 2^25, "
                                        f"{m} matches", dt)
+        if a.cpu_mt_threads > 1:
+            t0 = time.perf_counter()
+            m2 = oracle.sidx_join_mt(r, s_, threads=a.cpu_mt_threads, count_only=True)
+            dt2 = time.perf_counter() - t0
+            mt = cpu_port(len(s_) / max(dt2 - build_s, 1e-9), "probes/s",
+                          f"pthread restatement of multi_joins/sortindex (serial quicksort build, then "
+                          f"{a.cpu_mt_threads} probe threads), same sample, {m2} matches", dt2, a.cpu_mt_threads)
+            res["cpu_baseline"]["multi_thread"] = {k: mt[k] for k in ("value", "unit", "cores", "sample")}
     return res
 
 

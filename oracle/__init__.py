@@ -55,6 +55,8 @@ def lib():
         L.oracle_nlj_dist_join.restype = U64
         L.oracle_hash_join_mt.argtypes = [P, U64, P, U64, U32, I, P, U64]
         L.oracle_hash_join_mt.restype = U64
+        L.oracle_sidx_join_mt.argtypes = [P, U64, P, U64, I, P, U64]
+        L.oracle_sidx_join_mt.restype = U64
         _lib = L
     return _lib
 
@@ -141,6 +143,21 @@ def hash_join_mt(R, S, nparts: int = 1 << 20, threads: int = 1, count_only: bool
     if n > cap:
         out = np.empty((n, 2), dtype=np.int32)
         fn(_ptr(R), len(R), _ptr(S), len(S), nparts, threads, _ptr(out), n)
+    return out[:n]
+
+
+def sidx_join_mt(R, S, threads: int = 1, count_only: bool = False):
+    """joins/multi_joins/sortindex/thread-hash-bnlj.cc:180-335 with per-thread output."""
+    R, S = _tuples(R), _tuples(S)
+    fn = lib().oracle_sidx_join_mt
+    n = fn(_ptr(R), len(R), _ptr(S), len(S), threads, None, 0)
+    if n == FAIL:
+        raise MemoryError("oracle allocation failed")
+    if count_only:
+        return n
+    out = np.empty((max(n, 1), 2), dtype=np.int32)
+    if fn(_ptr(R), len(R), _ptr(S), len(S), threads, _ptr(out), n) == FAIL:
+        raise MemoryError("oracle allocation failed")
     return out[:n]
 
 

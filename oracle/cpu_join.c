@@ -482,3 +482,95 @@ uint64_t oracle_hash_join_mt(const otuple *R, uint64_t nR, const otuple *S, uint
   free(sloc);
   return total;
 }
+
+/* ---- N-core sorted-index join ----------------------------------------------
+ * The pthread sorted-index join of joins/multi_joins/sortindex/
+ * thread-hash-bnlj.cc:180-335: the index is built as in sortIndex_cpu (copy
+ * {val, adr}, quicksort, order check), then workers take LT_BUF-sized chunks
+ * of S from a shared cursor (LTGetter, :210) and binary-search + expand each
+ * probe (executor, :254).  Fixes: a shared atomic counter instead of the mutex
+ * around the cursor, per-thread result buffers instead of a mutex per result
+ * row (:276, :292), and the expansions are bounds-checked as in
+ * oracle_sidx_join. */
+
+typedef struct {
+  const otuple *R, *S, *b;
+  uint64_t nR, nS;
+  uint64_t *next; /* shared S cursor */
+  opair *buf;
+  uint64_t n, cap;
+  int count_only;
+} sidx_task;
+
+static int sidx_push(sidx_task *t, int32_t rp, int32_t sp) {
+  if (!t->count_only) {
+    if (t->n == t->cap) {
+      uint64_t nc = t->cap ? t->cap * 2 : 4096;
+      opair *nb = (opair *)realloc(t->buf, nc * sizeof(opair));
+      if (!nb) return -1;
+      t->buf = nb;
+      t->cap = nc;
+    }
+    t->buf[t->n].r_payload = rp;
+    t->buf[t->n].s_payload = sp;
+  }
+  t->n++;
+  return 0;
+}
+
+static void *sidx_probe_thread(void *arg) {
+  sidx_task *t = (sidx_task *)arg;
+  const uint64_t chunk = 4096; /* the reference's LT_BUF */
+  for (;;) {
+    const uint64_t j0 = __atomic_fetch_add(t->next, chunk, __ATOMIC_RELAXED);
+    if (j0 >= t->nS) break;
+    const uint64_t j1 = j0 + chunk < t->nS ? j0 + chunk : t->nS;
+    for (uint64_t j = j0; j < j1; ++j) {
+      const int32_t v = t->S[j].key;
+      const uint64_t bidx = ref_search(t->b, v, t->nR);
+      uint64_t x = bidx;
+      while (t->b[x].key == v) {
+        if (sidx_push(t, t->R[t->b[x].payload].payload, t->S[j].payload)) return (void *)1;
+        if (x == 0) break;
+        x--;
+      }
+      for (x = bidx + 1; x < t->nR && t->b[x].key == v; ++x)
+        if (sidx_push(t, t->R[t->b[x].payload].payload, t->S[j].payload)) return (void *)1;
+    }
+  }
+  return NULL;
+}
+
+uint64_t oracle_sidx_join_mt(const otuple *R, uint64_t nR, const otuple *S, uint64_t nS, int nthreads, opair *out,
+                             uint64_t cap) {
+  if (nR == 0) return 0;
+  if (nthreads < 1) nthreads = 1;
+  if (nthreads > 256) nthreads = 256;
+  otuple *b = (otuple *)malloc(nR * sizeof(otuple));
+  if (!b || oracle_sidx_build(R, nR, b)) {
+    free(b);
+    return UINT64_MAX;
+  }
+  pthread_t th[256];
+  sidx_task tk[256];
+  uint64_t next = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    tk[t] = (sidx_task){R, S, b, nR, nS, &next, NULL, 0, 0, out == NULL};
+    pthread_create(&th[t], NULL, sidx_probe_thread, &tk[t]);
+  }
+  int failed = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    void *rv = NULL;
+    pthread_join(th[t], &rv);
+    failed |= rv != NULL;
+  }
+  uint64_t total = 0;
+  for (int t = 0; t < nthreads; ++t) {
+    if (out)
+      for (uint64_t i = 0; i < tk[t].n && total + i < cap; ++i) out[total + i] = tk[t].buf[i];
+    total += tk[t].n;
+    free(tk[t].buf);
+  }
+  free(b);
+  return failed ? UINT64_MAX : total;
+}

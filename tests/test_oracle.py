@@ -50,7 +50,7 @@ def test_oracle_partition_bits_equals_modulo(join_ori):
     assert np.array_equal(a, b) and np.array_equal(sa, sb)
 
 
-@pytest.mark.parametrize("algo", ["hash", "hidx", "sidx", "nlj", "mt"])
+@pytest.mark.parametrize("algo", ["hash", "hidx", "sidx", "nlj", "mt", "sidx_mt"])
 def test_join_ori_join_counts(join_ori, algo):
     doc, R, S = join_ori
     Re, Se = ref_to_eo(R), ref_to_eo(S)
@@ -60,6 +60,7 @@ def test_join_ori_join_counts(join_ori, algo):
         "sidx": lambda: oracle.sidx_join(Re, Se),
         "nlj": lambda: oracle.nlj_join(Re, Se),
         "mt": lambda: oracle.hash_join_mt(Re, Se, nparts=4, threads=4),
+        "sidx_mt": lambda: oracle.sidx_join_mt(Re, Se, threads=4),
     }[algo]
     pairs = fn()
     assert len(pairs) == doc["join"]["pairs"]
@@ -80,7 +81,8 @@ def test_oracles_agree_with_sort_reduction_random():
         want = brute_join(R, S)
         assert len(want) == join_count_closed_form(R, S) if nR and nS else len(want) == 0
         for got in (oracle.hash_join(R, S, nparts=64), oracle.hidx_join(R, S, nbuckets=1024),
-                    oracle.sidx_join(R, S), oracle.nlj_join(R, S), oracle.hash_join_mt(R, S, nparts=64, threads=3)):
+                    oracle.sidx_join(R, S), oracle.nlj_join(R, S), oracle.hash_join_mt(R, S, nparts=64, threads=3),
+                    oracle.sidx_join_mt(R, S, threads=3)):
             assert same_multiset(got, want)
 
 
