@@ -2,7 +2,12 @@
  * eo_join — command-line driver with the reference binaries' argument order.
  *
  *   eo_join <op> <|S|> <|R|> [seed] [selectivity]
- *     op: hash | nlj | hidx | sidx | learned
+ *     op: hash | nlj | hidx | sidx | learned | stream-hash | stream-nlj
+ *
+ * stream-*: the inputs are held in pinned host memory and joined by
+ * eo_stream_join tile by tile (R and S tiles of 2^18 tuples, the reference
+ * NLJ's PART of joins/blockJoin_gpu/main_street/tuple.h:13), the rows copied
+ * back to host memory.
  *
  * Replaces `./HashExe <left=|S|> <right=|R|>` (joins/hashJoin_gpu/main_street/
  * hj.cc:1277-1290) and the NLJ / index executables of the same form
@@ -27,7 +32,8 @@ static int fail(const char *what, eo_status st) {
 
 int main(int argc, char **argv) {
   if (argc < 4) {
-    fprintf(stderr, "usage: %s <hash|nlj|hidx|sidx|learned> <|S|> <|R|> [seed] [selectivity]\n", argv[0]);
+    fprintf(stderr, "usage: %s <hash|nlj|hidx|sidx|learned|stream-hash|stream-nlj> <|S|> <|R|> [seed] [selectivity]\n",
+            argv[0]);
     return 2;
   }
   const char *op = argv[1];
@@ -41,6 +47,8 @@ int main(int argc, char **argv) {
   else if (!strcmp(op, "hidx")) kind = 2;
   else if (!strcmp(op, "sidx")) kind = 3;
   else if (!strcmp(op, "learned")) kind = 4;
+  else if (!strcmp(op, "stream-hash")) kind = 5;
+  else if (!strcmp(op, "stream-nlj")) kind = 6;
   if (kind < 0) {
     fprintf(stderr, "eo_join: unknown op '%s'\n", op);
     return 2;
@@ -58,6 +66,42 @@ int main(int argc, char **argv) {
   o.where = EO_DEVICE;
   if ((st = eo_gen_build(ctx, R, nR, seed, &o)) != EO_OK) return fail("eo_gen_build", st);
   if ((st = eo_gen_probe(ctx, S, nS, nR, sel, seed + 1, &o)) != EO_OK) return fail("eo_gen_probe", st);
+
+  if (kind >= 5) {  /* host-resident inputs, streamed tile by tile */
+    const uint64_t part = 1ull << 18;
+    eo_tuple *hR = NULL, *hS = NULL;
+    eo_pair *hO = NULL;
+    if ((st = eo_host_alloc(ctx, (nR ? nR : 1) * sizeof(eo_tuple), (void **)&hR)) != EO_OK) return fail("host R", st);
+    if ((st = eo_host_alloc(ctx, (nS ? nS : 1) * sizeof(eo_tuple), (void **)&hS)) != EO_OK) return fail("host S", st);
+    if ((st = eo_copy_d2h(ctx, hR, R, nR * sizeof(eo_tuple))) != EO_OK) return fail("copy R", st);
+    if ((st = eo_copy_d2h(ctx, hS, S, nS * sizeof(eo_tuple))) != EO_OK) return fail("copy S", st);
+    eo_stream_stats ss;
+    eo_stream_opts so;
+    memset(&so, 0, sizeof(so));
+    so.struct_size = sizeof(so);
+    so.op = kind == 5 ? EO_STREAM_HASH : EO_STREAM_NLJ;
+    so.r_tile = part;
+    so.s_chunk = part;
+    uint64_t n = 0, n2 = 0;
+    if ((st = eo_stream_join(ctx, hR, nR, hS, nS, NULL, 0, &n, &so)) != EO_OK) return fail("count", st);
+    if ((st = eo_host_alloc(ctx, (n ? n : 1) * sizeof(eo_pair), (void **)&hO)) != EO_OK) return fail("host out", st);
+    so.stats = &ss;
+    if ((st = eo_stream_join(ctx, hR, nR, hS, nS, hO, n, &n2, &so)) != EO_OK) return fail("join", st);
+    printf("op=%s |S|=%llu |R|=%llu seed=0x%llx selectivity=%.3f\n", op, (unsigned long long)nS,
+           (unsigned long long)nR, (unsigned long long)seed, sel);
+    printf("count: %llu\n", (unsigned long long)n2);
+    printf("wall time: %.3f ms (%u R tiles x %u S chunks, %.1f MB in, %.1f MB out)\n", ss.ms_total, ss.r_tiles,
+           ss.r_tiles ? ss.s_chunks / ss.r_tiles : 0, ss.h2d_bytes / 1e6, ss.d2h_bytes / 1e6);
+    for (uint64_t i = 0; i < (n2 < 3 ? n2 : 3); ++i)
+      printf("pair %llu: R payload %d, S payload %d\n", (unsigned long long)i, hO[i].r_payload, hO[i].s_payload);
+    eo_host_free(ctx, hO);
+    eo_host_free(ctx, hS);
+    eo_host_free(ctx, hR);
+    eo_dev_free(ctx, S);
+    eo_dev_free(ctx, R);
+    eo_ctx_destroy(ctx);
+    return 0;
+  }
 
   eo_index *ix = NULL;
   eo_join_stats stats;

@@ -50,3 +50,25 @@ def test_c4_full_size_learned_equals_binary(ctx):
     assert np.array_equal(ll[present], lb[present])
     pos = np.minimum(ll[~present], n - 1).astype(np.int64)
     assert not np.any(keys[pos] == S[~present, 0])
+
+
+def test_c2_full_size_streamed_from_host(ctx):
+    """The C2 relations held in pinned host memory and joined by eo_stream_join
+    (S in 16 chunks of 2^24): the same properties as the in-HBM join."""
+    nR, nS = 1 << 27, 1 << 28
+    R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS)
+    hR, hS, hO = eo.HostArray(ctx, R.nbytes), eo.HostArray(ctx, S.nbytes), eo.HostArray(ctx, 8 * nS)
+    try:
+        vR, vS, vO = hR.view(np.int32, R.shape), hS.view(np.int32, S.shape), hO.view(np.int32, (nS, 2))
+        vR[:] = R
+        vS[:] = S
+        st = _abi.StreamStats()
+        n, status = ctx.stream_join(vR, vS, out=vO, stats=st)
+        assert status == _abi.EO_OK and n == nS and st.pinned == 7 and st.s_chunks == 16
+        pairs = vO.copy()
+    finally:
+        hO.close()
+        hS.close()
+        hR.close()
+    assert np.array_equal(np.bincount(pairs[:, 1], minlength=nS), np.ones(nS, dtype=np.int64))
+    assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])

@@ -121,7 +121,7 @@ def test_serve_budget_admission(ctx):
         t.free()
 
 
-@pytest.mark.parametrize("op", ["hash", "nlj", "hidx", "sidx", "learned"])
+@pytest.mark.parametrize("op", ["hash", "nlj", "hidx", "sidx", "learned", "stream-hash", "stream-nlj"])
 def test_cli_counts(op):
     """eo_join <op> <|S|> <|R|>: with selectivity 1 every S key matches exactly one R key."""
     import os
