@@ -428,8 +428,14 @@ def run_c3_hidx(a):
     R, S = gen.c3()
     ctx = eo.Context(0)
     stats = _abi.JoinStats()
-    ix = ctx.hidx_build(R, stats=stats)
-    build_ms = stats.ms_build
+    builds = []
+    for it in range(1 + 5):  # the first build also loads the kernels' code: not timed
+        ix = ctx.hidx_build(R, stats=stats)
+        if it:
+            builds.append(stats.ms_build)
+        if it < 5:
+            ix.free()
+    build_ms = sorted(builds)[len(builds) // 2]
     dS = eo.DeviceArray.from_host(ctx, S)
     L = ctx.lib
     n = ctypes.c_uint64()
@@ -473,8 +479,15 @@ def run_c4(a):
     R, S = gen.c4(nR=nR, nS=nS)
     ctx = eo.Context(0)
     stats = _abi.JoinStats()
-    ix = ctx.sidx_build(R, learned=True, stats=stats)
+    builds = []
+    for it in range(1 + 3):  # the first build also loads the kernels' code: not timed
+        ix = ctx.sidx_build(R, learned=True, stats=stats)
+        if it:
+            builds.append(stats.ms_build)
+        if it < 3:
+            ix.free()
     info = ix.info()
+    info["build_ms"] = sorted(builds)[len(builds) // 2]
     dS = eo.DeviceArray.from_host(ctx, S)
     L = ctx.lib
     pos = eo.DeviceArray(ctx, nS * 8)
