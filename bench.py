@@ -9,9 +9,12 @@ when the timed region starts; pairs written to a device buffer).
                algorithmic bytes per launch / its mean launch time vs 8 TB/s
   pipeline     whole join: 24(|R|+|S|) + 8|out| bytes / join time vs 8 TB/s
   cpu_baseline the CPU_JOIN hash join restated in oracle/ on a bounded sample
+  batched      the metric's second half, from the C5 stream (4096 joins of
+               log-uniform sizes): batched throughput, speedup over per-join
+               calls, p50/p99 latency under Poisson arrivals (--no-c5 skips it)
 
-Other configurations (parity cases; reported in the design notes, not the
-driver's line): --config c3-nlj | c3-hidx | c4 | c5 | nlj-dist.
+Other configurations print their own lines: --config c1 | c3-nlj | c3-hidx |
+c4 | c5 | nlj-dist | stream.
 """
 from __future__ import annotations
 
@@ -195,7 +198,7 @@ def run_c2(a):
     pipe = alg_bytes / dt / 1e9
     ceiling = copy_ceiling(ctx)
     res = {
-        "metric": "join tuples/s & %HBM peak, 2^27 x 2^28 hash join",
+        "metric": "join tuples/s & %HBM peak, 2^27\u22c82^28 hash join; p99 latency, batched joins",
         "value": (nR + nS) / dt,
         "unit": "tuples/s",
         "n_gpus": 1,
@@ -637,7 +640,9 @@ def run_c5(a):
     kw = dict(max_batch=a.c5_max_batch, slots=a.c5_slots)
     out = {}
     counts = None
-    for name, mode in (("per_join", _abi.EO_SERVE_PER_JOIN), ("batched", _abi.EO_SERVE_BATCHED)):
+    modes = [m for m in (("per_join", _abi.EO_SERVE_PER_JOIN), ("batched", _abi.EO_SERVE_BATCHED))
+             if a.c5_modes in ("both", m[0])]
+    for name, mode in modes:
         ctx.serve(reqs, None, mode=mode, **kw)  # warm-up: arenas grow to their steady size
         walls = []
         for _ in range(max(1, a.c5_reps)):
@@ -651,6 +656,12 @@ def run_c5(a):
         out[name] = {"wall_ms": float(np.median(walls)), "tuples_per_s": tuples / (np.median(walls) * 1e-3),
                      "batches": st["batches"], "largest_batch": st["largest_batch"]}
         log(f"{name}: {np.median(walls):.1f} ms for {n_req} requests ({tuples / np.median(walls) / 1e6:.2f} G tuples/s)")
+    if a.c5_modes != "both":  # one mode only (profiling): closed loop only
+        for _, t, d in tables:
+            t.free()
+            d.close()
+        ctx.close()
+        return {"metric": "C5 join stream, one mode", "closed_loop": out}
     capacity_qps = n_req / (out["batched"]["wall_ms"] * 1e-3)
     qps = a.c5_qps if a.c5_qps else round(0.7 * capacity_qps, -2)
     arrival = np.cumsum(rng.exponential(1000.0 / qps, size=n_req))
@@ -693,10 +704,11 @@ def main(argv=None) -> int:
     ap.add_argument("--c5-tables", type=int, default=64)
     ap.add_argument("--c5-reuse", type=float, default=0.25)
     ap.add_argument("--c5-budget", type=float, default=200e9)
-    ap.add_argument("--c5-max-batch", type=int, default=512)
+    ap.add_argument("--c5-max-batch", type=int, default=1024)
     ap.add_argument("--c5-slots", type=int, default=2)
     ap.add_argument("--c5-reps", type=int, default=3)
     ap.add_argument("--c5-qps", type=float, default=0)
+    ap.add_argument("--c5-modes", default="both", choices=["both", "batched", "per_join"])
     ap.add_argument("--log2-r", type=int, default=27)
     ap.add_argument("--theta", type=float, default=0.0)
     ap.add_argument("--selectivity", type=float, default=1.0)
@@ -710,12 +722,24 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-mt-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="threads of the additional N-core CPU baseline (0/1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="c2: skip the batched C5 stream the default line carries")
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): the library has no multi-GPU path",
               file=sys.stderr)
         return 2
+    if a.config == "c2" and not a.no_c5 and a.log2_r == 27:
+        # the metric's second half: the batched C5 stream (throughput, speedup, p50/p99 latency)
+        res = run_c2(a)
+        c5 = run_c5(a)
+        res["batched"] = {"workload": c5["config"]["workload"], "tuples_per_s": c5["value"],
+                          "speedup_vs_per_join": c5["speedup_batched_vs_per_join"],
+                          "qps": c5["poisson"]["qps"], "p50_ms": c5["poisson"]["batched"]["p50_ms"],
+                          "p99_ms": c5["poisson"]["batched"]["p99_ms"],
+                          "per_join_p99_ms": c5["poisson"]["per_join"]["p99_ms"]}
+        print(json.dumps(res), flush=True)
+        return 0
     fn = {"c1": run_c1, "c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,
           "nlj-dist": run_nlj_dist, "stream": run_stream}[a.config]
     print(json.dumps(fn(a)), flush=True)
