@@ -343,7 +343,7 @@ eo_status eo_index_join(eo_ctx *ctx, const eo_index *index, const eo_tuple *S,
 /* Per probe, the index position its search starts from: the lower bound of
  * S[i].key (sorted index; method BINARY or LEARNED — equal for every key
  * present in R) or the bucket start (hash index).  pos: nS uint64, host or
- * device per opts->where.  Used to judge the learned model against binary
+ * device per opts->where.  Used to check the learned model against binary
  * search probe by probe. */
 eo_status eo_index_lookup(eo_ctx *ctx, const eo_index *index, const eo_tuple *S,
                           uint64_t nS, int method, uint64_t *pos,
