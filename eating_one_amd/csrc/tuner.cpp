@@ -5,9 +5,11 @@
 // p_num from |R| / 64 and 6 bits per pass).  Here the planner's choice
 // (plan.cpp: partitions sized to one LDS table) is a starting point: for one
 // size class (ceil log2 |R|, ceil log2 |S|) eo_tune_hash_join times every
-// candidate on the device — radix bits around the planner's, one or two
-// passes, three S spans per probe tile — and keeps the fastest in the
-// context's table.  Joins and batches that leave radix_bits, passes,
+// candidate on the device — the LDS table size (a full 17408-tuple table,
+// one 1024-thread workgroup per CU, or a half table with two workgroups per
+// CU), radix bits around the planner's for that table, one or two passes,
+// three S spans per probe tile — and keeps the fastest in the context's
+// table.  Joins and batches that leave radix_bits, passes,
 // lds_tuples and s_tile at 0 then use the table's entry for their class.
 // Every candidate computes the same result (tested), so tuning only changes
 // speed.
@@ -40,6 +42,7 @@ void apply_tuned(const eo_ctx *c, uint64_t nR, uint64_t nS, eo_join_opts *o) {
   if (it == c->tuned.end()) return;
   o->radix_bits = it->second.radix_bits ? it->second.radix_bits : -1;
   o->passes = it->second.passes;
+  o->lds_tuples = it->second.lds_tuples;
   o->s_tile = it->second.s_tile;
 }
 
@@ -55,32 +58,39 @@ eo_status eo_tune_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_
     return EO_EINVAL;
   reps = std::max(1, std::min(reps, 16));
   std::memset(res, 0, sizeof(*res));
-  HashPlan base;
-  eo_status st = plan_hash_join(nR, nS, nullptr, &base, c->props.cus);
-  if (st != EO_OK) return st;
-  // candidates: bits b0-1 .. b0+2, passes 1 (bits <= 11) or 2, span auto / x2 / x0.5
+  eo_status st = EO_OK;
+  // candidates per LDS table (0: the full table, 8192: the half table):
+  // bits b0-1 .. b0+2 around the planner's for that table, passes 1
+  // (bits <= 11) or 2, span auto / x2 / x0.5
   struct Cand {
-    int bits, passes, s_tile;
+    int bits, passes, lds, s_tile;
   };
   std::vector<Cand> cands;
-  for (int b = std::max(0, base.bits - 1); b <= std::min(kMaxJoinBits, base.bits + 2); ++b) {
-    for (int p = 1; p <= 2; ++p) {
-      if (b == 0 && p == 2) continue;
-      if (b > 0 && p == 1 && b > kMaxPassBits) continue;
-      if (p == 2 && (b < 2 || b > 2 * kMaxPassBits)) continue;
-      HashPlan hp;
-      eo_join_opts o{};
-      o.radix_bits = b ? b : -1;
-      o.passes = b ? p : 0;
-      if (plan_hash_join(nR, nS, &o, &hp, c->props.cus) != EO_OK) continue;
-      for (int k = 0; k < 3; ++k) {
-        uint32_t span = hp.s_span;
-        if (k == 1) span = std::min<uint32_t>(span * 2, 1u << 18);
-        if (k == 2) span = std::max<uint32_t>(span / 2, 8192u);
-        if (k > 0 && span == hp.s_span) continue;
-        cands.push_back({b, b ? p : 0, k == 0 ? 0 : (int)span});
+  for (int lds : {0, 8192}) {
+    eo_join_opts lo{};
+    lo.lds_tuples = lds;
+    HashPlan base;
+    if (plan_hash_join(nR, nS, &lo, &base, c->props.cus) != EO_OK) continue;
+    for (int b = std::max(0, base.bits - 1); b <= std::min(kMaxJoinBits, base.bits + 2); ++b) {
+      for (int p = 1; p <= 2; ++p) {
+        if (b == 0 && p == 2) continue;
+        if (b > 0 && p == 1 && b > kMaxPassBits) continue;
+        if (p == 2 && (b < 2 || b > 2 * kMaxPassBits)) continue;
+        HashPlan hp;
+        eo_join_opts o{};
+        o.radix_bits = b ? b : -1;
+        o.passes = b ? p : 0;
+        o.lds_tuples = lds;
+        if (plan_hash_join(nR, nS, &o, &hp, c->props.cus) != EO_OK) continue;
+        for (int k = 0; k < 3; ++k) {
+          uint32_t span = hp.s_span;
+          if (k == 1) span = std::min<uint32_t>(span * 2, 1u << 18);
+          if (k == 2) span = std::max<uint32_t>(span / 2, 8192u);
+          if (k > 0 && span == hp.s_span) continue;
+          cands.push_back({b, b ? p : 0, lds, k == 0 ? 0 : (int)span});
+        }
+        if (b == 0) break;
       }
-      if (b == 0) break;
     }
   }
   c->tuned.erase(tune_key(nR, nS));  // the baseline below is the untuned planner
@@ -107,16 +117,17 @@ eo_status eo_tune_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_
   for (const Cand &cd : cands) {
     o.radix_bits = cd.bits ? cd.bits : -1;
     o.passes = cd.passes;
+    o.lds_tuples = cd.lds;
     o.s_tile = cd.s_tile;
     double ms = 0;
     st = time_plan(o, &ms);
     if (st != EO_OK) return st;
-    if (ms < best.ms) best = {cd.bits, cd.passes, 0, cd.s_tile, ms};
+    if (ms < best.ms) best = {cd.bits, cd.passes, cd.lds, cd.s_tile, ms};
   }
   c->tuned[tune_key(nR, nS)] = best;
   res->radix_bits = best.radix_bits;
   res->passes = best.passes;
-  res->lds_tuples = 0;
+  res->lds_tuples = best.lds_tuples;
   res->s_tile = best.s_tile;
   res->ms = best.ms;
   res->default_ms = def_ms;
@@ -131,6 +142,7 @@ eo_status eo_tune_lookup(eo_ctx *c, uint64_t nR, uint64_t nS, eo_tune_result *re
   if (it == c->tuned.end()) return EO_EAGAIN;
   res->radix_bits = it->second.radix_bits;
   res->passes = it->second.passes;
+  res->lds_tuples = it->second.lds_tuples;
   res->s_tile = it->second.s_tile;
   res->ms = it->second.ms;
   return EO_OK;
@@ -152,10 +164,10 @@ eo_status eo_tune_save(eo_ctx *c, const char *path) {
     set_last_error(std::string("cannot write ") + path);
     return EO_EINVAL;
   }
-  std::fprintf(f, "eojoin-tune 1\n");
+  std::fprintf(f, "eojoin-tune 2\n");  // rows: log2|R| log2|S| bits passes lds_tuples s_tile ms
   for (const auto &kv : c->tuned)
-    std::fprintf(f, "%d %d %d %d %d %.6f\n", kv.first.first, kv.first.second, kv.second.radix_bits,
-                 kv.second.passes, kv.second.s_tile, kv.second.ms);
+    std::fprintf(f, "%d %d %d %d %d %d %.6f\n", kv.first.first, kv.first.second, kv.second.radix_bits,
+                 kv.second.passes, kv.second.lds_tuples, kv.second.s_tile, kv.second.ms);
   const bool ok = std::fclose(f) == 0;
   return ok ? EO_OK : EO_EINVAL;
 }
@@ -168,19 +180,22 @@ eo_status eo_tune_load(eo_ctx *c, const char *path) {
     return EO_EINVAL;
   }
   int ver = 0;
-  if (std::fscanf(f, "eojoin-tune %d", &ver) != 1 || ver != 1) {
+  if (std::fscanf(f, "eojoin-tune %d", &ver) != 1 || (ver != 1 && ver != 2)) {
     std::fclose(f);
     set_last_error("not an eojoin tuning table");
     return EO_EINVAL;
   }
   std::vector<std::pair<TuneKey, TunedPlan>> rows;
-  int kr, ks, b, p, st;
+  int kr, ks, b, p, lds = 0, st;
   double ms;
   int got;
-  while ((got = std::fscanf(f, "%d %d %d %d %d %lf", &kr, &ks, &b, &p, &st, &ms)) == 6) {
+  const int fields = ver == 1 ? 6 : 7;  // version 1 rows have no lds_tuples column
+  while ((got = ver == 1 ? std::fscanf(f, "%d %d %d %d %d %lf", &kr, &ks, &b, &p, &st, &ms)
+                         : std::fscanf(f, "%d %d %d %d %d %d %lf", &kr, &ks, &b, &p, &lds, &st, &ms)) == fields) {
     eo_join_opts o{};
     o.radix_bits = b ? b : -1;
     o.passes = p;
+    o.lds_tuples = lds;
     o.s_tile = st;
     HashPlan hp;
     if (kr < 0 || kr > 31 || ks < 0 || ks > 31 || st < 0 ||
@@ -189,7 +204,7 @@ eo_status eo_tune_load(eo_ctx *c, const char *path) {
       set_last_error("invalid tuning table row");
       return EO_EINVAL;
     }
-    rows.push_back({{kr, ks}, {b, p, 0, st, ms}});
+    rows.push_back({{kr, ks}, {b, p, lds, st, ms}});
   }
   const bool clean_end = got == EOF;
   std::fclose(f);

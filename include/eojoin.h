@@ -459,8 +459,10 @@ typedef struct eo_tune_result {
   uint32_t reserved;
 } eo_tune_result;
 
-/* Times candidate plans (radix bits around the planner's choice, one or two
- * passes, three S spans) for the size class of (|R|, |S|) = (ceil log2 nR,
+/* Times candidate plans (the LDS table size — a full table with one
+ * workgroup per CU, or a half table with two — radix bits around the
+ * planner's choice for it, one or two passes, three S spans) for the size
+ * class of (|R|, |S|) = (ceil log2 nR,
  * ceil log2 nS) on the given device inputs, best of `reps` runs each, and
  * stores the fastest in the ctx's table.  eo_hash_join and eo_batch_submit
  * use the table for requests whose opts leave radix_bits, passes,

@@ -22,11 +22,16 @@ def test_tune_then_join_equals_oracle(ctx, tmp_path):
     assert res["ms"] <= res["default_ms"] * 1.5
     entry = ctx.tune_lookup(len(R), len(S))
     assert entry["radix_bits"] == res["radix_bits"] and entry["passes"] == res["passes"]
+    assert entry["lds_tuples"] == res["lds_tuples"] and res["lds_tuples"] in (0, 8192)
     # the plain call now runs the tuned plan; same multiset as the oracle
     st = eo.JoinStats()
     got = ctx.hash_join(R, S, stats=st)
     assert st.radix_bits == res["radix_bits"] and st.passes == res["passes"]
+    assert st.lds_tuples == (res["lds_tuples"] or 16384)
     assert same_multiset(got, want)
+    # every LDS table size the tuner tries gives the oracle's rows
+    for lds in (8192, 16384):
+        assert same_multiset(ctx.hash_join(R, S, lds_tuples=lds), want)
     # a size class that was not tuned is untouched
     assert ctx.tune_lookup(len(R) * 4, len(S)) is None
     # save / clear / load round trip
@@ -36,6 +41,7 @@ def test_tune_then_join_equals_oracle(ctx, tmp_path):
     assert ctx.tune_lookup(len(R), len(S)) is None
     ctx.tune_load(path)
     assert ctx.tune_lookup(len(R), len(S))["radix_bits"] == res["radix_bits"]
+    assert ctx.tune_lookup(len(R), len(S))["lds_tuples"] == res["lds_tuples"]
     # batches use the table too
     b = ctx.batch()
     (tag, n, status), = b.run([eo.request(R=dR, nR=len(R), S=dS, nS=len(S), out=dout, out_cap=len(want))])
@@ -56,3 +62,9 @@ def test_tune_load_rejects_bad_tables(ctx, tmp_path):
     with pytest.raises(eo.EOError):
         ctx.tune_load(str(p))
     assert ctx.tune_lookup(1 << 17, 1 << 19) is None
+    # a version-1 table (no LDS column) still loads
+    p.write_text("eojoin-tune 1\n17 19 4 1 0 1.0\n")
+    ctx.tune_load(str(p))
+    e = ctx.tune_lookup(1 << 17, 1 << 19)
+    assert e["radix_bits"] == 4 and e["lds_tuples"] == 0
+    ctx.tune_clear()
