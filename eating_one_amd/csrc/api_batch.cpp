@@ -27,6 +27,7 @@ namespace eo {
 namespace {
 
 constexpr int kMaxGroupBits = kMaxPassBits;  // one fast pass per group
+constexpr uint64_t kMaxTileDescs = 1ull << 21;  // 64 MiB of probe tile descriptors per batch
 
 // Where one relation of one join comes from and goes to.
 struct SideInfo {
@@ -75,7 +76,8 @@ struct eo_batch {
   };
   std::vector<Group> groups;
   size_t o_blob = 0, blob_bytes = 0, o_cursor = 0, o_head = 0, o_counts = 0, o_cscr = 0, o_tmp = 0, o_pscr = 0,
-         o_pair = 0, o_tiles = 0, o_tscr = 0;
+         o_pair = 0, o_tiles = 0, o_tscr = 0, o_tdesc = 0;
+  uint32_t tiles_cap = 0;  // probe tile descriptors written ahead of the probe
   size_t o_joins_blob = 0;
   uint32_t npairs = 0;
   bool wide = false, outer = false;  // batch flags: EO_F_RESULT16, EO_F_LEFT_OUTER
@@ -114,6 +116,12 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   uint64_t max_tmp = 0;
   size_t max_pscr = 0;
   uint64_t npairs = 0;
+  uint64_t tiles_est = 0;
+  // S tuples of the whole batch: the probe tiles of all joins share one
+  // persistent launch, so each join's S spans are planned against this pool
+  // (long spans, few LDS table builds) rather than as if it ran alone
+  uint64_t s_pool = 0;
+  for (uint32_t i = 0; i < n; ++i) s_pool += std::min<uint64_t>(reqs[i].nS, kMaxRows);
   for (uint32_t i = 0; i < n; ++i) {
     const eo_request &q = reqs[i];
     const uint64_t nR = q.table ? q.table->n : q.nR;
@@ -136,11 +144,13 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
       to.passes = q.table->hp.passes;
       eo_status st = plan_hash_join(nR, q.nS, &to, &J.hp, cus);
       if (st != EO_OK) return st;
+      if (to.s_tile == 0) J.hp.s_span = plan_span(nR, q.nS, J.hp.lds_cap, J.hp.bits, s_pool, cus);
     } else {
       eo_join_opts to = po;
       apply_tuned(c, nR, q.nS, &to);
       eo_status st = plan_hash_join(nR, q.nS, &to, &J.hp, cus);
       if (st != EO_OK) return st;
+      if (to.s_tile == 0) J.hp.s_span = plan_span(nR, q.nS, J.hp.lds_cap, J.hp.bits, s_pool, cus);
     }
     const uint32_t P = 1u << J.hp.bits;
     auto side = [&](SideInfo &sd, const eo_tuple *in, uint64_t cnt, int which) {
@@ -172,6 +182,11 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
     side(J.s, q.S, q.nS, 1);
     npairs += P;
     if (npairs >= (1ull << 31)) return EO_EINVAL;
+    if (nR && q.nS) {
+      // probe tiles if the keys spread evenly, with a spare S span per partition
+      const uint64_t chunks = std::max<uint64_t>(1, (nR / P + kBatchChunk - 1) / kBatchChunk);
+      tiles_est += P * chunks * ((q.nS / P + J.hp.s_span - 1) / J.hp.s_span + 1);
+    }
     b->req_join[i] = (int)b->joins.size();
     b->joins.push_back(J);
   }
@@ -228,6 +243,10 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   b->o_pair = L.add(npairs * sizeof(uint32_t));
   b->o_tiles = L.add((npairs + 1) * sizeof(uint32_t));
   b->o_tscr = L.add(scan_scratch_bytes_u32(npairs + 1));
+  // descriptors for the expected tiles (at most 64 MiB); tiles beyond them,
+  // e.g. of skewed partitions, resolve themselves in the probe
+  b->tiles_cap = (uint32_t)std::min<uint64_t>(tiles_est, kMaxTileDescs);
+  b->o_tdesc = L.add((size_t)b->tiles_cap * sizeof(BatchTile));
   b->npairs = (uint32_t)npairs;
   return EO_OK;
 }
@@ -338,8 +357,11 @@ eo_status enqueue_batch(eo_batch *b, hipStream_t s) {
   const BatchJoin *dj = reinterpret_cast<const BatchJoin *>(A + b->o_blob + b->o_joins_blob);
   uint32_t *pair_join = reinterpret_cast<uint32_t *>(A + b->o_pair);
   uint32_t *tile_start = reinterpret_cast<uint32_t *>(A + b->o_tiles);
-  EO_TRY_HIP(batch_plan(dj, nj, pair_join, b->npairs, tile_start, A + b->o_tscr, c->d_err, s), "batch plan");
-  EO_TRY_HIP(batch_probe(dj, pair_join, tile_start, b->npairs, head, c->props.cus, s), "batch probe");
+  BatchTile *tdesc = reinterpret_cast<BatchTile *>(A + b->o_tdesc);
+  EO_TRY_HIP(batch_plan(dj, nj, pair_join, b->npairs, tile_start, tdesc, b->tiles_cap, A + b->o_tscr, c->d_err, s),
+             "batch plan");
+  EO_TRY_HIP(batch_probe(dj, pair_join, tile_start, b->npairs, tdesc, b->tiles_cap, head, c->props.cus, s),
+             "batch probe");
   if (b->outer)
     EO_TRY_HIP(batch_emit_unmatched(dj, nj, reinterpret_cast<const uint32_t *>(A + b->o_blob + b->o_emit),
                                     b->emit_blocks, s),

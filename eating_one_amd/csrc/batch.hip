@@ -31,6 +31,7 @@ using tile::kFNT;
 using tile::SpanRef;
 
 constexpr int kBHistNT = 512;
+static_assert(kFCap == (int)kBatchChunk, "batch tile chunk is the probe table's capacity");
 
 __device__ __forceinline__ uint32_t find_seg(const BatchSeg *segs, uint32_t nseg, uint32_t g) {
   uint32_t lo = 0, hi = nseg;  // segs[lo].stripe_base <= g < segs[hi].stripe_base
@@ -41,6 +42,27 @@ __device__ __forceinline__ uint32_t find_seg(const BatchSeg *segs, uint32_t nseg
   return lo;
 }
 
+// Descriptors read from the device tables, their pointers marked global (see
+// as_global: flat accesses would make every LDS wait also wait for them).
+__device__ __forceinline__ BatchSeg load_seg(const BatchSeg *segs, uint32_t i) {
+  BatchSeg sg = segs[i];
+  sg.in = as_global(sg.in);
+  sg.out = as_global(sg.out);
+  sg.part_start = as_global(sg.part_start);
+  return sg;
+}
+__device__ __forceinline__ BatchJoin load_join(const BatchJoin *joins, uint32_t j) {
+  BatchJoin J = joins[j];
+  J.R = as_global(J.R);
+  J.S = as_global(J.S);
+  J.r_start = as_global(J.r_start);
+  J.s_start = as_global(J.s_start);
+  J.out = as_global(J.out);
+  J.cursor = as_global(J.cursor);
+  J.matched = as_global(J.matched);
+  return J;
+}
+
 template <int BITS>
 __global__ __launch_bounds__(kBHistNT) void k_bseg_hist(const BatchSeg *__restrict__ segs, uint32_t nseg,
                                                         uint32_t total_stripes, uint64_t len,
@@ -49,7 +71,7 @@ __global__ __launch_bounds__(kBHistNT) void k_bseg_hist(const BatchSeg *__restri
   __shared__ uint32_t hist[F];
   const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
   if (g >= total_stripes) return;
-  const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+  const BatchSeg sg = load_seg(segs, find_seg(segs, nseg, g));
   const uint32_t s = g - sg.stripe_base;
   const uint64_t beg = (uint64_t)s * len;
   const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
@@ -67,7 +89,7 @@ __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__rest
   __shared__ tile::ScatterShared<BITS> sh;
   const uint32_t g = xcd_local_index(blockIdx.x, gridDim.x);
   if (g >= total_stripes) return;
-  const BatchSeg sg = segs[find_seg(segs, nseg, g)];
+  const BatchSeg sg = load_seg(segs, find_seg(segs, nseg, g));
   const uint32_t s = g - sg.stripe_base;
   const uint64_t beg = (uint64_t)s * len;
   const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
@@ -85,7 +107,7 @@ __global__ void k_bseg_starts(const BatchSeg *__restrict__ segs, uint32_t nseg, 
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t per = (uint64_t)F + 1;
   if (i >= (uint64_t)nseg * per) return;
-  const BatchSeg sg = segs[i / per];
+  const BatchSeg sg = load_seg(segs, (uint32_t)(i / per));
   const uint32_t d = (uint32_t)(i % per);
   if (d == F) {
     sg.part_start[F] = (uint32_t)sg.n;
@@ -98,7 +120,7 @@ __global__ void k_bseg_starts(const BatchSeg *__restrict__ segs, uint32_t nseg, 
 // pair_join[pair_base + p] = j for every partition p of join j (one workgroup per join).
 __global__ void k_bpairs(const BatchJoin *__restrict__ joins, uint32_t *__restrict__ pair_join) {
   const uint32_t j = blockIdx.x;
-  const BatchJoin J = joins[j];
+  const BatchJoin J = load_join(joins, j);
   for (uint32_t p = threadIdx.x; p < J.P; p += blockDim.x) pair_join[J.pair_base + p] = j;
 }
 
@@ -111,51 +133,104 @@ __global__ void k_bplan(const BatchJoin *__restrict__ joins, const uint32_t *__r
     tiles[npairs] = 0;
     return;
   }
-  const BatchJoin J = joins[pair_join[q]];
+  const BatchJoin J = load_join(joins, pair_join[q]);
   const uint32_t p = q - J.pair_base;
   const uint32_t rn = J.r_start[p + 1] - J.r_start[p];
   const uint32_t sn = J.s_start[p + 1] - J.s_start[p];
   tiles[q] = (rn && sn) ? ((rn + kFCap - 1) / kFCap) * ((sn + J.s_span - 1) / J.s_span) : 0u;
 }
 
+// Descriptor of probe tile t of pair q (tile_start[q] <= t < tile_start[q + 1]):
+// its R chunk and S span, numbered chunk-minor.
+__device__ __forceinline__ BatchTile tile_of_pair(const BatchJoin *joins, const uint32_t *pair_join,
+                                                  const uint32_t *tile_start, uint32_t q, uint32_t t) {
+  const uint32_t jn = pair_join[q];
+  const BatchJoin J = load_join(joins, jn);
+  const uint32_t p = q - J.pair_base;
+  const uint32_t local = t - tile_start[q];
+  const uint32_t rb = J.r_start[p], rn = J.r_start[p + 1] - rb;
+  const uint32_t sb = J.s_start[p], sn = J.s_start[p + 1] - sb;
+  const uint32_t nc = (rn + kFCap - 1) / kFCap;
+  const uint32_t c = local % nc, k = local / nc;
+  BatchTile T;
+  T.R = J.R + rb + c * kFCap;
+  T.S = J.S;
+  T.j = jn;
+  T.r_n = min((uint32_t)kFCap, rn - c * kFCap);
+  T.s_beg = sb + k * J.s_span;
+  T.s_n = min(J.s_span, sn - k * J.s_span);
+  return T;
+}
+
+// The descriptor of every probe tile t < min(total, cap).
+__global__ void k_btiles(const BatchJoin *__restrict__ joins, const uint32_t *__restrict__ pair_join,
+                         const uint32_t *__restrict__ tile_start, uint32_t npairs, BatchTile *__restrict__ tiles,
+                         uint32_t cap) {
+  const uint32_t total = tile_start[npairs];
+  const uint32_t n = min(total, cap);
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = npairs;  // tile_start[lo] <= t < tile_start[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (tile_start[mid] <= t) lo = mid; else hi = mid;
+    }
+    tiles[t] = tile_of_pair(joins, pair_join, tile_start, lo, t);
+  }
+}
+
+__device__ __forceinline__ const uint2 *uniform_ptr(const uint2 *p) {
+  const uint64_t v = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return as_global((const uint2 *)(uintptr_t)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 // Persistent probe: every workgroup takes tiles from the queue until it is
 // empty; every workgroup leaves the loop once its ticket passes the total.
+// A workgroup takes its next ticket as it starts a tile, so that the atomic's
+// round trip overlaps that tile's loads instead of delaying the next tile;
+// every ticket below the total that a workgroup takes it also runs.  Wave 0
+// puts the tile's descriptor in LDS: read from the table written by the plan
+// (one load before the tile's data loads) or, past the table, found by search.
 __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ joins,
                                                  const uint32_t *__restrict__ pair_join,
                                                  const uint32_t *__restrict__ tile_start, uint32_t npairs,
+                                                 const BatchTile *__restrict__ tiles, uint32_t tiles_cap,
                                                  uint32_t *queue_head) {
   __shared__ tile::ProbeShared<tile::ProbeBig> sh;
-  __shared__ uint32_t item[2];
+  __shared__ BatchTile cur;
+  __shared__ uint32_t item;
   const uint32_t total = tile_start[npairs];
+  uint32_t ticket = 0;  // thread 0: the ticket of the next tile
+  if (threadIdx.x == 0) ticket = atomic_add_agent(queue_head, 1u);
   for (;;) {
     if (threadIdx.x < kWave) {
-      uint32_t t = 0;
-      if (threadIdx.x == 0) t = atomic_add_agent(queue_head, 1u);
-      t = __shfl(t, 0);
-      uint32_t q = 0;
-      if (t < total) q = wave_find(tile_start, npairs, t);
-      if (threadIdx.x == 0) {
-        item[0] = t;
-        item[1] = q;
+      const uint32_t t = __shfl(ticket, 0);
+      if (t < total) {
+        BatchTile T;
+        if (t < tiles_cap) T = tiles[t];
+        else T = tile_of_pair(joins, pair_join, tile_start, wave_find(tile_start, npairs, t), t);
+        if (threadIdx.x == 0) cur = T;
       }
+      if (threadIdx.x == 0) item = t;
     }
     __syncthreads();
-    const uint32_t t = item[0], q = item[1];
+    const uint32_t t = item;
+    const BatchTile T = cur;
     __syncthreads();
     if (t >= total) break;
-    const BatchJoin J = joins[pair_join[q]];
-    const uint32_t p = q - J.pair_base;
-    const uint32_t local = t - tile_start[q];
-    const uint32_t rb = J.r_start[p], rn = J.r_start[p + 1] - rb;
-    const uint32_t sb = J.s_start[p], sn = J.s_start[p + 1] - sb;
-    const uint32_t nc = (rn + kFCap - 1) / kFCap;
-    const uint32_t c = local % nc, j = local / nc;
+    const BatchJoin J = load_join(joins, uniform_u32(T.j));
     SpanRef ref;
-    ref.r_beg = rb + c * kFCap;
-    ref.r_n = min((uint32_t)kFCap, rn - c * kFCap);
-    ref.s_beg = sb + j * J.s_span;
-    ref.s_n = min(J.s_span, sn - j * J.s_span);
-    tile::probe_tile<tile::ProbeBig>(sh, J.R, J.S, ref, J.out, J.out_cap, J.cursor, J.wide != 0, J.matched);
+    ref.r_beg = 0;
+    ref.r_n = uniform_u32(T.r_n);
+    ref.s_beg = uniform_u32(T.s_beg);
+    ref.s_n = uniform_u32(T.s_n);
+    if (threadIdx.x == 0) ticket = atomic_add_agent(queue_head, 1u);
+    tile::probe_tile<tile::ProbeBig>(sh, uniform_ptr(T.R), uniform_ptr(T.S), ref, J.out, J.out_cap, J.cursor,
+                                     J.wide != 0, J.matched);
   }
 }
 
@@ -167,7 +242,7 @@ __global__ __launch_bounds__(256) void k_bemit(const BatchJoin *__restrict__ joi
   __shared__ uint64_t red[256 / kWave + 1];
   __shared__ uint64_t base_sh;
   const uint32_t j = wave_find(emit_start, njoins, blockIdx.x);
-  const BatchJoin J = joins[j];
+  const BatchJoin J = load_join(joins, j);
   const uint64_t wd = (uint64_t)(blockIdx.x - emit_start[j]) * 256 + threadIdx.x;
   uint32_t unm = 0;
   if (wd * 32 < J.nS) {
@@ -229,14 +304,20 @@ hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *sc
 }
 
 hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_join, uint32_t npairs,
-                      uint32_t *tile_start, void *scan_scratch, uint32_t *err, hipStream_t stream) {
+                      uint32_t *tile_start, BatchTile *tiles, uint32_t tiles_cap, void *scan_scratch, uint32_t *err,
+                      hipStream_t stream) {
   if (njoins == 0) return hipSuccess;
   hipLaunchKernelGGL(k_bpairs, dim3(njoins), dim3(256), 0, stream, joins, pair_join);
   hipLaunchKernelGGL(k_bplan, dim3((npairs + 1 + 255) / 256), dim3(256), 0, stream, joins, pair_join, npairs,
                      tile_start);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return scan_u32(tile_start, tile_start, (uint64_t)npairs + 1, nullptr, scan_scratch, err, stream);
+  e = scan_u32(tile_start, tile_start, (uint64_t)npairs + 1, nullptr, scan_scratch, err, stream);
+  if (e != hipSuccess || tiles_cap == 0 || npairs == 0) return e;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(((uint64_t)tiles_cap + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_btiles, dim3(blocks), dim3(256), 0, stream, joins, pair_join, tile_start, npairs, tiles,
+                     tiles_cap);
+  return hipGetLastError();
 }
 
 hipError_t batch_emit_unmatched(const BatchJoin *joins, uint32_t njoins, const uint32_t *emit_start,
@@ -248,11 +329,12 @@ hipError_t batch_emit_unmatched(const BatchJoin *joins, uint32_t njoins, const u
 }
 
 hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,
-                       uint32_t *queue_head, int cus, hipStream_t stream) {
+                       const BatchTile *tiles, uint32_t tiles_cap, uint32_t *queue_head, int cus,
+                       hipStream_t stream) {
   if (npairs == 0) return hipSuccess;
   EO_TIMED("batch_probe", stream,
            hipLaunchKernelGGL(k_bprobe, dim3((uint32_t)std::max(cus, 1)), dim3(kFNT), 0, stream, joins, pair_join,
-                              tile_start, npairs, queue_head));
+                              tile_start, npairs, tiles, tiles_cap, queue_head));
   return hipGetLastError();
 }
 

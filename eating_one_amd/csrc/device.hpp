@@ -217,6 +217,18 @@ __device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p) {
 __device__ __forceinline__ void st_release(uint64_t *p, uint64_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Returns p, stating that it points to global memory (not LDS, not scratch).
+// Pointers read from memory (descriptor tables) are otherwise generic, and the
+// compiler accesses them with flat instructions, which the LDS wait counter
+// also tracks: every LDS wait would then also wait for those loads and stores.
+// Rebuilt from its address as a global-address-space pointer (a plain cast
+// pair would fold away), the compiler emits global instructions, as it does
+// for kernel arguments.
+template <typename T>
+__device__ __forceinline__ T *as_global(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+
 __device__ __forceinline__ uint32_t atomic_add_agent(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

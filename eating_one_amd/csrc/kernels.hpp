@@ -216,21 +216,37 @@ struct BatchJoin {
   uint32_t *matched;         // left outer join: bitmap over S positions (zero on entry), or NULL
   uint64_t nS;
 };
+// One probe tile of a batch, resolved ahead of the probe so that a workgroup
+// starts a tile's loads after a single descriptor read: R chunk (already
+// offset), the join's S and output, and the S span.
+constexpr uint32_t kBatchChunk = 17408;  // R tuples per probe tile: the batch probe's LDS table
+struct BatchTile {
+  const uint2 *R;            // first tuple of the R chunk
+  const uint2 *S;            // the join's S (partitioned)
+  uint32_t j;                // join
+  uint32_t r_n;              // R tuples of the chunk
+  uint32_t s_beg, s_n;       // S span
+};
 uint64_t batch_stripe_len();
 hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
                                  hipStream_t stream);
-// Writes pair_join (npairs u32: the join of every (join, partition) pair) and
-// tile_start (npairs + 1 u32: scanned probe tile numbers of each pair).
+// Writes pair_join (npairs u32: the join of every (join, partition) pair),
+// tile_start (npairs + 1 u32: scanned probe tile numbers of each pair) and
+// the descriptors of the first min(total, tiles_cap) tiles.
 hipError_t batch_plan(const BatchJoin *joins, uint32_t njoins, uint32_t *pair_join, uint32_t npairs,
-                      uint32_t *tile_start, void *scan_scratch, uint32_t *err, hipStream_t stream);
+                      uint32_t *tile_start, BatchTile *tiles, uint32_t tiles_cap, void *scan_scratch, uint32_t *err,
+                      hipStream_t stream);
 // Left outer joins of a batch: one null row per unmatched S tuple of every
 // join with a bitmap; emit_start (njoins + 1 u32) is the prefix of each
 // join's unmatched_blocks(nS), emit_start[njoins] the grid.
 hipError_t batch_emit_unmatched(const BatchJoin *joins, uint32_t njoins, const uint32_t *emit_start,
                                 uint32_t blocks, hipStream_t stream);
-// *queue_head must be zero on entry; grid = one workgroup per CU
+// *queue_head must be zero on entry; grid = one workgroup per CU.  Tiles
+// below tiles_cap read their descriptor; later ones resolve their pair by
+// search.
 hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const uint32_t *tile_start, uint32_t npairs,
-                       uint32_t *queue_head, int cus, hipStream_t stream);
+                       const BatchTile *tiles, uint32_t tiles_cap, uint32_t *queue_head, int cus,
+                       hipStream_t stream);
 
 // ---- gen.hip ----------------------------------------------------------------
 hipError_t gen_build(eo_tuple *out, uint64_t n, uint64_t seed, hipStream_t s);

@@ -60,21 +60,25 @@ eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPl
   const int st_opt = o ? o->s_tile : 0;
   if (st_opt < 0) return EO_EINVAL;
   p.s_tile = st_opt ? (uint32_t)st_opt : 16384u;
-  // Fused probe: S spans short enough to give every CU a few tiles, and at
-  // most twice the mean S partition, so that a partition grown by skewed keys
-  // splits into tiles no longer than the others (a long tile late in the grid
-  // would hold the whole join up); long enough to amortise the LDS build.
-  {
-    const uint64_t parts = 1ull << p.bits;
-    const uint64_t want_tiles = (uint64_t)std::max(cus, 8) * 4;
-    uint64_t span = nS * ((nR + p.lds_cap - 1) / p.lds_cap) / std::max<uint64_t>(1, want_tiles);
-    span = std::min<uint64_t>(span, 2 * ((nS + parts - 1) / parts));
-    uint32_t s2 = 8192;
-    while (s2 < span && s2 < (1u << 18)) s2 <<= 1;
-    p.s_span = st_opt ? std::max<uint32_t>(8192, (uint32_t)st_opt) : s2;
-  }
+  p.s_span = st_opt ? std::max<uint32_t>(8192, (uint32_t)st_opt) : plan_span(nR, nS, p.lds_cap, p.bits, nS, cus);
   *out = p;
   return EO_OK;
+}
+
+uint32_t plan_span(uint64_t nR, uint64_t nS, uint32_t lds_cap, int bits, uint64_t s_pool, int cus) {
+  // Fused probe: S spans short enough to give every CU a few tiles of the
+  // s_pool S tuples that share the GPU (this join's, or a whole batch's), and
+  // at most twice the mean S partition, so that a partition grown by skewed
+  // keys splits into tiles no longer than the others (a long tile late in the
+  // grid would hold the whole join up); long enough to amortise the LDS build.
+  const uint64_t parts = 1ull << bits;
+  const uint64_t want_tiles = (uint64_t)std::max(cus, 8) * 4;
+  const uint64_t chunks = (nR + lds_cap - 1) / std::max<uint32_t>(lds_cap, 1);
+  uint64_t span = std::max(s_pool, nS) * chunks / want_tiles;
+  span = std::min<uint64_t>(span, 2 * ((nS + parts - 1) / parts));
+  uint32_t s2 = 8192;
+  while (s2 < span && s2 < (1u << 18)) s2 <<= 1;
+  return s2;
 }
 
 }  // namespace eo

@@ -29,4 +29,9 @@ constexpr int kMaxJoinBits = 22;
 // Returns EO_OK and fills *out, or EO_EINVAL for inconsistent overrides.
 eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *opts, HashPlan *out, int cus = 256);
 
+// S tuples per fused-probe tile of a join of nR x nS planned with lds_cap and
+// bits, when s_pool S tuples (>= nS: the join's own, or a batch's) share the
+// GPU: a power of two in [8192, 2^18].
+uint32_t plan_span(uint64_t nR, uint64_t nS, uint32_t lds_cap, int bits, uint64_t s_pool, int cus);
+
 }  // namespace eo

@@ -12,7 +12,7 @@ import pytest
 import eating_one_amd as eo
 import oracle
 from eating_one_amd import DeviceArray, gen
-from common import same_multiset
+from common import brute_join, same_multiset
 
 pytestmark = pytest.mark.gpu
 
@@ -37,9 +37,9 @@ def _mixed_inputs(seed, n_req, lo=10, hi=20):
 class _Req:
     """Device buffers of one request."""
 
-    def __init__(self, ctx, R, S, cap=None, table=None, s_offset=0):
+    def __init__(self, ctx, R, S, cap=None, table=None, s_offset=0, want=None):
         self.R, self.S = R, S
-        self.want = _oracle(R, S)
+        self.want = _oracle(R, S) if want is None else want
         self.cap = len(self.want) if cap is None else cap
         self.dR = None if table is not None else DeviceArray.from_host(ctx, R)
         # s_offset tuples of padding in front of S: a device pointer that is only 8-byte aligned
@@ -72,6 +72,31 @@ def test_batch_mixed_sizes_equal_oracle_and_solo(ctx):
         # the same request alone
         solo = ctx.hash_join(r.R, r.S)
         assert same_multiset(solo, got), i
+    b.free()
+
+
+def test_batch_hot_partition_tiles_beyond_descriptors(ctx):
+    """Keys that are all multiples of 2^12 land in one partition: that join has
+    far more probe tiles (R chunks x S spans) than the evenly-spread estimate
+    the batch writes descriptors for, so its later tiles resolve by search in
+    the probe; every result still equals the oracle's and the solo join's."""
+    rng = np.random.default_rng(77)
+    nR, nS = 1 << 17, 1 << 20  # 8 R chunks x 8 S spans = 64 tiles; 16 estimated
+    rk = (rng.permutation(nR).astype(np.int64) << 12).astype(np.int32)
+    R = np.stack([rk, rng.integers(0, 1 << 30, nR).astype(np.int32)], 1)
+    S = np.stack([rk[rng.integers(0, nR, nS)], np.arange(nS, dtype=np.int32)], 1)
+    # the oracle's nested loop over one hot partition is quadratic: sort + equal_range instead
+    reqs = [_Req(ctx, R, S, want=brute_join(R, S))]
+    reqs += [_Req(ctx, r, s) for r, s in _mixed_inputs(78, 6, lo=12, hi=16)]  # their spare estimate: < 30 tiles
+    b = ctx.batch()
+    res = b.run([r.request(tag=i) for i, r in enumerate(reqs)])
+    for i, (r, (tag, n, st)) in enumerate(zip(reqs, res)):
+        assert st == 0 and n == len(r.want), i
+        got = r.pairs(n)
+        assert same_multiset(got, r.want), i
+        if i == 0:
+            assert n == nS
+            assert same_multiset(ctx.hash_join(r.R, r.S), got)
     b.free()
 
 
