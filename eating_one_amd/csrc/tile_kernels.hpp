@@ -11,23 +11,25 @@ namespace eo {
 namespace tile {
 
 // ---- fused probe tile ----------------------------------------------------------
-// ProbeCfg<CAP, LOG2NB>: an LDS table of up to CAP R tuples in 2^LOG2NB hash
-// buckets, built and probed by a 1024-thread workgroup (8 S tuples per thread
-// per batch).  ProbeBig (17408 tuples, ~156 KB) runs one workgroup per CU;
-// ProbeSmall (8704 tuples, ~78 KB) lets two share a CU.
-template <int CAP, int LOG2NB>
+// ProbeCfg<CAP, LOG2NB, SPT>: an LDS table of up to CAP R tuples in 2^LOG2NB
+// hash buckets, built and probed by a 1024-thread workgroup (SPT S tuples per
+// thread per batch).  ProbeBig (17408 tuples, ~156 KB) runs one workgroup per
+// CU with up to 128 VGPRs; ProbeSmall (8704 tuples, ~78 KB) lets two share a
+// CU, which leaves 64 VGPRs: 4 S tuples per thread keep its build (9 R tuples
+// per thread plus the first S batch in flight) within them.
+template <int CAP, int LOG2NB, int S_PER_THREAD>
 struct ProbeCfg {
   static constexpr int NT = 1024;
   static constexpr int Cap = CAP;
   static constexpr int Log2NB = LOG2NB;
   static constexpr int NB = 1 << LOG2NB;
-  static constexpr int SPT = 8;
+  static constexpr int SPT = S_PER_THREAD;
   static constexpr int Batch = NT * SPT;
   static constexpr int RPT = (CAP + NT - 1) / NT;
   static constexpr int BPT = NB / NT;
 };
-using ProbeBig = ProbeCfg<17408, 13>;   // a 16K-tuple partition plus binomial spread
-using ProbeSmall = ProbeCfg<8704, 12>;  // an 8K-tuple partition plus spread
+using ProbeBig = ProbeCfg<17408, 13, 8>;   // a 16K-tuple partition plus binomial spread
+using ProbeSmall = ProbeCfg<8704, 12, 4>;  // an 8K-tuple partition plus spread
 constexpr int kFNT = 1024;
 constexpr int kFCap = ProbeBig::Cap;
 
@@ -41,8 +43,10 @@ struct ProbeShared {
   uint32_t pays[C::Cap];
   uint16_t off[C::NB + 2];
   uint32_t scan_tmp[C::NT / kWave + 1];
-  uint32_t wtot[C::SPT * (C::NT / kWave)];
-  uint64_t base_sh;
+  // per S batch, by batch parity: match totals of each (tuple slot, wave),
+  // then their exclusive prefix; the batch's output base
+  uint32_t wtot[2][C::SPT * (C::NT / kWave)];
+  uint64_t base_sh[2];
 };
 
 // Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= C::Cap) and probes the S
@@ -118,7 +122,10 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
 
   // ---- probe the S span batch by batch ----
   const int w = threadIdx.x / kWave, lane = lane_id();
-  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += C::Batch) {
+  constexpr int NV = C::SPT * NW;             // match totals per batch
+  constexpr int VPL = (NV + kWave - 1) / kWave;  // of them per lane of wave 0
+  uint32_t par = 0;
+  for (uint32_t b0 = 0; b0 < ref.s_n; b0 += C::Batch, par ^= 1u) {
     uint2 nxt[C::SPT];
     if (b0 + C::Batch < ref.s_n) load_batch(b0 + C::Batch, nxt);
     uint32_t m[C::SPT], fp[C::SPT];
@@ -147,20 +154,38 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
     for (int u = 0; u < C::SPT; ++u) {
       const uint32_t inc = wave_incl_scan(m[u]);
       wex[u] = inc - m[u];
-      if (lane == kWave - 1) sh.wtot[u * NW + w] = inc;
+      if (lane == kWave - 1) sh.wtot[par][u * NW + w] = inc;
     }
     __syncthreads();
-    uint32_t batch_total;
-    const uint32_t v = threadIdx.x < C::SPT * NW ? sh.wtot[threadIdx.x] : 0u;
-    const uint32_t vex = block_excl_scan<C::NT, uint32_t>(v, sh.scan_tmp, &batch_total);
-    if (threadIdx.x < C::SPT * NW) sh.wtot[threadIdx.x] = vex;
-    if (threadIdx.x == 0) sh.base_sh = batch_total ? atomic_add_agent(cursor, (uint64_t)batch_total) : 0ull;
+    // wave 0 alone turns the NV totals into exclusive offsets (VPL consecutive
+    // ones per lane) and reserves the batch's output; one barrier publishes it
+    if (w == 0) {
+      uint32_t a[VPL], sum = 0;
+#pragma unroll
+      for (int k = 0; k < VPL; ++k) {
+        const int i = lane * VPL + k;
+        a[k] = i < NV ? sh.wtot[par][i] : 0u;
+        sum += a[k];
+      }
+      const uint32_t inc = wave_incl_scan(sum);
+      uint32_t ex = inc - sum;
+#pragma unroll
+      for (int k = 0; k < VPL; ++k) {
+        const int i = lane * VPL + k;
+        if (i < NV) sh.wtot[par][i] = ex;
+        ex += a[k];
+      }
+      const uint32_t batch_total = read_lane(inc, kWave - 1);
+      if (lane == 0) sh.base_sh[par] = batch_total ? atomic_add_agent(cursor, (uint64_t)batch_total) : 0ull;
+    }
     __syncthreads();
-    const uint64_t base = sh.base_sh;
+    // no barrier after the writes: the next batch uses the other parity's
+    // slots, and its first barrier orders everything after this batch's reads
+    const uint64_t base = sh.base_sh[par];
 #pragma unroll
     for (int u = 0; u < C::SPT; ++u) {
       if (m[u] == 0) continue;
-      uint64_t pos = base + sh.wtot[u * NW + w] + wex[u];
+      uint64_t pos = base + sh.wtot[par][u * NW + w] + wex[u];
       if (m[u] == 1) {
         if (pos < out_cap) emit_pair<true>(out, pos, fp[u], cur[u].y, cur[u].x, wide);
       } else {
@@ -174,7 +199,6 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
         }
       }
     }
-    __syncthreads();  // sh.wtot / sh.base_sh reuse
 #pragma unroll
     for (int u = 0; u < C::SPT; ++u) cur[u] = nxt[u];
   }

@@ -59,14 +59,14 @@ eo_status eo_tune_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_
   reps = std::max(1, std::min(reps, 16));
   std::memset(res, 0, sizeof(*res));
   eo_status st = EO_OK;
-  // candidates per LDS table (0: the full table, 8192: the half table):
+  // candidates per LDS table (16384: the full table, 8192: the half table):
   // bits b0-1 .. b0+2 around the planner's for that table, passes 1
   // (bits <= 11) or 2, span auto / x2 / x0.5
   struct Cand {
     int bits, passes, lds, s_tile;
   };
   std::vector<Cand> cands;
-  for (int lds : {0, 8192}) {
+  for (int lds : {16384, 8192}) {
     eo_join_opts lo{};
     lo.lds_tuples = lds;
     HashPlan base;

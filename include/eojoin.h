@@ -451,7 +451,7 @@ eo_status eo_batch_replay(eo_batch *batch);
 typedef struct eo_tune_result {
   int32_t radix_bits;   /* chosen plan (0 bits = no partitioning) */
   int32_t passes;
-  int32_t lds_tuples;   /* 0: the planner's LDS table size */
+  int32_t lds_tuples;   /* 16384 or 8192 (0 in a table saved before the LDS axis: the planner's) */
   int32_t s_tile;       /* 0: the planner's S span */
   double ms;            /* best device time of the chosen plan */
   double default_ms;    /* device time of the untuned planner's plan */

@@ -22,12 +22,12 @@ def test_tune_then_join_equals_oracle(ctx, tmp_path):
     assert res["ms"] <= res["default_ms"] * 1.5
     entry = ctx.tune_lookup(len(R), len(S))
     assert entry["radix_bits"] == res["radix_bits"] and entry["passes"] == res["passes"]
-    assert entry["lds_tuples"] == res["lds_tuples"] and res["lds_tuples"] in (0, 8192)
+    assert entry["lds_tuples"] == res["lds_tuples"] and res["lds_tuples"] in (16384, 8192)
     # the plain call now runs the tuned plan; same multiset as the oracle
     st = eo.JoinStats()
     got = ctx.hash_join(R, S, stats=st)
     assert st.radix_bits == res["radix_bits"] and st.passes == res["passes"]
-    assert st.lds_tuples == (res["lds_tuples"] or 16384)
+    assert st.lds_tuples == res["lds_tuples"]
     assert same_multiset(got, want)
     # every LDS table size the tuner tries gives the oracle's rows
     for lds in (8192, 16384):
