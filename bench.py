@@ -7,7 +7,8 @@ when the timed region starts; pairs written to a device buffer).
   roofline     the dominant kernel (largest device time per join, from HIP
                events recorded around every launch on the join's stream):
                algorithmic bytes per launch / its mean launch time vs 8 TB/s
-  pipeline     whole join: 24(|R|+|S|) + 8|out| bytes / join time vs 8 TB/s
+  pipeline     whole join: 24(|R|+|S|) + 8|out| bytes / join time vs 8 TB/s,
+               and each kernel family's algorithmic GB/s
   cpu_baseline the CPU_JOIN hash join restated in oracle/ on a bounded sample
   batched      the metric's second half, from the C5 stream (4096 joins of
                log-uniform sizes): batched throughput, speedup over per-join
@@ -219,7 +220,10 @@ def run_c2(a):
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "ms_per_launch": dom_mean_ms},
         "pipeline": {"alg_bytes": alg_bytes, "achieved_gbs": pipe, "frac": pipe / HBM_PEAK_GBS,
-                     "kernels_ms_per_join": {k: v[0] / a.steps for k, v in kern.items()}},
+                     "kernels_ms_per_join": {k: v[0] / a.steps for k, v in kern.items()},
+                     # every kernel family: algorithmic bytes per launch / mean launch time
+                     "kernels_gbs": {k: kernel_alg_bytes(k, nR, nS, n_out) / (v[0] / v[1] * 1e-3) / 1e9
+                                     for k, v in kern.items() if kernel_alg_bytes(k, nR, nS, n_out)}},
         "copy_ceiling_gbs": ceiling,
     }
     if a.host_leg:
