@@ -222,11 +222,11 @@ __device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shif
 template <int BITS>
 struct ScatterShared {
   uint2 stage[kFastTile];
-  uint32_t cnt[1 << BITS];
-  uint32_t tbase[(1 << BITS) + 1];
-  uint32_t cursor[1 << BITS];
-  uint32_t scan_tmp[kFastNT / kWave + 1];
-};
+  uint32_t cnt[1 << BITS];     // this tile's digit counts (LDS atomics)
+  uint32_t tbase[1 << BITS];   // this tile's digit starts in stage
+  uint32_t cursor[1 << BITS];  // next output position of each digit
+  uint32_t pbase[1 << BITS];   // this tile: output position of stage slot i (digit d) = pbase[d] + i (mod 2^32)
+};  // 160 KB at 11 bits: the whole LDS of a gfx950 workgroup
 
 // Scatters in[sbeg, send) (in + sbeg 16-byte aligned, or sbeg > 0) by digit
 // into out, digit d starting at sh.cursor[d] (set by the caller, followed by a
@@ -236,7 +236,6 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
                                                uint64_t send, uint64_t nbound, int shift, uint32_t xm,
                                                uint2 *__restrict__ out, uint32_t *err) {
   constexpr int F = 1 << BITS;
-  constexpr int DPT = (F + kFastNT - 1) / kFastNT;
   // a stripe whose first tuple is only 8-byte aligned is read from one tuple
   // earlier (16-byte pairs); that tuple is not part of the stripe: skip it
   const uint64_t skip = ((uintptr_t)(in + sbeg) & 15) ? 1 : 0;
@@ -283,24 +282,33 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
       if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&sh.cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
     }
     __syncthreads();
-    {
-      uint32_t loc[DPT];
+    // wave 0: the tile's digit starts in staging (exclusive scan of the counts,
+    // DPL consecutive digits per lane), each digit's output base for this tile,
+    // the cursors advanced past it and the counts cleared for the next tile
+    if (threadIdx.x < kWave) {
+      constexpr int DPL = (F + kWave - 1) / kWave;
+      uint32_t loc[DPL];
       uint32_t sum = 0;
 #pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const int d = threadIdx.x * DPT + k;
+      for (int k = 0; k < DPL; ++k) {
+        const int d = threadIdx.x * DPL + k;
         loc[k] = d < F ? sh.cnt[d] : 0u;
         sum += loc[k];
       }
-      uint32_t tot;
-      uint32_t ex = block_excl_scan<kFastNT, uint32_t>(sum, sh.scan_tmp, &tot);
+      const uint32_t inc = wave_incl_scan(sum);
+      uint32_t ex = inc - sum;
 #pragma unroll
-      for (int k = 0; k < DPT; ++k) {
-        const int d = threadIdx.x * DPT + k;
-        if (d < F) sh.tbase[d] = ex;
+      for (int k = 0; k < DPL; ++k) {
+        const int d = threadIdx.x * DPL + k;
+        if (d < F) {
+          const uint32_t c = sh.cursor[d];
+          sh.tbase[d] = ex;
+          sh.pbase[d] = c - ex;
+          sh.cursor[d] = c + loc[k];
+          sh.cnt[d] = 0;
+        }
         ex += loc[k];
       }
-      if (threadIdx.x == 0) sh.tbase[F] = tn - tskip;
     }
     __syncthreads();
 #pragma unroll
@@ -313,20 +321,16 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
     }
     __syncthreads();
     load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
+    // no barrier after the flush: the next tile touches stage, tbase and pbase
+    // only after its first barrier, which every wave reaches after its flush
     for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
       const uint2 t = sh.stage[i];
-      const uint32_t d = digit_of(t.x, xm, shift, F - 1);
-      const uint64_t pos = (uint64_t)sh.cursor[d] + (i - sh.tbase[d]);
+      const uint64_t pos = (uint32_t)(sh.pbase[digit_of(t.x, xm, shift, F - 1)] + i);
       if (pos < nbound) out[pos] = t;
       else atomic_or_agent(err, kErrOverflow);  // inconsistent counts: report, never write outside
     }
-    __syncthreads();
-    for (int d = threadIdx.x; d < F; d += kFastNT) {
-      sh.cursor[d] += sh.tbase[d + 1] - sh.tbase[d];
-      sh.cnt[d] = 0;
-    }
-    __syncthreads();
   }
+  __syncthreads();
 }
 
 // ---- LDS histogram of one stripe --------------------------------------------------
