@@ -1,7 +1,7 @@
 """GPU parity of the resident indexes (hash, sorted, learned) through the C ABI.
 
 Oracles: joins/CPU_JOIN/inner_cpu (hash index) and joins/CPU_JOIN/sortIndex_cpu
-(sorted index) restated in oracle/cpu_join.c; the learned model is judged probe
+(sorted index) restated in oracle/cpu_join.c; the learned model is checked probe
 by probe against the plain binary search of the same index.
 """
 import numpy as np
@@ -74,7 +74,7 @@ def test_indexes_random(ctx, seed, nR, nS, kr):
             ix.free()
 
 
-def _judge_learned(ctx, R, S):
+def _check_learned(ctx, R, S):
     ix = ctx.sidx_build(R, learned=True)
     try:
         info = ix.info()
@@ -97,13 +97,13 @@ def _judge_learned(ctx, R, S):
 
 def test_learned_index_c4_shape(ctx):
     R, S = gen.c4(nR=1 << 21, nS=1 << 22)
-    info = _judge_learned(ctx, R, S)
+    info = _check_learned(ctx, R, S)
     assert info["max_err"] < 1 << 16
 
 
 def test_learned_index_duplicates_and_uniform(ctx):
     R, S = gen.uniform(5, 1 << 20, 1 << 20, 1 << 18)
-    _judge_learned(ctx, R, S)
+    _check_learned(ctx, R, S)
 
 
 @pytest.mark.parametrize("kind", ["hash", "sorted", "learned"])
