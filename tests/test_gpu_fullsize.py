@@ -72,3 +72,48 @@ def test_c2_full_size_streamed_from_host(ctx):
         hR.close()
     assert np.array_equal(np.bincount(pairs[:, 1], minlength=nS), np.ones(nS, dtype=np.int64))
     assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])
+
+
+def test_two_pass_half_tables_outer_and_deterministic(ctx):
+    """|R| = 2^26 needs two partition passes, so the fused join takes half LDS
+    tables (13 bits); selectivity 0.5 (absent S keys are >= |R|).  Properties:
+    inner rows = exactly the present S tuples once each with equal keys; the
+    left outer join adds one null row per absent S tuple; the deterministic
+    join (full tables) gives the same rows in the same order twice, and the
+    same set as the fused join."""
+    nR, nS = 1 << 26, 1 << 27
+    R, S = gen.c2(seed=0xE01C5, nR=nR, nS=nS, selectivity=0.5)
+    present = S[:, 0] < nR
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    dout = eo.DeviceArray(ctx, nS * 8)
+    try:
+        st = eo.JoinStats()
+        n, status = ctx.hash_join_device(dR, nR, dS, nS, dout, nS, stats=st)
+        assert status == 0 and n == int(present.sum())
+        assert st.lds_tuples == 8192 and st.passes == 2 and st.radix_bits == 13
+        pairs = dout.to_host(np.int32, (n, 2))
+        cnt = np.bincount(pairs[:, 1], minlength=nS)
+        assert cnt.max() == 1 and np.array_equal(cnt.astype(bool), present)
+        assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])
+
+        n2, status = ctx.hash_join_device(dR, nR, dS, nS, dout, nS, left_outer=True)
+        assert status == 0 and n2 == nS
+        rows = dout.to_host(np.int32, (nS, 2))
+        null = rows[:, 0] == _abi.EO_NULL_PAYLOAD
+        assert np.array_equal(np.bincount(rows[:, 1], minlength=nS), np.ones(nS, dtype=np.int64))
+        assert np.array_equal(np.sort(rows[null, 1]), np.flatnonzero(~present))
+        assert np.array_equal(R[rows[~null, 0], 0], S[rows[~null, 1], 0])
+
+        runs = []
+        for _ in range(2):
+            st = eo.JoinStats()
+            n3, status = ctx.hash_join_device(dR, nR, dS, nS, dout, nS, deterministic=True, stats=st)
+            assert status == 0 and n3 == n and st.lds_tuples == 16384
+            runs.append(dout.to_host(np.int32, (n, 2)))
+        assert np.array_equal(runs[0], runs[1])
+        a = runs[0][np.argsort(runs[0][:, 1], kind="stable")]
+        b = pairs[np.argsort(pairs[:, 1], kind="stable")]
+        assert np.array_equal(a, b)
+    finally:
+        for x in (dout, dS, dR):
+            x.close()
