@@ -150,6 +150,16 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
       apply_tuned(c, nR, q.nS, &to);
       eo_status st = plan_hash_join(nR, q.nS, &to, &J.hp, cus);
       if (st != EO_OK) return st;
+      // The batch's segmented one-pass scatter slows with every bit from 8 on
+      // (4.2, 3.7, 3.1 TB/s at 8, 9, 10 bits): a join whose S is under a third
+      // of its R takes one bit less, so each partition holds two LDS tables'
+      // worth of R and its S is probed twice (C5 stream: 106 -> 100 ms; single
+      // joins of such shapes gain nothing, so the planner keeps its choice).
+      if (!to.radix_bits && J.hp.passes == 1 && J.hp.bits >= 8 && q.nS * 3 < nR) {
+        to.radix_bits = J.hp.bits - 1;
+        to.passes = 1;
+        if ((st = plan_hash_join(nR, q.nS, &to, &J.hp, cus)) != EO_OK) return st;
+      }
       if (to.s_tile == 0) J.hp.s_span = plan_span(nR, q.nS, J.hp.lds_cap, J.hp.bits, s_pool, cus);
     }
     const uint32_t P = 1u << J.hp.bits;
