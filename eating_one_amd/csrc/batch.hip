@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
   __shared__ uint32_t item;
   const uint32_t total = tile_start[npairs];
   uint32_t ticket = 0;  // thread 0: the ticket of the next tile
-  if (threadIdx.x == 0) ticket = atomic_add_agent(queue_head, 1u);
+  if (threadIdx.x == 0) ticket = ticket_next(queue_head);
   for (;;) {
     if (threadIdx.x < kWave) {
       const uint32_t t = __shfl(ticket, 0);
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
     ref.r_n = uniform_u32(T.r_n);
     ref.s_beg = uniform_u32(T.s_beg);
     ref.s_n = uniform_u32(T.s_n);
-    if (threadIdx.x == 0) ticket = atomic_add_agent(queue_head, 1u);
+    if (threadIdx.x == 0) ticket = ticket_next(queue_head);
     tile::probe_tile<tile::ProbeBig>(sh, uniform_ptr(T.R), uniform_ptr(T.S), ref, J.out, J.out_cap, J.cursor,
                                      J.wide != 0, J.matched);
   }

@@ -229,6 +229,16 @@ __device__ __forceinline__ T *as_global(T *p) {
   return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
 }
 
+// Takes the next ticket of a work queue (agent scope): returns *p and adds 1
+// (the increment wraps only at 2^32).  An atomic add issued by one lane is
+// rewritten by the compiler into a wave-combined form that reads its result
+// right away, which makes the issuing wave wait for the round trip at once;
+// the increment is left as is, so the wait falls at the first use of the
+// ticket and a ticket taken ahead overlaps the work in between.
+__device__ __forceinline__ uint32_t ticket_next(uint32_t *p) {
+  return __builtin_amdgcn_atomic_inc32(p, 0xffffffffu, __ATOMIC_RELAXED, "agent");
+}
+
 __device__ __forceinline__ uint32_t atomic_add_agent(uint32_t *p, uint32_t v) {
   return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
