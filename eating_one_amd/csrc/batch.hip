@@ -9,7 +9,8 @@
 //      scatter launch partition every relation of the group (each stripe
 //      finds its relation by binary search in a descriptor table);
 //   2. plan: one thread per (join, partition) pair counts its probe tiles,
-//      a scan numbers them;
+//      a scan numbers them, and one thread per tile (up to an estimate of
+//      the count) writes the tile's descriptor;
 //   3. probe: a persistent kernel (one workgroup per CU) drains a device work
 //      queue of (join, partition, R chunk, S span) tiles with an atomic
 //      ticket and runs the fused LDS probe on each, writing into the join's
