@@ -541,7 +541,7 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const size_t o_ubs = L.add(ublocks ? scan_scratch_bytes_u64(ublocks) : 0);
   const size_t o_pscr = L.add(std::max(partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus)));
   const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
-  const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_tile + 1024;
+  const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_span + 1024;
   const size_t o_tc = L.add(tiles_est * sizeof(uint64_t) + scan_scratch_bytes_u64(tiles_est));
   EO_TRY_HIP(hipStreamSynchronize(s), "sync");
   EO_TRY_HIP(c->arena.reserve(L.total), "arena");
@@ -580,13 +580,10 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   clk.mark();  // 1
 
   const bool deterministic = o && (o->flags & EO_F_DETERMINISTIC);
-  JoinTilePlan jp{P, hp.lds_cap, hp.s_tile, 0, wide};
-  if (deterministic)
-    EO_TRY_HIP(join_plan_count(jp, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s), "tile plan");
-  else
-    EO_TRY_HIP(join_fused_plan(P, hp.s_span, join_fused_cap(hp.lds_cap), r_start, s_start, tile_start, small,
-                               A + o_jscr, c->d_err, s),
-               "tile plan");
+  // both orders use the fused probe's tiles: (partition, R chunk, S span)
+  const uint32_t cap = join_fused_cap(hp.lds_cap);
+  EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s),
+             "tile plan");
   uint64_t n_tiles = 0;
   st = read_u64(c, small, &n_tiles, 1, s);
   if (st != EO_OK) return st;
@@ -663,7 +660,8 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   uint32_t *bits = outer ? (uint32_t *)(A + o_bits) : nullptr;
   uint64_t *ub = (uint64_t *)(A + o_ub);
   if (outer) EO_TRY_HIP(fill_u32(bits, 0u, matched_words(nS), s), "match bitmap reset");
-  EO_TRY_HIP(join_count(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, bits, tile_cnt, s), "count");
+  EO_TRY_HIP(span_count(P, hp.s_span, cap, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, bits, tile_cnt, s),
+             "count");
   EO_TRY_HIP(scan_u64(tile_cnt, tile_cnt, n_tiles, small + 1, B + o_cscr, c->d_err, s), "count scan");
   if (outer) {
     EO_TRY_HIP(count_unmatched(nS, bits, ub, s), "unmatched count");
@@ -699,8 +697,8 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     }
   }
   if (total) {
-    hipError_t e = inner ? join_write(jp, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start, nullptr, tile_cnt,
-                                      dout, total, s)
+    hipError_t e = inner ? span_write(P, hp.s_span, cap, (uint32_t)n_tiles, Rp, r_start, Sp, s_start, tile_start,
+                                      tile_cnt, dout, total, wide, s)
                          : hipSuccess;
     if (e == hipSuccess && outer) e = emit_unmatched_at(Sp, nS, bits, ub, inner, dout, total, wide, s);
     clk.mark();  // 3

@@ -93,29 +93,8 @@ hipError_t partition_bounds(const eo_tuple *sorted, uint64_t n, int bits, int sh
                             uint32_t xor_mask, uint32_t *part_start, hipStream_t stream);
 
 // ---- hashjoin.hip -----------------------------------------------------------
-struct JoinTilePlan {
-  uint32_t parts;      // number of partitions P
-  uint32_t lds_cap;    // R tuples per LDS table (C)
-  uint32_t s_tile;     // S tuples per tile
-  int log2_buckets;    // LDS bucket count = 2^log2_buckets
-  bool wide = false;   // write 16-byte RESULT rows (see emit_pair)
-};
-// Tile planning: tiles_per_part -> exclusive scan -> tile_part map.
+// Scratch of join_fused_plan's scan over the P + 1 tile counts.
 size_t join_plan_scratch_bytes(uint32_t parts);
-hipError_t join_plan_count(const JoinTilePlan &jp, const uint32_t *r_start, const uint32_t *s_start,
-                           uint32_t *tile_start /*P+1*/, uint64_t *n_tiles_dev, void *scan_scratch,
-                           uint32_t *err, hipStream_t stream);
-hipError_t join_plan_map(const JoinTilePlan &jp, const uint32_t *tile_start, uint32_t *tile_part,
-                         hipStream_t stream);
-// matched (optional): bitmap over S positions, bits set for S tuples with a match.
-hipError_t join_count(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
-                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      uint32_t *matched, uint64_t *tile_count, hipStream_t stream);
-hipError_t join_write(const JoinTilePlan &jp, uint32_t n_tiles, const eo_tuple *R, const uint32_t *r_start,
-                      const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      const uint32_t *tile_part, const uint64_t *tile_off, eo_pair *out, uint64_t out_cap,
-                      hipStream_t stream);
-uint32_t join_max_lds_cap();
 // Fused single-pass probe.  Tiles are (partition, R-chunk of join_fused_cap()
 // tuples, S span of s_span tuples); join_fused_plan counts and scans them
 // (tile_start: P+1 u32, *n_tiles_dev = total).  *out_cursor must be zero on
@@ -133,6 +112,16 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
                       eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched,
                       hipStream_t stream);
+// Deterministic order on the fused tiles (join_fused_plan's tile_start):
+// span_count writes every tile's match count (and sets matched bits);
+// span_write writes every tile's rows from tile_off[t] on, in an order fixed
+// by the inputs.
+hipError_t span_count(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
+                      const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                      uint32_t *matched, uint64_t *tile_count, hipStream_t stream);
+hipError_t span_write(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
+                      const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
+                      const uint64_t *tile_off, eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream);
 // Left outer join: bitmap words over n S positions, and the pass that emits
 // one null-padded row per S tuple whose bit is clear (rows reserved from
 // *cursor; out == NULL counts only).

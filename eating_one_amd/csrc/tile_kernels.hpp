@@ -45,21 +45,31 @@ struct ProbeShared {
   uint32_t scan_tmp[C::NT / kWave + 1];
   // per S batch, by batch parity: match totals of each (tuple slot, wave),
   // then their exclusive prefix; the batch's output base
-  uint32_t wtot[2][C::SPT * (C::NT / kWave)];
+  alignas(8) uint32_t wtot[2][C::SPT * (C::NT / kWave)];  // also the u64 scratch of a count's sum
   uint64_t base_sh[2];
 };
 
+// What probe_tile does with the matches of a tile.
+constexpr int kProbeFused = 0;    // write rows at ranges reserved from *cursor, one atomic per batch
+constexpr int kProbeCount = 1;    // count only: return the tile's match count
+constexpr int kProbeOrdered = 2;  // write rows from position base on, in an order fixed by the inputs
+
 // Builds the LDS table of R[ref.r_beg, +r_n) (r_n <= C::Cap) and probes the S
-// span S[ref.s_beg, +s_n) against it, writing (R payload, S payload) pairs at
-// positions reserved from *cursor; positions >= out_cap are counted only.
-// wide: 16-byte RESULT rows instead of pairs (see emit_pair).  matched
-// (optional bitmap over S positions): sets the bit of every S tuple that
-// matched, for a left outer join.
+// span S[ref.s_beg, +s_n) against it.  kProbeFused writes (R payload, S
+// payload) pairs at positions reserved from *cursor; kProbeOrdered writes them
+// from `base` on: the table's bucket order is fixed (lanes sharing a bucket
+// ranked by lane, waves taking turns) and batches take consecutive ranges, so
+// every run writes the same rows at the same positions; kProbeCount writes
+// nothing and returns the number of matches.  Positions >= out_cap are
+// counted only.  wide: 16-byte RESULT rows instead of pairs (see emit_pair).
+// matched (optional bitmap over S positions): sets the bit of every S tuple
+// that matched, for a left outer join.
 // All threads of the workgroup call it; it leaves LDS free for reuse.
-template <typename C>
-__device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__restrict__ R, const uint2 *__restrict__ S,
-                                           const SpanRef &ref, uint2 *__restrict__ out, uint64_t out_cap,
-                                           uint64_t *cursor, bool wide = false, uint32_t *matched = nullptr) {
+template <typename C, int MODE = kProbeFused>
+__device__ __forceinline__ uint64_t probe_tile(ProbeShared<C> &sh, const uint2 *__restrict__ R,
+                                               const uint2 *__restrict__ S, const SpanRef &ref,
+                                               uint2 *__restrict__ out, uint64_t out_cap, uint64_t *cursor,
+                                               bool wide = false, uint32_t *matched = nullptr, uint64_t base = 0) {
   constexpr int NW = C::NT / kWave;
   auto load_batch = [&](uint32_t b0, uint2 *dst) {
 #pragma unroll
@@ -82,12 +92,43 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
     const uint32_t i = k * C::NT + threadIdx.x;
     if (i < ref.r_n) r[k] = R[ref.r_beg + i];
   }
+  if (MODE != kProbeOrdered) {
 #pragma unroll
-  for (int k = 0; k < C::RPT; ++k) {
-    const uint32_t i = k * C::NT + threadIdx.x;
-    if (i < ref.r_n) {
-      hh[k] = bucket_hash((int32_t)r[k].x, C::Log2NB);
-      rk[k] = atomicAdd(&cnt[hh[k]], 1u);
+    for (int k = 0; k < C::RPT; ++k) {
+      const uint32_t i = k * C::NT + threadIdx.x;
+      if (i < ref.r_n) {
+        hh[k] = bucket_hash((int32_t)r[k].x, C::Log2NB);
+        rk[k] = atomicAdd(&cnt[hh[k]], 1u);
+      }
+    }
+  } else {
+    // ranks fixed by the input: within a wave, lanes whose tuples share a
+    // bucket are ranked by lane and the group's first lane takes the group's
+    // ranks with one atomic; the waves take turns
+    // (packed per tuple in rk until the turn: rank in group | leader << 8 | size << 16)
+    const int wv = threadIdx.x / kWave;
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int k = 0; k < C::RPT; ++k) {
+      const bool valid = k * C::NT + threadIdx.x < ref.r_n;
+      hh[k] = valid ? bucket_hash((int32_t)r[k].x, C::Log2NB) : 0u;
+      const uint64_t peers = match_peers<C::Log2NB>(hh[k], __ballot(valid));
+      const uint32_t lead = valid ? (uint32_t)(__ffsll((unsigned long long)peers) - 1) : (uint32_t)lane_id();
+      rk[k] = (uint32_t)__popcll(peers & lt) | lead << 8 | (uint32_t)__popcll(peers) << 16;
+    }
+    for (int turn = 0; turn < C::NT / kWave; ++turn) {
+      if (wv == turn) {
+#pragma unroll
+        for (int k = 0; k < C::RPT; ++k) {
+          const bool valid = k * C::NT + threadIdx.x < ref.r_n;
+          const uint32_t before = rk[k] & 0xffu;
+          uint32_t first = 0;
+          if (valid && before == 0) first = atomicAdd(&cnt[hh[k]], rk[k] >> 16);
+          first = __shfl(first, (int)((rk[k] >> 8) & 0xffu));
+          rk[k] = first + before;
+        }
+      }
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -125,6 +166,8 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
   constexpr int NV = C::SPT * NW;             // match totals per batch
   constexpr int VPL = (NV + kWave - 1) / kWave;  // of them per lane of wave 0
   uint32_t par = 0;
+  uint64_t mine = 0;        // kProbeCount: this thread's matches
+  uint64_t running = base;  // kProbeOrdered, wave 0: the next batch's first position
   for (uint32_t b0 = 0; b0 < ref.s_n; b0 += C::Batch, par ^= 1u) {
     uint2 nxt[C::SPT];
     if (b0 + C::Batch < ref.s_n) load_batch(b0 + C::Batch, nxt);
@@ -138,7 +181,7 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
         const uint32_t e1 = sh.off[h + 1];
         for (uint32_t e = sh.off[h]; e < e1; ++e) {
           if (sh.keys[e] == cur[u].x) {
-            if (m[u] == 0) fp[u] = sh.pays[e];
+            if (MODE != kProbeCount && m[u] == 0) fp[u] = sh.pays[e];
             ++m[u];
           }
         }
@@ -148,6 +191,14 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
 #pragma unroll
       for (int u = 0; u < C::SPT; ++u)
         mark_bits(matched, (uint64_t)ref.s_beg + b0 + (uint64_t)u * C::NT + (uint64_t)w * kWave, __ballot(m[u] != 0));
+    }
+    if (MODE == kProbeCount) {
+#pragma unroll
+      for (int u = 0; u < C::SPT; ++u) {
+        mine += m[u];
+        cur[u] = nxt[u];
+      }
+      continue;
     }
     uint32_t wex[C::SPT];
 #pragma unroll
@@ -176,7 +227,12 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
         ex += a[k];
       }
       const uint32_t batch_total = read_lane(inc, kWave - 1);
-      if (lane == 0) sh.base_sh[par] = batch_total ? atomic_add_agent(cursor, (uint64_t)batch_total) : 0ull;
+      if (MODE == kProbeOrdered) {
+        if (lane == 0) sh.base_sh[par] = running;
+        running += batch_total;
+      } else if (lane == 0) {
+        sh.base_sh[par] = batch_total ? atomic_add_agent(cursor, (uint64_t)batch_total) : 0ull;
+      }
     }
     __syncthreads();
     // no barrier after the writes: the next batch uses the other parity's
@@ -203,6 +259,8 @@ __device__ __forceinline__ void probe_tile(ProbeShared<C> &sh, const uint2 *__re
     for (int u = 0; u < C::SPT; ++u) cur[u] = nxt[u];
   }
   __syncthreads();
+  if (MODE == kProbeCount) return block_sum<C::NT, uint64_t>(mine, reinterpret_cast<uint64_t *>(sh.wtot[0]));
+  return 0;
 }
 
 // ---- fast radix scatter of one stripe ---------------------------------------------
