@@ -129,10 +129,11 @@ bool opts_ok(const eo_join_opts *o) {
 }
 
 eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
-                             eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s, bool stable) {
+                             eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s, bool stable,
+                             bool fixed) {
   const int cus = ctx->props.cus;
   if (hp.passes == 1) {
-    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, aligned16(in) && !stable);
+    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, aligned16(in) && !stable && !fixed);
     EO_TRY_HIP(partition_pass(pp, in, out, part_start, scratch, ctx->d_err, s), "partition pass");
     return EO_OK;
   }
@@ -140,11 +141,11 @@ eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const 
   if (stable) {
     // fall through to the stable LSD passes below
   } else if (aligned16(in) && aligned16(tmp) && msd2_supported(b_hi, b_lo)) {
-    EO_TRY_HIP(partition_msd2(n, b_hi, b_lo, in, tmp, out, part_start, scratch, ctx->d_err, cus, s),
+    EO_TRY_HIP(partition_msd2(n, b_hi, b_lo, in, tmp, out, part_start, scratch, ctx->d_err, cus, s, fixed),
                "partition (two passes, one histogram)");
     return EO_OK;
   }
-  if (!stable && aligned16(in) && aligned16(tmp)) {
+  if (!stable && !fixed && aligned16(in) && aligned16(tmp)) {
     const uint32_t F1 = 1u << b_hi;
     uint32_t *start1 = (uint32_t *)scratch;
     char *rest = (char *)scratch + ((F1 + 1) * sizeof(uint32_t) + 255) / 256 * 256;
@@ -563,11 +564,15 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const eo_tuple *Rp = dR, *Sp = dS;
   if (hp.passes) {
     eo_tuple *rp = (eo_tuple *)(A + o_rp), *sp = (eo_tuple *)(A + o_sp), *tmp = (eo_tuple *)(A + o_tmp);
-    // deterministic order needs stable partitions (the fast ones rank by atomics)
-    const bool stable = o && (o->flags & EO_F_DETERMINISTIC);
-    st = partition_relation(c, dR, nR, hp, tmp, rp, r_start, A + o_pscr, s, stable);
+    // deterministic order needs partitions whose positions the inputs fix
+    // (the fast passes rank by atomics): the ordered two-pass partition, or
+    // stable passes, which a deterministic left outer join takes so that its
+    // null rows come in S order within each partition
+    const bool det = o && (o->flags & EO_F_DETERMINISTIC);
+    const bool stable = det && outer, fixed = det && !outer;
+    st = partition_relation(c, dR, nR, hp, tmp, rp, r_start, A + o_pscr, s, stable, fixed);
     if (st != EO_OK) return st;
-    st = partition_relation(c, dS, nS, hp, tmp, sp, s_start, A + o_pscr, s, stable);
+    st = partition_relation(c, dS, nS, hp, tmp, sp, s_start, A + o_pscr, s, stable, fixed);
     if (st != EO_OK) return st;
     Rp = rp;
     Sp = sp;

@@ -107,10 +107,12 @@ inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 // input: one fast pass, or two fast passes MSD-first (high digit over the
 // whole array, then the low digit inside every first-pass partition).
 // Otherwise, or with stable = true (order within a partition = input order),
-// stable LSD passes (tmp holds the first pass).
+// stable LSD passes (tmp holds the first pass).  fixed = true: positions
+// fixed by the input (the same on every run): the two fast passes with
+// ordered ranks where they apply, else the stable passes.
 eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const HashPlan &hp, eo_tuple *tmp,
                              eo_tuple *out, uint32_t *part_start, void *scratch, hipStream_t s,
-                             bool stable = false);
+                             bool stable = false, bool fixed = false);
 size_t partition_scratch(uint64_t n, const HashPlan &hp, int cus);
 TuneKey tune_key(uint64_t nR, uint64_t nS);
 // Fills radix_bits / passes / s_tile of *o from the tuning table when the

@@ -169,8 +169,11 @@ hipError_t index_probe(int kind, bool write, uint64_t n, int log2nb, const uint3
 constexpr int kMsd2MaxBits = 14;
 bool msd2_supported(int bits_hi, int bits_lo);
 size_t msd2_scratch_bytes(uint64_t n, int bits_hi, int bits_lo, int cus);
+// ordered: every tuple lands at a position fixed by the input (the same on
+// every run; not the input's order within a partition).
 hipError_t partition_msd2(uint64_t n, int bits_hi, int bits_lo, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
-                          uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream);
+                          uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream,
+                          bool ordered = false);
 
 // ---- batch.hip --------------------------------------------------------------
 // One relation of one join to partition in a batch group (all of a group's

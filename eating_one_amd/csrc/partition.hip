@@ -241,11 +241,11 @@ __global__ __launch_bounds__(kPartNT) void k_part_scatter(const uint2 *__restric
 // issued before the flush so that HBM reads overlap the writes.  Within a
 // partition the tuple order depends on atomic order; the multiset per
 // partition equals the stable scatter's.
-template <int BITS>
+template <int BITS, bool ORDERED = false>
 __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__restrict__ in, StripeGeo geo, int shift,
                                                                uint32_t xm, const uint32_t *__restrict__ offsets,
                                                                uint2 *__restrict__ out, uint32_t *err) {
-  __shared__ tile::ScatterShared<BITS> sh;
+  __shared__ tile::ScatterShared<BITS, ORDERED> sh;
   // segmented: the grid is an upper bound, so spread the real stripes over all XCDs
   const uint32_t g = geo.seg_start ? blockIdx.x : xcd_local_index(blockIdx.x, gridDim.x);
   StripeRef sr;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kFastNT) void k_part_scatter_fast(const uint2 *__re
     sh.cursor[d] = offsets[sr.cidx0 + (uint64_t)d * sr.cstride];
     sh.cnt[d] = 0;
   }
-  tile::scatter_stripe<BITS>(sh, in, sr.beg, sr.end, geo.n, shift, xm, out, err);
+  tile::scatter_stripe<BITS, ORDERED>(sh, in, sr.beg, sr.end, geo.n, shift, xm, out, err);
 }
 
 __global__ void k_part_starts(const uint32_t *offsets, uint32_t G, uint32_t K, uint32_t F, uint64_t n,
@@ -444,14 +444,14 @@ __global__ void k_msd_groups(const uint32_t *__restrict__ off1, uint32_t G, uint
 // keep every CU busy to the end.  A group is a run of consecutive first-pass
 // chunks of one segment dh; it scatters by the low digit into the final
 // partitions, whose starts off2 the combined histogram already holds.
-template <int BL>
+template <int BL, bool ORDERED = false>
 __global__ __launch_bounds__(kFastNT) void k_msd_scatter(const uint2 *__restrict__ in, uint64_t n,
                                                          const uint32_t *__restrict__ off1, uint32_t G, uint32_t K,
                                                          uint32_t F1, const uint32_t *__restrict__ grp_base,
                                                          uint32_t *ticket, const uint32_t *__restrict__ off2,
                                                          uint2 *__restrict__ out, uint32_t *err) {
   constexpr uint32_t F2 = 1u << BL;
-  __shared__ tile::ScatterShared<BL> sh;
+  __shared__ tile::ScatterShared<BL, ORDERED> sh;
   __shared__ uint32_t s_w;
   const uint32_t total = grp_base[F1];
   for (;;) {
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(kFastNT) void k_msd_scatter(const uint2 *__restrict
       sh.cnt[d] = 0;
     }
     __syncthreads();
-    tile::scatter_stripe<BL>(sh, in, beg, end, n, 0, 0u, out, err);
+    tile::scatter_stripe<BL, ORDERED>(sh, in, beg, end, n, 0, 0u, out, err);
     __syncthreads();  // sh and s_w are reused by the next group
   }
 }
@@ -504,7 +504,7 @@ Msd2Layout msd2_layout(const PartPassPlan &p1, int bl) {
   return L;
 }
 
-template <int BH, int BL>
+template <int BH, int BL, bool ORDERED>
 hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, uint32_t workers, const eo_tuple *in, eo_tuple *tmp,
                        eo_tuple *out, uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
   constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL);
@@ -526,7 +526,7 @@ hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, uint32_t work
   if ((e = scan_u32(c1, c1, L.m1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
   if ((e = scan_u32(c2, c2, L.m2, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
   EO_TIMED("part_scatter", stream,
-           hipLaunchKernelGGL(k_part_scatter_fast<BH>, dim3(p1.stripes), dim3(kFastNT), 0, stream, vin, geo, BL, 0u,
+           hipLaunchKernelGGL((k_part_scatter_fast<BH, ORDERED>), dim3(p1.stripes), dim3(kFastNT), 0, stream, vin, geo, BL, 0u,
                               c1, reinterpret_cast<uint2 *>(tmp), err));
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_msd_groups, dim3((F1 + 1 + 255) / 256), dim3(256), 0, stream, c1, p1.stripes, p1.hist_split, F1,
@@ -536,7 +536,7 @@ hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, uint32_t work
   const uint64_t max_groups = F1 + p1.n / group_len + 1;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(max_groups, std::max<uint32_t>(workers, 1));
   EO_TIMED("seg_scatter", stream,
-           hipLaunchKernelGGL(k_msd_scatter<BL>, dim3(grid), dim3(kFastNT), 0, stream,
+           hipLaunchKernelGGL((k_msd_scatter<BL, ORDERED>), dim3(grid), dim3(kFastNT), 0, stream,
                               reinterpret_cast<const uint2 *>(tmp), p1.n, c1, p1.stripes, p1.hist_split, F1, grp,
                               grp + F1 + 1, c2, reinterpret_cast<uint2 *>(out), err));
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -686,7 +686,8 @@ size_t msd2_scratch_bytes(uint64_t n, int bh, int bl, int cus) {
 }
 
 hipError_t partition_msd2(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
-                          uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream) {
+                          uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream,
+                          bool ordered) {
   if (!msd2_supported(bh, bl)) return hipErrorInvalidValue;
   if (((uintptr_t)in & 15) || ((uintptr_t)tmp & 15) || ((uintptr_t)out & 7)) return hipErrorInvalidValue;
   const PartPassPlan p1 = msd2_pass1(n, bh, bl, cus);
@@ -695,7 +696,8 @@ hipError_t partition_msd2(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tup
   switch (bh * 16 + bl) {
 #define EO_MSD_CASE(BH, BL) \
   case BH * 16 + BL:        \
-    return launch_msd2<BH, BL>(p1, L, workers, in, tmp, out, part_start, scratch, err, stream);
+    return ordered ? launch_msd2<BH, BL, true>(p1, L, workers, in, tmp, out, part_start, scratch, err, stream) \
+                   : launch_msd2<BH, BL, false>(p1, L, workers, in, tmp, out, part_start, scratch, err, stream);
 #define EO_MSD_ROW(BH) \
   EO_MSD_CASE(BH, 1) EO_MSD_CASE(BH, 2) EO_MSD_CASE(BH, 3) EO_MSD_CASE(BH, 4) EO_MSD_CASE(BH, 5) EO_MSD_CASE(BH, 6) \
   EO_MSD_CASE(BH, 7)

@@ -277,8 +277,19 @@ __device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shif
   return ((key ^ xm) >> shift) & mask;
 }
 
+// ORDERED: ranks fixed by the input (per-wave digit counters, no LDS
+// atomics), so every run writes every tuple at the same position; up to 7
+// bits (the counters take 16 waves x 2^BITS words).
+template <int BITS, bool ORDERED>
+struct ScatterWaveCounts {  // per-wave digit counts, then each wave's offset within its digit
+  uint32_t wc[kFastNT / kWave][1 << BITS];
+};
 template <int BITS>
-struct ScatterShared {
+struct ScatterWaveCounts<BITS, false> {};
+
+template <int BITS, bool ORDERED = false>
+struct ScatterShared : ScatterWaveCounts<BITS, ORDERED> {
+  static_assert(!ORDERED || BITS <= 7, "ordered scatter: per-wave counters fit up to 7 bits");
   uint2 stage[kFastTile];
   uint32_t cnt[1 << BITS];     // this tile's digit counts (LDS atomics)
   uint32_t tbase[1 << BITS];   // this tile's digit starts in stage
@@ -289,8 +300,8 @@ struct ScatterShared {
 // Scatters in[sbeg, send) (in + sbeg 16-byte aligned, or sbeg > 0) by digit
 // into out, digit d starting at sh.cursor[d] (set by the caller, followed by a
 // barrier).  Positions >= nbound are never written (kErrOverflow is raised).
-template <int BITS>
-__device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const uint2 *__restrict__ in, uint64_t sbeg,
+template <int BITS, bool ORDERED = false>
+__device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh, const uint2 *__restrict__ in, uint64_t sbeg,
                                                uint64_t send, uint64_t nbound, int shift, uint32_t xm,
                                                uint2 *__restrict__ out, uint32_t *err) {
   constexpr int F = 1 << BITS;
@@ -333,13 +344,50 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
         if (t0 + (uint64_t)j * kFastNT + threadIdx.x == nfull) cur[j] = make_uint4(last.x, last.y, 0u, 0u);
     }
     uint32_t rk[2 * kFastPairs];
+    if constexpr (!ORDERED) {
 #pragma unroll
-    for (int j = 0; j < kFastPairs; ++j) {
-      const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
-      if (e >= tskip && e < tn) rk[2 * j] = atomicAdd(&sh.cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
-      if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&sh.cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
+      for (int j = 0; j < kFastPairs; ++j) {
+        const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
+        if (e >= tskip && e < tn) rk[2 * j] = atomicAdd(&sh.cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
+        if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&sh.cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
+      }
+      __syncthreads();
+    } else {
+      // each wave ranks its own tuples in (register, lane) order: the lanes
+      // that share a digit are ranked by lane, the group's first lane moves
+      // the wave's counter of that digit (distinct digits, distinct words)
+      const int w = threadIdx.x / kWave;
+      const uint64_t lt = lanemask_lt();
+      for (int d = lane_id(); d < F; d += kWave) sh.wc[w][d] = 0;
+#pragma unroll
+      for (int q = 0; q < 2 * kFastPairs; ++q) {
+        const int j = q >> 1;
+        const uint32_t e = 2u * (j * kFastNT + threadIdx.x) + (q & 1);
+        const bool valid = e >= tskip && e < tn;
+        const uint32_t d = digit_of((q & 1) ? cur[j].z : cur[j].x, xm, shift, F - 1);
+        const uint64_t peers = match_peers<BITS>(d, __ballot(valid));
+        const uint32_t before = (uint32_t)__popcll(peers & lt);
+        const int lead = valid ? __ffsll((unsigned long long)peers) - 1 : lane_id();
+        uint32_t first = 0;
+        if (valid && before == 0) {
+          first = sh.wc[w][d];
+          sh.wc[w][d] = first + (uint32_t)__popcll(peers);
+        }
+        rk[q] = __shfl(first, lead) + before;
+      }
+      __syncthreads();
+      // per digit: the tile's count, and each wave's offset within the digit
+      for (int d = threadIdx.x; d < F; d += kFastNT) {
+        uint32_t run = 0;
+        for (int v = 0; v < kFastNT / kWave; ++v) {
+          const uint32_t c = sh.wc[v][d];
+          sh.wc[v][d] = run;
+          run += c;
+        }
+        sh.cnt[d] = run;
+      }
+      __syncthreads();
     }
-    __syncthreads();
     // wave 0: the tile's digit starts in staging (exclusive scan of the counts,
     // DPL consecutive digits per lane), each digit's output base for this tile,
     // the cursors advanced past it and the counts cleared for the next tile
@@ -369,13 +417,16 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS> &sh, const ui
       }
     }
     __syncthreads();
+    // a tuple's stage slot: its digit's start, (ORDERED) its wave's offset within the digit, its rank
+    auto slot = [&](uint32_t d, uint32_t r) -> uint32_t {
+      if constexpr (ORDERED) return sh.tbase[d] + sh.wc[threadIdx.x / kWave][d] + r;
+      else return sh.tbase[d] + r;
+    };
 #pragma unroll
     for (int j = 0; j < kFastPairs; ++j) {
       const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
-      if (e >= tskip && e < tn)
-        sh.stage[sh.tbase[digit_of(cur[j].x, xm, shift, F - 1)] + rk[2 * j]] = make_uint2(cur[j].x, cur[j].y);
-      if (e + 1 < tn)
-        sh.stage[sh.tbase[digit_of(cur[j].z, xm, shift, F - 1)] + rk[2 * j + 1]] = make_uint2(cur[j].z, cur[j].w);
+      if (e >= tskip && e < tn) sh.stage[slot(digit_of(cur[j].x, xm, shift, F - 1), rk[2 * j])] = make_uint2(cur[j].x, cur[j].y);
+      if (e + 1 < tn) sh.stage[slot(digit_of(cur[j].z, xm, shift, F - 1), rk[2 * j + 1])] = make_uint2(cur[j].z, cur[j].w);
     }
     __syncthreads();
     load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
