@@ -23,15 +23,12 @@ eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *o, HashPl
 
   const int bits_opt = o ? o->radix_bits : 0;
   if (bits_opt < -1 || bits_opt > kMaxJoinBits) return EO_EINVAL;
-  // A fused join that needs two partition passes anyway takes half tables:
-  // the extra radix bit lands in the second pass (which moves the same bytes),
+  // A join that needs two partition passes anyway takes half tables: the
+  // extra radix bit lands in the second pass (which moves the same bytes),
   // and two probe workgroups per CU hide each other's barrier and atomic
-  // round trips (C2, 2^27 x 2^28: probe 1.32 -> 1.08 ms, join 5.09 -> 4.93 ms).
-  // The deterministic count/write kernels gain nothing from that (C2: 20.5
-  // ms with half tables, 19.7 ms with full ones).
-  const bool det = o && (o->flags & EO_F_DETERMINISTIC);
-  if (!cap_opt && !bits_opt && !det && nR > 16384 && ceil_log2((nR + 16383) / 16384) > kMaxPassBits)
-    p.lds_cap = 8192u;
+  // round trips (C2, 2^27 x 2^28: probe 1.32 -> 1.08 ms, join 5.09 -> 4.93 ms;
+  // deterministic order 7.35 -> 6.86 ms).
+  if (!cap_opt && !bits_opt && nR > 16384 && ceil_log2((nR + 16383) / 16384) > kMaxPassBits) p.lds_cap = 8192u;
   if (bits_opt == -1) {
     p.bits = 0;
   } else if (bits_opt > 0) {

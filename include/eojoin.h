@@ -107,7 +107,7 @@ typedef struct eo_join_opts {
   int32_t passes;        /* partition passes, 1 or 2 (0 = autotune) */
   int32_t lds_tuples;    /* R tuples per LDS table: 4096, 8192 or 16384; 0 = the
                             tuned entry, else the planner's (16384, or 8192 for a
-                            fused join whose |R| needs two partition passes) */
+                            join whose |R| needs two partition passes) */
   int32_t s_tile;        /* S tuples per join tile (0 = autotune) */
   int32_t flags;         /* EO_F_* bits */
   int32_t reserved;

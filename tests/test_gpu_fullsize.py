@@ -79,8 +79,8 @@ def test_two_pass_half_tables_outer_and_deterministic(ctx):
     tables (13 bits); selectivity 0.5 (absent S keys are >= |R|).  Properties:
     inner rows = exactly the present S tuples once each with equal keys; the
     left outer join adds one null row per absent S tuple; the deterministic
-    join (full tables) gives the same rows in the same order twice, and the
-    same set as the fused join."""
+    join (ordered partition, fixed table order) gives the same rows in the
+    same order twice, and the same set as the fused join."""
     nR, nS = 1 << 26, 1 << 27
     R, S = gen.c2(seed=0xE01C5, nR=nR, nS=nS, selectivity=0.5)
     present = S[:, 0] < nR
@@ -108,7 +108,7 @@ def test_two_pass_half_tables_outer_and_deterministic(ctx):
         for _ in range(2):
             st = eo.JoinStats()
             n3, status = ctx.hash_join_device(dR, nR, dS, nS, dout, nS, deterministic=True, stats=st)
-            assert status == 0 and n3 == n and st.lds_tuples == 16384
+            assert status == 0 and n3 == n and st.lds_tuples == 8192
             runs.append(dout.to_host(np.int32, (n, 2)))
         assert np.array_equal(runs[0], runs[1])
         a = runs[0][np.argsort(runs[0][:, 1], kind="stable")]
