@@ -133,7 +133,10 @@ eo_status partition_relation(eo_ctx *ctx, const eo_tuple *in, uint64_t n, const 
                              bool fixed) {
   const int cus = ctx->props.cus;
   if (hp.passes == 1) {
-    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, aligned16(in) && !stable && !fixed);
+    // fixed order: the ordered fast scatter up to 7 bits, else the stable one
+    const bool fast = aligned16(in) && !stable && (!fixed || hp.bits_pass1 <= 7);
+    PartPassPlan pp = plan_partition_pass(n, hp.bits_pass1, 0, 0u, cus, fast);
+    pp.ordered = fast && fixed;
     EO_TRY_HIP(partition_pass(pp, in, out, part_start, scratch, ctx->d_err, s), "partition pass");
     return EO_OK;
   }

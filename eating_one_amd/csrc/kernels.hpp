@@ -60,6 +60,7 @@ struct PartPassPlan {
   uint64_t stripe_len; // tuples per stripe (even)
   uint32_t hist_split; // K: histogram workgroups per stripe
   bool fast;           // unstable LDS-atomic-ranked scatter (needs 16-byte aligned input)
+  bool ordered = false;  // fast, <= 7 bits: ranks fixed by the input (the same positions on every run)
 };
 // fast = false: stable scatter (ballot-ranked), equal to the CPU_JOIN partition.
 PartPassPlan plan_partition_pass(uint64_t n, int bits, int shift, uint32_t xor_mask, int cus, bool fast = false);

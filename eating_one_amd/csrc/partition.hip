@@ -294,6 +294,16 @@ hipError_t launch_pass(const PartPassPlan &p, const eo_tuple *in, eo_tuple *out,
   if (e != hipSuccess) return e;
   if (p.fast) {
     if ((uintptr_t)in & 15) return hipErrorInvalidValue;  // stripe 0 starts at the array start
+    if constexpr (BITS <= 7) {
+      if (p.ordered) {
+        EO_TIMED("part_scatter", stream,
+                 hipLaunchKernelGGL((k_part_scatter_fast<BITS, true>), dim3(p.stripes), dim3(kFastNT), 0, stream, vin,
+                                    geo, p.shift, p.xor_mask, counts, vout, err));
+        return hipGetLastError();
+      }
+    } else {
+      if (p.ordered) return hipErrorInvalidValue;  // per-wave counters fit up to 7 bits
+    }
     EO_TIMED("part_scatter", stream, hipLaunchKernelGGL(k_part_scatter_fast<BITS>, dim3(p.stripes), dim3(kFastNT), 0, stream, vin, geo, p.shift,
                        p.xor_mask, counts, vout, err));
   } else {

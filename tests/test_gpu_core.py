@@ -106,11 +106,13 @@ def test_hash_join_plans(ctx, bits, passes, cap, det):
     assert same_multiset(got, want)
 
 
-@pytest.mark.parametrize("bits,passes,cap", [(-1, 0, 16384), (8, 1, 8192), (11, 1, 4096), (12, 2, 4096),
-                                             (13, 2, 16384)])
+@pytest.mark.parametrize("bits,passes,cap", [(-1, 0, 16384), (5, 1, 16384), (8, 1, 8192), (11, 1, 4096),
+                                             (12, 2, 4096), (13, 2, 16384)])
 def test_hash_join_deterministic_order_repeats(ctx, bits, passes, cap):
     """EO_F_DETERMINISTIC: the same rows in the same order on every run, with
-    duplicate keys on both sides (stable partitions, fixed bucket order)."""
+    duplicate keys on both sides (partitions with positions fixed by the
+    input — ordered fast passes up to 7 bits per pass, else stable ones — and
+    a fixed bucket order)."""
     R, S = gen.c3(seed=bits + 7, nR=70000, nS=110000, key_range=1 << 14)
     R = np.concatenate([R, R[:5000]])  # more duplicates, some several LDS chunks deep
     runs = [ctx.hash_join(R, S, radix_bits=bits, passes=passes, lds_tuples=cap, deterministic=True)
