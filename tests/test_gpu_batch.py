@@ -100,6 +100,27 @@ def test_batch_hot_partition_tiles_beyond_descriptors(ctx):
     b.free()
 
 
+def test_batch_small_probe_sides_take_one_bit_less(ctx):
+    """Joins whose S is under a third of R and whose R needs a one-pass
+    partition of 8+ bits are partitioned one bit coarser in a batch (each
+    partition then spans two LDS tables); rows equal the oracle's and the
+    solo join's, beside ordinary joins in the same batch."""
+    inputs = [gen.c2(seed=301, nR=(1 << 22) + 77, nS=1 << 18, selectivity=0.7),
+              gen.c2(seed=302, nR=3 << 21, nS=(1 << 20) + 5, selectivity=1.0),
+              gen.c2(seed=303, nR=1 << 21, nS=1 << 21, selectivity=0.5)]
+    inputs += _mixed_inputs(304, 4, lo=12, hi=18)
+    reqs = [_Req(ctx, R, S) for R, S in inputs]
+    b = ctx.batch()
+    res = b.run([r.request(tag=i) for i, r in enumerate(reqs)])
+    for i, (r, (tag, n, st)) in enumerate(zip(reqs, res)):
+        assert st == 0 and n == len(r.want), i
+        got = r.pairs(n)
+        assert same_multiset(got, r.want), i
+        if i < 3:
+            assert same_multiset(ctx.hash_join(r.R, r.S), got), i
+    b.free()
+
+
 def test_batch_resident_tables_and_count_only(ctx):
     rng = np.random.default_rng(7)
     tabs = []
