@@ -271,6 +271,8 @@ __device__ __forceinline__ uint64_t probe_tile(ProbeShared<C> &sh, const uint2 *
 constexpr int kFastNT = 1024;
 constexpr int kFastPairs = 8;                         // uint4 loads (2 tuples) per thread per tile
 constexpr int kFastTile = kFastNT * kFastPairs * 2;   // 16384 tuples: a 128 KB staging tile
+// gfx9 buffer resource word 3 for untyped loads (32-bit data format, no swizzle)
+constexpr int kBufferRsrcWord3 = 0x00020000;
 constexpr int kFastWgPerCu = 1;                       // one workgroup per CU (LDS)
 
 __device__ __forceinline__ uint32_t digit_of(uint32_t key, uint32_t xm, int shift, uint32_t mask) {
@@ -322,20 +324,45 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
   const bool odd_tail = nfull < npairs;
   uint2 last = make_uint2(0u, 0u);
   if (odd_tail) last = in[beg + 2 * nfull];
-  uint4 cur[kFastPairs];
-  auto load_tile = [&](uint64_t t0pair, uint4 *dst) {
+  // Loads j in [j0, j1) of the tile at pair t0pair, through a buffer resource
+  // over that tile's complete pairs: lanes past the end read zero in hardware
+  // (no memory access), and all loads share one 32-bit offset register
+  // instead of a 64-bit address each.
+  // The next tile's first half is read at the top of each tile when its 16
+  // registers fit beside the rest (1024 threads leave 128 VGPRs; the ordered
+  // ranks and the 10- and 11-bit scans need them), else the whole next tile
+  // is read once this tile is staged.
+  constexpr int kHalf = (!ORDERED && BITS <= 9) ? kFastPairs / 2 : 0;
+  auto load_part = [&](uint64_t t0pair, uint4 *dst, int j0, int j1) {
+    if constexpr (BITS >= 10) {  // plain loads: the resource's registers tip these scans into spills
 #pragma unroll
-    for (int j = 0; j < kFastPairs; ++j) {
-      const uint64_t pi = t0pair + (uint64_t)j * kFastNT + threadIdx.x;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (pi < nfull) v = in4[pi];
-      dst[j] = v;
+      for (int j = j0; j < j1; ++j) {
+        const uint64_t pi = t0pair + (uint64_t)j * kFastNT + threadIdx.x;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (pi < nfull) v = in4[pi];
+        dst[j] = v;
+      }
+      return;
+    }
+    const uint64_t left = t0pair < nfull ? nfull - t0pair : 0;
+    const uint32_t nrec = (uint32_t)min<uint64_t>(left, (uint64_t)kFastNT * kFastPairs) * 16u;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(in4 + min(t0pair, nfull)), 0, (int)nrec, kBufferRsrcWord3);
+#pragma unroll
+    for (int j = j0; j < j1; ++j) {
+      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)((j * kFastNT + threadIdx.x) * 16u), 0, 0);
+      dst[j] = make_uint4(v.x, v.y, v.z, v.w);
     }
   };
-  load_tile(0, cur);
+  uint4 cur[kFastPairs];
+  uint4 pre[kHalf > 0 ? kHalf : 1];  // the next tile's first half, read while this tile is ranked and staged
+  load_part(0, cur, 0, kFastPairs);
   __syncthreads();
 
   for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
+    const uint64_t tnext = t0 + (uint64_t)kFastNT * kFastPairs;
+    load_part(tnext, pre, 0, kHalf);
     const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples read in this tile
     const uint32_t tskip = t0 == 0 ? (uint32_t)skip : 0u;                            // lead-in tuple, not ours
     if (odd_tail) {
@@ -429,7 +456,11 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
       if (e + 1 < tn) sh.stage[slot(digit_of(cur[j].z, xm, shift, F - 1), rk[2 * j + 1])] = make_uint2(cur[j].z, cur[j].w);
     }
     __syncthreads();
-    load_tile(t0 + (uint64_t)kFastNT * kFastPairs, cur);  // next tile's reads overlap this tile's writes
+    // the next tile: its first half arrived during this tile's rank phase, its
+    // second half is read while this tile's runs are written
+#pragma unroll
+    for (int j = 0; j < kHalf; ++j) cur[j] = pre[j];
+    load_part(tnext, cur, kHalf, kFastPairs);
     // no barrier after the flush: the next tile touches stage, tbase and pbase
     // only after its first barrier, which every wave reaches after its flush
     for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
