@@ -328,11 +328,11 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
   // over that tile's complete pairs: lanes past the end read zero in hardware
   // (no memory access), and all loads share one 32-bit offset register
   // instead of a 64-bit address each.
-  // The next tile's first half is read at the top of each tile when its 16
-  // registers fit beside the rest (1024 threads leave 128 VGPRs; the ordered
-  // ranks and the 10- and 11-bit scans need them), else the whole next tile
-  // is read once this tile is staged.
-  constexpr int kHalf = (!ORDERED && BITS <= 9) ? kFastPairs / 2 : 0;
+  // The next tile's first kPre loads are issued at the top of each tile when
+  // their 4 kPre registers fit beside the rest (1024 threads leave 128 VGPRs:
+  // all 8 spill, 6 fit but time the same as 4, and the ordered ranks and the
+  // 10- and 11-bit scans leave no room), the rest once this tile is staged.
+  constexpr int kPre = (!ORDERED && BITS <= 9) ? kFastPairs / 2 : 0;
   auto load_part = [&](uint64_t t0pair, uint4 *dst, int j0, int j1) {
     if constexpr (BITS >= 10) {  // plain loads: the resource's registers tip these scans into spills
 #pragma unroll
@@ -356,13 +356,13 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
     }
   };
   uint4 cur[kFastPairs];
-  uint4 pre[kHalf > 0 ? kHalf : 1];  // the next tile's first half, read while this tile is ranked and staged
+  uint4 pre[kPre > 0 ? kPre : 1];  // the next tile's first loads, in flight while this tile is ranked and staged
   load_part(0, cur, 0, kFastPairs);
   __syncthreads();
 
   for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
     const uint64_t tnext = t0 + (uint64_t)kFastNT * kFastPairs;
-    load_part(tnext, pre, 0, kHalf);
+    load_part(tnext, pre, 0, kPre);
     const uint32_t tn = (uint32_t)min((uint64_t)kFastTile, (end - beg) - 2 * t0);  // tuples read in this tile
     const uint32_t tskip = t0 == 0 ? (uint32_t)skip : 0u;                            // lead-in tuple, not ours
     if (odd_tail) {
@@ -456,11 +456,11 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
       if (e + 1 < tn) sh.stage[slot(digit_of(cur[j].z, xm, shift, F - 1), rk[2 * j + 1])] = make_uint2(cur[j].z, cur[j].w);
     }
     __syncthreads();
-    // the next tile: its first half arrived during this tile's rank phase, its
-    // second half is read while this tile's runs are written
+    // the next tile: its first kPre loads arrived during this tile's rank
+    // phase, the rest are read while this tile's runs are written
 #pragma unroll
-    for (int j = 0; j < kHalf; ++j) cur[j] = pre[j];
-    load_part(tnext, cur, kHalf, kFastPairs);
+    for (int j = 0; j < kPre; ++j) cur[j] = pre[j];
+    load_part(tnext, cur, kPre, kFastPairs);
     // no barrier after the flush: the next tile touches stage, tbase and pbase
     // only after its first barrier, which every wave reaches after its flush
     for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
