@@ -112,3 +112,14 @@ def test_cli_usage_and_no_gpu_error():
     assert r.returncode == 2 and "<|S|> <|R|>" in r.stderr
     r = subprocess.run([exe, "bogus", "10", "10"], capture_output=True, text=True)
     assert r.returncode == 2
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The bench measures one GPU: --gpus above 1 exits non-zero with a message
+    before touching the device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
