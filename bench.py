@@ -81,6 +81,11 @@ def cpu_baseline_hash(sample_log2_r: int, theta: float, sel: float, threads: int
         n = oracle.hash_join_mt(R, S, nparts=nparts, threads=threads, count_only=True)
     dt = time.perf_counter() - t0
     model, ncpu = host_info()
+    # what the timed region produces: the 1-thread restatement counts matches
+    # without writing pairs; the pthread one writes every pair into its
+    # per-thread buffers (as the reference's threads write results)
+    output = ("pairs written to per-thread buffers" if threads > 1
+              else "count only: matches counted, no pairs written")
     return {
         "value": (nR + nS) / dt,
         "unit": "tuples/s",
@@ -89,7 +94,7 @@ def cpu_baseline_hash(sample_log2_r: int, theta: float, sel: float, threads: int
         "sample": f"oracle/cpu_join.c restatement of CPU_JOIN hash_cpu (partition + per-partition nested loop, "
                   f"timer scope hj.cc:209-230{'; pthread version of multi_joins/hash' if threads > 1 else ''}), "
                   f"|R|=2^{sample_log2_r}, |S|=2^{sample_log2_r + 1}, "
-                  f"theta={theta}, selectivity={sel}, {nparts} partitions, {n} matches, {dt:.2f} s on "
+                  f"theta={theta}, selectivity={sel}, {nparts} partitions, {n} matches ({output}), {dt:.2f} s on "
                   f"{threads} of {ncpu} host threads ({model})",
     }
 
@@ -114,7 +119,7 @@ def cpu_baseline_nlj(S, log2_r_sample: int):
     dt = time.perf_counter() - t0
     return cpu_port(len(r) * len(S) / dt, "compares/s",
                     f"oracle/cpu_join.c restatement of CPU_JOIN block_cpu, |R| = 2^{log2_r_sample} of C3's R, "
-                    f"|S| = {len(S)}, {n} matches", dt)
+                    f"|S| = {len(S)}, {n} matches (count only: no pairs written)", dt)
 
 
 def read_traffic(kernel):
@@ -439,7 +444,7 @@ def run_nlj_dist(a):
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = cpu_port(len(r) * float(n) / dt, "pairs/s",
                                        f"oracle/cpu_join.c restatement of CPU_JOIN 3d_cpu, |R| = 2^14 of the "
-                                       f"2^17 points, |S| = 2^17, {m} matches", dt)
+                                       f"2^17 points, |S| = 2^17, {m} matches (count only: no pairs written)", dt)
     return res
 
 
@@ -485,7 +490,8 @@ def run_c3_hidx(a):
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = cpu_port(len(S) / dt, "probes/s",
                                        f"oracle/cpu_join.c restatement of CPU_JOIN inner_cpu (2^27-bucket index "
-                                       f"build + probe, timer scope inner-idx-bnlj.cc:142-186), full C3, {m} matches",
+                                       f"build + probe, timer scope inner-idx-bnlj.cc:142-186), full C3, {m} matches "
+                                       f"(count only: no pairs written)",
                                        dt)
     return res
 
@@ -557,14 +563,15 @@ def run_c4(a):
                                        f"oracle/cpu_join.c restatement of CPU_JOIN sortIndex_cpu (quicksort build "
                                        f"{build_s:.2f} s, then binary-search probes; timer scopes "
                                        f"inner-idx-bnlj.cc:204-238), C4 shape at |R| = 2^24, |S| = 2^25, "
-                                       f"{m} matches", dt)
+                                       f"{m} matches (count only: no pairs written)", dt)
         if a.cpu_mt_threads > 1:
             t0 = time.perf_counter()
             m2 = oracle.sidx_join_mt(r, s_, threads=a.cpu_mt_threads, count_only=True)
             dt2 = time.perf_counter() - t0
             mt = cpu_port(len(s_) / max(dt2 - build_s, 1e-9), "probes/s",
                           f"pthread restatement of multi_joins/sortindex (serial quicksort build, then "
-                          f"{a.cpu_mt_threads} probe threads), same sample, {m2} matches", dt2, a.cpu_mt_threads)
+                          f"{a.cpu_mt_threads} probe threads), same sample, {m2} matches (count only)", dt2,
+                          a.cpu_mt_threads)
             res["cpu_baseline"]["multi_thread"] = {k: mt[k] for k in ("value", "unit", "cores", "sample")}
     return res
 
@@ -604,7 +611,7 @@ def run_c1(a):
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = cpu_port((len(R) + len(S)) / dt, "tuples/s",
                                        f"oracle/cpu_join.c restatement of CPU_JOIN hash_cpu (2^20 partitions, "
-                                       f"timer scope hj.cc:209-230), full C1, {m} matches", dt)
+                                       f"timer scope hj.cc:209-230), full C1, {m} matches (count only: no pairs written)", dt)
     return res
 
 
@@ -615,45 +622,19 @@ def run_c5(a):
     import numpy as np
 
     import eating_one_amd as eo
-    from eating_one_amd import _abi
+    from eating_one_amd import _abi, gen
 
-    rng = np.random.default_rng(a.seed if a.seed is not None else 0xE01C5)
+    rng = np.random.default_rng((a.seed if a.seed is not None else 0xE01C5) + 1)  # arrival times
     n_req = a.c5_requests
     ctx = eo.Context(0, hbm_budget=int(a.c5_budget))
-    # 64 resident build tables; 25 % of the requests reuse one of them
-    t_sizes = (2 ** rng.uniform(a.c5_lo, a.c5_hi, size=a.c5_tables)).astype(np.int64)
-    tables = []
-    for k, nR in enumerate(t_sizes):
-        d = eo.DeviceArray(ctx, 8 * int(nR))
-        ctx.gen_build(d, int(nR), seed=0x7AB1E000 + k)
-        tables.append((int(nR), ctx.table_create(d, n=int(nR)), d))
-    use_table = rng.random(n_req) < a.c5_reuse
-    table_id = rng.integers(0, a.c5_tables, size=n_req)
-    nR = (2 ** rng.uniform(a.c5_lo, a.c5_hi, size=n_req)).astype(np.int64)
-    nR = np.where(use_table, t_sizes[table_id], nR)
-    nS = (2 ** rng.uniform(a.c5_lo, a.c5_hi, size=n_req)).astype(np.int64)
-    sel = rng.uniform(0.1, 1.0, size=n_req)
-    pad = lambda n: (n + 1) // 2 * 2  # 16-byte aligned starts
-    r_off = np.concatenate([[0], np.cumsum(np.where(use_table, 0, pad(nR)))])
-    s_off = np.concatenate([[0], np.cumsum(pad(nS))])
-    log(f"C5: {n_req} requests, |R|,|S| = 2^U, U~U[{a.c5_lo},{a.c5_hi}]; R {8 * r_off[-1] / 1e9:.1f} GB, "
-        f"S {8 * s_off[-1] / 1e9:.1f} GB, out {8 * s_off[-1] / 1e9:.1f} GB; {int(use_table.sum())} on tables")
-    dR = eo.DeviceArray(ctx, 8 * max(int(r_off[-1]), 1))
-    dS = eo.DeviceArray(ctx, 8 * max(int(s_off[-1]), 1))
-    dO = eo.DeviceArray(ctx, 8 * max(int(s_off[-1]), 1))
     t0 = time.time()
-    reqs = []
-    for i in range(n_req):
-        if use_table[i]:
-            t = tables[int(table_id[i])][1]
-            Rp = None
-        else:
-            t = None
-            Rp = dR.ptr + 8 * int(r_off[i])
-            ctx.gen_build(dR, int(nR[i]), seed=0xB0000000 + i, offset=8 * int(r_off[i]))
-        ctx.gen_probe(dS, int(nS[i]), int(nR[i]), float(sel[i]), seed=0x50000000 + i, offset=8 * int(s_off[i]))
-        reqs.append(eo.request(R=Rp, nR=int(nR[i]), S=dS.ptr + 8 * int(s_off[i]), nS=int(nS[i]),
-                               out=dO.ptr + 8 * int(s_off[i]), out_cap=int(nS[i]), table=t, tag=i))
+    # 64 resident build tables; 25 % of the requests reuse one of them
+    stream = gen.C5Stream(ctx, n_req=n_req, lo=a.c5_lo, hi=a.c5_hi, n_tables=a.c5_tables, reuse=a.c5_reuse,
+                          seed=a.seed if a.seed is not None else 0xE01C5)
+    reqs, nR, nS = stream.reqs, stream.nR, stream.nS
+    log(f"C5: {n_req} requests, |R|,|S| = 2^U, U~U[{a.c5_lo},{a.c5_hi}]; R {8 * stream.r_off[-1] / 1e9:.1f} GB, "
+        f"S {8 * stream.s_off[-1] / 1e9:.1f} GB, out {8 * stream.s_off[-1] / 1e9:.1f} GB; "
+        f"{int(stream.use_table.sum())} on tables")
     log(f"generated in {time.time() - t0:.1f} s")
     tuples = int(nR.sum() + nS.sum())
     kw = dict(max_batch=a.c5_max_batch, slots=a.c5_slots)
@@ -676,9 +657,7 @@ def run_c5(a):
                      "batches": st["batches"], "largest_batch": st["largest_batch"]}
         log(f"{name}: {np.median(walls):.1f} ms for {n_req} requests ({tuples / np.median(walls) / 1e6:.2f} G tuples/s)")
     if a.c5_modes != "both":  # one mode only (profiling): closed loop only
-        for _, t, d in tables:
-            t.free()
-            d.close()
+        stream.free()
         ctx.close()
         return {"metric": "C5 join stream, one mode", "closed_loop": out}
     capacity_qps = n_req / (out["batched"]["wall_ms"] * 1e-3)
@@ -701,11 +680,55 @@ def run_c5(a):
                       "slots": a.c5_slots},
            "closed_loop": out, "speedup_batched_vs_per_join": out["per_join"]["wall_ms"] / out["batched"]["wall_ms"],
            "poisson": {"qps": qps, **{k: v for k, v in lat_res.items()}}}
-    for _, t, d in tables:
-        t.free()
-        d.close()
+    stream.free()
     ctx.close()
     return res
+
+
+def run_copy(a):
+    """The HBM ceiling the join kernels are compared against: a sweep of the
+    bandwidth probe (copy, read-only and write-only passes over 1 and 4 GiB;
+    16-byte words in flight per thread, workgroup size, workgroups per CU,
+    streaming or normal loads and stores).  Prints every point to stderr and
+    one JSON line with the best of each mode."""
+    import eating_one_amd as eo
+    from eating_one_amd import _abi
+
+    ctx = eo.Context(0)
+    best = {}
+    rows = []
+    modes = {"copy": _abi.EO_PROBE_COPY, "read": _abi.EO_PROBE_READ, "write": _abi.EO_PROBE_WRITE}
+    for size in (1 << 30, 1 << 32):
+        for mname, mode in modes.items():
+            for unroll in (1, 2, 4, 8):
+                for threads in (256, 512, 1024):
+                    for bpc in (0, 4, 8, 16, 32):
+                        if bpc and bpc * threads > 2048 * 4:  # more threads than a CU holds
+                            continue
+                        for flags in (0, 1, 2, 3):
+                            if mode == _abi.EO_PROBE_READ and flags & 2 or mode == _abi.EO_PROBE_WRITE and flags & 1:
+                                continue
+                            if bpc == 0 and unroll > 1:  # one word per thread: nothing to unroll
+                                continue
+                            o = _abi.ProbeOpts(ctypes.sizeof(_abi.ProbeOpts), mode, size, unroll, threads, bpc,
+                                               flags, 5, 0)
+                            g = ctypes.c_double()
+                            st = ctx.lib.eo_bandwidth_probe_ex(ctx.handle, ctypes.byref(o), ctypes.byref(g))
+                            if st != 0:
+                                raise RuntimeError(f"eo_bandwidth_probe_ex failed ({st})")
+                            key = (mname, size)
+                            row = {"mode": mname, "gib": size >> 30, "unroll": unroll, "threads": threads,
+                                   "blocks_per_cu": bpc, "nt_load": bool(flags & 1), "nt_store": bool(flags & 2),
+                                   "gbs": round(g.value, 1)}
+                            rows.append(row)
+                            if key not in best or g.value > best[key]["gbs"]:
+                                best[key] = row
+            log(f"{mname} {size >> 30} GiB best: {best[(mname, size)]}")
+    ctx.close()
+    top = {f"{m}_{sz >> 30}gib": r for (m, sz), r in best.items()}
+    return {"metric": "HBM bandwidth ceiling (bandwidth probe sweep)", "unit": "GB/s",
+            "value": max(r["gbs"] for (m, _), r in best.items() if m == "copy"), "best": top, "points": len(rows),
+            "sweep": rows if a.copy_all else None}
 
 
 def main(argv=None) -> int:
@@ -714,7 +737,7 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c2",
-                    choices=["c1", "c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist", "stream"])
+                    choices=["c1", "c2", "c3-nlj", "c3-hidx", "c4", "c5", "nlj-dist", "stream", "copy"])
     ap.add_argument("--s-chunk", type=int, default=0, help="stream: S tuples per chunk (0: library default)")
     ap.add_argument("--seed", type=int, default=None)
     ap.add_argument("--c5-requests", type=int, default=4096)
@@ -743,6 +766,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="c2: skip the batched C5 stream the default line carries")
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
+    ap.add_argument("--copy-all", action="store_true", help="copy: print every sweep point in the JSON line")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU (asked for --gpus {a.gpus}): the library has no multi-GPU path",
@@ -760,7 +784,7 @@ def main(argv=None) -> int:
         print(json.dumps(res), flush=True)
         return 0
     fn = {"c1": run_c1, "c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,
-          "nlj-dist": run_nlj_dist, "stream": run_stream}[a.config]
+          "nlj-dist": run_nlj_dist, "stream": run_stream, "copy": run_copy}[a.config]
     print(json.dumps(fn(a)), flush=True)
     return 0
 

@@ -341,8 +341,13 @@ class Context:
     def batch(self) -> "Batch":
         return Batch(self)
 
-    def serve(self, reqs, arrival_ms=None, mode=_abi.EO_SERVE_BATCHED, max_batch=0, slots=0, hbm_budget=0):
-        """Serve a request stream (eo_serve_run).  Returns (results, latency_ms, stats dict)."""
+    def serve(self, reqs, arrival_ms=None, mode=_abi.EO_SERVE_BATCHED, max_batch=0, slots=0, hbm_budget=0,
+              deadline_ms=None, cancel=None):
+        """Serve a request stream (eo_serve_run).  Returns (results, latency_ms, stats dict).
+
+        deadline_ms: per-request admission deadlines (ms after arrival, <= 0: none);
+        cancel: a uint32 numpy array of per-request cancel flags, which another
+        thread may set while the loop runs."""
         n = len(reqs)
         arr = (_abi.Request * max(n, 1))(*reqs)
         res = (_abi.Result * max(n, 1))()
@@ -352,6 +357,16 @@ class Context:
             a_np = np.ascontiguousarray(arrival_ms, dtype=np.float64)
             a = a_np.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         o = _abi.ServeOpts(ctypes.sizeof(_abi.ServeOpts), mode, max_batch, slots, hbm_budget)
+        if deadline_ms is not None:
+            d_np = np.ascontiguousarray(deadline_ms, dtype=np.float64)
+            if len(d_np) != n:
+                raise ValueError("one deadline per request")
+            o.deadline_ms = d_np.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        if cancel is not None:
+            if not (isinstance(cancel, np.ndarray) and cancel.dtype == np.uint32 and len(cancel) == n
+                    and cancel.flags.c_contiguous):
+                raise ValueError("cancel must be a contiguous uint32 array with one flag per request")
+            o.cancel = cancel.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
         st = _abi.ServeStats()
         _check(self.lib.eo_serve_run(self.handle, arr, a, n, ctypes.byref(o), res,
                                      lat.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(st)),

@@ -133,6 +133,26 @@ class ServeOpts(ctypes.Structure):
         ("max_batch", U32),
         ("slots", U32),
         ("hbm_budget", U64),
+        ("deadline_ms", ctypes.POINTER(ctypes.c_double)),
+        ("cancel", ctypes.POINTER(U32)),
+    ]
+
+
+EO_PROBE_COPY, EO_PROBE_READ, EO_PROBE_WRITE = 0, 1, 2
+EO_PROBE_NT_LOAD, EO_PROBE_NT_STORE = 1, 2
+
+
+class ProbeOpts(ctypes.Structure):
+    _fields_ = [
+        ("struct_size", U32),
+        ("mode", I32),
+        ("bytes", U64),
+        ("unroll", I32),
+        ("threads", I32),
+        ("blocks_per_cu", I32),
+        ("flags", U32),
+        ("iters", I32),
+        ("reserved", I32),
     ]
 
 
@@ -148,6 +168,10 @@ class ServeStats(ctypes.Structure):
         ("batches", U32),
         ("largest_batch", U32),
         ("peak_admitted", U64),
+        ("refused", U32),
+        ("expired", U32),
+        ("cancelled", U32),
+        ("reserved", U32),
     ]
 
 
@@ -201,6 +225,7 @@ EO_REF_FILE_RAW = 1
 # name -> (restype, argtypes)
 SIGNATURES = {
     "eo_abi_version": (U32, []),
+    "eo_build_source_hash": (ctypes.c_char_p, []),
     "eo_status_string": (ctypes.c_char_p, [INT]),
     "eo_last_error": (ctypes.c_char_p, []),
     "eo_ctx_create": (INT, [ctypes.POINTER(P), INT, ctypes.POINTER(CtxOpts)]),
@@ -215,6 +240,7 @@ SIGNATURES = {
     "eo_copy_d2h": (INT, [P, P, P, U64]),
     "eo_copy_d2d": (INT, [P, P, P, U64]),
     "eo_bandwidth_probe": (INT, [P, U64, INT, ctypes.POINTER(ctypes.c_double)]),
+    "eo_bandwidth_probe_ex": (INT, [P, ctypes.POINTER(ProbeOpts), ctypes.POINTER(ctypes.c_double)]),
     "eo_exclusive_scan_u32": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_exclusive_scan_u64": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),

@@ -25,7 +25,6 @@ SOURCES = [
     "hashjoin.hip",
     "nlj.hip",
     "index.hip",
-    "learned.hip",
     "gen.hip",
     "util.hip",
     "batch.hip",
@@ -35,11 +34,41 @@ SOURCES = [
     "api_io.cpp",
     "plan.cpp",
     "tuner.cpp",
+    "tune_table.cpp",
+    "status.cpp",
     "serve.cpp",
+    "serve_queue.cpp",
     "stream.cpp",
 ]
 
 ARCH = os.environ.get("EO_OFFLOAD_ARCH", "gfx950")
+
+# Files whose text the source hash covers: every C/C++/HIP source and header
+# of the library and the public headers.
+_HASHED_EXT = (".hip", ".cpp", ".hpp", ".c", ".h")
+
+
+def source_hash() -> str:
+    """SHA-256 (first 16 hex digits) over the library's sources and headers.
+
+    The build embeds it in libeojoin.so (eo_build_source_hash), so a library
+    built from other sources than the tree's is detected before it is used.
+    """
+    import hashlib
+
+    h = hashlib.sha256()
+    files = []
+    for d in (CSRC, INCLUDE):
+        for name in sorted(os.listdir(d)):
+            if name.endswith(_HASHED_EXT):
+                files.append(os.path.join(d, name))
+    for f in files:
+        h.update(os.path.relpath(f, ROOT).encode())
+        h.update(b"\0")
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+        h.update(b"\0")
+    return h.hexdigest()[:16]
 
 # Packed-FP32 instructions are disabled for every HIP translation unit.
 HIPCC_FLAGS = [
@@ -63,9 +92,9 @@ def hipcc() -> str:
     return path
 
 
-def _compile(src: str) -> str:
+def _compile(src: str, defines: list[str]) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [hipcc(), "-x", "hip", *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), "-x", "hip", *HIPCC_FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -75,12 +104,16 @@ def _compile(src: str) -> str:
 def build(jobs: int | None = None, verbose: bool = False) -> str:
     """Compiles every source and links eating_one_amd/libeojoin.so. Returns its path."""
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    missing = [s for s in SOURCES if not os.path.exists(os.path.join(CSRC, s))]
+    if missing:
+        raise RuntimeError(f"sources listed in SOURCES are missing: {missing}")
+    defines = [f'-DEO_SOURCE_HASH="{source_hash()}"']
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda src: _compile(src, defines), SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    # every symbol must resolve at link time, not at dlopen or first call
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

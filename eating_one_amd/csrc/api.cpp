@@ -18,10 +18,6 @@
 
 namespace eo {
 
-static thread_local std::string g_last_error;
-
-void set_last_error(const std::string &msg) { g_last_error = msg; }
-
 eo_status hip_fail(hipError_t e, const char *what) {
   set_last_error(std::string(what) + ": " + hipGetErrorString(e));
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return EO_ENOMEM;
@@ -189,27 +185,6 @@ using namespace eo;
 
 extern "C" {
 
-uint32_t eo_abi_version(void) { return EOJOIN_ABI_VERSION; }
-
-const char *eo_status_string(eo_status st) {
-  switch (st) {
-    case EO_OK: return "ok";
-    case EO_EINVAL: return "invalid argument";
-    case EO_ENOMEM: return "out of memory";
-    case EO_ECAPACITY: return "output capacity too small";
-    case EO_EHIP: return "HIP runtime error";
-    case EO_EBUDGET: return "HBM budget exceeded";
-    case EO_ENODEV: return "no usable GPU";
-    case EO_EAGAIN: return "not ready";
-    case EO_ECANCELED: return "cancelled";
-    case EO_ETIMEDOUT: return "deadline expired";
-    case EO_EINTERNAL: return "internal error";
-  }
-  return "unknown status";
-}
-
-const char *eo_last_error(void) { return g_last_error.c_str(); }
-
 eo_status eo_ctx_create(eo_ctx **out, int device, const eo_ctx_opts *opts) {
   if (!out) return EO_EINVAL;
   *out = nullptr;
@@ -363,6 +338,32 @@ eo_status eo_bandwidth_probe(eo_ctx *c, uint64_t bytes, int iters, double *gbs) 
   float ms = 0.f;
   EO_TRY_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
   *gbs = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+  return EO_OK;
+}
+
+eo_status eo_bandwidth_probe_ex(eo_ctx *c, const eo_probe_opts *o, double *gbs) {
+  if (!c || !o || !gbs || o->struct_size < sizeof(eo_probe_opts)) return EO_EINVAL;
+  if (o->bytes == 0 || o->bytes % 16 || o->iters < 1 || o->mode < EO_PROBE_COPY || o->mode > EO_PROBE_WRITE)
+    return EO_EINVAL;
+  if (o->unroll != 1 && o->unroll != 2 && o->unroll != 4 && o->unroll != 8) return EO_EINVAL;
+  if (o->threads != 0 && (o->threads < 64 || o->threads > 1024 || o->threads % 64)) return EO_EINVAL;
+  if (o->blocks_per_cu < 0 || o->blocks_per_cu > 64 || (o->flags & ~3u)) return EO_EINVAL;
+  if (o->blocks_per_cu == 0 && o->bytes / 16 / (o->threads ? o->threads : 256) >= (1ull << 31)) return EO_EINVAL;
+  DeviceGuard g(c->device);
+  EO_TRY_HIP(hipStreamSynchronize(c->stream), "sync");
+  const uint64_t span = (o->bytes + 255) / 256 * 256;
+  EO_TRY_HIP(c->arena.reserve(2 * span + 4096), "arena");
+  char *a = c->arena.base, *b = c->arena.base + span;
+  uint32_t *sink = (uint32_t *)(c->arena.base + 2 * span);
+  EO_TRY_HIP(bandwidth_kernel(*o, a, b, sink, c->props.cus, c->stream), "warm launch");
+  EO_TRY_HIP(hipEventRecord(c->ev[0], c->stream), "event");
+  for (int i = 0; i < o->iters; ++i) EO_TRY_HIP(bandwidth_kernel(*o, a, b, sink, c->props.cus, c->stream), "launch");
+  EO_TRY_HIP(hipEventRecord(c->ev[1], c->stream), "event");
+  EO_TRY_HIP(hipEventSynchronize(c->ev[1]), "event sync");
+  float ms = 0.f;
+  EO_TRY_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
+  const double moved = (o->mode == EO_PROBE_COPY ? 2.0 : 1.0) * (double)o->bytes * o->iters;
+  *gbs = moved / (ms * 1e-3) / 1e9;
   return EO_OK;
 }
 
@@ -592,17 +593,9 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const uint32_t cap = join_fused_cap(hp.lds_cap);
   EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s),
              "tile plan");
-  uint64_t n_tiles = 0;
-  st = read_u64(c, small, &n_tiles, 1, s);
-  if (st != EO_OK) return st;
-  if (n_tiles >= (1ull << 31)) {
-    set_last_error("too many join tiles");
-    return EO_EINTERNAL;
-  }
   if (stats) {
     stats->radix_bits = (uint32_t)hp.bits;
     stats->passes = (uint32_t)hp.passes;
-    stats->tiles = (uint32_t)n_tiles;
     stats->lds_tuples = hp.lds_cap;
   }
   eo_pair *dout = out;
@@ -611,7 +604,9 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
 
   if (!deterministic) {
     // One fused probe kernel; with out == NULL (or a too small buffer) it counts
-    // every match and writes only the pairs that fit.
+    // every match and writes only the pairs that fit.  The probe's grid does
+    // not depend on the tile count, so the join reads back once: the tile
+    // count, the match count and the device error flags together.
     uint64_t cap_dev = out ? out_cap : 0;
     if (host && out && out_cap) {
       EO_TRY_HIP(hipMallocAsync((void **)&dout, out_cap * row, s), "output staging");
@@ -621,22 +616,31 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     uint32_t *matched = outer ? (uint32_t *)(A + o_bits) : nullptr;
     if (outer) EO_TRY_HIP(fill_u32(matched, 0u, matched_words(nS), s), "match bitmap reset");
     clk.mark();  // 2
-    hipError_t e = join_fused(P, hp.s_span, join_fused_cap(hp.lds_cap), (uint32_t)n_tiles, Rp, r_start, Sp, s_start,
-                              tile_start, dout, cap_dev, small + 1, wide, matched, s);
+    hipError_t e = join_fused(P, hp.s_span, cap, join_fused_grid(P, nR, nS, tiles_est, c->props.cus), small, Rp, r_start, Sp,
+                              s_start, tile_start, dout, cap_dev, small + 1, wide, matched, c->d_err, s);
     if (e == hipSuccess && outer) e = emit_unmatched(Sp, nS, matched, small + 1, dout, cap_dev, wide, s);
     clk.mark();  // 3
+    uint64_t words[3] = {0, 0, 0};  // tiles, matches, error flags
+    if (e == hipSuccess) e = take_error_flags(c->d_err, small + 2, s);
     if (e == hipSuccess) {
-      st = read_u64(c, small + 1, &total, 1, s);
-      if (st == EO_OK && staged && total)  // on overflow: the out_cap rows that were written
+      st = read_u64(c, small, words, 3, s);
+      total = words[1];
+      if (st == EO_OK && !words[2] && staged && total) {  // on overflow: the out_cap rows that were written
         e = hipMemcpyAsync(out, dout, std::min(total, out_cap) * row, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+      }
     }
     if (staged) (void)hipFreeAsync(dout, s);
     if (e != hipSuccess) return hip_fail(e, "fused probe");
     if (st != EO_OK) return st;
-    EO_TRY_HIP(hipStreamSynchronize(s), "sync");
-    st = check_device_errors(c, s);
+    if (words[2]) {
+      set_last_error("device error flags 0x" + std::to_string(words[2]) +
+                     (words[0] >= (1ull << 31) ? " (2^31 or more join tiles)" : ""));
+      return EO_EINTERNAL;
+    }
     *n_out = total;
     if (stats) {
+      stats->tiles = (uint32_t)words[0];
       stats->n_out = total;
       stats->ms_partition = clk.ms(0, 1);
       stats->ms_count = 0.0;
@@ -644,9 +648,19 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
       stats->ms_total = clk.ms(0, 3);
       tmr.finish();
     }
-    if (st != EO_OK) return st;
     return (out && total > out_cap) ? EO_ECAPACITY : EO_OK;
   }
+
+  // Deterministic order: the count and write grids are exact, so the tile
+  // count is read first.
+  uint64_t n_tiles = 0;
+  st = read_u64(c, small, &n_tiles, 1, s);
+  if (st != EO_OK) return st;
+  if (n_tiles >= (1ull << 31)) {
+    set_last_error("too many join tiles");
+    return EO_EINTERNAL;
+  }
+  if (stats) stats->tiles = (uint32_t)n_tiles;
 
   // Deterministic order: count -> scan -> write, nothing written on EO_ECAPACITY.
   char *B = A + o_tc;

@@ -66,6 +66,9 @@ struct eo_batch {
   bool timed = false;
   eo_join_stats *stats = nullptr;
   bool in_flight = false;
+  // set once a submit has enqueued its whole batch; cleared when a submit
+  // starts, so a batch whose planning or enqueue failed is never replayed
+  bool replayable = false;
   // last submitted batch
   std::vector<eo_request> reqs;
   std::vector<eo::JoinInfo> joins;
@@ -532,6 +535,7 @@ eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const
   eo_ctx *c = b->ctx;
   DeviceGuard g(c->device);
   drop_graph(b);
+  b->replayable = false;
   b->reqs.assign(reqs, reqs + n);
   Layout L;
   size_t blob_bytes = 0;
@@ -588,6 +592,7 @@ eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const
   }
   EO_TRY_HIP(hipEventRecord(b->done, s), "event");
   b->in_flight = true;
+  b->replayable = true;
   return EO_OK;
 }
 
@@ -597,7 +602,10 @@ eo_status eo_batch_replay(eo_batch *b) {
     set_last_error("eo_batch_replay: a batch is still in flight on this object");
     return EO_EINVAL;
   }
-  if (b->reqs.empty() && b->joins.empty()) return EO_EINVAL;
+  if (!b->replayable || (b->reqs.empty() && b->joins.empty())) {
+    set_last_error("eo_batch_replay: no batch was submitted successfully on this object");
+    return EO_EINVAL;
+  }
   eo_ctx *c = b->ctx;
   DeviceGuard g(c->device);
   hipStream_t s = b->stream;

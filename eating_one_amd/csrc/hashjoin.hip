@@ -59,20 +59,34 @@ __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_st
 
 // The half-size table variant is limited to 64 VGPRs so that two 16-wave
 // workgroups fit on a CU (8 waves per SIMD).
+//
+// The grid is sized on the host without knowing the tile count (no readback
+// between planning and probing): block b runs tiles b, b + grid, ... below the
+// planned total *n_tiles, so a grid of one block per partition covers the
+// usual one-tile partitions exactly and the extra span tiles of skewed
+// partitions go to the blocks that started first.
 template <typename C>
 __global__ __launch_bounds__(kFNT) __attribute__((amdgpu_waves_per_eu(C::Cap <= 8704 ? 8 : 1)))
 void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
                                                      const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
-                                                     uint64_t *out_cursor, int wide, uint32_t *matched) {
+                                                     uint64_t *out_cursor, int wide, uint32_t *matched,
+                                                     const uint64_t *__restrict__ n_tiles, uint32_t *err) {
   __shared__ tile::ProbeShared<C> sh;
+  const uint64_t nt = *n_tiles;
+  if (nt >= (1ull << 31)) {  // tile ids are 32-bit: report, probe nothing
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomic_or_agent(err, kErrOverflow);
+    return;
+  }
   // round-robin over the XCDs: the span tiles of one hot (skewed) partition
   // are consecutive and must not all land on one XCD
-  const uint32_t t = blockIdx.x;
-  SpanRef ref;
-  if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
-  tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched);
+  for (uint32_t t = blockIdx.x; t < (uint32_t)nt; t += gridDim.x) {
+    if (t != blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
+    SpanRef ref;
+    if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+    tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched);
+  }
 }
 
 // Deterministic order on the fused probe's tiles: a count pass (one count per
@@ -85,11 +99,13 @@ void k_span_count(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
                   const uint32_t *__restrict__ s_start, const uint32_t *__restrict__ tile_start, uint32_t P,
                   uint32_t s_span, uint64_t *__restrict__ tile_count, uint32_t *matched) {
   __shared__ tile::ProbeShared<C> sh;
-  const uint32_t t = blockIdx.x;
-  SpanRef ref;
-  if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
-  const uint64_t n = tile::probe_tile<C, tile::kProbeCount>(sh, R, S, ref, nullptr, 0, nullptr, false, matched);
-  if (threadIdx.x == 0) tile_count[t] = n;
+  for (uint32_t t = blockIdx.x;; t += gridDim.x) {  // a grid below the tile count loops
+    if (t != blockIdx.x) __syncthreads();
+    SpanRef ref;
+    if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+    const uint64_t n = tile::probe_tile<C, tile::kProbeCount>(sh, R, S, ref, nullptr, 0, nullptr, false, matched);
+    if (threadIdx.x == 0) tile_count[t] = n;
+  }
 }
 
 template <typename C>
@@ -99,10 +115,12 @@ void k_span_write(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
                   uint32_t s_span, const uint64_t *__restrict__ tile_off, uint2 *__restrict__ out, uint64_t out_cap,
                   int wide) {
   __shared__ tile::ProbeShared<C> sh;
-  const uint32_t t = blockIdx.x;
-  SpanRef ref;
-  if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
-  tile::probe_tile<C, tile::kProbeOrdered>(sh, R, S, ref, out, out_cap, nullptr, wide != 0, nullptr, tile_off[t]);
+  for (uint32_t t = blockIdx.x;; t += gridDim.x) {  // a grid below the tile count loops
+    if (t != blockIdx.x) __syncthreads();
+    SpanRef ref;
+    if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+    tile::probe_tile<C, tile::kProbeOrdered>(sh, R, S, ref, out, out_cap, nullptr, wide != 0, nullptr, tile_off[t]);
+  }
 }
 
 __global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,
@@ -141,24 +159,44 @@ hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const 
   return scan_u32(tile_start, tile_start, n, n_tiles_dev, scan_scratch, err, stream);
 }
 
-hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
-                      const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched,
-                      hipStream_t stream) {
-  if (n_tiles == 0) return hipSuccess;
-  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+// Largest grid of 1024-thread blocks one dispatch takes (2^32 work-items),
+// in whole rounds over the 8 XCDs.
+constexpr uint64_t kMaxFNTBlocks = (0xffffffffull / kFNT) / kXcds * kXcds;
+
+// One block per tile, up to the dispatch limit (the kernels loop beyond it).
+static uint32_t exact_grid(uint32_t n_tiles) {
+  return (uint32_t)std::min<uint64_t>(((uint64_t)n_tiles + kXcds - 1) / kXcds * kXcds, kMaxFNTBlocks);
+}
+
+uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tiles_est, int cus) {
+  // one block per partition that can hold a tile (the common one tile per
+  // partition; a tile needs an R and an S tuple, so at most min(|R|, |S|)
+  // partitions have one), at least two blocks per CU, never more than the
+  // estimated tile count, and within the dispatch limit of 2^32 work-items
+  uint64_t g = std::min<uint64_t>(parts, std::min(nR, nS));
+  g = std::max<uint64_t>(g, 2ull * (uint64_t)cus);
+  g = std::min<uint64_t>(g, std::max<uint64_t>(tiles_est, 1));
+  g = (g + kXcds - 1) / kXcds * kXcds;
+  return (uint32_t)std::min(g, kMaxFNTBlocks);
+}
+
+hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t grid, const uint64_t *n_tiles_dev,
+                      const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
+                      const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
+                      uint32_t *matched, uint32_t *err, hipStream_t stream) {
+  if (grid == 0) return hipSuccess;
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
   uint2 *vout = reinterpret_cast<uint2 *>(out);
   if (cap == (uint32_t)tile::ProbeSmall::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
                                 s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
-                                matched));
+                                matched, n_tiles_dev, err));
   else if (cap == (uint32_t)tile::ProbeBig::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
                                 s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
-                                matched));
+                                matched, n_tiles_dev, err));
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -168,7 +206,7 @@ hipError_t span_count(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
                       uint32_t *matched, uint64_t *tile_count, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
-  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+  const uint32_t grid = exact_grid(n_tiles);
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
   if (cap == (uint32_t)tile::ProbeSmall::Cap)
     EO_TIMED("probe_count", stream,
@@ -187,7 +225,7 @@ hipError_t span_write(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_
                       const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
                       const uint64_t *tile_off, eo_pair *out, uint64_t out_cap, bool wide, hipStream_t stream) {
   if (n_tiles == 0) return hipSuccess;
-  const uint32_t grid = (n_tiles + kXcds - 1) / kXcds * kXcds;
+  const uint32_t grid = exact_grid(n_tiles);
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
   uint2 *vout = reinterpret_cast<uint2 *>(out);
   if (cap == (uint32_t)tile::ProbeSmall::Cap)

@@ -282,8 +282,8 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
   const uint64_t s0 = t * kIdxTile;
   uint2 s[kIdxU];
   uint64_t b[kIdxU];
-  uint32_t m[kIdxU];
-  uint32_t total = 0;
+  uint32_t m[kIdxU];  // one probe's matches: at most the index size (< 2^31)
+  uint64_t total = 0;  // eight probes' matches can pass 2^32
   uint32_t nul = 0;
 #pragma unroll
   for (int u = 0; u < kIdxU; ++u) {
@@ -305,10 +305,10 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
       for (uint64_t j = b[u]; j < e && ix.keys[j] == s[u].x; ++j) ++m[u];
     }
     if (outer && m[u] == 0) nul |= 1u << u;  // left outer join: one null-padded row
-    total += m[u] + ((nul >> u) & 1u);
+    total += (uint64_t)m[u] + ((nul >> u) & 1u);
   }
   if (!WRITE) {
-    const uint64_t tot = block_sum<kIdxNT, uint64_t>((uint64_t)total, red);
+    const uint64_t tot = block_sum<kIdxNT, uint64_t>(total, red);
     if (threadIdx.x == 0) {
       if (tile_cnt) tile_cnt[t] = tot;
       else if (cursor && tot) atomic_add_agent(cursor, tot);
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(kIdxNT) void k_index_probe(IndexView ix, const uint
   }
   // output offsets: scanned tile counts, or a range reserved with one atomic
   uint64_t tile_total;
-  const uint64_t excl = block_excl_scan<kIdxNT, uint64_t>((uint64_t)total, red, &tile_total);
+  const uint64_t excl = block_excl_scan<kIdxNT, uint64_t>(total, red, &tile_total);
   uint64_t pos;
   if (tile_cnt) {
     pos = tile_cnt[t] + excl;

@@ -47,6 +47,10 @@ hipError_t fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
 // ---- util.hip ---------------------------------------------------------------
 // Streaming 16-byte copy of `bytes` (a multiple of 16) with a grid of 8 WGs per CU.
 hipError_t copy_bytes(const void *src, void *dst, uint64_t bytes, int cus, hipStream_t stream);
+// One launch of the bandwidth probe kernel that `o` describes (copy, read or
+// write of o.bytes); sink: a device word array of o.threads entries.
+hipError_t bandwidth_kernel(const eo_probe_opts &o, const void *src, void *dst, uint32_t *sink, int cus,
+                            hipStream_t s);
 
 // ---- partition.hip ----------------------------------------------------------
 // One stable radix-partition pass over `bits` (1..kMaxPassBits) digit bits.
@@ -107,12 +111,21 @@ uint32_t join_fused_cap(uint32_t lds_cap);
 hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
                            const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
                            uint32_t *err, hipStream_t stream);
+// Grid of join_fused for `parts` partitions of nR x nS tuples, sized without
+// the tile count (tiles_est: the host's estimate of it).
+uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tiles_est, int cus);
+// The probe runs the *n_tiles_dev planned tiles over `grid` blocks (any grid
+// is correct; a tile count of 2^31 or more raises kErrOverflow in *err and
+// probes nothing).
 // wide: out holds 16-byte RESULT rows {R payload, key, S payload, key};
 // matched (optional): bitmap over S positions of the tuples that matched.
-hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t n_tiles, const eo_tuple *R,
-                      const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start, const uint32_t *tile_start,
-                      eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched,
-                      hipStream_t stream);
+hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t grid, const uint64_t *n_tiles_dev,
+                      const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
+                      const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
+                      uint32_t *matched, uint32_t *err, hipStream_t stream);
+// Copies the device error flags into *dst (one u64 word) and clears them, so
+// that one readback returns results and flags together.
+hipError_t take_error_flags(uint32_t *err, uint64_t *dst, hipStream_t stream);
 // Deterministic order on the fused tiles (join_fused_plan's tile_start):
 // span_count writes every tile's match count (and sets matched bits);
 // span_write writes every tile's rows from tile_off[t] on, in an order fixed

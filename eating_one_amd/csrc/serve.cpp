@@ -13,6 +13,8 @@
 //   - whenever a batch slot is free and requests are waiting, all waiting
 //     requests that fit are submitted as one batch (continuous batching);
 //     `slots` batches run at once on their own streams;
+//   - a waiting request that is cancelled or outlives its admission deadline
+//     leaves the queue without running (the policy is eo::ServeQueue);
 //   - latency = completion (event observed by the loop) - arrival.
 // Mode EO_SERVE_PER_JOIN serves the same stream one eo_hash_join call per
 // request, the per-join launch sequence the batched mode is measured against.
@@ -21,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -29,6 +32,7 @@
 #include "eojoin.h"
 #include "kernels.hpp"
 #include "plan.hpp"
+#include "serve_queue.hpp"
 #include "table.hpp"
 
 
@@ -104,19 +108,25 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
                                   const eo_serve_opts *o, eo_result *results, double *latency_ms,
                                   eo_serve_stats *stats) {
   if (!c || (n && (!reqs || !results))) return EO_EINVAL;
-  if (o && o->struct_size != 0 && o->struct_size < sizeof(eo_serve_opts)) return EO_EINVAL;
+  // a caller built against the struct without deadline_ms / cancel passes the shorter size
+  const size_t v1_size = offsetof(eo_serve_opts, deadline_ms);
+  if (o && o->struct_size != 0 && o->struct_size < v1_size) return EO_EINVAL;
+  const bool has_v2 = o && (o->struct_size == 0 || o->struct_size >= sizeof(eo_serve_opts));
+  const double *deadline = has_v2 ? o->deadline_ms : nullptr;
+  const volatile uint32_t *cancel = has_v2 ? o->cancel : nullptr;
   const int mode = o ? o->mode : EO_SERVE_BATCHED;
   if (mode != EO_SERVE_BATCHED && mode != EO_SERVE_PER_JOIN) return EO_EINVAL;
   for (uint32_t i = 1; arrival_ms && i < n; ++i)
     if (!(arrival_ms[i] >= arrival_ms[i - 1])) return EO_EINVAL;
+  for (uint32_t i = 0; deadline && i < n; ++i)
+    if (deadline[i] != deadline[i]) return EO_EINVAL;  // NaN
   const uint32_t max_batch = (o && o->max_batch) ? o->max_batch : 512u;
   const uint32_t nslots = (o && o->slots) ? std::min<uint32_t>(o->slots, 8u) : 2u;
   const uint64_t budget = (o && o->hbm_budget) ? o->hbm_budget : c->hbm_budget;
-  const uint64_t batch_cap = budget / nslots;
   DeviceGuard g(c->device);
   std::vector<double> lat(n, 0.0);
+  std::vector<uint8_t> ran(n, 0);
   eo_serve_stats st{};
-  auto arrival = [&](uint32_t i) { return arrival_ms ? arrival_ms[i] : 0.0; };
 
   // batch objects (each with its stream and grow-only arena) persist in the
   // ctx between calls, so a warmed-up loop allocates nothing
@@ -135,42 +145,72 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
   std::vector<uint64_t> rbytes(n);
   for (uint32_t i = 0; i < n; ++i) rbytes[i] = request_bytes(reqs[i], c->props.cus);
 
+  ServeQueueParams qp;
+  qp.budget = budget;
+  qp.batch_cap = mode == EO_SERVE_BATCHED ? budget / nslots : budget;
+  qp.max_batch = mode == EO_SERVE_BATCHED ? max_batch : 1u;
+  ServeQueue queue(n, arrival_ms, rbytes.data(), deadline, cancel, qp);
+  std::vector<std::pair<uint32_t, eo_status>> gone;
+  uint32_t done = 0;
+  // requests that leave the queue without running
+  auto retire = [&](double now) {
+    gone.clear();
+    queue.retire(now, &gone);
+    for (const auto &r : gone) {
+      eo_result res{};
+      res.tag = reqs[r.first].tag;
+      res.status = r.second;
+      results[r.first] = res;
+      lat[r.first] = std::max(0.0, now - queue.arrival(r.first));  // 0: cancelled before it arrived
+      st.refused += r.second == EO_EBUDGET;
+      st.expired += r.second == EO_ETIMEDOUT;
+      st.cancelled += r.second == EO_ECANCELED;
+    }
+    done += (uint32_t)gone.size();
+    return !gone.empty();
+  };
+
   const Clock::time_point t0 = Clock::now();
-  uint32_t next = 0, done = 0;
   uint64_t in_flight_bytes = 0;
   eo_status fail = EO_OK;
+  std::vector<uint32_t> idx;
 
   if (mode == EO_SERVE_PER_JOIN) {
     eo_join_opts jo{};
     jo.struct_size = sizeof(jo);
     jo.where = EO_DEVICE;
-    for (uint32_t i = 0; i < n && fail == EO_OK; ++i) {
-      while (ms_since(t0) < arrival(i)) std::this_thread::yield();
+    while (done < n && fail == EO_OK) {
+      retire(ms_since(t0));
+      uint64_t bytes = 0;
+      bool exclusive = false;
+      if (!queue.take(ms_since(t0), false, 0, &idx, &bytes, &exclusive)) {
+        if (queue.drained()) break;
+        std::this_thread::yield();
+        continue;
+      }
+      const uint32_t i = idx[0];
       const eo_request &q = reqs[i];
       eo_result r{};
       r.tag = q.tag;
-      if (rbytes[i] > budget) {
-        r.status = EO_EBUDGET;
-      } else {
-        const eo_tuple *R = q.table ? q.table->part : q.R;
-        const uint64_t nR = q.table ? q.table->n : q.nR;
-        uint64_t cnt = 0;
-        eo_status e = eo_hash_join(c, R, nR, q.S, q.nS, q.out, q.out ? q.out_cap : 0, &cnt, &jo);
-        if (e != EO_OK && e != EO_ECAPACITY) fail = e;
-        r.n_out = cnt;
-        r.status = e == EO_ECAPACITY ? EO_ECAPACITY : EO_OK;
-      }
-      lat[i] = ms_since(t0) - arrival(i);
+      const eo_tuple *R = q.table ? q.table->part : q.R;
+      const uint64_t nR = q.table ? q.table->n : q.nR;
+      uint64_t cnt = 0;
+      eo_status e = eo_hash_join(c, R, nR, q.S, q.nS, q.out, q.out ? q.out_cap : 0, &cnt, &jo);
+      if (e != EO_OK && e != EO_ECAPACITY) fail = e;
+      r.n_out = cnt;
+      r.status = e == EO_ECAPACITY ? EO_ECAPACITY : EO_OK;
+      lat[i] = ms_since(t0) - queue.arrival(i);
+      ran[i] = 1;
       results[i] = r;
       st.n_out += r.n_out;
       st.batches += 1;
       st.largest_batch = 1;
-      st.peak_admitted = std::max(st.peak_admitted, rbytes[i]);
+      st.peak_admitted = std::max(st.peak_admitted, bytes);
+      ++done;
     }
-    done = n;
   }
 
-  while (done < n && fail == EO_OK) {
+  while (mode == EO_SERVE_BATCHED && done < n && fail == EO_OK) {
     bool progress = false;
     // completions
     for (auto &s : slots) {
@@ -191,7 +231,8 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
       for (size_t k = 0; k < s.idx.size(); ++k) {
         const uint32_t i = s.idx[k];
         results[i] = s.res[k];
-        lat[i] = now - arrival(i);
+        lat[i] = now - queue.arrival(i);
+        ran[i] = 1;
         st.n_out += s.res[k].n_out;
       }
       done += (uint32_t)s.idx.size();
@@ -206,46 +247,20 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
       progress = true;
     }
     if (fail != EO_OK) break;
-    // refuse requests that can never fit
-    while (next < n && rbytes[next] > budget && arrival(next) <= ms_since(t0)) {
-      eo_result r{};
-      r.tag = reqs[next].tag;
-      r.status = EO_EBUDGET;
-      results[next] = r;
-      lat[next] = 0.0;
-      ++next;
-      ++done;
-      progress = true;
-    }
-    // admission into a free slot
+    // cancelled, expired and too large requests leave the queue
+    progress |= retire(ms_since(t0));
+    // admission into free slots
     bool any_busy = false;
     for (auto &s : slots) any_busy |= s.busy;
     for (auto &s : slots) {
-      if (s.busy || next >= n) continue;
-      const double now = ms_since(t0);
-      if (arrival(next) > now) break;
-      if (rbytes[next] > budget) break;  // refused above once it has arrived
-      s.idx.clear();
+      if (s.busy) continue;
+      uint64_t bytes = 0;
+      bool exclusive = false;
+      if (!queue.take(ms_since(t0), any_busy, in_flight_bytes, &s.idx, &bytes, &exclusive)) break;
       s.reqs.clear();
-      s.bytes = 0;
-      s.exclusive = false;
-      if (rbytes[next] > batch_cap) {
-        if (any_busy) break;  // wait until the device is idle
-        s.exclusive = true;
-        s.idx.push_back(next);
-        s.reqs.push_back(reqs[next]);
-        s.bytes = rbytes[next];
-        ++next;
-      } else {
-        while (next < n && arrival(next) <= now && s.idx.size() < max_batch && rbytes[next] <= batch_cap &&
-               s.bytes + rbytes[next] <= batch_cap && in_flight_bytes + s.bytes + rbytes[next] <= budget) {
-          s.idx.push_back(next);
-          s.reqs.push_back(reqs[next]);
-          s.bytes += rbytes[next];
-          ++next;
-        }
-      }
-      if (s.idx.empty()) break;
+      for (uint32_t i : s.idx) s.reqs.push_back(reqs[i]);
+      s.bytes = bytes;
+      s.exclusive = exclusive;
       eo_status e = eo_batch_submit(s.b, s.reqs.data(), (uint32_t)s.reqs.size(), nullptr);
       if (e != EO_OK) {
         fail = e;
@@ -271,11 +286,13 @@ extern "C" eo_status eo_serve_run(eo_ctx *c, const eo_request *reqs, const doubl
     return fail;
   }
   st.wall_ms = ms_since(t0);
+  std::vector<double> ran_lat;
   for (uint32_t i = 0; i < n; ++i) {
     const uint64_t nR = reqs[i].table ? reqs[i].table->n : reqs[i].nR;
     st.tuples += nR + reqs[i].nS;
+    if (ran[i]) ran_lat.push_back(lat[i]);
   }
-  percentiles(lat, &st);
+  percentiles(ran_lat, &st);
   if (latency_ms)
     for (uint32_t i = 0; i < n; ++i) latency_ms[i] = lat[i];
   if (stats) *stats = st;

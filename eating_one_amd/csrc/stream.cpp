@@ -178,7 +178,6 @@ struct Streamer {
     const eo_tuple *probe_side = dS[b];
     hipError_t e = hipSuccess;
     if (hash) {
-      uint64_t n_tiles = 0;
       if (hp.passes) {
         eo_status s = partition_relation(c, dS[b], n, hp, tmp, sp, s_start, pscr, cs);
         if (s != EO_OK) return s;
@@ -190,14 +189,11 @@ struct Streamer {
       const uint32_t cap_lds = join_fused_cap(hp.lds_cap);
       EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap_lds, r_start, s_start, tile_start, small, jscr, c->d_err, cs),
                  "tile plan");
-      eo_status s = read_u64(c, small, &n_tiles, 1, cs);
-      if (s != EO_OK) return s;
-      if (n_tiles >= (1ull << 31)) {
-        set_last_error("too many join tiles");
-        return EO_EINTERNAL;
-      }
-      e = join_fused(P, hp.s_span, cap_lds, (uint32_t)n_tiles, hp.passes ? rp : dR, r_start, probe_side, s_start,
-                     tile_start, dst, cap, small + 1, wide, outer ? matched : nullptr, cs);
+      // the probe's grid does not need the tile count: no readback before it
+      const uint64_t tiles_est = (uint64_t)P + nr / cap_lds + n / hp.s_span + 1024;
+      e = join_fused(P, hp.s_span, cap_lds, join_fused_grid(P, nr, n, tiles_est, c->props.cus), small, hp.passes ? rp : dR,
+                     r_start, probe_side, s_start, tile_start, dst, cap, small + 1, wide, outer ? matched : nullptr,
+                     c->d_err, cs);
     } else {
       e = nlj_fused(plan_nlj(nr, n), dR, dS[b], small + 1, dst, cap, wide, outer ? matched : nullptr, cs);
     }

@@ -142,3 +142,82 @@ def uniform(seed: int, nR: int, nS: int, key_range: int, lo: int = 0):
     R = _tuples(rng.integers(lo, lo + key_range, size=nR, dtype=np.int64).astype(np.int32))
     S = _tuples(rng.integers(lo, lo + key_range, size=nS, dtype=np.int64).astype(np.int32))
     return R, S
+
+
+class C5Stream:
+    """The C5 request stream, generated on the device (eo_gen_build /
+    eo_gen_probe) and held in three device buffers: R of the requests without
+    a resident table, S, and one output slot of |S| pairs per request.
+
+    Per request i: |R|, |S| = 2^U with U ~ U[lo, hi]; with probability `reuse`
+    the build side is one of `n_tables` resident tables (eo_table); R keys are
+    a seeded permutation of [0, |R|); S keys hit R with probability sel ~
+    U[0.1, 1] and otherwise lie in a disjoint range, so the join count of
+    request i is exactly the number of its S keys below |R|."""
+
+    def __init__(self, ctx, n_req=4096, lo=10.0, hi=24.0, n_tables=64, reuse=0.25, seed=0xE01C5):
+        import eating_one_amd as eo
+
+        rng = np.random.default_rng(seed)
+        self.ctx, self.n = ctx, n_req
+        t_sizes = (2 ** rng.uniform(lo, hi, size=n_tables)).astype(np.int64)
+        self.tables = []  # (|R|, eo_table, the DeviceArray it was built from)
+        for k, nR in enumerate(t_sizes):
+            d = eo.DeviceArray(ctx, 8 * int(nR))
+            ctx.gen_build(d, int(nR), seed=0x7AB1E000 + k)
+            self.tables.append((int(nR), ctx.table_create(d, n=int(nR)), d))
+        self.use_table = rng.random(n_req) < reuse
+        self.table_id = rng.integers(0, n_tables, size=n_req)
+        nR = (2 ** rng.uniform(lo, hi, size=n_req)).astype(np.int64)
+        self.nR = np.where(self.use_table, t_sizes[self.table_id], nR)
+        self.nS = (2 ** rng.uniform(lo, hi, size=n_req)).astype(np.int64)
+        self.sel = rng.uniform(0.1, 1.0, size=n_req)
+        pad = lambda n: (n + 1) // 2 * 2  # 16-byte aligned starts
+        self.r_off = np.concatenate([[0], np.cumsum(np.where(self.use_table, 0, pad(self.nR)))])
+        self.s_off = np.concatenate([[0], np.cumsum(pad(self.nS))])
+        self.dR = eo.DeviceArray(ctx, 8 * max(int(self.r_off[-1]), 1))
+        self.dS = eo.DeviceArray(ctx, 8 * max(int(self.s_off[-1]), 1))
+        self.dO = eo.DeviceArray(ctx, 8 * max(int(self.s_off[-1]), 1))
+        self.reqs = []
+        for i in range(n_req):
+            if self.use_table[i]:
+                t, Rp = self.tables[int(self.table_id[i])][1], None
+            else:
+                t, Rp = None, self.dR.ptr + 8 * int(self.r_off[i])
+                ctx.gen_build(self.dR, int(self.nR[i]), seed=0xB0000000 + i, offset=8 * int(self.r_off[i]))
+            ctx.gen_probe(self.dS, int(self.nS[i]), int(self.nR[i]), float(self.sel[i]), seed=0x50000000 + i,
+                          offset=8 * int(self.s_off[i]))
+            self.reqs.append(eo.request(R=Rp, nR=int(self.nR[i]), S=self.dS.ptr + 8 * int(self.s_off[i]),
+                                        nS=int(self.nS[i]), out=self.dO.ptr + 8 * int(self.s_off[i]),
+                                        out_cap=int(self.nS[i]), table=t, tag=i))
+
+    def _read(self, arr, off_tuples, n, cols=2):
+        import ctypes
+
+        out = np.empty((n, cols), dtype=np.int32)
+        if n:
+            st = self.ctx.lib.eo_copy_d2h(self.ctx.handle, out.ctypes.data, arr.ptr + 8 * int(off_tuples), out.nbytes)
+            if st != 0:
+                raise RuntimeError(f"eo_copy_d2h failed ({st})")
+        return out
+
+    def R(self, i):
+        """Build side of request i (host copy)."""
+        if self.use_table[i]:
+            nR, _, d = self.tables[int(self.table_id[i])]
+            return self._read(d, 0, nR)
+        return self._read(self.dR, self.r_off[i], int(self.nR[i]))
+
+    def S(self, i):
+        return self._read(self.dS, self.s_off[i], int(self.nS[i]))
+
+    def out(self, i, n):
+        return self._read(self.dO, self.s_off[i], n)
+
+    def free(self):
+        for _, t, d in self.tables:
+            t.free()
+            d.close()
+        for d in (self.dO, self.dS, self.dR):
+            d.close()
+        self.tables = []

@@ -151,6 +151,10 @@ typedef struct eo_device_info {
 /* ---- library / context --------------------------------------------------- */
 
 uint32_t eo_abi_version(void);
+/* Hash of the sources the library was built from (16 hex digits), so that a
+   caller can tell a stale build from a current one.  No reference counterpart
+   (the reference rebuilds its cubins with make). */
+const char *eo_build_source_hash(void);
 const char *eo_status_string(eo_status st);
 /* Message of the last failure on this host thread (HIP error text etc.). */
 const char *eo_last_error(void);
@@ -182,6 +186,29 @@ eo_status eo_copy_d2d(eo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
  * mean device bandwidth in GB/s (read + write bytes / time).  The measured
  * HBM ceiling reported beside every roofline fraction. */
 eo_status eo_bandwidth_probe(eo_ctx *ctx, uint64_t bytes, int iters, double *gbs);
+
+/* Bandwidth probe with an explicit kernel shape, for measuring the HBM
+ * ceiling the join kernels are compared against: a copy, a read-only or a
+ * write-only pass over `bytes` (multiple of 16) with `unroll` 16-byte words
+ * in flight per thread (1, 2, 4, 8), `threads` per workgroup (64..1024,
+ * 0 = 256), `blocks_per_cu` workgroups per CU (0 = one thread per word, no
+ * grid-stride loop) and nontemporal loads / stores.  *gbs = bytes moved (read
+ * + write) per second, mean over `iters` launches after one untimed launch.
+ * No reference counterpart (the reference reports no bandwidth). */
+enum { EO_PROBE_COPY = 0, EO_PROBE_READ = 1, EO_PROBE_WRITE = 2 };
+enum { EO_PROBE_NT_LOAD = 1, EO_PROBE_NT_STORE = 2 };
+typedef struct eo_probe_opts {
+  uint32_t struct_size;  /* sizeof(eo_probe_opts) */
+  int32_t mode;          /* EO_PROBE_COPY / READ / WRITE */
+  uint64_t bytes;
+  int32_t unroll;
+  int32_t threads;
+  int32_t blocks_per_cu;
+  uint32_t flags;        /* EO_PROBE_NT_LOAD | EO_PROBE_NT_STORE */
+  int32_t iters;
+  int32_t reserved;
+} eo_probe_opts;
+eo_status eo_bandwidth_probe_ex(eo_ctx *ctx, const eo_probe_opts *opts, double *gbs);
 
 /* Seeded synthetic relations written straight into device memory (R, S are
  * device pointers), for request streams too large to build on the host.
@@ -485,11 +512,19 @@ eo_status eo_tune_load(eo_ctx *ctx, const char *path);
 enum { EO_SERVE_BATCHED = 0, EO_SERVE_PER_JOIN = 1 };
 
 typedef struct eo_serve_opts {
-  uint32_t struct_size;  /* sizeof(eo_serve_opts) */
+  uint32_t struct_size;  /* sizeof(eo_serve_opts); a struct that ends at hbm_budget
+                            (no deadline_ms / cancel fields) is accepted as well */
   int32_t mode;          /* EO_SERVE_BATCHED, or EO_SERVE_PER_JOIN: one eo_hash_join per request */
   uint32_t max_batch;    /* requests per batch (0 = 512) */
   uint32_t slots;        /* batches in flight at once (0 = 2, at most 8) */
   uint64_t hbm_budget;   /* bytes admitted at once (0 = the ctx budget, 200 GB by default) */
+  /* Per request, optional (NULL = none): a request not admitted within
+     deadline_ms[i] (> 0) of its arrival completes with EO_ETIMEDOUT without
+     running; a request whose cancel[i] is non-zero before it is admitted
+     completes with EO_ECANCELED without running (the flags may be set from
+     another thread while the loop runs).  Admitted requests run to the end. */
+  const double *deadline_ms;
+  const volatile uint32_t *cancel;
 } eo_serve_opts;
 
 typedef struct eo_serve_stats {
@@ -501,6 +536,10 @@ typedef struct eo_serve_stats {
   uint32_t batches;
   uint32_t largest_batch;
   uint64_t peak_admitted;  /* largest admitted bytes in flight */
+  uint32_t refused;        /* requests that completed with EO_EBUDGET */
+  uint32_t expired;        /* ... EO_ETIMEDOUT */
+  uint32_t cancelled;      /* ... EO_ECANCELED */
+  uint32_t reserved;
 } eo_serve_stats;
 
 /* Serves a stream of requests: request i arrives arrival_ms[i] after the call
@@ -508,7 +547,10 @@ typedef struct eo_serve_stats {
  * request's inputs, partition scratch and output buffer count while it is in
  * flight), continuous batching into eo_batch objects (one per slot, each on
  * its own stream).  results[i] and latency_ms[i] (optional) per request; a
- * request larger than the budget gets EO_EBUDGET.  No reference counterpart:
+ * request larger than the budget gets EO_EBUDGET, an expired or cancelled one
+ * EO_ETIMEDOUT or EO_ECANCELED (n_out 0, its output untouched; latency = the
+ * time it left the queue).  The latency percentiles cover the requests that
+ * ran.  No reference counterpart:
  * the reference runs one join per process (hj.cc:1189-1269). */
 eo_status eo_serve_run(eo_ctx *ctx, const eo_request *reqs, const double *arrival_ms,
                        uint32_t n, const eo_serve_opts *opts, eo_result *results,

@@ -16,7 +16,12 @@ def pytest_configure(config):
 def ctx():
     """One library context on GPU 0 for the whole GPU session."""
     import eating_one_amd as eo
+    from eating_one_amd import _abi
+    from eating_one_amd import build as eob
 
+    # the GPU suite must exercise the library built from these sources
+    built = _abi.load().eo_build_source_hash().decode()
+    assert built == eob.source_hash(), f"libeojoin.so ({built}) is stale against the tree ({eob.source_hash()})"
     c = eo.Context(0)
     yield c
     c.close()

@@ -39,6 +39,22 @@ def test_ctypes_signatures_cover_header():
     assert set(_abi.SIGNATURES) <= names
 
 
+def test_library_built_from_the_tree_sources():
+    """The library embeds a hash of the sources it was built from; a stale build fails here."""
+    from eating_one_amd import build as eob
+
+    L = _abi.load()
+    assert L.eo_build_source_hash().decode() == eob.source_hash(), "libeojoin.so is stale: rebuild it"
+
+
+def test_build_rejects_missing_sources(monkeypatch):
+    from eating_one_amd import build as eob
+
+    monkeypatch.setattr(eob, "SOURCES", eob.SOURCES + ["no_such_file.hip"])
+    with pytest.raises(RuntimeError, match="missing"):
+        eob.build()
+
+
 def test_status_strings_and_version():
     L = _abi.load()
     assert L.eo_abi_version() == 1
@@ -86,10 +102,11 @@ def test_struct_layouts_match_the_header(tmp_path):
     src.write_text('''#include <stdio.h>
 #include "eojoin.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(eo_join_opts), sizeof(eo_join_stats),
-         sizeof(eo_kernel_stat), sizeof(eo_ctx_opts), sizeof(eo_device_info), sizeof(eo_index_info),
-         sizeof(eo_request), sizeof(eo_result), sizeof(eo_stream_opts), sizeof(eo_stream_stats),
-         sizeof(eo_tuple));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(eo_join_opts),
+         sizeof(eo_join_stats), sizeof(eo_kernel_stat), sizeof(eo_ctx_opts), sizeof(eo_device_info),
+         sizeof(eo_index_info), sizeof(eo_request), sizeof(eo_result), sizeof(eo_stream_opts),
+         sizeof(eo_stream_stats), sizeof(eo_serve_opts), sizeof(eo_serve_stats), sizeof(eo_probe_opts),
+         sizeof(eo_tune_result), sizeof(eo_tuple));
   return 0;
 }
 ''')
@@ -98,7 +115,8 @@ int main(void) {
     got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     want = [ctypes.sizeof(t) for t in (_abi.JoinOpts, _abi.JoinStats, _abi.KernelStat, _abi.CtxOpts,
                                        _abi.DeviceInfo, _abi.IndexInfo, _abi.Request, _abi.Result,
-                                       _abi.StreamOpts, _abi.StreamStats)] + [8]
+                                       _abi.StreamOpts, _abi.StreamStats, _abi.ServeOpts, _abi.ServeStats,
+                                       _abi.ProbeOpts, _abi.TuneResult)] + [8]
     assert got == want
 
 

@@ -196,6 +196,28 @@ def test_batch_rejects_bad_requests(ctx):
     b.free()
 
 
+def test_batch_replay_refused_after_failed_submit(ctx):
+    """A replay after a submit that failed (here: a request above the row limit) launches nothing."""
+    R, S = gen.c2(seed=6, nR=2000, nS=3000)
+    r = _Req(ctx, R, S)
+    too_big = eo.request(R=r.dR, nR=len(R), S=r.s_ptr, nS=1 << 31, out=r.dout, out_cap=r.cap)
+    # a fresh batch object whose first submit fails
+    b = ctx.batch()
+    with pytest.raises(eo.EOError):
+        b.submit([too_big])
+    assert ctx.lib.eo_batch_replay(b.handle) == eo._abi.EO_EINVAL
+    # a good batch, then a failed submit on the same object: the old graph is gone too
+    assert b.run([r.request(tag=1)])[0][1] == len(r.want)
+    b.replay()
+    assert b.wait()[0][1] == len(r.want)
+    with pytest.raises(eo.EOError):
+        b.submit([r.request(tag=2), too_big])
+    assert ctx.lib.eo_batch_replay(b.handle) == eo._abi.EO_EINVAL
+    # and the object still works
+    assert b.run([r.request(tag=3)])[0][1] == len(r.want)
+    b.free()
+
+
 def test_batch_graph_replay_matches_eager(ctx):
     inputs = _mixed_inputs(61, 10, lo=10, hi=18)
     reqs = [_Req(ctx, R, S) for R, S in inputs]

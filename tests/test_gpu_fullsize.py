@@ -117,3 +117,45 @@ def test_two_pass_half_tables_outer_and_deterministic(ctx):
     finally:
         for x in (dout, dS, dR):
             x.close()
+
+
+@pytest.mark.parametrize("theta,sel,bits,passes", [(0.9, 0.1, 0, 0), (0.5, 1.0, 0, 0), (0.0, 1.0, 11, 1)],
+                         ids=["zipf0.9-sel0.1", "zipf0.5-sel1.0", "one-pass-11-bits"])
+def test_c2_full_size_variants(ctx, theta, sel, bits, passes):
+    """The other C2 shapes at 2^27 x 2^28: Zipf-skewed probe keys (hot keys
+    land in a few partitions, which the planner splits into span tiles),
+    10 % selectivity, and a forced one-pass 11-bit partition.  Properties:
+    count = the S keys present in R, each present S payload exactly once,
+    equal keys in every pair; the deterministic join repeats its rows in the
+    same order and equals the fused join as a set."""
+    nR, nS = 1 << 27, 1 << 28
+    print(f"generating C2 theta={theta} selectivity={sel}", flush=True)
+    R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS, theta=theta, selectivity=sel)
+    present = S[:, 0] < nR
+    n_present = int(present.sum())
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    dout = eo.DeviceArray(ctx, n_present * 8)
+    try:
+        print(f"fused join, radix_bits={bits} passes={passes}", flush=True)
+        st = eo.JoinStats()
+        n, status = ctx.hash_join_device(dR, nR, dS, nS, dout, n_present, radix_bits=bits, passes=passes, stats=st)
+        assert status == 0 and n == n_present
+        if bits:
+            assert st.radix_bits == bits and st.passes == passes
+        pairs = dout.to_host(np.int32, (n, 2))
+        cnt = np.bincount(pairs[:, 1], minlength=nS)
+        assert cnt.max() == 1 and np.array_equal(cnt.astype(bool), present)
+        assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])
+        print("deterministic join, twice", flush=True)
+        runs = []
+        for _ in range(2):
+            n2, status = ctx.hash_join_device(dR, nR, dS, nS, dout, n_present, radix_bits=bits, passes=passes,
+                                              deterministic=True)
+            assert status == 0 and n2 == n
+            runs.append(dout.to_host(np.int32, (n, 2)))
+        assert np.array_equal(runs[0], runs[1])
+        by_s = lambda p: p[np.argsort(p[:, 1])]
+        assert np.array_equal(by_s(runs[0]), by_s(pairs))
+    finally:
+        for x in (dout, dS, dR):
+            x.close()

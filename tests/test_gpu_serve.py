@@ -136,3 +136,76 @@ def test_cli_counts(op):
     assert sum(l.startswith("pair ") for l in lines) == 3
     r = subprocess.run([exe, op, str(nS), str(nR), "7", "0.0"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "count: 0" in r.stdout
+
+
+def test_serve_deadlines_and_cancellation(ctx):
+    """Requests cancelled before admission, or waiting past their admission
+    deadline, complete without running (output untouched); the others equal
+    the oracle.  One slot and batches of one make later requests wait."""
+    tables, items, reqs = _stream(ctx, 51, 24, 12, 18, reuse=0.0)
+    n = len(reqs)
+    sentinel = np.full((1, 2), -7, np.int32)
+    for it in items:  # a sentinel row where each output starts
+        if len(it["want"]):
+            it["dout"].ctx.lib.eo_copy_h2d(ctx.handle, it["dout"].ptr, sentinel.ctypes.data, 8)
+    cancel = np.zeros(n, np.uint32)
+    cancel[[3, 4, 17]] = 1
+    deadline = np.zeros(n)
+    deadline[1::2] = 1e-3  # odd requests must start within a microsecond of arriving
+    res, lat, st = ctx.serve(reqs, None, slots=1, max_batch=1, deadline_ms=deadline, cancel=cancel)
+    ran = 0
+    for i, ((tag, cnt, status), it) in enumerate(zip(res, items)):
+        assert tag == i
+        if cancel[i]:
+            assert status == eo._abi.EO_ECANCELED and cnt == 0, i
+        elif status == eo._abi.EO_ETIMEDOUT:
+            assert deadline[i] > 0 and cnt == 0, i
+        else:
+            assert status == 0 and cnt == len(it["want"]), i
+            got = it["dout"].to_host(np.int32, (cnt, 2)) if cnt else np.empty((0, 2), np.int32)
+            assert same_multiset(got, it["want"]), i
+            ran += 1
+            continue
+        if len(it["want"]):  # did not run: the output is untouched
+            assert np.array_equal(it["dout"].to_host(np.int32, (1, 2)), sentinel), i
+    assert st["cancelled"] == 3
+    # all requests arrive at once and run one at a time: most odd ones expire
+    assert st["expired"] >= 5
+    assert ran == n - st["cancelled"] - st["expired"]
+    assert st["refused"] == 0
+    # no deadlines and no flags: everything runs
+    res2, _, st2 = ctx.serve(reqs, None, slots=1, max_batch=1)
+    assert all(r[2] == 0 and r[1] == len(it["want"]) for r, it in zip(res2, items))
+    assert st2["expired"] == st2["cancelled"] == 0
+    for _, t in tables:
+        t.free()
+
+
+def test_serve_cancel_from_another_thread(ctx):
+    """Flags set while the loop runs stop the requests not yet admitted."""
+    import threading
+    import time
+    tables, items, reqs = _stream(ctx, 53, 16, 12, 16, reuse=0.0)
+    n = len(reqs)
+    cancel = np.zeros(n, np.uint32)
+    arrival = np.linspace(0.0, 400.0, n)  # one request every ~27 ms
+
+    def stop_later():
+        time.sleep(0.1)
+        cancel[:] = 1
+
+    th = threading.Thread(target=stop_later)
+    th.start()
+    res, lat, st = ctx.serve(reqs, arrival, cancel=cancel)
+    th.join()
+    statuses = [r[2] for r in res]
+    assert statuses[0] == 0 and statuses[-1] == eo._abi.EO_ECANCELED
+    first_cancelled = statuses.index(eo._abi.EO_ECANCELED)
+    assert all(s == 0 for s in statuses[:first_cancelled])
+    assert all(s == eo._abi.EO_ECANCELED for s in statuses[first_cancelled:])
+    for (tag, cnt, status), it in zip(res[:first_cancelled], items):
+        assert cnt == len(it["want"])
+    assert st["cancelled"] == n - first_cancelled
+    assert st["wall_ms"] < 350  # the loop ended at the cancellation, not at the last arrival
+    for _, t in tables:
+        t.free()
