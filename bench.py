@@ -386,11 +386,26 @@ def run_c3_nlj(a):
     dom = max(ms.items(), key=lambda x: x[1])
     total_ms = sum(ms.values())
     rate = compares / (dom[1] * 1e-3)
+    # measured issue rates: the NLJ's own compare loop without hits, and plain
+    # integer adds (eo_valu_probe), best over workgroups per CU
+    measured = {}
+    for name, kind in (("compare_loop", _abi.EO_VALU_COMPARE), ("int_add", _abi.EO_VALU_ADD)):
+        best = 0.0
+        for bpc in (2, 4, 8):
+            r = ctypes.c_double()
+            assert L.eo_valu_probe(ctx.handle, kind, bpc, 4 if kind == _abi.EO_VALU_COMPARE else 64, 3,
+                                   ctypes.byref(r)) == 0
+            best = max(best, r.value)
+        measured[name] = best
+        log(f"measured {name}: {best / 1e12:.2f} T/s ({best / peak:.2f} of 1 per lane per clock)")
     res = {"metric": "NLJ compares/s, 2^18 x 2^18 (C3)", "value": compares / (total_ms * 1e-3),
            "unit": "compares/s", "ms_per_step": total_ms, "n_out": n.value, "kernels_ms": ms,
            "deterministic_ms": sum(det) / len(det),
            "roofline": {"bound": "valu", "achieved": rate / 1e9, "peak": peak / 1e9, "unit": "Gcompares/s",
-                        "frac": rate / peak, "kernel": dom[0]}}
+                        "frac": rate / peak, "kernel": dom[0],
+                        "measured_compare_loop": measured["compare_loop"] / 1e9,
+                        "measured_int_add": measured["int_add"] / 1e9,
+                        "frac_of_compare_loop": rate / measured["compare_loop"]}}
     ctx.close()
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline_nlj(S, 16)
@@ -681,8 +696,45 @@ def run_c5(a):
            "closed_loop": out, "speedup_batched_vs_per_join": out["per_join"]["wall_ms"] / out["batched"]["wall_ms"],
            "poisson": {"qps": qps, **{k: v for k, v in lat_res.items()}}}
     stream.free()
+    res["graph"] = graph_leg(ctx, a)
     ctx.close()
     return res
+
+
+def graph_leg(ctx, a, n_req=1024, lo=10, hi=16, reps=20):
+    """A fixed batch of small joins (C5 shapes, |R|, |S| = 2^U, U ~ U[lo, hi])
+    run eagerly (eo_batch_submit: host planning, descriptor upload, launches)
+    and replayed from its hipGraph (eo_batch_replay: one graph launch), each
+    followed by eo_batch_wait; medians of `reps` runs, rows checked equal."""
+    import numpy as np
+
+    from eating_one_amd import gen
+
+    s = gen.C5Stream(ctx, n_req=n_req, lo=lo, hi=hi, n_tables=16, reuse=0.25, seed=0x6EA9)
+    be, bg = ctx.batch(), ctx.batch()  # a new submit discards a batch object's graph: one object each
+    eager, graph = [], []
+    first = be.run(s.reqs)
+    assert bg.run(s.reqs) == first
+    bg.replay()
+    assert bg.wait() == first  # capture, instantiate and the first replay
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = be.run(s.reqs)
+        eager.append((time.perf_counter() - t0) * 1e3)
+        assert res == first
+        t0 = time.perf_counter()
+        bg.replay()
+        res = bg.wait()
+        graph.append((time.perf_counter() - t0) * 1e3)
+        assert res == first
+    be.free()
+    bg.free()
+    tuples = int(s.nR.sum() + s.nS.sum())
+    s.free()
+    e, g = float(np.median(eager)), float(np.median(graph))
+    log(f"graph leg: {n_req} joins of 2^{lo}..2^{hi}: eager {e:.3f} ms, graph replay {g:.3f} ms")
+    return {"workload": f"one batch of {n_req} joins, |R|,|S|=2^U U~U[{lo},{hi}], 16 resident tables for 25%",
+            "tuples": tuples, "eager_ms": e, "graph_replay_ms": g, "speedup": e / g}
 
 
 def run_copy(a):
@@ -724,10 +776,30 @@ def run_copy(a):
                             if key not in best or g.value > best[key]["gbs"]:
                                 best[key] = row
             log(f"{mname} {size >> 30} GiB best: {best[(mname, size)]}")
+    # the radix scatter's write pattern: sequential reads, writes in runs of
+    # run_bytes sent to scattered places (1 GiB)
+    scatter = {}
+    for run in (128, 256, 512, 1024, 2048, 4096, 16384, 65536):
+        top = None
+        for unroll, threads, bpc in ((1, 256, 0), (4, 256, 8), (4, 1024, 2), (2, 512, 8)):
+            for flags in (1, 3):
+                o = _abi.ProbeOpts(ctypes.sizeof(_abi.ProbeOpts), _abi.EO_PROBE_SCATTER, 1 << 30, unroll, threads,
+                                   bpc, flags, 5, run)
+                g = ctypes.c_double()
+                if ctx.lib.eo_bandwidth_probe_ex(ctx.handle, ctypes.byref(o), ctypes.byref(g)) != 0:
+                    raise RuntimeError("eo_bandwidth_probe_ex (scatter) failed")
+                row = {"mode": "scatter", "run_bytes": run, "unroll": unroll, "threads": threads,
+                       "blocks_per_cu": bpc, "nt_store": bool(flags & 2), "gbs": round(g.value, 1)}
+                rows.append(row)
+                if top is None or g.value > top["gbs"]:
+                    top = row
+        scatter[run] = top
+        log(f"scatter runs of {run} B best: {top}")
     ctx.close()
     top = {f"{m}_{sz >> 30}gib": r for (m, sz), r in best.items()}
     return {"metric": "HBM bandwidth ceiling (bandwidth probe sweep)", "unit": "GB/s",
-            "value": max(r["gbs"] for (m, _), r in best.items() if m == "copy"), "best": top, "points": len(rows),
+            "value": max(r["gbs"] for (m, _), r in best.items() if m == "copy"), "best": top,
+            "scatter_by_run_bytes": scatter, "points": len(rows),
             "sweep": rows if a.copy_all else None}
 
 
@@ -780,7 +852,8 @@ def main(argv=None) -> int:
                           "speedup_vs_per_join": c5["speedup_batched_vs_per_join"],
                           "qps": c5["poisson"]["qps"], "p50_ms": c5["poisson"]["batched"]["p50_ms"],
                           "p99_ms": c5["poisson"]["batched"]["p99_ms"],
-                          "per_join_p99_ms": c5["poisson"]["per_join"]["p99_ms"]}
+                          "per_join_p99_ms": c5["poisson"]["per_join"]["p99_ms"],
+                          "graph": c5["graph"]}
         print(json.dumps(res), flush=True)
         return 0
     fn = {"c1": run_c1, "c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,

@@ -138,8 +138,9 @@ class ServeOpts(ctypes.Structure):
     ]
 
 
-EO_PROBE_COPY, EO_PROBE_READ, EO_PROBE_WRITE = 0, 1, 2
+EO_PROBE_COPY, EO_PROBE_READ, EO_PROBE_WRITE, EO_PROBE_SCATTER = 0, 1, 2, 3
 EO_PROBE_NT_LOAD, EO_PROBE_NT_STORE = 1, 2
+EO_VALU_COMPARE, EO_VALU_ADD = 0, 1
 
 
 class ProbeOpts(ctypes.Structure):
@@ -152,7 +153,7 @@ class ProbeOpts(ctypes.Structure):
         ("blocks_per_cu", I32),
         ("flags", U32),
         ("iters", I32),
-        ("reserved", I32),
+        ("run_bytes", I32),
     ]
 
 
@@ -241,6 +242,7 @@ SIGNATURES = {
     "eo_copy_d2d": (INT, [P, P, P, U64]),
     "eo_bandwidth_probe": (INT, [P, U64, INT, ctypes.POINTER(ctypes.c_double)]),
     "eo_bandwidth_probe_ex": (INT, [P, ctypes.POINTER(ProbeOpts), ctypes.POINTER(ctypes.c_double)]),
+    "eo_valu_probe": (INT, [P, INT, INT, INT, INT, ctypes.POINTER(ctypes.c_double)]),
     "eo_exclusive_scan_u32": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_exclusive_scan_u64": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),

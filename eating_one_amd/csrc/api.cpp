@@ -343,8 +343,12 @@ eo_status eo_bandwidth_probe(eo_ctx *c, uint64_t bytes, int iters, double *gbs) 
 
 eo_status eo_bandwidth_probe_ex(eo_ctx *c, const eo_probe_opts *o, double *gbs) {
   if (!c || !o || !gbs || o->struct_size < sizeof(eo_probe_opts)) return EO_EINVAL;
-  if (o->bytes == 0 || o->bytes % 16 || o->iters < 1 || o->mode < EO_PROBE_COPY || o->mode > EO_PROBE_WRITE)
+  if (o->bytes == 0 || o->bytes % 16 || o->iters < 1 || o->mode < EO_PROBE_COPY || o->mode > EO_PROBE_SCATTER)
     return EO_EINVAL;
+  if (o->mode == EO_PROBE_SCATTER) {
+    const uint64_t rb = (uint64_t)o->run_bytes, words = o->bytes / 16;
+    if (rb < 16 || rb > 65536 || (rb & (rb - 1)) || (words & (words - 1)) || words < rb / 16) return EO_EINVAL;
+  }
   if (o->unroll != 1 && o->unroll != 2 && o->unroll != 4 && o->unroll != 8) return EO_EINVAL;
   if (o->threads != 0 && (o->threads < 64 || o->threads > 1024 || o->threads % 64)) return EO_EINVAL;
   if (o->blocks_per_cu < 0 || o->blocks_per_cu > 64 || (o->flags & ~3u)) return EO_EINVAL;
@@ -362,8 +366,30 @@ eo_status eo_bandwidth_probe_ex(eo_ctx *c, const eo_probe_opts *o, double *gbs) 
   EO_TRY_HIP(hipEventSynchronize(c->ev[1]), "event sync");
   float ms = 0.f;
   EO_TRY_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
-  const double moved = (o->mode == EO_PROBE_COPY ? 2.0 : 1.0) * (double)o->bytes * o->iters;
+  const double moved =
+      (o->mode == EO_PROBE_COPY || o->mode == EO_PROBE_SCATTER ? 2.0 : 1.0) * (double)o->bytes * o->iters;
   *gbs = moved / (ms * 1e-3) / 1e9;
+  return EO_OK;
+}
+
+eo_status eo_valu_probe(eo_ctx *c, int kind, int blocks_per_cu, int reps, int iters, double *per_s) {
+  if (!c || !per_s || (kind != EO_VALU_COMPARE && kind != EO_VALU_ADD) || blocks_per_cu < 1 || blocks_per_cu > 32 ||
+      reps < 1 || iters < 1)
+    return EO_EINVAL;
+  DeviceGuard g(c->device);
+  EO_TRY_HIP(hipStreamSynchronize(c->stream), "sync");
+  const uint32_t blocks = (uint32_t)blocks_per_cu * (uint32_t)c->props.cus;
+  EO_TRY_HIP(c->arena.reserve((size_t)blocks * sizeof(uint64_t) + 256), "arena");
+  uint64_t *sink = (uint64_t *)c->arena.base;
+  double ops = 0;
+  EO_TRY_HIP(valu_kernel(kind, blocks, (uint32_t)reps, sink, c->stream, &ops), "warm launch");
+  EO_TRY_HIP(hipEventRecord(c->ev[0], c->stream), "event");
+  for (int i = 0; i < iters; ++i) EO_TRY_HIP(valu_kernel(kind, blocks, (uint32_t)reps, sink, c->stream, &ops), "launch");
+  EO_TRY_HIP(hipEventRecord(c->ev[1], c->stream), "event");
+  EO_TRY_HIP(hipEventSynchronize(c->ev[1]), "event sync");
+  float ms = 0.f;
+  EO_TRY_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
+  *per_s = ops * iters / (ms * 1e-3);
   return EO_OK;
 }
 

@@ -45,7 +45,7 @@ hipError_t fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t stream);
 hipError_t fill_u64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t stream);
 
 // ---- util.hip ---------------------------------------------------------------
-// Streaming 16-byte copy of `bytes` (a multiple of 16) with a grid of 8 WGs per CU.
+// Streaming 16-byte copy of `bytes` (a multiple of 16), one word per thread.
 hipError_t copy_bytes(const void *src, void *dst, uint64_t bytes, int cus, hipStream_t stream);
 // One launch of the bandwidth probe kernel that `o` describes (copy, read or
 // write of o.bytes); sink: a device word array of o.threads entries.
@@ -123,6 +123,9 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t gr
                       const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
                       const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
                       uint32_t *matched, uint32_t *err, hipStream_t stream);
+// One launch of the integer VALU probe `kind` (EO_VALU_COMPARE / EO_VALU_ADD)
+// over `blocks` 256-thread blocks; *ops = the operations it performs.
+hipError_t valu_kernel(int kind, uint32_t blocks, uint32_t reps, uint64_t *sink, hipStream_t s, double *ops);
 // Copies the device error flags into *dst (one u64 word) and clears them, so
 // that one readback returns results and flags together.
 hipError_t take_error_flags(uint32_t *err, uint64_t *dst, hipStream_t stream);

@@ -192,10 +192,12 @@ eo_status eo_bandwidth_probe(eo_ctx *ctx, uint64_t bytes, int iters, double *gbs
  * write-only pass over `bytes` (multiple of 16) with `unroll` 16-byte words
  * in flight per thread (1, 2, 4, 8), `threads` per workgroup (64..1024,
  * 0 = 256), `blocks_per_cu` workgroups per CU (0 = one thread per word, no
- * grid-stride loop) and nontemporal loads / stores.  *gbs = bytes moved (read
+ * grid-stride loop) and nontemporal loads / stores; EO_PROBE_SCATTER is a
+ * copy with run-scattered writes (a radix scatter's write pattern; bytes / 16
+ * / run words must be a power of two).  *gbs = bytes moved (read
  * + write) per second, mean over `iters` launches after one untimed launch.
  * No reference counterpart (the reference reports no bandwidth). */
-enum { EO_PROBE_COPY = 0, EO_PROBE_READ = 1, EO_PROBE_WRITE = 2 };
+enum { EO_PROBE_COPY = 0, EO_PROBE_READ = 1, EO_PROBE_WRITE = 2, EO_PROBE_SCATTER = 3 };
 enum { EO_PROBE_NT_LOAD = 1, EO_PROBE_NT_STORE = 2 };
 typedef struct eo_probe_opts {
   uint32_t struct_size;  /* sizeof(eo_probe_opts) */
@@ -206,9 +208,20 @@ typedef struct eo_probe_opts {
   int32_t blocks_per_cu;
   uint32_t flags;        /* EO_PROBE_NT_LOAD | EO_PROBE_NT_STORE */
   int32_t iters;
-  int32_t reserved;
+  int32_t run_bytes;     /* EO_PROBE_SCATTER: a copy whose writes go in runs of this
+                            many bytes (power of two, 16..65536) to scattered places */
 } eo_probe_opts;
 eo_status eo_bandwidth_probe_ex(eo_ctx *ctx, const eo_probe_opts *opts, double *gbs);
+
+/* Integer VALU issue-rate probe, the ceiling of the nested-loop join's
+ * compare roofline: EO_VALU_COMPARE runs the NLJ's inner loop (keys in
+ * registers against a key tile in LDS, each compare's wave mask
+ * OR-accumulated) with no matches, EO_VALU_ADD independent 32-bit add
+ * chains.  `blocks_per_cu` 256-thread workgroups per CU (1..32), `reps`
+ * passes; *per_s = compares (or adds) per second over `iters` launches after
+ * one untimed launch.  No reference counterpart. */
+enum { EO_VALU_COMPARE = 0, EO_VALU_ADD = 1 };
+eo_status eo_valu_probe(eo_ctx *ctx, int kind, int blocks_per_cu, int reps, int iters, double *per_s);
 
 /* Seeded synthetic relations written straight into device memory (R, S are
  * device pointers), for request streams too large to build on the host.
