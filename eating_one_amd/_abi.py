@@ -222,6 +222,7 @@ EO_INDEX_HASH, EO_INDEX_SORTED = 0, 1
 EO_LOOKUP_DEFAULT, EO_LOOKUP_BINARY, EO_LOOKUP_LEARNED = 0, 1, 2
 EO_F_BINARY_SEARCH = 4
 EO_REF_FILE_RAW = 1
+EO_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -247,6 +248,7 @@ SIGNATURES = {
     "eo_exclusive_scan_u64": (INT, [P, P, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_radix_partition": (INT, [P, P, U64, INT, INT, P, P, ctypes.POINTER(JoinOpts)]),
     "eo_hash_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
+    "eo_hash_join_noalloc": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_nlj_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_nlj_dist_join": (INT, [P, P, U64, P, U64, ctypes.c_double, P, U64, PU64, ctypes.POINTER(JoinOpts)]),
     "eo_stream_join": (INT, [P, P, U64, P, U64, P, U64, PU64, ctypes.POINTER(StreamOpts)]),

@@ -38,6 +38,7 @@ SOURCES = [
     "status.cpp",
     "serve.cpp",
     "serve_queue.cpp",
+    "noalloc.hip",
     "stream.cpp",
 ]
 

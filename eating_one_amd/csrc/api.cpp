@@ -222,6 +222,13 @@ eo_status eo_ctx_create(eo_ctx **out, int device, const eo_ctx_opts *opts) {
     eo_ctx_destroy(c);
     return st;
   }
+  // the device address of the no-allocation entry point's scratch, taken
+  // once per device with the device current
+  eo_status st = noalloc_prepare(device);
+  if (st != EO_OK) {
+    eo_ctx_destroy(c);
+    return st;
+  }
   *out = c;
   return EO_OK;
 }
@@ -522,8 +529,12 @@ eo_status eo_radix_partition(eo_ctx *c, const eo_tuple *in, uint64_t n, int bits
 
 // ---- hash join ----------------------------------------------------------------
 
-eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
-                       uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o) {
+}  // extern "C"
+
+namespace eo {
+
+eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
+                         uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o, const FixedScratch *fx) {
   if (!c || !n_out || !opts_ok(o) || nR > kMaxRows || nS > kMaxRows) return EO_EINVAL;
   if ((nR && !R) || (nS && !S) || (out_cap && !out)) return EO_EINVAL;
   *n_out = 0;
@@ -574,9 +585,19 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
   const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
   const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_span + 1024;
   const size_t o_tc = L.add(tiles_est * sizeof(uint64_t) + scan_scratch_bytes_u64(tiles_est));
-  EO_TRY_HIP(hipStreamSynchronize(s), "sync");
-  EO_TRY_HIP(c->arena.reserve(L.total), "arena");
-  char *A = c->arena.base;
+  char *A = nullptr;
+  if (fx) {  // the caller's fixed scratch: no allocation, no synchronisation
+    if (L.total > fx->cap) {
+      set_last_error("hash join needs " + std::to_string(L.total) + " bytes of scratch, the fixed buffer holds " +
+                     std::to_string(fx->cap));
+      return EO_EINVAL;
+    }
+    A = fx->base;
+  } else {
+    EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+    EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+    A = c->arena.base;
+  }
   const eo_tuple *dR = host ? (eo_tuple *)(A + o_rin) : R;
   const eo_tuple *dS = host ? (eo_tuple *)(A + o_sin) : S;
   uint32_t *r_start = (uint32_t *)(A + o_rs);
@@ -771,6 +792,15 @@ eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple
     tmr.finish();
   }
   return st;
+}
+
+}  // namespace eo
+
+extern "C" {
+
+eo_status eo_hash_join(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
+                       uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o) {
+  return hash_join_impl(c, R, nR, S, nS, out, out_cap, n_out, o, nullptr);
 }
 
 // ---- nested-loop join -----------------------------------------------------------

@@ -121,6 +121,20 @@ void apply_tuned(const eo_ctx *c, uint64_t nR, uint64_t nS, eo_join_opts *o);
 // Folds a launch timer's events into stats->kernels[] (stream synchronised).
 void fold_timer(const LaunchTimer &t, eo_join_stats *stats);
 
+// Device memory the caller provides in place of the context arena.
+struct FixedScratch {
+  char *base;
+  size_t cap;
+};
+// eo_hash_join; with fx, its scratch comes from [fx->base, fx->base + cap)
+// (a join that needs more is refused with EO_EINVAL) and nothing is
+// allocated or synchronised before the kernels are enqueued.
+eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tuple *S, uint64_t nS, eo_pair *out,
+                         uint64_t out_cap, uint64_t *n_out, const eo_join_opts *o, const FixedScratch *fx);
+// Takes the device address of the no-allocation scratch on `device` (the
+// current device) once; called by eo_ctx_create.
+eo_status noalloc_prepare(int device);
+
 // True when [p, p + bytes) lies inside one eo_host_alloc block of ctx.
 bool is_pinned(const eo_ctx *ctx, const void *p, size_t bytes);
 

@@ -269,6 +269,23 @@ eo_status eo_hash_join(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
                        uint64_t out_cap, uint64_t *n_out,
                        const eo_join_opts *opts);
 
+/* Hash join that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that must not call hipMalloc).  Same results as
+ * eo_hash_join with the same arguments; R, S and out are device pointers
+ * (opts->where = EO_DEVICE, required) and EO_F_DETERMINISTIC is refused.
+ * The join's scratch comes from one fixed device buffer of
+ * EO_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends (about
+ * 8 (|R| + |S|) bytes plus small tables: a one-pass join of |R| + |S| up to
+ * about 7.5 million tuples); a join that needs more returns EO_EINVAL with
+ * the sizes in eo_last_error().  All calls on one GPU share the buffer:
+ * calls that overlap on one GPU (several host threads or contexts) race. */
+#define EO_NOALLOC_SCRATCH_BYTES (64ull << 20)
+eo_status eo_hash_join_noalloc(eo_ctx *ctx, const eo_tuple *R, uint64_t nR,
+                               const eo_tuple *S, uint64_t nS, eo_pair *out,
+                               uint64_t out_cap, uint64_t *n_out,
+                               const eo_join_opts *opts);
+
 /* Block nested-loop join: replaces join() of
  * joins/blockJoin_gpu/main_street/nestLoopJoin.cc:180-655 (CPU oracle
  * joins/CPU_JOIN/block_cpu/nestLoopJoin.cc:61-77). */
