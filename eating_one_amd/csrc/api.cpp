@@ -563,6 +563,18 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
     so.join = &eff;
     return eo_stream_join(c, R, nR, S, nS, out, out_cap, n_out, &so);
   }
+  // An inner join in the fused order builds its LDS tables on the smaller
+  // relation and streams the larger one past them (the probe writes each row
+  // with the payloads exchanged back): the partition bits follow the smaller
+  // side, and a side that fits one table is not partitioned at all.
+  const bool swapped = !outer && !(o && (o->flags & EO_F_DETERMINISTIC)) && nS < nR;
+  if (swapped) {
+    std::swap(R, S);
+    std::swap(nR, nS);
+    eff = o ? *o : eo_join_opts{};
+    apply_tuned(c, nR, nS, &eff);
+    if ((st = plan_hash_join(nR, nS, &eff, &hp, c->props.cus)) != EO_OK) return st;
+  }
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
   const uint32_t P = 1u << hp.bits;
@@ -664,7 +676,7 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
     if (outer) EO_TRY_HIP(fill_u32(matched, 0u, matched_words(nS), s), "match bitmap reset");
     clk.mark();  // 2
     hipError_t e = join_fused(P, hp.s_span, cap, join_fused_grid(P, nR, nS, tiles_est, c->props.cus), small, Rp, r_start, Sp,
-                              s_start, tile_start, dout, cap_dev, small + 1, wide, matched, c->d_err, s);
+                              s_start, tile_start, dout, cap_dev, small + 1, wide, matched, c->d_err, s, swapped);
     if (e == hipSuccess && outer) e = emit_unmatched(Sp, nS, matched, small + 1, dout, cap_dev, wide, s);
     clk.mark();  // 3
     uint64_t words[3] = {0, 0, 0};  // tiles, matches, error flags

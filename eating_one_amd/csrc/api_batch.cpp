@@ -46,6 +46,7 @@ struct JoinInfo {
   SideInfo r, s;
   uint64_t bits_word = 0;  // left outer: first word of this join's bitmap in the batch's bitmap region
   uint32_t eblocks = 0;    // left outer: blocks of the null-row kernel
+  bool swapped = false;    // r is the request's S and s its R (built on the smaller relation)
 };
 
 }  // namespace
@@ -123,8 +124,12 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   // S tuples of the whole batch: the probe tiles of all joins share one
   // persistent launch, so each join's S spans are planned against this pool
   // (long spans, few LDS table builds) rather than as if it ran alone
-  uint64_t s_pool = 0;
-  for (uint32_t i = 0; i < n; ++i) s_pool += std::min<uint64_t>(reqs[i].nS, kMaxRows);
+  uint64_t s_pool = 0;  // probe-side tuples (the larger relation of a join built on its smaller one)
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint64_t nr = reqs[i].table ? reqs[i].table->n : reqs[i].nR;
+    const uint64_t np = (!reqs[i].table && !b->outer && reqs[i].nS < nr) ? nr : reqs[i].nS;
+    s_pool += std::min<uint64_t>(np, kMaxRows);
+  }
   for (uint32_t i = 0; i < n; ++i) {
     const eo_request &q = reqs[i];
     const uint64_t nR = q.table ? q.table->n : q.nR;
@@ -134,6 +139,16 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
     if (q.nS == 0 || (nR == 0 && !b->outer)) continue;  // no rows (an outer join has a null row per S tuple)
     JoinInfo J;
     J.req = i;
+    // build side (LDS tables) and probe side: an inner join without a
+    // resident table builds on its smaller relation (the probe exchanges the
+    // payloads of its rows back), so that the partition bits follow it
+    const eo_tuple *bld = q.R, *prb = q.S;
+    uint64_t nB = nR, nP = q.nS;
+    if (!q.table && !b->outer && q.nS < nR) {
+      std::swap(bld, prb);
+      std::swap(nB, nP);
+      J.swapped = true;
+    }
     if (b->outer) {
       J.bits_word = b->bits_words;
       b->bits_words += (matched_words(q.nS) + 63) / 64 * 64;  // 256-byte aligned per join
@@ -150,20 +165,10 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
       if (to.s_tile == 0) J.hp.s_span = plan_span(nR, q.nS, J.hp.lds_cap, J.hp.bits, s_pool, cus);
     } else {
       eo_join_opts to = po;
-      apply_tuned(c, nR, q.nS, &to);
-      eo_status st = plan_hash_join(nR, q.nS, &to, &J.hp, cus);
+      apply_tuned(c, nB, nP, &to);
+      eo_status st = plan_hash_join(nB, nP, &to, &J.hp, cus);
       if (st != EO_OK) return st;
-      // The batch's segmented one-pass scatter slows with every bit from 8 on
-      // (4.2, 3.7, 3.1 TB/s at 8, 9, 10 bits): a join whose S is under a third
-      // of its R takes one bit less, so each partition holds two LDS tables'
-      // worth of R and its S is probed twice (C5 stream: 106 -> 100 ms; single
-      // joins of such shapes gain nothing, so the planner keeps its choice).
-      if (!to.radix_bits && J.hp.passes == 1 && J.hp.bits >= 8 && q.nS * 3 < nR) {
-        to.radix_bits = J.hp.bits - 1;
-        to.passes = 1;
-        if ((st = plan_hash_join(nR, q.nS, &to, &J.hp, cus)) != EO_OK) return st;
-      }
-      if (to.s_tile == 0) J.hp.s_span = plan_span(nR, q.nS, J.hp.lds_cap, J.hp.bits, s_pool, cus);
+      if (to.s_tile == 0) J.hp.s_span = plan_span(nB, nP, J.hp.lds_cap, J.hp.bits, s_pool, cus);
     }
     const uint32_t P = 1u << J.hp.bits;
     auto side = [&](SideInfo &sd, const eo_tuple *in, uint64_t cnt, int which) {
@@ -190,15 +195,15 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
       J.r.table_part = q.table->part;
       J.r.table_start = q.table->start;
     } else {
-      side(J.r, q.R, nR, 0);
+      side(J.r, bld, nB, 0);
     }
-    side(J.s, q.S, q.nS, 1);
+    side(J.s, prb, nP, 1);
     npairs += P;
     if (npairs >= (1ull << 31)) return EO_EINVAL;
-    if (nR && q.nS) {
+    if (nB && nP) {
       // probe tiles if the keys spread evenly, with a spare S span per partition
-      const uint64_t chunks = std::max<uint64_t>(1, (nR / P + kBatchChunk - 1) / kBatchChunk);
-      tiles_est += P * chunks * ((q.nS / P + J.hp.s_span - 1) / J.hp.s_span + 1);
+      const uint64_t chunks = std::max<uint64_t>(1, (nB / P + kBatchChunk - 1) / kBatchChunk);
+      tiles_est += P * chunks * ((nP / P + J.hp.s_span - 1) / J.hp.s_span + 1);
     }
     b->req_join[i] = (int)b->joins.size();
     b->joins.push_back(J);
@@ -209,10 +214,10 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   b->o_joins_blob = 0;
   b->groups.clear();
   uint64_t max_counts = 0;
-  const uint64_t len = batch_stripe_len();
   for (int bits = 1; bits <= kMaxGroupBits; ++bits) {
     auto &mem = group_members[bits];
     if (mem.empty()) continue;
+    const uint64_t len = batch_stripe_len(bits);
     eo_batch::Group G{};
     G.g.bits = bits;
     G.g.nseg = (uint32_t)mem.size();
@@ -302,6 +307,7 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
     bj.wide = b->wide ? 1u : 0u;
     bj.matched = b->outer ? reinterpret_cast<uint32_t *>(A + b->o_bits) + J.bits_word : nullptr;
     bj.nS = J.s.n;
+    bj.swapped = J.swapped ? 1u : 0u;
     pair_base += bj.P;
     hj[j] = bj;
   }
@@ -314,11 +320,11 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
     }
     es[b->joins.size()] = acc;
   }
-  const uint64_t len = batch_stripe_len();
   size_t gi = 0;
   for (int bits = 1; bits <= kMaxGroupBits; ++bits) {
     auto &mem = group_members[bits];
     if (mem.empty()) continue;
+    const uint64_t len = batch_stripe_len(bits);
     eo_batch::Group &G = b->groups[gi++];
     const size_t off = reinterpret_cast<size_t>(G.g.segs);
     BatchSeg *hs = reinterpret_cast<BatchSeg *>(H + off);

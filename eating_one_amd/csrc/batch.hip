@@ -96,26 +96,15 @@ __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__rest
   const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
   const uint64_t base = offsets[sg.count_base];  // tuples of the group's earlier relations
   for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
-    sh.cursor[d] = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes + s] - base);
+    const uint32_t c = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes + s] - base);
+    sh.cursor[d] = c;
     sh.cnt[d] = 0;
+    // the relation's first stripe starts every digit: its offsets are the
+    // partition starts (relative to the relation)
+    if (s == 0) sg.part_start[d] = c;
   }
+  if (s == 0 && threadIdx.x == 0) sg.part_start[1 << BITS] = (uint32_t)sg.n;
   tile::scatter_stripe<BITS>(sh, sg.in, beg, end, sg.n, 0, 0u, sg.out, err);
-}
-
-// part_start[d] of every relation of the group, relative to the relation.
-__global__ void k_bseg_starts(const BatchSeg *__restrict__ segs, uint32_t nseg, uint32_t F,
-                              const uint64_t *__restrict__ offsets) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t per = (uint64_t)F + 1;
-  if (i >= (uint64_t)nseg * per) return;
-  const BatchSeg sg = load_seg(segs, (uint32_t)(i / per));
-  const uint32_t d = (uint32_t)(i % per);
-  if (d == F) {
-    sg.part_start[F] = (uint32_t)sg.n;
-  } else {
-    const uint64_t base = offsets[sg.count_base];
-    sg.part_start[d] = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes] - base);
-  }
 }
 
 // pair_join[pair_base + p] = j for every partition p of join j (one workgroup per join).
@@ -196,6 +185,8 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
 // every ticket below the total that a workgroup takes it also runs.  Wave 0
 // puts the tile's descriptor in LDS: read from the table written by the plan
 // (one load before the tile's data loads) or, past the table, found by search.
+// (A grid of one workgroup per estimated tile, taking only the overflow from
+// the queue, measured slower: C5 batched 99.9 -> 104.9 ms.)
 __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ joins,
                                                  const uint32_t *__restrict__ pair_join,
                                                  const uint32_t *__restrict__ tile_start, uint32_t npairs,
@@ -231,7 +222,7 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
     ref.s_n = uniform_u32(T.s_n);
     if (threadIdx.x == 0) ticket = ticket_next(queue_head);
     tile::probe_tile<tile::ProbeBig>(sh, uniform_ptr(T.R), uniform_ptr(T.S), ref, J.out, J.out_cap, J.cursor,
-                                     J.wide != 0, J.matched);
+                                     J.wide != 0, J.matched, 0, uniform_u32(J.swapped) != 0);
   }
 }
 
@@ -278,16 +269,18 @@ hipError_t launch_group(const BatchGroup &g, uint64_t *counts, void *scan_scratc
   EO_TIMED("batch_scatter", stream,
            hipLaunchKernelGGL(k_bseg_scatter<BITS>, dim3(grid), dim3(kFastNT), 0, stream, g.segs, g.nseg,
                               g.total_stripes, g.stripe_len, counts, err));
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  const uint64_t nst = (uint64_t)g.nseg * ((1u << BITS) + 1);
-  hipLaunchKernelGGL(k_bseg_starts, dim3((uint32_t)((nst + 255) / 256)), dim3(256), 0, stream, g.segs, g.nseg,
-                     1u << BITS, counts);
   return hipGetLastError();
 }
 
 }  // namespace
 
-uint64_t batch_stripe_len() { return 1ull << 16; }
+uint64_t batch_stripe_len(int bits) {
+  // One 16K-tuple tile per partition workgroup up to 8 bits (workgroups
+  // handed out in address order keep the scatter's writes near each other);
+  // longer stripes for more digits, whose counts (2^bits u64 per stripe,
+  // written, scanned and read) would otherwise pass 6 % of the bytes moved.
+  return bits <= 8 ? (uint64_t)tile::kFastTile : bits == 9 ? 2ull * tile::kFastTile : 4ull * tile::kFastTile;
+}
 
 hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
                                  hipStream_t stream) {

@@ -72,7 +72,8 @@ void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
                                                      const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
                                                      uint64_t *out_cursor, int wide, uint32_t *matched,
-                                                     const uint64_t *__restrict__ n_tiles, uint32_t *err) {
+                                                     const uint64_t *__restrict__ n_tiles, uint32_t *err,
+                                                     int swapped) {
   __shared__ tile::ProbeShared<C> sh;
   const uint64_t nt = *n_tiles;
   if (nt >= (1ull << 31)) {  // tile ids are 32-bit: report, probe nothing
@@ -85,7 +86,7 @@ void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
     if (t != blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
     SpanRef ref;
     if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
-    tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched);
+    tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched, 0, swapped != 0);
   }
 }
 
@@ -183,7 +184,7 @@ uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tile
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t grid, const uint64_t *n_tiles_dev,
                       const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
                       const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
-                      uint32_t *matched, uint32_t *err, hipStream_t stream) {
+                      uint32_t *matched, uint32_t *err, hipStream_t stream, bool swapped) {
   if (grid == 0) return hipSuccess;
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
   uint2 *vout = reinterpret_cast<uint2 *>(out);
@@ -191,12 +192,12 @@ hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t gr
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
                                 s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
-                                matched, n_tiles_dev, err));
+                                matched, n_tiles_dev, err, (int)swapped));
   else if (cap == (uint32_t)tile::ProbeBig::Cap)
     EO_TIMED("probe", stream,
              hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
                                 s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
-                                matched, n_tiles_dev, err));
+                                matched, n_tiles_dev, err, (int)swapped));
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

@@ -119,10 +119,12 @@ uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tile
 // probes nothing).
 // wide: out holds 16-byte RESULT rows {R payload, key, S payload, key};
 // matched (optional): bitmap over S positions of the tuples that matched.
+// swapped: R is the caller's S and S its R (the smaller relation built into
+// the tables); the rows are written with the payloads exchanged.
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t grid, const uint64_t *n_tiles_dev,
                       const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
                       const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
-                      uint32_t *matched, uint32_t *err, hipStream_t stream);
+                      uint32_t *matched, uint32_t *err, hipStream_t stream, bool swapped = false);
 // One launch of the integer VALU probe `kind` (EO_VALU_COMPARE / EO_VALU_ADD)
 // over `blocks` 256-thread blocks; *ops = the operations it performs.
 hipError_t valu_kernel(int kind, uint32_t blocks, uint32_t reps, uint64_t *sink, hipStream_t s, double *ops);
@@ -224,6 +226,8 @@ struct BatchJoin {
   uint32_t wide;             // 16-byte RESULT rows
   uint32_t *matched;         // left outer join: bitmap over S positions (zero on entry), or NULL
   uint64_t nS;
+  uint32_t swapped;          // R and S are the request's S and R (rows written with the payloads exchanged)
+  uint32_t pad;
 };
 // One probe tile of a batch, resolved ahead of the probe so that a workgroup
 // starts a tile's loads after a single descriptor read: R chunk (already
@@ -236,7 +240,7 @@ struct BatchTile {
   uint32_t r_n;              // R tuples of the chunk
   uint32_t s_beg, s_n;       // S span
 };
-uint64_t batch_stripe_len();
+uint64_t batch_stripe_len(int bits);  // tuples per partition workgroup of a `bits`-bit group
 hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
                                  hipStream_t stream);
 // Writes pair_join (npairs u32: the join of every (join, partition) pair),

@@ -372,42 +372,106 @@ hipError_t launch_seg_pass(const SegPassPlan &p, const eo_tuple *in, eo_tuple *o
 }
 
 // ---- two-pass MSD partition from one histogram ------------------------------------
-// The first pass's histogram counts the combined digit bin = key & (2^(bh+bl) - 1)
-// = dh * F2 + dl of every sub-stripe.  Summed over dl it gives the first pass's
-// counts; summed over the sub-stripes of stripe g (k_msd_reduce, a transpose,
-// instead of device-scope atomics, which bypass the per-XCD L2) it gives
-// counts2[bin * G + g]: how many tuples of each (high, low) digit pair the
-// first-pass stripe g holds.  The first
-// pass writes stripe g's tuples of high digit dh as one contiguous chunk
-// (dh, g), chunks ordered by g inside segment dh, so after one exclusive scan
-// of counts2 (order dh, dl, g) the tuples of digit pair (dh, dl) that come
-// from chunks g0..g1-1 have one contiguous output range starting at
+// The histogram counts the combined digit bin = key & (2^(bh+bl) - 1) =
+// dh * F2 + dl.  The first pass runs one workgroup per kFastTile-tuple tile
+// (grid = the tile count, no loop over a stripe): a radix scatter writes into
+// every digit's region at once, and with persistent stripe workgroups the
+// writes in flight are spread over every stripe's share of every region,
+// while tiles handed out in address order (consecutive tiles on one XCD)
+// keep them near each other, each region filled from a few moving fronts.
+// Its offsets come from per-tile counts of the high digit dh, which the
+// histogram workgroup of each coarse stripe g (TPS consecutive tiles) writes
+// as it walks its tiles, laid out [dh][tile] so that one exclusive scan gives
+// tile t's run start at off1[dh * T + t].  The same workgroup accumulates the
+// combined-digit histogram of the whole stripe and writes it as one row; a
+// transpose (k_msd_reduce, instead of device-scope atomics, which bypass the
+// per-XCD L2) turns the rows into counts2[bin * G + g].  The first pass writes
+// stripe g's tuples of high digit dh (its tiles' runs) as one contiguous
+// chunk (dh, g), chunks ordered by g inside segment dh, so after one exclusive
+// scan of counts2 (order dh, dl, g) the tuples of digit pair (dh, dl) that
+// come from chunks g0..g1-1 have one contiguous output range starting at
 // off2[(dh * F2 + dl) * G + g0].  The second pass therefore needs no
-// histogram of its own: its stripes are groups of consecutive chunks of one
+// histogram of its own: its work items are groups of consecutive chunks of one
 // segment, sized so that the groups fill the GPU in about one wave.
-template <int TB>
-__global__ __launch_bounds__(kPartNT) void k_msd_hist(const uint2 *__restrict__ in, StripeGeo geo, int bl,
-                                                      uint32_t *__restrict__ counts1,
-                                                      uint32_t *__restrict__ partial) {
-  constexpr uint32_t FT = 1u << TB;
+template <int TB, int BL>
+__global__ __launch_bounds__(kPartNT) void k_msd_hist_tiles(const uint2 *__restrict__ in, uint64_t n, uint32_t G,
+                                                            uint32_t TPS, uint32_t *__restrict__ counts1,
+                                                            uint32_t *__restrict__ partial) {
+  constexpr uint32_t FT = 1u << TB, F1 = FT >> BL;
+  constexpr uint32_t NW = kPartNT / kWave;
+  constexpr uint32_t kPairsPerTile = kFastTile / 2;
+  constexpr uint32_t kGroups = kPairsPerTile / (4 * kPartNT);  // 4-load groups per thread per tile
+  static_assert(kPairsPerTile % (4 * kPartNT) == 0, "a tile is whole 4-load groups");
   __shared__ uint32_t hist[FT];
-  const uint32_t gk = xcd_local_index(blockIdx.x, geo.G * geo.K);
-  const uint32_t g = gk / geo.K, k = gk % geo.K;
-  StripeRef sr;
-  stripe_of<1>(geo, g, k, &sr);
+  __shared__ uint32_t th_w[NW][F1];  // per-wave tile counts of the high digit
+  const uint32_t g = xcd_local_index(blockIdx.x, G);
+  uint32_t *th = th_w[threadIdx.x / kWave];
   for (uint32_t b = threadIdx.x; b < FT; b += kPartNT) hist[b] = 0;
+  for (uint32_t b = threadIdx.x; b < NW * F1; b += kPartNT) (&th_w[0][0])[b] = 0;
   __syncthreads();
-  tile::hist_stripe<TB, kPartNT, true>(hist, in, sr.beg, sr.end, 0, 0u);
-  __syncthreads();
-  const uint32_t F2 = 1u << bl, F1 = FT >> bl;
-  for (uint32_t dh = threadIdx.x; dh < F1; dh += kPartNT) {
-    uint32_t sum = 0;
-    for (uint32_t dl = 0; dl < F2; ++dl) sum += hist[dh * F2 + ((dl + dh) & (F2 - 1))];  // skewed: fewer bank conflicts
-    counts1[((uint64_t)dh * geo.G + g) * geo.K + k] = sum;
+  const uint4 *in4 = reinterpret_cast<const uint4 *>(in);
+  const uint64_t npairs = n / 2;  // complete pairs; an odd last tuple is counted once at the end
+  auto count = [&](uint32_t key) {
+    const uint32_t b = key & (FT - 1);
+    atomicAdd(&hist[b], 1u);
+    atomicAdd(&th[b >> BL], 1u);
+  };
+  // (loads of the next tile issued before each tile's barrier measured no
+  // faster: 0.270 against 0.264 ms per C2 launch)
+  for (uint32_t k = 0; k < TPS; ++k) {
+    const uint64_t t = (uint64_t)g * TPS + k;
+    const uint64_t p0 = t * kPairsPerTile;
+    for (uint32_t j = 0; j < kGroups; ++j) {
+      uint4 q[4];
+      bool ok[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint64_t p = p0 + (uint64_t)(j * 4 + u) * kPartNT + threadIdx.x;
+        ok[u] = p < npairs;
+        q[u] = ok[u] ? ld_stream(in4 + p) : make_uint4(0u, 0u, 0u, 0u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (!ok[u]) continue;
+        count(q[u].x);
+        count(q[u].z);
+      }
+    }
+    // the odd last tuple of the input belongs to the tile that holds it
+    if ((n & 1) && threadIdx.x == 0 && (n - 1) / kFastTile == t) count(in[n - 1].x);
+    __syncthreads();
+    for (uint32_t d = threadIdx.x; d < F1; d += kPartNT) {
+      uint32_t c = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < NW; ++w) {
+        c += th_w[w][d];
+        th_w[w][d] = 0;
+      }
+      counts1[((uint64_t)d * G + g) * TPS + k] = c;
+    }
+    __syncthreads();
   }
-  // the whole combined-digit histogram of this sub-stripe, one coalesced row
-  uint32_t *row = partial + (uint64_t)(g * geo.K + k) * FT;
+  uint32_t *row = partial + (uint64_t)g * FT;
   for (uint32_t b = threadIdx.x; b < FT; b += kPartNT) row[b] = hist[b];
+}
+
+// First pass, one workgroup per tile: tile t's run of digit d starts at
+// offsets[d * T + t] (the exclusive scan of the per-tile counts, digit-major).
+template <int BITS, bool ORDERED>
+__global__ __launch_bounds__(kFastNT) void k_tile_scatter(const uint2 *__restrict__ in, uint64_t n, int shift,
+                                                          const uint32_t *__restrict__ offsets,
+                                                          uint2 *__restrict__ out, uint32_t *err) {
+  __shared__ tile::ScatterShared<BITS, ORDERED> sh;
+  const uint32_t T = gridDim.x;
+  const uint32_t t = xcd_local_index(blockIdx.x, T);  // consecutive tiles on one XCD
+  const uint64_t beg = (uint64_t)t * kFastTile;
+  if (beg >= n) return;
+  const uint64_t end = beg + kFastTile < n ? beg + kFastTile : n;
+  for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
+    sh.cursor[d] = offsets[(uint64_t)d * T + t];
+    sh.cnt[d] = 0;
+  }
+  tile::scatter_stripe<BITS, ORDERED>(sh, in, beg, end, n, shift, 0u, out, err);
 }
 
 // counts2[b * G + g] = sum over k of partial[(g * K + k) * FT + b]: a tiled
@@ -492,17 +556,32 @@ __global__ void k_msd_starts(const uint32_t *__restrict__ off2, uint32_t G, uint
   if (i == FT) part_start[FT] = (uint32_t)n;
 }
 
+// Geometry of the two-pass partition: T tiles of kFastTile tuples (the
+// first pass's workgroups), G coarse stripes of TPS tiles each (the
+// histogram's workgroups, and the second pass's unit of offsets).
+struct Msd2Geo {
+  uint32_t G, TPS, T;
+};
+
+Msd2Geo msd2_geo(uint64_t n, int cus) {
+  const uint64_t tiles = std::max<uint64_t>(1, (n + kFastTile - 1) / kFastTile);
+  uint64_t g = std::min<uint64_t>(tiles, (uint64_t)std::max(cus, kXcds));  // one histogram workgroup per CU
+  g = (g + kXcds - 1) / kXcds * kXcds;
+  const uint64_t tps = (tiles + g - 1) / g;
+  return Msd2Geo{(uint32_t)g, (uint32_t)tps, (uint32_t)(g * tps)};
+}
+
 struct Msd2Layout {
   size_t o_c1, o_c2, o_part, o_grp, o_scan, total;
   uint64_t m1, m2, mp;
 };
 
-Msd2Layout msd2_layout(const PartPassPlan &p1, int bl) {
+Msd2Layout msd2_layout(const Msd2Geo &geo, int bh, int bl) {
   Msd2Layout L{};
-  const uint64_t F1 = 1ull << p1.bits;
-  L.m1 = (uint64_t)p1.stripes * p1.hist_split * F1;
-  L.m2 = (F1 << bl) * p1.stripes;
-  L.mp = L.m2 * p1.hist_split;
+  const uint64_t F1 = 1ull << bh;
+  L.m1 = (uint64_t)geo.T * F1;
+  L.m2 = (F1 << bl) * geo.G;
+  L.mp = L.m2;  // one combined-digit row per coarse stripe
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   L.o_c1 = 0;
   L.o_c2 = L.o_c1 + al(L.m1 * 4);
@@ -515,43 +594,44 @@ Msd2Layout msd2_layout(const PartPassPlan &p1, int bl) {
 }
 
 template <int BH, int BL, bool ORDERED>
-hipError_t launch_msd2(const PartPassPlan &p1, uint64_t group_len, uint32_t workers, const eo_tuple *in, eo_tuple *tmp,
-                       eo_tuple *out, uint32_t *part_start, void *scratch, uint32_t *err, hipStream_t stream) {
+hipError_t launch_msd2(uint64_t n, const Msd2Geo &geo, uint64_t group_len, uint32_t workers, const eo_tuple *in,
+                       eo_tuple *tmp, eo_tuple *out, uint32_t *part_start, void *scratch, uint32_t *err,
+                       hipStream_t stream) {
   constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL);
-  const Msd2Layout L = msd2_layout(p1, BL);
+  const Msd2Layout L = msd2_layout(geo, BH, BL);
   char *sc = (char *)scratch;
   uint32_t *c1 = (uint32_t *)(sc + L.o_c1), *c2 = (uint32_t *)(sc + L.o_c2), *grp = (uint32_t *)(sc + L.o_grp);
   uint32_t *part = (uint32_t *)(sc + L.o_part);
   void *scan_scratch = sc + L.o_scan;
   const uint2 *vin = reinterpret_cast<const uint2 *>(in);
-  const StripeGeo geo{p1.n, p1.stripe_len, p1.stripes, p1.hist_split, nullptr, nullptr, 0};
   hipError_t e;
   EO_TIMED("part_hist", stream,
-           hipLaunchKernelGGL(k_msd_hist<BH + BL>, dim3(p1.stripes * p1.hist_split), dim3(kPartNT), 0, stream, vin,
-                              geo, BL, c1, part));
+           hipLaunchKernelGGL((k_msd_hist_tiles<BH + BL, BL>), dim3(geo.G), dim3(kPartNT), 0, stream, vin, n, geo.G,
+                              geo.TPS, c1, part));
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_msd_reduce, dim3((FT + 63) / 64, (p1.stripes + 63) / 64), dim3(256), 0, stream, part,
-                     p1.stripes, p1.hist_split, FT, c2);
+  hipLaunchKernelGGL(k_msd_reduce, dim3((FT + 63) / 64, (geo.G + 63) / 64), dim3(256), 0, stream, part, geo.G, 1u,
+                     FT, c2);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = scan_u32(c1, c1, L.m1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
   if ((e = scan_u32(c2, c2, L.m2, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
   EO_TIMED("part_scatter", stream,
-           hipLaunchKernelGGL((k_part_scatter_fast<BH, ORDERED>), dim3(p1.stripes), dim3(kFastNT), 0, stream, vin, geo, BL, 0u,
-                              c1, reinterpret_cast<uint2 *>(tmp), err));
+           hipLaunchKernelGGL((k_tile_scatter<BH, ORDERED>), dim3(geo.T), dim3(kFastNT), 0, stream, vin, n, BL, c1,
+                              reinterpret_cast<uint2 *>(tmp), err));
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_msd_groups, dim3((F1 + 1 + 255) / 256), dim3(256), 0, stream, c1, p1.stripes, p1.hist_split, F1,
-                     p1.n, group_len, grp);
+  // the second pass's chunks are (high digit, coarse stripe): its first tile's
+  // run starts the chunk, so the stripe's tiles act as its sub-stripes
+  hipLaunchKernelGGL(k_msd_groups, dim3((F1 + 1 + 255) / 256), dim3(256), 0, stream, c1, geo.G, geo.TPS, F1, n,
+                     group_len, grp);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = scan_u32(grp, grp, F1 + 1, nullptr, scan_scratch, err, stream)) != hipSuccess) return e;
-  const uint64_t max_groups = F1 + p1.n / group_len + 1;
+  const uint64_t max_groups = F1 + n / group_len + 1;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(max_groups, std::max<uint32_t>(workers, 1));
   EO_TIMED("seg_scatter", stream,
            hipLaunchKernelGGL((k_msd_scatter<BL, ORDERED>), dim3(grid), dim3(kFastNT), 0, stream,
-                              reinterpret_cast<const uint2 *>(tmp), p1.n, c1, p1.stripes, p1.hist_split, F1, grp,
-                              grp + F1 + 1, c2, reinterpret_cast<uint2 *>(out), err));
+                              reinterpret_cast<const uint2 *>(tmp), n, c1, geo.G, geo.TPS, F1, grp, grp + F1 + 1,
+                              c2, reinterpret_cast<uint2 *>(out), err));
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_msd_starts, dim3((FT + 1 + 255) / 256), dim3(256), 0, stream, c2, p1.stripes, FT, p1.n,
-                     part_start);
+  hipLaunchKernelGGL(k_msd_starts, dim3((FT + 1 + 255) / 256), dim3(256), 0, stream, c2, geo.G, FT, n, part_start);
   return hipGetLastError();
 }
 
@@ -672,17 +752,6 @@ bool msd2_supported(int bits_hi, int bits_lo) {
   return bits_hi >= 1 && bits_lo >= 1 && bits_hi <= 7 && bits_lo <= 7 && bits_hi + bits_lo <= kMsd2MaxBits;
 }
 
-static PartPassPlan msd2_pass1(uint64_t n, int bh, int bl, int cus) {
-  // The combined histogram writes one 2^(bh+bl)-bin row per sub-stripe
-  // (64 KB at 14 bits), and a transpose sums the rows: only as many
-  // sub-stripes as keep one histogram workgroup per CU (C2: one per stripe;
-  // the 64 MB of rows that four per stripe wrote cost 0.12 ms per join).
-  PartPassPlan p = plan_partition_pass(n, bh, bl, 0u, cus, true);
-  const uint32_t per_cu = (uint32_t)((std::max(cus, 1) + p.stripes - 1) / p.stripes);
-  p.hist_split = std::max<uint32_t>(1, std::min(p.hist_split, per_cu));
-  return p;
-}
-
 uint64_t msd2_group_len(uint64_t n, int cus) {
   // kMsdGroupsPerWg groups per persistent second-pass workgroup (kFastWgPerCu
   // per CU): small enough that the dynamic hand-out evens out segments of
@@ -692,7 +761,7 @@ uint64_t msd2_group_len(uint64_t n, int cus) {
 }
 
 size_t msd2_scratch_bytes(uint64_t n, int bh, int bl, int cus) {
-  return msd2_layout(msd2_pass1(n, bh, bl, cus), bl).total;
+  return msd2_layout(msd2_geo(n, cus), bh, bl).total;
 }
 
 hipError_t partition_msd2(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
@@ -700,14 +769,14 @@ hipError_t partition_msd2(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tup
                           bool ordered) {
   if (!msd2_supported(bh, bl)) return hipErrorInvalidValue;
   if (((uintptr_t)in & 15) || ((uintptr_t)tmp & 15) || ((uintptr_t)out & 7)) return hipErrorInvalidValue;
-  const PartPassPlan p1 = msd2_pass1(n, bh, bl, cus);
+  const Msd2Geo geo = msd2_geo(n, cus);
   const uint64_t L = msd2_group_len(n, cus);
   const uint32_t workers = (uint32_t)std::max(cus, 8) * tile::kFastWgPerCu;
   switch (bh * 16 + bl) {
 #define EO_MSD_CASE(BH, BL) \
   case BH * 16 + BL:        \
-    return ordered ? launch_msd2<BH, BL, true>(p1, L, workers, in, tmp, out, part_start, scratch, err, stream) \
-                   : launch_msd2<BH, BL, false>(p1, L, workers, in, tmp, out, part_start, scratch, err, stream);
+    return ordered ? launch_msd2<BH, BL, true>(n, geo, L, workers, in, tmp, out, part_start, scratch, err, stream) \
+                   : launch_msd2<BH, BL, false>(n, geo, L, workers, in, tmp, out, part_start, scratch, err, stream);
 #define EO_MSD_ROW(BH) \
   EO_MSD_CASE(BH, 1) EO_MSD_CASE(BH, 2) EO_MSD_CASE(BH, 3) EO_MSD_CASE(BH, 4) EO_MSD_CASE(BH, 5) EO_MSD_CASE(BH, 6) \
   EO_MSD_CASE(BH, 7)

@@ -65,7 +65,7 @@ uint64_t request_bytes(const eo_request &q, int cus) {
   b += sizeof(BatchTile) * P * std::max<uint64_t>(1, (nR / P + kBatchChunk - 1) / kBatchChunk) *
        ((q.nS / P + hp.s_span - 1) / hp.s_span + 1);
   if (hp.bits == 0) return b;
-  const uint64_t len = batch_stripe_len();
+  const uint64_t len = batch_stripe_len(hp.bits);
   b += 8 * q.nS + 4 * (P + 1) + 8 * (((q.nS + len - 1) / len) << hp.bits) + 8 * P;
   if (!q.table) b += 8 * nR + 4 * (P + 1) + 8 * (((nR + len - 1) / len) << hp.bits);
   if (hp.passes == 2) b += 8 * std::max(nR, q.nS);

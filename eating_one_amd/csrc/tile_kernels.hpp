@@ -63,13 +63,16 @@ constexpr int kProbeOrdered = 2;  // write rows from position base on, in an ord
 // nothing and returns the number of matches.  Positions >= out_cap are
 // counted only.  wide: 16-byte RESULT rows instead of pairs (see emit_pair).
 // matched (optional bitmap over S positions): sets the bit of every S tuple
-// that matched, for a left outer join.
+// that matched, for a left outer join.  swapped: the table side is the
+// caller's S and the streamed side its R (an inner join built on its smaller
+// relation), so every row is written with the two payloads exchanged.
 // All threads of the workgroup call it; it leaves LDS free for reuse.
 template <typename C, int MODE = kProbeFused>
 __device__ __forceinline__ uint64_t probe_tile(ProbeShared<C> &sh, const uint2 *__restrict__ R,
                                                const uint2 *__restrict__ S, const SpanRef &ref,
                                                uint2 *__restrict__ out, uint64_t out_cap, uint64_t *cursor,
-                                               bool wide = false, uint32_t *matched = nullptr, uint64_t base = 0) {
+                                               bool wide = false, uint32_t *matched = nullptr, uint64_t base = 0,
+                                               bool swapped = false) {
   constexpr int NW = C::NT / kWave;
   auto load_batch = [&](uint32_t b0, uint2 *dst) {
 #pragma unroll
@@ -243,13 +246,19 @@ __device__ __forceinline__ uint64_t probe_tile(ProbeShared<C> &sh, const uint2 *
       if (m[u] == 0) continue;
       uint64_t pos = base + sh.wtot[par][u * NW + w] + wex[u];
       if (m[u] == 1) {
-        if (pos < out_cap) emit_pair<true>(out, pos, fp[u], cur[u].y, cur[u].x, wide);
+        if (pos < out_cap) {
+          if (swapped) emit_pair<true>(out, pos, cur[u].y, fp[u], cur[u].x, wide);
+          else emit_pair<true>(out, pos, fp[u], cur[u].y, cur[u].x, wide);
+        }
       } else {
         const uint32_t h = bucket_hash((int32_t)cur[u].x, C::Log2NB);
         const uint32_t e1 = sh.off[h + 1];
         for (uint32_t e = sh.off[h]; e < e1; ++e) {
           if (sh.keys[e] == cur[u].x) {
-            if (pos < out_cap) emit_pair<true>(out, pos, sh.pays[e], cur[u].y, cur[u].x, wide);
+            if (pos < out_cap) {
+              if (swapped) emit_pair<true>(out, pos, cur[u].y, sh.pays[e], cur[u].x, wide);
+              else emit_pair<true>(out, pos, sh.pays[e], cur[u].y, cur[u].x, wide);
+            }
             ++pos;
           }
         }

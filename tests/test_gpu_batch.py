@@ -100,11 +100,11 @@ def test_batch_hot_partition_tiles_beyond_descriptors(ctx):
     b.free()
 
 
-def test_batch_small_probe_sides_take_one_bit_less(ctx):
-    """Joins whose S is under a third of R and whose R needs a one-pass
-    partition of 8+ bits are partitioned one bit coarser in a batch (each
-    partition then spans two LDS tables); rows equal the oracle's and the
-    solo join's, beside ordinary joins in the same batch."""
+def test_batch_small_probe_sides_build_on_s(ctx):
+    """Joins whose S is smaller than R build their LDS tables on S in a batch
+    (and the partition bits follow S); rows come out as (R payload, S
+    payload) and equal the oracle's and the solo join's, beside ordinary
+    joins in the same batch."""
     inputs = [gen.c2(seed=301, nR=(1 << 22) + 77, nS=1 << 18, selectivity=0.7),
               gen.c2(seed=302, nR=3 << 21, nS=(1 << 20) + 5, selectivity=1.0),
               gen.c2(seed=303, nR=1 << 21, nS=1 << 21, selectivity=0.5)]
@@ -258,7 +258,8 @@ def test_batch_stats_and_footprint(ctx):
     q = [r.request(tag=i) for i, r in enumerate(reqs)]
     b = ctx.batch()
     fp = b.footprint(q)
-    assert fp > sum(8 * (len(r.S)) for r in reqs if len(r.R) > 16384)
+    # a join whose smaller side passes one LDS table partitions both relations
+    assert fp > sum(8 * (len(r.R) + len(r.S)) for r in reqs if min(len(r.R), len(r.S)) > 16384)
     st = eo.JoinStats()
     res = b.run(q, stats=st)
     assert st.n_out == sum(n for _, n, _ in res)

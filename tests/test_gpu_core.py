@@ -386,3 +386,33 @@ def test_two_pass_hot_key(ctx, hot):
     # every S tuple matches the single R tuple with its key (R payload = row)
     assert np.array_equal(np.sort(got[:, 1]), np.arange(nS))
     assert np.array_equal(R[got[:, 0], 0], S[got[:, 1], 0])
+
+
+@pytest.mark.parametrize("nR,nS,key_range", [(1 << 20, 5000, 1 << 20), (300000, 20000, 1 << 14), (200000, 199999, 1 << 16),
+                                           (1 << 22, 1 << 18, 1 << 22)])
+def test_hash_join_builds_on_the_smaller_side(ctx, nR, nS, key_range):
+    """An inner join with |S| < |R| builds its tables on S and streams R past
+    them; the rows still come out as (R payload, S payload), in every row
+    layout, and equal the oracle (also count only and on capacity overflow)."""
+    R, S = gen.uniform(seed=nR % 977 + nS, nR=nR, nS=nS, key_range=key_range)
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    st = eo.JoinStats()
+    got = ctx.hash_join(R, S, stats=st)
+    assert same_multiset(got, want)
+    # the plan follows the smaller relation (a side of one LDS table is not partitioned)
+    if nS <= 8192:
+        assert st.radix_bits == 0
+    wide = ctx.hash_join(R, S, result16=True)
+    assert len(wide) == len(want)
+    assert same_multiset(wide[:, [0, 2]], want)
+    assert np.array_equal(wide[:, 1], wide[:, 3])
+    assert np.array_equal(R[wide[:, 0], 0], wide[:, 1]) and np.array_equal(S[wide[:, 2], 0], wide[:, 3])
+    assert ctx.hash_join(R, S, count_only=True) == len(want)
+    # a deterministic or left outer join keeps R as the build side: same rows
+    assert same_multiset(ctx.hash_join(R, S, deterministic=True), want)
+    dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
+    small = eo.DeviceArray(ctx, 8 * 100)
+    n, status = ctx.hash_join_device(dR, nR, dS, nS, small, 100)
+    assert status == eo._abi.EO_ECAPACITY and n == len(want)
+    rows = small.to_host(np.int32, (100, 2))
+    assert np.array_equal(R[rows[:, 0], 0], S[rows[:, 1], 0])
