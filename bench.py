@@ -675,6 +675,7 @@ def run_c5(a):
         stream.free()
         ctx.close()
         return {"metric": "C5 join stream, one mode", "closed_loop": out}
+    alg_bytes = 24 * tuples + 8 * int(sum(counts))
     capacity_qps = n_req / (out["batched"]["wall_ms"] * 1e-3)
     qps = a.c5_qps if a.c5_qps else round(0.7 * capacity_qps, -2)
     arrival = np.cumsum(rng.exponential(1000.0 / qps, size=n_req))
@@ -694,6 +695,10 @@ def run_c5(a):
                       "requests": n_req, "tuples": tuples, "n_out": int(sum(counts)), "max_batch": a.c5_max_batch,
                       "slots": a.c5_slots},
            "closed_loop": out, "speedup_batched_vs_per_join": out["per_join"]["wall_ms"] / out["batched"]["wall_ms"],
+           # the C2 formula per request (read, write and re-read both inputs, write the pairs)
+           "roofline": {"bound": "hbm", "alg_bytes": alg_bytes,
+                        "achieved": alg_bytes / (out["batched"]["wall_ms"] * 1e-3) / 1e9, "peak": 8000.0,
+                        "unit": "GB/s", "frac": alg_bytes / (out["batched"]["wall_ms"] * 1e-3) / 8e12},
            "poisson": {"qps": qps, **{k: v for k, v in lat_res.items()}}}
     stream.free()
     res["graph"] = graph_leg(ctx, a)
@@ -853,7 +858,7 @@ def main(argv=None) -> int:
                           "qps": c5["poisson"]["qps"], "p50_ms": c5["poisson"]["batched"]["p50_ms"],
                           "p99_ms": c5["poisson"]["batched"]["p99_ms"],
                           "per_join_p99_ms": c5["poisson"]["per_join"]["p99_ms"],
-                          "graph": c5["graph"]}
+                          "roofline": c5["roofline"], "graph": c5["graph"]}
         print(json.dumps(res), flush=True)
         return 0
     fn = {"c1": run_c1, "c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,

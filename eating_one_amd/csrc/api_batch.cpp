@@ -166,6 +166,8 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
     } else {
       eo_join_opts to = po;
       apply_tuned(c, nB, nP, &to);
+      // partitions sized for the batch probe's half-size LDS table
+      if (!to.lds_tuples) to.lds_tuples = 8192;
       eo_status st = plan_hash_join(nB, nP, &to, &J.hp, cus);
       if (st != EO_OK) return st;
       if (to.s_tile == 0) J.hp.s_span = plan_span(nB, nP, J.hp.lds_cap, J.hp.bits, s_pool, cus);
@@ -408,7 +410,12 @@ eo_status eo_table_create(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_jo
   *out = nullptr;
   DeviceGuard g(c->device);
   HashPlan hp;
-  eo_status st = plan_hash_join(nR, nR, o, &hp, c->props.cus);
+  // partitions sized for the batch probe's half-size LDS table, unless the
+  // caller chose a table size
+  eo_join_opts to{};
+  if (o) to = *o;
+  if (!to.lds_tuples) to.lds_tuples = 8192;
+  eo_status st = plan_hash_join(nR, nR, &to, &hp, c->props.cus);
   if (st != EO_OK) return st;
   hipStream_t s = pick_stream(c, o);
   const bool host = where_of(o) == EO_HOST;

@@ -27,12 +27,13 @@ namespace eo {
 namespace {
 
 using tile::kFastNT;
-using tile::kFCap;
+using BProbe = tile::ProbeSmall;  // the batch probe's LDS table: two workgroups per CU
+constexpr uint32_t kFCap = BProbe::Cap;
 using tile::kFNT;
 using tile::SpanRef;
 
 constexpr int kBHistNT = 512;
-static_assert(kFCap == (int)kBatchChunk, "batch tile chunk is the probe table's capacity");
+static_assert(kFCap == kBatchChunk, "batch tile chunk is the probe table's capacity");
 
 __device__ __forceinline__ uint32_t find_seg(const BatchSeg *segs, uint32_t nseg, uint32_t g) {
   uint32_t lo = 0, hi = nseg;  // segs[lo].stripe_base <= g < segs[hi].stripe_base
@@ -187,12 +188,12 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
 // (one load before the tile's data loads) or, past the table, found by search.
 // (A grid of one workgroup per estimated tile, taking only the overflow from
 // the queue, measured slower: C5 batched 99.9 -> 104.9 ms.)
-__global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ joins,
+__global__ __launch_bounds__(kFNT) __attribute__((amdgpu_waves_per_eu(8))) void k_bprobe(const BatchJoin *__restrict__ joins,
                                                  const uint32_t *__restrict__ pair_join,
                                                  const uint32_t *__restrict__ tile_start, uint32_t npairs,
                                                  const BatchTile *__restrict__ tiles, uint32_t tiles_cap,
                                                  uint32_t *queue_head) {
-  __shared__ tile::ProbeShared<tile::ProbeBig> sh;
+  __shared__ tile::ProbeShared<BProbe> sh;
   __shared__ BatchTile cur;
   __shared__ uint32_t item;
   const uint32_t total = tile_start[npairs];
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(kFNT) void k_bprobe(const BatchJoin *__restrict__ j
     ref.s_beg = uniform_u32(T.s_beg);
     ref.s_n = uniform_u32(T.s_n);
     if (threadIdx.x == 0) ticket = ticket_next(queue_head);
-    tile::probe_tile<tile::ProbeBig>(sh, uniform_ptr(T.R), uniform_ptr(T.S), ref, J.out, J.out_cap, J.cursor,
+    tile::probe_tile<BProbe>(sh, uniform_ptr(T.R), uniform_ptr(T.S), ref, J.out, J.out_cap, J.cursor,
                                      J.wide != 0, J.matched, 0, uniform_u32(J.swapped) != 0);
   }
 }
@@ -327,8 +328,8 @@ hipError_t batch_probe(const BatchJoin *joins, const uint32_t *pair_join, const 
                        hipStream_t stream) {
   if (npairs == 0) return hipSuccess;
   EO_TIMED("batch_probe", stream,
-           hipLaunchKernelGGL(k_bprobe, dim3((uint32_t)std::max(cus, 1)), dim3(kFNT), 0, stream, joins, pair_join,
-                              tile_start, npairs, tiles, tiles_cap, queue_head));
+           hipLaunchKernelGGL(k_bprobe, dim3(2u * (uint32_t)std::max(cus, 1)), dim3(kFNT), 0, stream, joins,
+                              pair_join, tile_start, npairs, tiles, tiles_cap, queue_head));
   return hipGetLastError();
 }
 

@@ -232,7 +232,9 @@ struct BatchJoin {
 // One probe tile of a batch, resolved ahead of the probe so that a workgroup
 // starts a tile's loads after a single descriptor read: R chunk (already
 // offset), the join's S and output, and the S span.
-constexpr uint32_t kBatchChunk = 17408;  // R tuples per probe tile: the batch probe's LDS table
+// R tuples per probe tile: the batch probe's LDS table (the half-size table,
+// two 1024-thread workgroups per CU)
+constexpr uint32_t kBatchChunk = 8704;
 struct BatchTile {
   const uint2 *R;            // first tuple of the R chunk
   const uint2 *S;            // the join's S (partitioned)

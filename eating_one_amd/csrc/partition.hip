@@ -528,6 +528,8 @@ __global__ __launch_bounds__(kFastNT) void k_msd_scatter(const uint2 *__restrict
   __shared__ tile::ScatterShared<BL, ORDERED> sh;
   __shared__ uint32_t s_w;
   const uint32_t total = grp_base[F1];
+  // (one group per workgroup, consecutive groups on one XCD, measured slower:
+  // 0.738 -> 0.772 ms per C2 launch)
   for (;;) {
     if (threadIdx.x == 0) s_w = atomicAdd(ticket, 1u);
     __syncthreads();
