@@ -54,6 +54,8 @@ def kernel_alg_bytes(name, nR, nS, n_out):
     per_launch_tuples = (nR + nS) / 2  # each family runs once over R and once over S
     if name in ("part_hist", "seg_hist"):
         return 8 * per_launch_tuples  # read every tuple
+    if name == "part_sample":
+        return 8 * per_launch_tuples / 16  # one 128-byte line in 16
     if name in ("part_scatter", "seg_scatter", "part_scatter_stable"):
         return 16 * per_launch_tuples  # read + write every tuple
     if name == "probe":
@@ -232,7 +234,10 @@ def run_c2(a):
                 "inputs resident in HBM",
         "config": {"workload": f"C2 radix hash join |R|=2^{a.log2_r} |S|=2^{a.log2_r + 1}, theta={a.theta}, "
                                f"selectivity={a.selectivity}",
-                   "radix_bits": int(stats.radix_bits), "passes": int(stats.passes), "n_out": int(n_out),
+                   "radix_bits": int(stats.radix_bits), "passes": int(stats.passes),
+                   "partitions": ("exact (histogram pass)", "sized from a 1/16 sample",
+                                  "sample too small, redone exactly")[min(int(stats.sampled), 2)],
+                   "n_out": int(n_out),
                    "device": info["arch"], "cus": info["cus"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,

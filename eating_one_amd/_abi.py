@@ -20,6 +20,7 @@ EO_HOST, EO_DEVICE = 0, 1
 EO_F_NO_SYNC_STATS, EO_F_DETERMINISTIC = 1, 2
 EO_F_RESULT16 = 8
 EO_F_LEFT_OUTER = 16
+EO_F_EXACT_PARTITIONS = 32
 EO_NULL_PAYLOAD = -(1 << 31)
 
 
@@ -43,7 +44,7 @@ class JoinStats(ctypes.Structure):
         ("tiles", U32),
         ("lds_tuples", U32),
         ("n_kernels", U32),
-        ("reserved", U32),
+        ("sampled", U32),
         ("kernels", KernelStat * EO_MAX_KERNEL_STATS),
     ]
 

@@ -538,6 +538,8 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
   if (!c || !n_out || !opts_ok(o) || nR > kMaxRows || nS > kMaxRows) return EO_EINVAL;
   if ((nR && !R) || (nS && !S) || (out_cap && !out)) return EO_EINVAL;
   *n_out = 0;
+  const eo_tuple *const R0 = R, *const S0 = S;  // the caller's sides, for a redo with exact partitions
+  const uint64_t nR0 = nR, nS0 = nS;
   HashPlan hp;
   eo_join_opts eff{};
   if (o) eff = *o;
@@ -578,22 +580,37 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
   DeviceGuard g(c->device);
   hipStream_t s = pick_stream(c, o);
   const uint32_t P = 1u << hp.bits;
+  // Fused inner joins of two passes size the partitions of each large enough
+  // relation from a sample instead of a histogram pass (gapped partitions,
+  // [start, end) each); the fixed-scratch entry point keeps the exact ones.
+  const bool deterministic = o && (o->flags & EO_F_DETERMINISTIC);
+  const bool gap_try = !fx && !outer && !deterministic && hp.passes == 2 && !(o && (o->flags & EO_F_EXACT_PARTITIONS));
+  const bool gapR = gap_try && gap_supported(nR, hp.bits_pass2, hp.bits_pass1);
+  const bool gapS = gap_try && gap_supported(nS, hp.bits_pass2, hp.bits_pass1);
+  const uint64_t rp_rows = gapR ? gap_out_rows(nR, hp.bits_pass2, hp.bits_pass1) : nR;
+  const uint64_t sp_rows = gapS ? gap_out_rows(nS, hp.bits_pass2, hp.bits_pass1) : nS;
+  const uint64_t tmp_rows = std::max(gapR ? gap_tmp_rows(nR, hp.bits_pass2, hp.bits_pass1) : nR,
+                                     gapS ? gap_tmp_rows(nS, hp.bits_pass2, hp.bits_pass1) : nS);
 
   Layout L;
   const size_t o_rin = L.add(host ? nR * sizeof(eo_tuple) : 0);
   const size_t o_sin = L.add(host ? nS * sizeof(eo_tuple) : 0);
-  const size_t o_rp = L.add(hp.passes ? nR * sizeof(eo_tuple) : 0);
-  const size_t o_sp = L.add(hp.passes ? nS * sizeof(eo_tuple) : 0);
-  const size_t o_tmp = L.add(hp.passes == 2 ? std::max(nR, nS) * sizeof(eo_tuple) : 0);
+  const size_t o_rp = L.add(hp.passes ? rp_rows * sizeof(eo_tuple) : 0);
+  const size_t o_sp = L.add(hp.passes ? sp_rows * sizeof(eo_tuple) : 0);
+  const size_t o_tmp = L.add(hp.passes == 2 ? tmp_rows * sizeof(eo_tuple) : 0);
   const size_t o_rs = L.add((P + 1) * sizeof(uint32_t));
   const size_t o_ss = L.add((P + 1) * sizeof(uint32_t));
+  const size_t o_re = L.add(gapR ? P * sizeof(uint32_t) : 0);
+  const size_t o_se = L.add(gapS ? P * sizeof(uint32_t) : 0);
   const size_t o_ts = L.add((P + 1) * sizeof(uint32_t));
   const size_t o_small = L.add(8 * sizeof(uint64_t));
   const size_t o_bits = L.add(outer ? matched_words(nS) * sizeof(uint32_t) : 0);
   const uint64_t ublocks = outer ? unmatched_blocks(nS) : 0;  // deterministic outer rows: block offsets
   const size_t o_ub = L.add(ublocks * sizeof(uint64_t));
   const size_t o_ubs = L.add(ublocks ? scan_scratch_bytes_u64(ublocks) : 0);
-  const size_t o_pscr = L.add(std::max(partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus)));
+  const size_t o_pscr = L.add(std::max({partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus),
+                                        gapR ? gap_scratch_bytes(nR, hp.bits_pass2, hp.bits_pass1) : (size_t)0,
+                                        gapS ? gap_scratch_bytes(nS, hp.bits_pass2, hp.bits_pass1) : (size_t)0}));
   const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
   const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_span + 1024;
   const size_t o_tc = L.add(tiles_est * sizeof(uint64_t) + scan_scratch_bytes_u64(tiles_est));
@@ -614,6 +631,7 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
   const eo_tuple *dS = host ? (eo_tuple *)(A + o_sin) : S;
   uint32_t *r_start = (uint32_t *)(A + o_rs);
   uint32_t *s_start = (uint32_t *)(A + o_ss);
+  const uint32_t *r_end = r_start + 1, *s_end = s_start + 1;  // contiguous partitions
   uint32_t *tile_start = (uint32_t *)(A + o_ts);
   uint64_t *small = (uint64_t *)(A + o_small);
   if (host) {
@@ -625,17 +643,29 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
   ScopedTimer tmr(c, stats);
   clk.mark();  // 0
   const eo_tuple *Rp = dR, *Sp = dS;
+  bool used_gap = false;
   if (hp.passes) {
     eo_tuple *rp = (eo_tuple *)(A + o_rp), *sp = (eo_tuple *)(A + o_sp), *tmp = (eo_tuple *)(A + o_tmp);
     // deterministic order needs partitions whose positions the inputs fix
     // (the fast passes rank by atomics): the ordered two-pass partition, or
     // stable passes, which a deterministic left outer join takes so that its
     // null rows come in S order within each partition
-    const bool det = o && (o->flags & EO_F_DETERMINISTIC);
-    const bool stable = det && outer, fixed = det && !outer;
-    st = partition_relation(c, dR, nR, hp, tmp, rp, r_start, A + o_pscr, s, stable, fixed);
+    const bool stable = deterministic && outer, fixed = deterministic && !outer;
+    auto part = [&](const eo_tuple *in, uint64_t n, bool gap, eo_tuple *dst, uint32_t *start, uint32_t *end,
+                    const uint32_t **end_out) -> eo_status {
+      if (gap && aligned16(in) && aligned16(tmp) && aligned16(dst)) {
+        EO_TRY_HIP(partition_gapped(n, hp.bits_pass2, hp.bits_pass1, in, tmp, dst, start, end, A + o_pscr, c->d_err,
+                                    c->props.cus, s),
+                   "partition (sampled regions)");
+        *end_out = end;
+        used_gap = true;
+        return EO_OK;
+      }
+      return partition_relation(c, in, n, hp, tmp, dst, start, A + o_pscr, s, stable, fixed);
+    };
+    st = part(dR, nR, gapR, rp, r_start, (uint32_t *)(A + o_re), &r_end);
     if (st != EO_OK) return st;
-    st = partition_relation(c, dS, nS, hp, tmp, sp, s_start, A + o_pscr, s, stable, fixed);
+    st = part(dS, nS, gapS, sp, s_start, (uint32_t *)(A + o_se), &s_end);
     if (st != EO_OK) return st;
     Rp = rp;
     Sp = sp;
@@ -647,15 +677,16 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
   }
   clk.mark();  // 1
 
-  const bool deterministic = o && (o->flags & EO_F_DETERMINISTIC);
   // both orders use the fused probe's tiles: (partition, R chunk, S span)
   const uint32_t cap = join_fused_cap(hp.lds_cap);
-  EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap, r_start, s_start, tile_start, small, A + o_jscr, c->d_err, s),
+  EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap, r_start, r_end, s_start, s_end, tile_start, small, A + o_jscr,
+                             c->d_err, s),
              "tile plan");
   if (stats) {
     stats->radix_bits = (uint32_t)hp.bits;
     stats->passes = (uint32_t)hp.passes;
     stats->lds_tuples = hp.lds_cap;
+    stats->sampled = used_gap ? 1u : 0u;
   }
   eo_pair *dout = out;
   bool staged = false;
@@ -675,8 +706,9 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
     uint32_t *matched = outer ? (uint32_t *)(A + o_bits) : nullptr;
     if (outer) EO_TRY_HIP(fill_u32(matched, 0u, matched_words(nS), s), "match bitmap reset");
     clk.mark();  // 2
-    hipError_t e = join_fused(P, hp.s_span, cap, join_fused_grid(P, nR, nS, tiles_est, c->props.cus), small, Rp, r_start, Sp,
-                              s_start, tile_start, dout, cap_dev, small + 1, wide, matched, c->d_err, s, swapped);
+    hipError_t e = join_fused(P, hp.s_span, cap, join_fused_grid(P, nR, nS, tiles_est, c->props.cus), small, Rp,
+                              r_start, r_end, Sp, s_start, s_end, tile_start, dout, cap_dev, small + 1, wide, matched,
+                              c->d_err, s, swapped);
     if (e == hipSuccess && outer) e = emit_unmatched(Sp, nS, matched, small + 1, dout, cap_dev, wide, s);
     clk.mark();  // 3
     uint64_t words[3] = {0, 0, 0};  // tiles, matches, error flags
@@ -684,6 +716,18 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
     if (e == hipSuccess) {
       st = read_u64(c, small, words, 3, s);
       total = words[1];
+      if (st == EO_OK && used_gap && words[2] == kErrCapacity) {
+        // a sampled region was too small: the rows are incomplete, so the
+        // join runs again from the caller's inputs with exact partitions
+        if (staged) (void)hipFreeAsync(dout, s);
+        eo_join_opts ex = o ? *o : eo_join_opts{};
+        if (!o) ex.where = EO_HOST;
+        ex.struct_size = sizeof(ex);
+        ex.flags |= EO_F_EXACT_PARTITIONS;
+        st = hash_join_impl(c, R0, nR0, S0, nS0, out, out_cap, n_out, &ex, fx);
+        if (stats) stats->sampled = 2u;
+        return st;
+      }
       if (st == EO_OK && !words[2] && staged && total) {  // on overflow: the out_cap rows that were written
         e = hipMemcpyAsync(out, dout, std::min(total, out_cap) * row, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -279,6 +279,7 @@ __device__ __forceinline__ uint32_t bucket_hash(int32_t key, int log2nb) {
 enum : uint32_t {
   kErrSpin = 1u,       // a bounded wait gave up
   kErrOverflow = 2u,   // a device-side capacity was exceeded
+  kErrCapacity = 4u,   // a partition region sized from a sample was too small (the join is redone exactly)
 };
 
 }  // namespace eo

@@ -40,14 +40,17 @@ namespace {
 // but not written.
 using namespace tile;
 
+// Partition p of R is [r_start[p], r_end[p]) (r_end = r_start + 1 for
+// contiguous partitions), of S [s_start[p], s_end[p]).
 __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_start, uint32_t P,
-                                             const uint32_t *r_start, const uint32_t *s_start, uint32_t s_span,
+                                             const uint32_t *r_start, const uint32_t *r_end,
+                                             const uint32_t *s_start, const uint32_t *s_end, uint32_t s_span,
                                              uint32_t cap, SpanRef *ref) {
   if (t >= tile_start[P]) return false;
   const uint32_t p = wave_find(tile_start, P, t);
   const uint32_t local = t - tile_start[p];
-  const uint32_t rb = r_start[p], rn = r_start[p + 1] - rb;
-  const uint32_t sb = s_start[p], sn = s_start[p + 1] - sb;
+  const uint32_t rb = r_start[p], rn = r_end[p] - rb;
+  const uint32_t sb = s_start[p], sn = s_end[p] - sb;
   const uint32_t nc = (rn + cap - 1) / cap;
   const uint32_t c = local % nc, j = local / nc;
   ref->r_beg = rb + c * cap;
@@ -68,8 +71,9 @@ __device__ __forceinline__ bool resolve_span(uint32_t t, const uint32_t *tile_st
 template <typename C>
 __global__ __launch_bounds__(kFNT) __attribute__((amdgpu_waves_per_eu(C::Cap <= 8704 ? 8 : 1)))
 void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_start,
-                                                     const uint2 *__restrict__ S, const uint32_t *__restrict__ s_start,
-                                                     const uint32_t *__restrict__ tile_start, uint32_t P,
+                  const uint32_t *__restrict__ r_end, const uint2 *__restrict__ S,
+                  const uint32_t *__restrict__ s_start, const uint32_t *__restrict__ s_end,
+                  const uint32_t *__restrict__ tile_start, uint32_t P,
                                                      uint32_t s_span, uint2 *__restrict__ out, uint64_t out_cap,
                                                      uint64_t *out_cursor, int wide, uint32_t *matched,
                                                      const uint64_t *__restrict__ n_tiles, uint32_t *err,
@@ -85,7 +89,7 @@ void k_join_fused(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
   for (uint32_t t = blockIdx.x; t < (uint32_t)nt; t += gridDim.x) {
     if (t != blockIdx.x) __syncthreads();  // the previous tile's LDS reads are done
     SpanRef ref;
-    if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+    if (!resolve_span(t, tile_start, P, r_start, r_end, s_start, s_end, s_span, C::Cap, &ref)) return;
     tile::probe_tile<C>(sh, R, S, ref, out, out_cap, out_cursor, wide != 0, matched, 0, swapped != 0);
   }
 }
@@ -103,7 +107,7 @@ void k_span_count(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
   for (uint32_t t = blockIdx.x;; t += gridDim.x) {  // a grid below the tile count loops
     if (t != blockIdx.x) __syncthreads();
     SpanRef ref;
-    if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+    if (!resolve_span(t, tile_start, P, r_start, r_start + 1, s_start, s_start + 1, s_span, C::Cap, &ref)) return;
     const uint64_t n = tile::probe_tile<C, tile::kProbeCount>(sh, R, S, ref, nullptr, 0, nullptr, false, matched);
     if (threadIdx.x == 0) tile_count[t] = n;
   }
@@ -119,21 +123,21 @@ void k_span_write(const uint2 *__restrict__ R, const uint32_t *__restrict__ r_st
   for (uint32_t t = blockIdx.x;; t += gridDim.x) {  // a grid below the tile count loops
     if (t != blockIdx.x) __syncthreads();
     SpanRef ref;
-    if (!resolve_span(t, tile_start, P, r_start, s_start, s_span, C::Cap, &ref)) return;
+    if (!resolve_span(t, tile_start, P, r_start, r_start + 1, s_start, s_start + 1, s_span, C::Cap, &ref)) return;
     tile::probe_tile<C, tile::kProbeOrdered>(sh, R, S, ref, out, out_cap, nullptr, wide != 0, nullptr, tile_off[t]);
   }
 }
 
-__global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *s_start, uint32_t P, uint32_t s_span,
-                             uint32_t cap, uint32_t *tiles) {
+__global__ void k_plan_spans(const uint32_t *r_start, const uint32_t *r_end, const uint32_t *s_start,
+                             const uint32_t *s_end, uint32_t P, uint32_t s_span, uint32_t cap, uint32_t *tiles) {
   const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p > P) return;
   if (p == P) {
     tiles[P] = 0;
     return;
   }
-  const uint32_t rn = r_start[p + 1] - r_start[p];
-  const uint32_t sn = s_start[p + 1] - s_start[p];
+  const uint32_t rn = r_end[p] - r_start[p];
+  const uint32_t sn = s_end[p] - s_start[p];
   tiles[p] = (rn && sn) ? ((rn + cap - 1) / cap) * ((sn + s_span - 1) / s_span) : 0u;
 }
 
@@ -150,11 +154,12 @@ uint32_t join_fused_cap(uint32_t lds_cap) {
 }
 
 hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
-                           const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
-                           uint32_t *err, hipStream_t stream) {
+                           const uint32_t *r_end, const uint32_t *s_start, const uint32_t *s_end,
+                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
+                           hipStream_t stream) {
   const uint32_t n = parts + 1;
-  hipLaunchKernelGGL(k_plan_spans, dim3((n + 255) / 256), dim3(256), 0, stream, r_start, s_start, parts, s_span, cap,
-                     tile_start);
+  hipLaunchKernelGGL(k_plan_spans, dim3((n + 255) / 256), dim3(256), 0, stream, r_start, r_end, s_start, s_end,
+                     parts, s_span, cap, tile_start);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return scan_u32(tile_start, tile_start, n, n_tiles_dev, scan_scratch, err, stream);
@@ -182,21 +187,22 @@ uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tile
 }
 
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t grid, const uint64_t *n_tiles_dev,
-                      const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
-                      const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
-                      uint32_t *matched, uint32_t *err, hipStream_t stream, bool swapped) {
+                      const eo_tuple *R, const uint32_t *r_start, const uint32_t *r_end, const eo_tuple *S,
+                      const uint32_t *s_start, const uint32_t *s_end, const uint32_t *tile_start, eo_pair *out,
+                      uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched, uint32_t *err,
+                      hipStream_t stream, bool swapped) {
   if (grid == 0) return hipSuccess;
   const uint2 *vR = reinterpret_cast<const uint2 *>(R), *vS = reinterpret_cast<const uint2 *>(S);
   uint2 *vout = reinterpret_cast<uint2 *>(out);
   if (cap == (uint32_t)tile::ProbeSmall::Cap)
     EO_TIMED("probe", stream,
-             hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
-                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
+             hipLaunchKernelGGL(k_join_fused<tile::ProbeSmall>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, r_end,
+                                vS, s_start, s_end, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
                                 matched, n_tiles_dev, err, (int)swapped));
   else if (cap == (uint32_t)tile::ProbeBig::Cap)
     EO_TIMED("probe", stream,
-             hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, vS,
-                                s_start, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
+             hipLaunchKernelGGL(k_join_fused<tile::ProbeBig>, dim3(grid), dim3(kFNT), 0, stream, vR, r_start, r_end,
+                                vS, s_start, s_end, tile_start, parts, s_span, vout, out_cap, out_cursor, (int)wide,
                                 matched, n_tiles_dev, err, (int)swapped));
   else
     return hipErrorInvalidValue;

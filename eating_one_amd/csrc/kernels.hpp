@@ -102,15 +102,18 @@ hipError_t partition_bounds(const eo_tuple *sorted, uint64_t n, int bits, int sh
 size_t join_plan_scratch_bytes(uint32_t parts);
 // Fused single-pass probe.  Tiles are (partition, R-chunk of join_fused_cap()
 // tuples, S span of s_span tuples); join_fused_plan counts and scans them
-// (tile_start: P+1 u32, *n_tiles_dev = total).  *out_cursor must be zero on
+// (tile_start: P+1 u32, *n_tiles_dev = total).  Partition p of R is
+// [r_start[p], r_end[p]) (contiguous partitions: r_end = r_start + 1), of S
+// [s_start[p], s_end[p]).  *out_cursor must be zero on
 // entry and holds the match count afterwards; pairs beyond out_cap are
 // counted, not written.
 // The probe's LDS table capacity for a planned lds_cap (8192 or less: the
 // half-size table, two workgroups per CU; otherwise the full-size table).
 uint32_t join_fused_cap(uint32_t lds_cap);
 hipError_t join_fused_plan(uint32_t parts, uint32_t s_span, uint32_t cap, const uint32_t *r_start,
-                           const uint32_t *s_start, uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch,
-                           uint32_t *err, hipStream_t stream);
+                           const uint32_t *r_end, const uint32_t *s_start, const uint32_t *s_end,
+                           uint32_t *tile_start, uint64_t *n_tiles_dev, void *scan_scratch, uint32_t *err,
+                           hipStream_t stream);
 // Grid of join_fused for `parts` partitions of nR x nS tuples, sized without
 // the tile count (tiles_est: the host's estimate of it).
 uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tiles_est, int cus);
@@ -122,9 +125,10 @@ uint32_t join_fused_grid(uint32_t parts, uint64_t nR, uint64_t nS, uint64_t tile
 // swapped: R is the caller's S and S its R (the smaller relation built into
 // the tables); the rows are written with the payloads exchanged.
 hipError_t join_fused(uint32_t parts, uint32_t s_span, uint32_t cap, uint32_t grid, const uint64_t *n_tiles_dev,
-                      const eo_tuple *R, const uint32_t *r_start, const eo_tuple *S, const uint32_t *s_start,
-                      const uint32_t *tile_start, eo_pair *out, uint64_t out_cap, uint64_t *out_cursor, bool wide,
-                      uint32_t *matched, uint32_t *err, hipStream_t stream, bool swapped = false);
+                      const eo_tuple *R, const uint32_t *r_start, const uint32_t *r_end, const eo_tuple *S,
+                      const uint32_t *s_start, const uint32_t *s_end, const uint32_t *tile_start, eo_pair *out,
+                      uint64_t out_cap, uint64_t *out_cursor, bool wide, uint32_t *matched, uint32_t *err,
+                      hipStream_t stream, bool swapped = false);
 // One launch of the integer VALU probe `kind` (EO_VALU_COMPARE / EO_VALU_ADD)
 // over `blocks` 256-thread blocks; *ops = the operations it performs.
 hipError_t valu_kernel(int kind, uint32_t blocks, uint32_t reps, uint64_t *sink, hipStream_t s, double *ops);
@@ -193,6 +197,22 @@ size_t msd2_scratch_bytes(uint64_t n, int bits_hi, int bits_lo, int cus);
 hipError_t partition_msd2(uint64_t n, int bits_hi, int bits_lo, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
                           uint32_t *part_start, void *scratch, uint32_t *err, int cus, hipStream_t stream,
                           bool ordered = false);
+
+// Two-pass MSD partition into regions sized from a 1/16 sample of the input
+// instead of a histogram pass: every tile reserves its runs from per-region
+// cursors.  The final partitions are [part_start[p], part_end[p]) with gaps
+// (part_start: 2^(bits_hi+bits_lo) + 1 entries); tmp holds gap_tmp_rows(n)
+// tuples, out gap_out_rows(n).  A region the sample underestimated raises
+// kErrCapacity in *err (its tuples are then incomplete: redo the join with
+// partition_msd2).  Needs n >= kGapMinRows and 16-byte aligned in, tmp, out.
+constexpr uint64_t kGapMinRows = 1ull << 20;
+bool gap_supported(uint64_t n, int bits_hi, int bits_lo);
+uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int bits_lo);
+uint64_t gap_out_rows(uint64_t n, int bits_hi, int bits_lo);
+size_t gap_scratch_bytes(uint64_t n, int bits_hi, int bits_lo);
+hipError_t partition_gapped(uint64_t n, int bits_hi, int bits_lo, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
+                            uint32_t *part_start, uint32_t *part_end, void *scratch, uint32_t *err, int cus,
+                            hipStream_t stream);
 
 // ---- batch.hip --------------------------------------------------------------
 // One relation of one join to partition in a batch group (all of a group's

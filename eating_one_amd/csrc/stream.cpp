@@ -187,13 +187,14 @@ struct Streamer {
         EO_TRY_HIP(fill_u32(s_start + 1, (uint32_t)n, 1, cs), "fill");
       }
       const uint32_t cap_lds = join_fused_cap(hp.lds_cap);
-      EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap_lds, r_start, s_start, tile_start, small, jscr, c->d_err, cs),
+      EO_TRY_HIP(join_fused_plan(P, hp.s_span, cap_lds, r_start, r_start + 1, s_start, s_start + 1, tile_start,
+                                 small, jscr, c->d_err, cs),
                  "tile plan");
       // the probe's grid does not need the tile count: no readback before it
       const uint64_t tiles_est = (uint64_t)P + nr / cap_lds + n / hp.s_span + 1024;
       e = join_fused(P, hp.s_span, cap_lds, join_fused_grid(P, nr, n, tiles_est, c->props.cus), small, hp.passes ? rp : dR,
-                     r_start, probe_side, s_start, tile_start, dst, cap, small + 1, wide, outer ? matched : nullptr,
-                     c->d_err, cs);
+                     r_start, r_start + 1, probe_side, s_start, s_start + 1, tile_start, dst, cap, small + 1, wide,
+                     outer ? matched : nullptr, c->d_err, cs);
     } else {
       e = nlj_fused(plan_nlj(nr, n), dR, dS[b], small + 1, dst, cap, wide, outer ? matched : nullptr, cs);
     }

@@ -311,11 +311,19 @@ struct ScatterShared : ScatterWaveCounts<BITS, ORDERED> {
 // Scatters in[sbeg, send) (in + sbeg 16-byte aligned, or sbeg > 0) by digit
 // into out, digit d starting at sh.cursor[d] (set by the caller, followed by a
 // barrier).  Positions >= nbound are never written (kErrOverflow is raised).
-template <int BITS, bool ORDERED = false>
+// RES (reserving mode): every tile takes the base of its run of digit d from
+// the global cursor res[d] with one atomic add, and sh.cursor[d] holds the end
+// of digit d's region instead: a run that would pass it is not written past
+// it and raises kErrCapacity (the regions were sized from an estimate).  The
+// atomics are issued as soon as the tile's counts are known and their results
+// are needed only after staging, so their round trip overlaps it.
+template <int BITS, bool ORDERED = false, bool RES = false>
 __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh, const uint2 *__restrict__ in, uint64_t sbeg,
                                                uint64_t send, uint64_t nbound, int shift, uint32_t xm,
-                                               uint2 *__restrict__ out, uint32_t *err) {
+                                               uint2 *__restrict__ out, uint32_t *err, uint32_t *res = nullptr) {
+  static_assert(!(RES && ORDERED), "reserved runs land in atomic order");
   constexpr int F = 1 << BITS;
+  constexpr int DPL = (F + kWave - 1) / kWave;  // digits per lane of wave 0
   // a stripe whose first tuple is only 8-byte aligned is read from one tuple
   // earlier (16-byte pairs); that tuple is not part of the stripe: skip it
   const uint64_t skip = ((uintptr_t)(in + sbeg) & 15) ? 1 : 0;
@@ -427,8 +435,8 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
     // wave 0: the tile's digit starts in staging (exclusive scan of the counts,
     // DPL consecutive digits per lane), each digit's output base for this tile,
     // the cursors advanced past it and the counts cleared for the next tile
+    uint32_t rbase[RES ? DPL : 1];  // RES: the reserved run bases, in flight during staging
     if (threadIdx.x < kWave) {
-      constexpr int DPL = (F + kWave - 1) / kWave;
       uint32_t loc[DPL];
       uint32_t sum = 0;
 #pragma unroll
@@ -443,10 +451,14 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
       for (int k = 0; k < DPL; ++k) {
         const int d = threadIdx.x * DPL + k;
         if (d < F) {
-          const uint32_t c = sh.cursor[d];
+          if constexpr (RES) {
+            rbase[k] = loc[k] ? atomic_add_agent(res + d, loc[k]) : 0u;
+          } else {
+            const uint32_t c = sh.cursor[d];
+            sh.pbase[d] = c - ex;
+            sh.cursor[d] = c + loc[k];
+          }
           sh.tbase[d] = ex;
-          sh.pbase[d] = c - ex;
-          sh.cursor[d] = c + loc[k];
           sh.cnt[d] = 0;
         }
         ex += loc[k];
@@ -464,6 +476,15 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
       if (e >= tskip && e < tn) sh.stage[slot(digit_of(cur[j].x, xm, shift, F - 1), rk[2 * j])] = make_uint2(cur[j].x, cur[j].y);
       if (e + 1 < tn) sh.stage[slot(digit_of(cur[j].z, xm, shift, F - 1), rk[2 * j + 1])] = make_uint2(cur[j].z, cur[j].w);
     }
+    if constexpr (RES) {
+      if (threadIdx.x < kWave) {
+#pragma unroll
+        for (int k = 0; k < DPL; ++k) {
+          const int d = threadIdx.x * DPL + k;
+          if (d < F) sh.pbase[d] = rbase[k] - sh.tbase[d];
+        }
+      }
+    }
     __syncthreads();
     // the next tile: its first kPre loads arrived during this tile's rank
     // phase, the rest are read while this tile's runs are written
@@ -474,9 +495,15 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
     // only after its first barrier, which every wave reaches after its flush
     for (uint32_t i = threadIdx.x; i < tn - tskip; i += kFastNT) {
       const uint2 t = sh.stage[i];
-      const uint64_t pos = (uint32_t)(sh.pbase[digit_of(t.x, xm, shift, F - 1)] + i);
-      if (pos < nbound) out[pos] = t;
-      else atomic_or_agent(err, kErrOverflow);  // inconsistent counts: report, never write outside
+      const uint32_t d = digit_of(t.x, xm, shift, F - 1);
+      const uint64_t pos = (uint32_t)(sh.pbase[d] + i);
+      if constexpr (RES) {
+        if (pos < sh.cursor[d]) out[pos] = t;
+        else atomic_or_agent(err, kErrCapacity);  // the region's estimate was too small
+      } else {
+        if (pos < nbound) out[pos] = t;
+        else atomic_or_agent(err, kErrOverflow);  // inconsistent counts: report, never write outside
+      }
     }
   }
   __syncthreads();

@@ -96,7 +96,9 @@ typedef struct eo_join_stats {
   uint32_t tiles;        /* join work tiles (partition x R-chunk x S-range) */
   uint32_t lds_tuples;   /* LDS table capacity used */
   uint32_t n_kernels;    /* valid entries of kernels[] */
-  uint32_t reserved;
+  uint32_t sampled;      /* hash join partitions: 0 exact (histogram pass), 1 sized
+                            from a sample, 2 sized from a sample that proved too
+                            small, so the join was redone exactly */
   eo_kernel_stat kernels[EO_MAX_KERNEL_STATS];
 } eo_join_stats;
 
@@ -134,6 +136,12 @@ typedef struct eo_join_opts {
  * eo_index_join; with EO_F_DETERMINISTIC the null rows follow the inner rows
  * in S order (the hash join's S order is its stable partition's). */
 #define EO_F_LEFT_OUTER 16
+/* Hash join: always partition from a full histogram pass.  By default a
+ * fused inner join of two passes and at least 2^20 tuples per relation sizes
+ * its partitions from a 1/16 sample instead (one read of the input less) and
+ * redoes the join exactly if the sample underestimated a partition; the rows
+ * are the same either way. */
+#define EO_F_EXACT_PARTITIONS 32
 #define EO_NULL_PAYLOAD ((int32_t)0x80000000)
 
 typedef struct eo_device_info {

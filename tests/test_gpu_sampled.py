@@ -1,0 +1,96 @@
+"""GPU parity of the hash join's sampled partitions: two-pass partitions whose
+regions are sized from a 1/16 sample of the input instead of a histogram pass
+(runs reserved with atomics, gaps between partitions), and the exact redo
+when the sample underestimated a region.  Every result is compared with the
+CPU oracle as a sorted multiset with an identical count.
+"""
+import numpy as np
+import pytest
+
+import eating_one_amd as eo
+import oracle
+from eating_one_amd import _abi, gen
+from common import join_count_closed_form, same_multiset
+
+pytestmark = pytest.mark.gpu
+
+# sizes off the 16384-tuple tile and odd: the last tile is partial and the
+# last tuple has no pair partner
+NR = (1 << 20) + 777
+NS = (1 << 21) + 4097
+
+
+def _join(ctx, R, S, **kw):
+    st = eo.JoinStats()
+    got = ctx.hash_join(R, S, stats=st, **kw)
+    return got, st
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.9])
+@pytest.mark.parametrize("bits", [2, 9, 13, 14])
+def test_sampled_partitions_equal_oracle(ctx, bits, theta):
+    R, S = gen.c2(seed=0x5A + bits, nR=NR, nS=NS, theta=theta, selectivity=1.0)
+    got, st = _join(ctx, R, S, radix_bits=bits, passes=2)
+    assert st.passes == 2 and st.radix_bits == bits
+    assert st.sampled == 1  # sized from the sample, no redo
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == join_count_closed_form(R, S)
+    assert same_multiset(got, want)
+
+
+def test_sampled_partitions_build_on_the_smaller_side(ctx):
+    """|S| < |R|: the inner join builds on S; both relations still large
+    enough for sampled partitions."""
+    R, S = gen.c3(seed=0x5B, nR=NS, nS=NR, key_range=1 << 21)
+    got, st = _join(ctx, R, S, radix_bits=13, passes=2)
+    assert st.sampled == 1
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == join_count_closed_form(R, S)
+    assert same_multiset(got, want)
+
+
+def test_sampled_and_exact_partitions_give_the_same_rows(ctx):
+    R, S = gen.c2(seed=0x5C, nR=NR, nS=NS, theta=0.5, selectivity=0.1)
+    a, sa = _join(ctx, R, S, radix_bits=14, passes=2)
+    b, sb = _join(ctx, R, S, radix_bits=14, passes=2, flags=_abi.EO_F_EXACT_PARTITIONS)
+    assert sa.sampled == 1 and sb.sampled == 0
+    assert same_multiset(a, b)
+    # paths that keep exact partitions: deterministic order, left outer join
+    _, sd = _join(ctx, R, S, radix_bits=14, passes=2, deterministic=True)
+    _, so = _join(ctx, R, S, radix_bits=14, passes=2, left_outer=True)
+    assert sd.sampled == 0 and so.sampled == 0
+
+
+def _sampled_lines(t, j):
+    """Line of tile t sampled for group j (the partition's sampling pattern,
+    restated to build an input that defeats it)."""
+    h = ((t * 64 + j) * 0x9E3779B1) & 0xFFFFFFFF
+    h ^= h >> 15
+    h = (h * 0x85EBCA77) & 0xFFFFFFFF
+    h ^= h >> 13
+    return j * 16 + (h >> 28)
+
+
+@pytest.mark.parametrize("side", ["S", "R"])
+def test_underestimated_region_redoes_the_join_exactly(ctx, side):
+    """Every line the sample reads holds keys spread over all partitions and
+    every other line one hot key: the sample sees almost nothing of the hot
+    partition, whose region overflows; the join is redone with exact
+    partitions and still returns every row."""
+    rng = np.random.default_rng(0x5D)
+    n = 1 << 21
+    keys = np.full(n, 5, dtype=np.int32)
+    tiles = n // 16384
+    for t in range(tiles):
+        for j in range(64):
+            line = _sampled_lines(t, j)
+            b = t * 16384 + line * 16
+            keys[b:b + 16] = rng.integers(0, 1 << 20, size=16, dtype=np.int32)
+    big = np.stack([keys, np.arange(n, dtype=np.int32)], 1)
+    small = np.stack([rng.permutation(1 << 20).astype(np.int32), np.arange(1 << 20, dtype=np.int32)], 1)
+    R, S = (small, big) if side == "S" else (big, small)
+    got, st = _join(ctx, R, S, radix_bits=14, passes=2)
+    assert st.sampled == 2  # the sampled attempt overflowed, the exact one ran
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == join_count_closed_form(R, S)
+    assert same_multiset(got, want)
