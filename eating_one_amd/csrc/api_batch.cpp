@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "ctx.hpp"
+#include "device.hpp"
 #include "eojoin.h"
 #include "kernels.hpp"
 #include "plan.hpp"
@@ -28,6 +29,8 @@ namespace {
 
 constexpr int kMaxGroupBits = kMaxPassBits;  // one fast pass per group
 constexpr uint64_t kMaxTileDescs = 1ull << 21;  // 64 MiB of probe tile descriptors per batch
+constexpr int kBatchSampleMaxBits = 8;          // sampled partitions up to 8 radix bits
+constexpr uint32_t kBatchSampleMinJoins = 256;  // and in batches of at least 256 requests
 
 // Where one relation of one join comes from and goes to.
 struct SideInfo {
@@ -35,6 +38,11 @@ struct SideInfo {
   uint64_t n = 0;
   int mode = 0;                  // 0: as is (P == 1), 1: group, 2: individual, 3: resident table
   size_t o_part = 0, o_start = 0;  // arena offsets of the partitioned copy and its starts
+  // mode 1 with partitions sized from a sample: the copy holds `budget`
+  // tuples, partitions [start, end) with gaps; region ends and cursors
+  bool sampled = false;
+  uint64_t budget = 0;
+  size_t o_end = 0, o_re = 0, o_cur = 0;
   size_t blob_start = 0;         // blob offset of {0, n} when mode == 0
   const eo_tuple *table_part = nullptr;
   const uint32_t *table_start = nullptr;
@@ -77,6 +85,7 @@ struct eo_batch {
   // enqueue parameters of the last batch (device addresses inside arena)
   struct Group {
     eo::BatchGroup g;
+    size_t o_hs = 0;  // sampled groups: the relations' sampled counts (arena)
   };
   std::vector<Group> groups;
   size_t o_blob = 0, blob_bytes = 0, o_cursor = 0, o_head = 0, o_counts = 0, o_cscr = 0, o_tmp = 0, o_pscr = 0,
@@ -85,6 +94,12 @@ struct eo_batch {
   size_t o_joins_blob = 0;
   uint32_t npairs = 0;
   bool wide = false, outer = false;  // batch flags: EO_F_RESULT16, EO_F_LEFT_OUTER
+  // partitions: exact (histogram) only, or sampled for large one-pass relations;
+  // any_sampled: some group of the batch in flight is sampled (its capacity
+  // flag follows the per-join counts in the cursor array and the readback)
+  bool exact = false, any_sampled = false;
+  eo_join_opts sub_opts{};  // the last submit's options (for an exact redo)
+  bool has_opts = false;
   size_t o_bits = 0, o_emit = 0;     // bitmap region (arena), emit_start (blob)
   uint64_t bits_words = 0;
   uint32_t emit_blocks = 0;
@@ -116,7 +131,8 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   po.where = EO_DEVICE;
   b->joins.clear();
   b->req_join.assign(n, -1);
-  group_members.assign(kMaxGroupBits + 1, {});
+  group_members.assign(2 * (kMaxGroupBits + 1), {});  // [bits]: exact groups, [kMaxGroupBits + 1 + bits]: sampled
+  b->any_sampled = false;
   uint64_t max_tmp = 0;
   size_t max_pscr = 0;
   uint64_t npairs = 0;
@@ -180,11 +196,31 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
         sd.mode = 0;
         return;
       }
-      sd.o_part = L.add(part_bytes(cnt));
+      const bool group = J.hp.passes == 1 && J.hp.bits <= kMaxGroupBits && aligned16(in);
+      // large relations of a large inner batch with few digits: partitions
+      // sized from a sample (no histogram pass; a batch whose sample proves
+      // too small is redone exactly when it is waited for).  Every tile
+      // reserves its runs with one atomic per digit on cursors that all tiles
+      // of its relation share: from 9 bits on (512 atomics per tile) that
+      // costs more than the histogram it replaces, and in a batch of few
+      // joins the tiles of one relation reserve all at once (measured, C5 in
+      // batches of 30 requests: 227 ms sampled up to 11 bits, 200 ms up to 8
+      // bits, 180 ms exact; in batches of 1024: 83.7 ms sampled up to 8 bits,
+      // 90.8 ms exact).
+      sd.sampled = group && n >= kBatchSampleMinJoins && J.hp.bits <= kBatchSampleMaxBits && !b->outer &&
+                   !b->exact && cnt >= kGapMinRows && gap_rows(cnt, P) + cnt < (1ull << 32);
+      sd.budget = sd.sampled ? gap_rows(cnt, P) : cnt;
+      sd.o_part = L.add(part_bytes(sd.budget));
       sd.o_start = L.add((P + 1) * sizeof(uint32_t));
-      if (J.hp.passes == 1 && J.hp.bits <= kMaxGroupBits && aligned16(in)) {
+      if (sd.sampled) {
+        sd.o_end = L.add(P * sizeof(uint32_t));
+        sd.o_re = L.add(P * sizeof(uint32_t));
+        sd.o_cur = L.add(P * sizeof(uint32_t));
+        b->any_sampled = true;
+      }
+      if (group) {
         sd.mode = 1;
-        group_members[J.hp.bits].push_back({(uint32_t)b->joins.size(), which});
+        group_members[J.hp.bits + (sd.sampled ? kMaxGroupBits + 1 : 0)].push_back({(uint32_t)b->joins.size(), which});
       } else {
         sd.mode = 2;
         if (J.hp.passes == 2) max_tmp = std::max(max_tmp, cnt);
@@ -216,27 +252,33 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   b->o_joins_blob = 0;
   b->groups.clear();
   uint64_t max_counts = 0;
-  for (int bits = 1; bits <= kMaxGroupBits; ++bits) {
-    auto &mem = group_members[bits];
-    if (mem.empty()) continue;
+  for (int gi = 1; gi < 2 * (kMaxGroupBits + 1); ++gi) {
+    const int bits = gi % (kMaxGroupBits + 1);
+    const bool sampled = gi > kMaxGroupBits;
+    auto &mem = group_members[gi];
+    if (bits == 0 || mem.empty()) continue;
     const uint64_t len = batch_stripe_len(bits);
     eo_batch::Group G{};
     G.g.bits = bits;
+    G.g.sampled = sampled ? 1 : 0;
     G.g.nseg = (uint32_t)mem.size();
     G.g.stripe_len = len;
     blob = (blob + 15) / 16 * 16;
     G.g.segs = reinterpret_cast<const BatchSeg *>(blob);  // blob offset until addresses are known
     blob += mem.size() * sizeof(BatchSeg);
-    uint64_t stripes = 0, counts = 0;
+    uint64_t stripes = 0, counts = 0, samp = 0;
     for (auto &m : mem) {
       const SideInfo &sd = m.second ? b->joins[m.first].s : b->joins[m.first].r;
       const uint64_t ns = (sd.n + len - 1) / len;
       stripes += ns;
-      counts += ns << bits;
+      if (sampled) samp += (sd.n + kPartTile * kBatchSampleTiles - 1) / (kPartTile * kBatchSampleTiles);
+      else counts += ns << bits;
     }
-    if (stripes >= (1ull << 31)) return EO_EINVAL;
+    if (stripes >= (1ull << 31) || samp >= (1ull << 31)) return EO_EINVAL;
     G.g.total_stripes = (uint32_t)stripes;
     G.g.ncounts = counts;
+    G.g.total_samp = (uint32_t)samp;
+    if (sampled) G.o_hs = L.add(mem.size() * (sizeof(uint32_t) << bits));
     max_counts = std::max(max_counts, counts);
     b->groups.push_back(G);
   }
@@ -254,7 +296,7 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
   blob_bytes = (blob + 15) / 16 * 16;
   b->o_blob = L.add(blob_bytes);
   b->o_bits = L.add(b->bits_words * sizeof(uint32_t));
-  b->o_cursor = L.add((size_t)std::max<uint32_t>(nj, 1) * sizeof(uint64_t));
+  b->o_cursor = L.add(((size_t)nj + 1) * sizeof(uint64_t));  // per-join counts, then the capacity flag
   b->o_head = L.add(sizeof(uint64_t));
   b->o_counts = L.add(max_counts * sizeof(uint64_t));
   b->o_cscr = L.add(max_counts ? scan_scratch_bytes_u64(max_counts) : 0);
@@ -283,7 +325,7 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
     const JoinInfo &J = b->joins[j];
     const eo_request &q = b->reqs[J.req];
     BatchJoin bj{};
-    auto src = [&](const SideInfo &sd, const uint2 **tup, const uint32_t **st) {
+    auto src = [&](const SideInfo &sd, const uint2 **tup, const uint32_t **st, const uint32_t **en) {
       if (sd.mode == 0) {
         *tup = reinterpret_cast<const uint2 *>(sd.in);
         uint32_t *h = reinterpret_cast<uint32_t *>(H + sd.blob_start);
@@ -297,9 +339,10 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
         *tup = reinterpret_cast<const uint2 *>(A + sd.o_part);
         *st = reinterpret_cast<const uint32_t *>(A + sd.o_start);
       }
+      *en = sd.sampled ? reinterpret_cast<const uint32_t *>(A + sd.o_end) : *st + 1;  // gapped or contiguous
     };
-    src(J.r, &bj.R, &bj.r_start);
-    src(J.s, &bj.S, &bj.s_start);
+    src(J.r, &bj.R, &bj.r_start, &bj.r_end);
+    src(J.s, &bj.S, &bj.s_start, &bj.s_end);
     bj.P = 1u << J.hp.bits;
     bj.s_span = J.hp.s_span;
     bj.out = reinterpret_cast<uint2 *>(q.out);
@@ -323,15 +366,17 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
     es[b->joins.size()] = acc;
   }
   size_t gi = 0;
-  for (int bits = 1; bits <= kMaxGroupBits; ++bits) {
-    auto &mem = group_members[bits];
-    if (mem.empty()) continue;
+  for (int gk = 1; gk < 2 * (kMaxGroupBits + 1); ++gk) {
+    const int bits = gk % (kMaxGroupBits + 1);
+    auto &mem = group_members[gk];
+    if (bits == 0 || mem.empty()) continue;
     const uint64_t len = batch_stripe_len(bits);
     eo_batch::Group &G = b->groups[gi++];
     const size_t off = reinterpret_cast<size_t>(G.g.segs);
     BatchSeg *hs = reinterpret_cast<BatchSeg *>(H + off);
-    uint32_t stripe_base = 0;
+    uint32_t stripe_base = 0, samp_base = 0;
     uint64_t count_base = 0;
+    uint32_t *hs_all = G.g.sampled ? reinterpret_cast<uint32_t *>(A + G.o_hs) : nullptr;
     for (size_t k = 0; k < mem.size(); ++k) {
       const SideInfo &sd = mem[k].second ? b->joins[mem[k].first].s : b->joins[mem[k].first].r;
       BatchSeg sg{};
@@ -342,11 +387,24 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
       sg.nstripes = (uint32_t)((sd.n + len - 1) / len);
       sg.count_base = count_base;
       sg.part_start = reinterpret_cast<uint32_t *>(A + sd.o_start);
+      if (G.g.sampled) {
+        const uint64_t unit = kPartTile * kBatchSampleTiles;
+        sg.samp_base = samp_base;
+        sg.nsamp = (uint32_t)((sd.n + unit - 1) / unit);
+        sg.hs = hs_all + ((uint64_t)k << bits);
+        sg.re = reinterpret_cast<uint32_t *>(A + sd.o_re);
+        sg.cur = reinterpret_cast<uint32_t *>(A + sd.o_cur);
+        sg.part_end = reinterpret_cast<uint32_t *>(A + sd.o_end);
+        sg.budget = sd.budget;
+        samp_base += sg.nsamp;
+      } else {
+        count_base += (uint64_t)sg.nstripes << bits;
+      }
       stripe_base += sg.nstripes;
-      count_base += (uint64_t)sg.nstripes << bits;
       hs[k] = sg;
     }
     G.g.segs = reinterpret_cast<const BatchSeg *>(D + off);
+    G.g.hs_all = hs_all;
   }
 }
 
@@ -359,13 +417,15 @@ eo_status enqueue_batch(eo_batch *b, hipStream_t s) {
   uint64_t *cursor = reinterpret_cast<uint64_t *>(A + b->o_cursor);
   uint32_t *head = reinterpret_cast<uint32_t *>(A + b->o_head);
   if (nj == 0) return EO_OK;
-  EO_TRY_HIP(fill_u64(cursor, 0, nj, s), "cursor reset");
+  EO_TRY_HIP(fill_u64(cursor, 0, (uint64_t)nj + 1, s), "cursor reset");  // counts and the capacity flag
+  uint32_t *cap_flag = reinterpret_cast<uint32_t *>(cursor + nj);
   EO_TRY_HIP(fill_u32(head, 0u, 1, s), "queue reset");
   if (b->outer)
     EO_TRY_HIP(fill_u32(reinterpret_cast<uint32_t *>(A + b->o_bits), 0u, b->bits_words, s), "match bitmap reset");
   uint64_t *counts = reinterpret_cast<uint64_t *>(A + b->o_counts);
   for (auto &G : b->groups)
-    EO_TRY_HIP(batch_partition_group(G.g, counts, A + b->o_cscr, c->d_err, s), "batch partition");
+    EO_TRY_HIP(batch_partition_group(G.g, counts, A + b->o_cscr, G.g.sampled ? cap_flag : c->d_err, s),
+               "batch partition");
   for (auto &J : b->joins) {
     for (const SideInfo *sd : {&J.r, &J.s}) {
       if (sd->mode != 2) continue;
@@ -387,7 +447,8 @@ eo_status enqueue_batch(eo_batch *b, hipStream_t s) {
     EO_TRY_HIP(batch_emit_unmatched(dj, nj, reinterpret_cast<const uint32_t *>(A + b->o_blob + b->o_emit),
                                     b->emit_blocks, s),
                "outer rows");
-  EO_TRY_HIP(hipMemcpyAsync(b->h_counts, cursor, nj * sizeof(uint64_t), hipMemcpyDeviceToHost, s), "count readback");
+  EO_TRY_HIP(hipMemcpyAsync(b->h_counts, cursor, ((size_t)nj + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, s),
+             "count readback");
   return EO_OK;
 }
 
@@ -539,8 +600,14 @@ eo_status eo_batch_footprint(eo_ctx *c, const eo_request *reqs, uint32_t n, uint
   return EO_OK;
 }
 
-eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_join_opts *o) {
-  if (!b || (n && !reqs) || !opts_ok(o)) return EO_EINVAL;
+}  // extern "C"
+
+namespace eo {
+namespace {
+
+// eo_batch_submit; exact: every partition from a histogram pass (the redo of
+// a batch whose sampled partitions proved too small).
+eo_status submit_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_join_opts *o, bool exact) {
   if (b->in_flight) {
     set_last_error("eo_batch_submit: a batch is still in flight on this object");
     return EO_EINVAL;
@@ -549,7 +616,10 @@ eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const
   DeviceGuard g(c->device);
   drop_graph(b);
   b->replayable = false;
-  b->reqs.assign(reqs, reqs + n);
+  b->exact = exact || (o && (o->flags & EO_F_EXACT_PARTITIONS));
+  b->has_opts = o != nullptr;
+  if (o) b->sub_opts = *o;
+  if (reqs != b->reqs.data()) b->reqs.assign(reqs, reqs + n);
   Layout L;
   size_t blob_bytes = 0;
   std::vector<std::vector<std::pair<uint32_t, int>>> members;
@@ -575,7 +645,7 @@ eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const
                "blob staging");
     b->h_blob_cap = std::max<size_t>(blob_bytes, 4096);
   }
-  const size_t ncnt = std::max<size_t>(b->joins.size(), 1);
+  const size_t ncnt = b->joins.size() + 1;  // the counts and the capacity flag
   if (ncnt > b->h_counts_cap) {
     if (b->h_counts) (void)hipHostFree(b->h_counts);
     b->h_counts = nullptr;
@@ -607,6 +677,16 @@ eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const
   b->in_flight = true;
   b->replayable = true;
   return EO_OK;
+}
+
+}  // namespace
+}  // namespace eo
+
+extern "C" {
+
+eo_status eo_batch_submit(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_join_opts *o) {
+  if (!b || (n && !reqs) || !opts_ok(o)) return EO_EINVAL;
+  return submit_batch(b, reqs, n, o, false);
 }
 
 eo_status eo_batch_replay(eo_batch *b) {
@@ -670,10 +750,25 @@ eo_status eo_batch_wait(eo_batch *b, eo_result *res, uint32_t n) {
   if (n != b->reqs.size()) return EO_EINVAL;
   eo_ctx *c = b->ctx;
   DeviceGuard g(c->device);
-  b->in_flight = false;
-  EO_TRY_HIP(hipEventSynchronize(b->done), "batch wait");
-  eo_status st = check_device_errors(c, b->stream);
-  if (st != EO_OK) return st;
+  bool redone = false;
+  for (;;) {
+    b->in_flight = false;
+    EO_TRY_HIP(hipEventSynchronize(b->done), "batch wait");
+    eo_status st = check_device_errors(c, b->stream);
+    if (st != EO_OK) return st;
+    const size_t nj = b->joins.size();
+    if (!(b->any_sampled && nj && (b->h_counts[nj] & kErrCapacity))) break;
+    // a sampled partition was too small: its join's rows are incomplete, so
+    // the whole batch runs again with exact partitions (same requests and
+    // options, stats included)
+    const eo_join_opts so = b->sub_opts;
+    eo_join_stats *stats = b->stats;
+    eo_join_opts ro = so;
+    ro.stats = stats;
+    st = submit_batch(b, b->reqs.data(), n, b->has_opts || stats ? &ro : nullptr, true);
+    if (st != EO_OK) return st;
+    redone = true;
+  }
   uint64_t total = 0;
   for (uint32_t i = 0; i < n; ++i) {
     const eo_request &q = b->reqs[i];
@@ -688,6 +783,7 @@ eo_status eo_batch_wait(eo_batch *b, eo_result *res, uint32_t n) {
   if (b->stats) {
     fold_timer(b->timer, b->stats);
     b->stats->n_out = total;
+    b->stats->sampled = redone ? 2u : (b->any_sampled ? 1u : 0u);
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, b->t0, b->t1) == hipSuccess) b->stats->ms_total = ms;
     b->stats = nullptr;

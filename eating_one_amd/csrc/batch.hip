@@ -34,6 +34,7 @@ using tile::SpanRef;
 
 constexpr int kBHistNT = 512;
 static_assert(kFCap == kBatchChunk, "batch tile chunk is the probe table's capacity");
+static_assert(tile::kFastTile == kPartTile, "the host's tile size is the scatter's");
 
 __device__ __forceinline__ uint32_t find_seg(const BatchSeg *segs, uint32_t nseg, uint32_t g) {
   uint32_t lo = 0, hi = nseg;  // segs[lo].stripe_base <= g < segs[hi].stripe_base
@@ -51,6 +52,10 @@ __device__ __forceinline__ BatchSeg load_seg(const BatchSeg *segs, uint32_t i) {
   sg.in = as_global(sg.in);
   sg.out = as_global(sg.out);
   sg.part_start = as_global(sg.part_start);
+  sg.hs = as_global(sg.hs);
+  sg.re = as_global(sg.re);
+  sg.cur = as_global(sg.cur);
+  sg.part_end = as_global(sg.part_end);
   return sg;
 }
 __device__ __forceinline__ BatchJoin load_join(const BatchJoin *joins, uint32_t j) {
@@ -59,6 +64,8 @@ __device__ __forceinline__ BatchJoin load_join(const BatchJoin *joins, uint32_t 
   J.S = as_global(J.S);
   J.r_start = as_global(J.r_start);
   J.s_start = as_global(J.s_start);
+  J.r_end = as_global(J.r_end);
+  J.s_end = as_global(J.s_end);
   J.out = as_global(J.out);
   J.cursor = as_global(J.cursor);
   J.matched = as_global(J.matched);
@@ -84,7 +91,9 @@ __global__ __launch_bounds__(kBHistNT) void k_bseg_hist(const BatchSeg *__restri
   for (int d = threadIdx.x; d < F; d += kBHistNT) counts[sg.count_base + (uint64_t)d * sg.nstripes + s] = hist[d];
 }
 
-template <int BITS>
+// RES (sampled groups): the stripe reserves its runs from the relation's
+// region cursors; otherwise its offsets come from the scanned counts.
+template <int BITS, bool RES = false>
 __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__restrict__ segs, uint32_t nseg,
                                                           uint32_t total_stripes, uint64_t len,
                                                           const uint64_t *__restrict__ offsets, uint32_t *err) {
@@ -95,17 +104,130 @@ __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__rest
   const uint32_t s = g - sg.stripe_base;
   const uint64_t beg = (uint64_t)s * len;
   const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
-  const uint64_t base = offsets[sg.count_base];  // tuples of the group's earlier relations
-  for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
-    const uint32_t c = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes + s] - base);
-    sh.cursor[d] = c;
-    sh.cnt[d] = 0;
-    // the relation's first stripe starts every digit: its offsets are the
-    // partition starts (relative to the relation)
-    if (s == 0) sg.part_start[d] = c;
+  if constexpr (RES) {
+    for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
+      sh.cursor[d] = sg.re[d];
+      sh.cnt[d] = 0;
+    }
+  } else {
+    const uint64_t base = offsets[sg.count_base];  // tuples of the group's earlier relations
+    for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
+      const uint32_t c = (uint32_t)(offsets[sg.count_base + (uint64_t)d * sg.nstripes + s] - base);
+      sh.cursor[d] = c;
+      sh.cnt[d] = 0;
+      // the relation's first stripe starts every digit: its offsets are the
+      // partition starts (relative to the relation)
+      if (s == 0) sg.part_start[d] = c;
+    }
+    if (s == 0 && threadIdx.x == 0) sg.part_start[1 << BITS] = (uint32_t)sg.n;
   }
-  if (s == 0 && threadIdx.x == 0) sg.part_start[1 << BITS] = (uint32_t)sg.n;
-  tile::scatter_stripe<BITS>(sh, sg.in, beg, end, sg.n, 0, 0u, sg.out, err);
+  tile::scatter_stripe<BITS, false, RES>(sh, sg.in, beg, end, sg.n, 0, 0u, sg.out, err, RES ? sg.cur : nullptr);
+}
+
+// ---- sampled groups ----------------------------------------------------------------
+// Workgroup u samples kBatchSampleTiles tiles of one relation (the tile's
+// sampled lines, as the single-join sampled partition reads them) into an LDS
+// histogram and adds it to the relation's counts hs with one atomic per
+// non-empty digit.
+__device__ __forceinline__ uint32_t find_samp(const BatchSeg *segs, uint32_t nseg, uint32_t u) {
+  uint32_t lo = 0, hi = nseg;  // segs[lo].samp_base <= u < segs[hi].samp_base
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (segs[mid].samp_base <= u) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(tile::kGapSampleNT) void k_bsamp(const BatchSeg *__restrict__ segs, uint32_t nseg,
+                                                              uint32_t total_samp) {
+  constexpr uint32_t F = 1u << BITS;
+  constexpr int NT = tile::kGapSampleNT, U = tile::kGapSampleUnroll;
+  __shared__ uint32_t hist[F];
+  const uint32_t u = blockIdx.x;
+  if (u >= total_samp) return;
+  const BatchSeg sg = load_seg(segs, find_samp(segs, nseg, u));
+  for (uint32_t d = threadIdx.x; d < F; d += NT) hist[d] = 0;
+  __syncthreads();
+  const uint32_t T = (uint32_t)((sg.n + tile::kFastTile - 1) / tile::kFastTile);
+  const uint32_t t0 = (u - sg.samp_base) * kBatchSampleTiles, t1 = min(T, t0 + kBatchSampleTiles);
+  const uint64_t npairs = sg.n / 2;
+  const uint4 *in4 = reinterpret_cast<const uint4 *>(sg.in);
+  const uint32_t j = threadIdx.x >> 3, q = threadIdx.x & 7;
+  for (uint32_t tb = t0; tb < t1; tb += U) {
+    uint4 v[U];
+    bool ok[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const uint32_t t = tb + (uint32_t)k;
+      const uint64_t p = (uint64_t)t * (tile::kFastTile / 2) +
+                         (uint64_t)tile::gap_line(t, j) * (tile::kGapLineTuples / 2) + q;
+      ok[k] = t < t1 && p < npairs;
+      v[k] = ok[k] ? ld_stream(in4 + p) : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (!ok[k]) continue;
+      atomicAdd(&hist[v[k].x & (F - 1)], 1u);
+      atomicAdd(&hist[v[k].z & (F - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t d = threadIdx.x; d < F; d += NT)
+    if (hist[d]) atomic_add_agent(sg.hs + d, hist[d]);
+}
+
+// One workgroup per relation: region d = [part_start[d], re[d]) sized by
+// gap_cap from the sampled counts, cursor cur[d] = part_start[d]; a total
+// beyond the relation's buffer (or an empty sample) raises kErrCapacity and
+// every region is clipped to the buffer.
+template <int BITS>
+__global__ __launch_bounds__(1024) void k_blayout(const BatchSeg *__restrict__ segs, uint32_t *err) {
+  constexpr uint32_t F = 1u << BITS, NT = 1024, BPT = F > NT ? F / NT : 1;
+  __shared__ uint64_t red[NT / kWave + 1];
+  const BatchSeg sg = load_seg(segs, blockIdx.x);
+  const uint32_t b0 = threadIdx.x * BPT;
+  uint32_t sv[BPT];
+  uint64_t loc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < BPT; ++k) {
+    sv[k] = b0 + k < F ? sg.hs[b0 + k] : 0u;
+    loc += sv[k];
+  }
+  const uint64_t ns = block_sum<NT, uint64_t>(loc, red);
+  const double f = ns ? (double)sg.n / (double)ns : 0.0;
+  uint64_t c[BPT];
+  uint64_t sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < BPT; ++k) {
+    c[k] = (ns && b0 + k < F) ? tile::gap_cap(sv[k], f) : 0;
+    sum += c[k];
+  }
+  uint64_t tot;
+  uint64_t at = block_excl_scan<NT, uint64_t>(sum, red, &tot);
+  if (threadIdx.x == 0 && (ns == 0 || tot > sg.budget)) atomic_or_agent(err, kErrCapacity);
+#pragma unroll
+  for (uint32_t k = 0; k < BPT; ++k) {
+    if (b0 + k < F) {
+      const uint64_t a = min(at, sg.budget), e = min(at + c[k], sg.budget);
+      sg.part_start[b0 + k] = (uint32_t)a;
+      sg.cur[b0 + k] = (uint32_t)a;
+      sg.re[b0 + k] = (uint32_t)e;
+    }
+    at += c[k];
+  }
+  if (threadIdx.x == 0) sg.part_start[F] = (uint32_t)min(tot, sg.budget);
+}
+
+// part_end[d] = the filled end of every partition of the group's relations.
+template <int BITS>
+__global__ void k_bends(const BatchSeg *__restrict__ segs, uint32_t nseg) {
+  constexpr uint32_t F = 1u << BITS;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (uint64_t)nseg * F) return;
+  const BatchSeg sg = load_seg(segs, (uint32_t)(i / F));
+  const uint32_t d = (uint32_t)(i % F);
+  sg.part_end[d] = min(sg.cur[d], sg.re[d]);
 }
 
 // pair_join[pair_base + p] = j for every partition p of join j (one workgroup per join).
@@ -126,8 +248,8 @@ __global__ void k_bplan(const BatchJoin *__restrict__ joins, const uint32_t *__r
   }
   const BatchJoin J = load_join(joins, pair_join[q]);
   const uint32_t p = q - J.pair_base;
-  const uint32_t rn = J.r_start[p + 1] - J.r_start[p];
-  const uint32_t sn = J.s_start[p + 1] - J.s_start[p];
+  const uint32_t rn = J.r_end[p] - J.r_start[p];
+  const uint32_t sn = J.s_end[p] - J.s_start[p];
   tiles[q] = (rn && sn) ? ((rn + kFCap - 1) / kFCap) * ((sn + J.s_span - 1) / J.s_span) : 0u;
 }
 
@@ -139,8 +261,8 @@ __device__ __forceinline__ BatchTile tile_of_pair(const BatchJoin *joins, const 
   const BatchJoin J = load_join(joins, jn);
   const uint32_t p = q - J.pair_base;
   const uint32_t local = t - tile_start[q];
-  const uint32_t rb = J.r_start[p], rn = J.r_start[p + 1] - rb;
-  const uint32_t sb = J.s_start[p], sn = J.s_start[p + 1] - sb;
+  const uint32_t rb = J.r_start[p], rn = J.r_end[p] - rb;
+  const uint32_t sb = J.s_start[p], sn = J.s_end[p] - sb;
   const uint32_t nc = (rn + kFCap - 1) / kFCap;
   const uint32_t c = local % nc, k = local / nc;
   BatchTile T;
@@ -261,6 +383,24 @@ __global__ __launch_bounds__(256) void k_bemit(const BatchJoin *__restrict__ joi
 template <int BITS>
 hipError_t launch_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err, hipStream_t stream) {
   const uint32_t grid = (g.total_stripes + kXcds - 1) / kXcds * kXcds;
+  if (g.sampled) {
+    // sample -> region layout -> scatter with reserved runs -> partition ends
+    hipError_t e = fill_u32(g.hs_all, 0u, (uint64_t)g.nseg << BITS, stream);
+    if (e != hipSuccess) return e;
+    EO_TIMED("batch_sample", stream,
+             hipLaunchKernelGGL(k_bsamp<BITS>, dim3(g.total_samp), dim3(tile::kGapSampleNT), 0, stream, g.segs,
+                                g.nseg, g.total_samp));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_blayout<BITS>, dim3(g.nseg), dim3(1024), 0, stream, g.segs, err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    EO_TIMED("batch_scatter", stream,
+             hipLaunchKernelGGL((k_bseg_scatter<BITS, true>), dim3(grid), dim3(kFastNT), 0, stream, g.segs, g.nseg,
+                                g.total_stripes, g.stripe_len, nullptr, err));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const uint64_t ends = (uint64_t)g.nseg << BITS;
+    hipLaunchKernelGGL(k_bends<BITS>, dim3((uint32_t)((ends + 255) / 256)), dim3(256), 0, stream, g.segs, g.nseg);
+    return hipGetLastError();
+  }
   EO_TIMED("batch_hist", stream,
            hipLaunchKernelGGL(k_bseg_hist<BITS>, dim3(grid), dim3(kBHistNT), 0, stream, g.segs, g.nseg,
                               g.total_stripes, g.stripe_len, counts));

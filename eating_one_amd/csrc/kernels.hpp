@@ -207,6 +207,9 @@ hipError_t partition_msd2(uint64_t n, int bits_hi, int bits_lo, const eo_tuple *
 // partition_msd2).  Needs n >= kGapMinRows and 16-byte aligned in, tmp, out.
 constexpr uint64_t kGapMinRows = 1ull << 20;
 bool gap_supported(uint64_t n, int bits_hi, int bits_lo);
+// Tuples of a buffer that holds `parts` sampled regions of a relation of n
+// (for n >= kGapMinRows: enough for any sample).
+uint64_t gap_rows(uint64_t n, uint64_t parts);
 uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int bits_lo);
 uint64_t gap_out_rows(uint64_t n, int bits_hi, int bits_lo);
 size_t gap_scratch_bytes(uint64_t n, int bits_hi, int bits_lo);
@@ -225,19 +228,34 @@ struct BatchSeg {
   uint32_t nstripes;
   uint64_t count_base;       // first count of this relation: counts laid out [rel][digit][stripe]
   uint32_t *part_start;      // 2^bits + 1 partition starts (output)
+  // sampled groups: partitions sized from a 1/16 sample (gaps between them)
+  uint32_t samp_base;        // first sample workgroup of this relation in the group
+  uint32_t nsamp;
+  uint32_t *hs;              // 2^bits sampled counts (zero on entry)
+  uint32_t *re, *cur;        // 2^bits region ends and cursors
+  uint32_t *part_end;        // 2^bits partition ends (output)
+  uint64_t budget;           // tuples of out
 };
 struct BatchGroup {
   int bits;
-  const BatchSeg *segs;      // device table, ordered by stripe_base
+  int sampled;               // partitions sized from a sample, runs reserved with atomics
+  const BatchSeg *segs;      // device table, ordered by stripe_base (and samp_base)
   uint32_t nseg;
   uint32_t total_stripes;
   uint64_t stripe_len;
-  uint64_t ncounts;          // sum over relations of 2^bits * nstripes
+  uint64_t ncounts;          // sum over relations of 2^bits * nstripes (exact groups)
+  uint32_t total_samp;       // sample workgroups (sampled groups)
+  uint32_t *hs_all;          // the group's sampled counts, nseg * 2^bits, contiguous
 };
+// Tuples per partition tile (the fast scatter's LDS staging tile), and the
+// tiles one sample workgroup of a sampled batch relation covers.
+constexpr uint64_t kPartTile = 16384;
+constexpr uint32_t kBatchSampleTiles = 16;
 // One join of a batch, as the persistent probe sees it.
 struct BatchJoin {
   const uint2 *R, *S;        // partitioned (or raw when P == 1)
   const uint32_t *r_start, *s_start;  // P + 1 each
+  const uint32_t *r_end, *s_end;      // partition ends: r_start + 1 when contiguous
   uint32_t P, s_span;
   uint2 *out;
   uint64_t out_cap;
@@ -263,6 +281,8 @@ struct BatchTile {
   uint32_t s_beg, s_n;       // S span
 };
 uint64_t batch_stripe_len(int bits);  // tuples per partition workgroup of a `bits`-bit group
+// err: the device flags; a sampled group raises kErrCapacity there when a
+// region proved too small.
 hipError_t batch_partition_group(const BatchGroup &g, uint64_t *counts, void *scan_scratch, uint32_t *err,
                                  hipStream_t stream);
 // Writes pair_join (npairs u32: the join of every (join, partition) pair),

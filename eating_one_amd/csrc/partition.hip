@@ -457,8 +457,9 @@ __global__ __launch_bounds__(kPartNT) void k_msd_hist_tiles(const uint2 *__restr
 
 // First pass, one workgroup per tile: tile t's run of digit d starts at
 // offsets[d * T + t] (the exclusive scan of the per-tile counts, digit-major).
-// RES: the runs are reserved from the region cursors res[d] instead, and
-// offsets[d] is the end of digit d's region.
+// RES: the runs are reserved from region cursors instead: grid T is a
+// multiple of 8 and the tiles of XCD range x = t / (T/8) use the cursors
+// res[x 2^BITS + d] and region ends offsets[x 2^BITS + d].
 template <int BITS, bool ORDERED, bool RES = false>
 __global__ __launch_bounds__(kFastNT) void k_tile_scatter(const uint2 *__restrict__ in, uint64_t n, int shift,
                                                           const uint32_t *__restrict__ offsets,
@@ -470,11 +471,15 @@ __global__ __launch_bounds__(kFastNT) void k_tile_scatter(const uint2 *__restric
   const uint64_t beg = (uint64_t)t * kFastTile;
   if (beg >= n) return;
   const uint64_t end = beg + kFastTile < n ? beg + kFastTile : n;
+  // RES: the tile's XCD range x owns regions [x 2^BITS, (x + 1) 2^BITS) of
+  // the ends (offsets) and cursors (res)
+  const uint32_t rx = RES ? t / (T / kXcds) : 0u;
   for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
-    sh.cursor[d] = RES ? offsets[d] : offsets[(uint64_t)d * T + t];
+    sh.cursor[d] = RES ? offsets[((uint64_t)rx << BITS) + d] : offsets[(uint64_t)d * T + t];
     sh.cnt[d] = 0;
   }
-  tile::scatter_stripe<BITS, ORDERED, RES>(sh, in, beg, end, n, shift, 0u, out, err, res);
+  tile::scatter_stripe<BITS, ORDERED, RES>(sh, in, beg, end, n, shift, 0u, out, err,
+                                           RES ? res + ((uint64_t)rx << BITS) : nullptr);
 }
 
 // counts2[b * G + g] = sum over k of partial[(g * K + k) * FT + b]: a tiled
@@ -642,44 +647,47 @@ hipError_t launch_msd2(uint64_t n, const Msd2Geo &geo, uint64_t group_len, uint3
 
 // ---- two-pass MSD partition into sampled regions ----------------------------
 // The exact two-pass partition above reads its whole input once for the
-// histogram before the first scatter.  This variant reads a sample instead:
-// of every kFastTile-tuple tile's 1024 lines of 128 bytes it reads one line
-// in each group of 16, at an offset hashed from (tile, group), so 1/16 of the
-// bytes, spread so that no arrangement that repeats with the tile or the line
-// skews it.  From the sample it sizes every first-pass region (high digit) and
-// every final partition (combined digit) at the scaled estimate plus kGapZ
-// standard deviations of the sampling error, and each scatter tile reserves
-// its runs from per-region cursors with one atomic per digit.  A region whose
-// estimate proves too small takes no tuple past its end and raises
-// kErrCapacity; the caller then redoes the join with the exact partition.
-// Every region starts on a 128-byte line, and the final partitions are
-// [part_start[p], part_end[p]) with gaps between them.
-constexpr uint32_t kGapLineTuples = 16;                           // 128-byte lines
-constexpr uint32_t kGapGroup = 16;                                // lines per sampled line
-constexpr uint32_t kGapLinesPerTile = kFastTile / kGapLineTuples / kGapGroup;  // 64
-constexpr int kGapSampleNT = kGapLinesPerTile * 8;                 // 8 threads (16-byte pairs) per line
-constexpr int kGapSampleUnroll = 8;                               // tiles in flight per workgroup
-constexpr uint32_t kGapSampleWgs = 256;                           // sample workgroups (rows of counts)
+// histogram before the first scatter.  This variant reads a sample instead
+// (tile::gap_line: one 128-byte line in 16, 1/16 of the bytes).  From it, it
+// sizes every final partition (combined digit) and every first-pass region at
+// the scaled estimate plus kGapZ standard deviations of the sampling error,
+// and each scatter tile reserves its runs from per-region cursors with one
+// atomic per digit.  The first pass keeps one region per (high digit, XCD):
+// the tiles of one XCD (a contiguous range of tiles) fill their own region of
+// each digit, so each cursor takes an eighth of the atomics and the runs that
+// meet in a cache line come from one L2.  A region whose estimate proves too
+// small takes no tuple past its end and raises kErrCapacity; the caller then
+// redoes the join with the exact partition.  Every region starts on a 128-byte
+// line, and the final partitions are [part_start[p], part_end[p]) with gaps
+// between them.
+using tile::gap_cap;
+using tile::gap_line;
+using tile::kGapLinesPerTile;
+using tile::kGapLineTuples;
+using tile::kGapPad;
+using tile::kGapSampleNT;
+using tile::kGapSampleUnroll;
+using tile::kGapZ;
+constexpr uint32_t kGapSampleWgs = 256;  // sample workgroups (rows of counts)
 constexpr uint32_t kGapLayoutNT = 1024;
-constexpr double kGapZ = 6.0;                                     // margin in standard deviations
-constexpr uint32_t kGapPad = 16;                                  // plus a fixed margin, in tuples
+// first-pass regions per high digit: one per XCD range of tiles
+constexpr uint32_t kGapRanges = kXcds;
 
-__host__ __device__ __forceinline__ uint32_t gap_line(uint32_t t, uint32_t j) {
-  uint32_t h = (t * kGapLinesPerTile + j) * 0x9e3779b1u;
-  h ^= h >> 15;
-  h *= 0x85ebca77u;
-  h ^= h >> 13;
-  return j * kGapGroup + (h >> 28);  // line of the tile, in [16 j, 16 j + 16)
-}
+// The XCD range of first-pass tile t in a grid of Tg (a multiple of 8) tiles:
+// k_tile_scatter's blockIdx -> tile mapping (xcd_local_index) gives range x
+// the tiles [x Tg/8, (x+1) Tg/8).
+__device__ __forceinline__ uint32_t gap_range(uint32_t t, uint32_t Tg) { return t / (Tg / kGapRanges); }
 
-// Sampled combined-digit counts, one row of 2^TB per workgroup (tiles b, b +
-// grid, ...: every workgroup samples the whole relation's spread).
-template <int TB>
+// Sampled counts, one row per workgroup (tiles b, b + grid, ...: every
+// workgroup samples the whole relation's spread): 2^TB combined-digit bins,
+// then the high-digit counts of each XCD range, [range][high digit].
+template <int TB, int BH>
 __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__restrict__ in4, uint64_t npairs,
-                                                             uint32_t T, uint32_t *__restrict__ rows) {
-  constexpr uint32_t FT = 1u << TB;
-  __shared__ uint32_t hist[FT];
-  for (uint32_t b = threadIdx.x; b < FT; b += kGapSampleNT) hist[b] = 0;
+                                                             uint32_t T, uint32_t Tg, uint32_t *__restrict__ rows) {
+  constexpr uint32_t FT = 1u << TB, F1 = 1u << BH, BL = TB - BH;
+  constexpr uint32_t COLS = FT + kGapRanges * F1;
+  __shared__ uint32_t hist[COLS];
+  for (uint32_t b = threadIdx.x; b < COLS; b += kGapSampleNT) hist[b] = 0;
   __syncthreads();
   const uint32_t j = threadIdx.x >> 3, q = threadIdx.x & 7;
   for (uint32_t t0 = blockIdx.x; t0 < T; t0 += gridDim.x * kGapSampleUnroll) {
@@ -695,59 +703,51 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
 #pragma unroll
     for (int u = 0; u < kGapSampleUnroll; ++u) {
       if (!ok[u]) continue;
+      const uint32_t hx = FT + gap_range(t0 + (uint32_t)u * gridDim.x, Tg) * F1;
       atomicAdd(&hist[v[u].x & (FT - 1)], 1u);
       atomicAdd(&hist[v[u].z & (FT - 1)], 1u);
+      atomicAdd(&hist[hx + ((v[u].x >> BL) & (F1 - 1))], 1u);
+      atomicAdd(&hist[hx + ((v[u].z >> BL) & (F1 - 1))], 1u);
     }
   }
   __syncthreads();
-  uint32_t *row = rows + (uint64_t)blockIdx.x * FT;
-  for (uint32_t b = threadIdx.x; b < FT; b += kGapSampleNT) row[b] = hist[b];
+  uint32_t *row = rows + (uint64_t)blockIdx.x * COLS;
+  for (uint32_t b = threadIdx.x; b < COLS; b += kGapSampleNT) row[b] = hist[b];
 }
 
-// hs[b] = sum of the G rows' counts of bin b: 64 bins per workgroup, its four
-// waves each summing a quarter of the rows (reads coalesced along the bins).
-__global__ __launch_bounds__(256) void k_gap_reduce(const uint32_t *__restrict__ rows, uint32_t G, uint32_t FT,
+// hs[b] = sum of the G rows' counts of column b: 64 columns per workgroup,
+// its four waves each summing a quarter of the rows (reads coalesced along
+// the columns).
+__global__ __launch_bounds__(256) void k_gap_reduce(const uint32_t *__restrict__ rows, uint32_t G, uint32_t cols,
                                                     uint32_t *__restrict__ hs) {
   __shared__ uint32_t part[4][64];
   const uint32_t b = blockIdx.x * 64 + (threadIdx.x & 63), qw = threadIdx.x >> 6;
   uint32_t sum = 0;
-  if (b < FT) {
+  if (b < cols) {
     const uint32_t g0 = qw * ((G + 3) / 4), g1 = min(G, g0 + (G + 3) / 4);
     uint32_t g = g0;
     for (; g + 8 <= g1; g += 8) {
       uint32_t c[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) c[u] = rows[(uint64_t)(g + u) * FT + b];
+      for (int u = 0; u < 8; ++u) c[u] = rows[(uint64_t)(g + u) * cols + b];
 #pragma unroll
       for (int u = 0; u < 8; ++u) sum += c[u];
     }
-    for (; g < g1; ++g) sum += rows[(uint64_t)g * FT + b];
+    for (; g < g1; ++g) sum += rows[(uint64_t)g * cols + b];
   }
   part[qw][threadIdx.x & 63] = sum;
   __syncthreads();
-  if (threadIdx.x < 64 && b < FT) hs[b] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+  if (threadIdx.x < 64 && b < cols)
+    hs[b] = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
 }
 
-// Region capacity for a sampled count s at scale f (tuples per sampled
-// tuple): f times the upper confidence bound of a Poisson mean given s
-// observations, chi^2(2(s + 1)) / 2 at kGapZ standard deviations by the
-// Wilson-Hilferty approximation, (s + 1) (1 - 1/(9 (s + 1)) + z/(3 sqrt(s +
-// 1)))^3 — about s + z sqrt(s) for large s and still a safe bound where a
-// partition was sampled a few times or never (thinning a count by 1/f varies
-// less than a Poisson draw) — plus kGapPad, rounded up to a whole line.
-__device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) {
-  const double m = (double)s + 1.0;
-  const double w = 1.0 - 1.0 / (9.0 * m) + kGapZ / (3.0 * sqrt(m));
-  const uint64_t k = (uint64_t)ceil(f * m * w * w * w) + kGapPad;
-  return (k + kGapLineTuples - 1) / kGapLineTuples * kGapLineTuples;
-}
-
-// One workgroup: the layout of both passes from the sampled counts hs[2^TB].
-// Final partition b: [part_start[b], re2[b]), cursor cur2[b] = part_start[b];
-// first-pass region of high digit h: [rs1[h], re1[h]), cursor cur1[h].  The
-// host sized the buffers (budget1, budget2 tuples) for any sample; a total
-// beyond them (or an empty sample) raises kErrCapacity, and every region is
-// clipped to its buffer.
+// One workgroup: the layout of both passes from the sampled counts hs.
+// Final partition b: [part_start[b], re2[b]), cursor cur2[b] = part_start[b].
+// First-pass region of (high digit h, range x), index k = x F1 + h: [rs1[k],
+// re1[k]), cursor cur1[k], laid out in (h, x) order so that the second pass
+// reads each digit's regions in a row.  The host sized the buffers (budget1,
+// budget2 tuples) for any sample; a total beyond them (or an empty sample)
+// raises kErrCapacity, and every region is clipped to its buffer.
 template <int BH, int BL>
 __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__restrict__ hs, uint64_t n,
                                                               uint64_t budget1, uint64_t budget2,
@@ -756,11 +756,11 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
                                                               uint32_t *__restrict__ part_start,
                                                               uint32_t *__restrict__ re2,
                                                               uint32_t *__restrict__ cur2, uint32_t *err) {
-  constexpr uint32_t FT = 1u << (BH + BL), F1 = 1u << BH, F2 = 1u << BL;
-  constexpr uint32_t BPT = FT >= kGapLayoutNT ? FT / kGapLayoutNT : 1;  // consecutive bins per thread (<= F2)
-  static_assert(F2 % BPT == 0, "a high digit's bins are whole threads");
+  constexpr uint32_t FT = 1u << (BH + BL), F1 = 1u << BH;
+  constexpr uint32_t BPT = FT >= kGapLayoutNT ? FT / kGapLayoutNT : 1;  // consecutive bins per thread
+  constexpr uint32_t NSEG = F1 * kGapRanges;
+  static_assert(NSEG <= kGapLayoutNT, "one first-pass region per thread");
   __shared__ uint64_t red[kGapLayoutNT / kWave + 1];
-  __shared__ uint32_t hsum[kGapLayoutNT];
   const uint32_t b0 = threadIdx.x * BPT;
   uint32_t sv[BPT];
   uint64_t loc = 0;
@@ -769,7 +769,6 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
     sv[k] = b0 + k < FT ? hs[b0 + k] : 0u;
     loc += sv[k];
   }
-  hsum[threadIdx.x] = (uint32_t)loc;
   const uint64_t ns = block_sum<kGapLayoutNT, uint64_t>(loc, red);
   const double f = ns ? (double)n / (double)ns : 0.0;
   if (ns == 0 && threadIdx.x == 0) atomic_or_agent(err, kErrCapacity);
@@ -795,54 +794,52 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
     at += c2[k];
   }
   if (threadIdx.x == 0) part_start[FT] = (uint32_t)min(tot2, budget2);
-  // first-pass regions: high digit h owns bins [h F2, (h + 1) F2), held by
-  // F2 / BPT consecutive threads
-  __syncthreads();
+  // first-pass regions: thread i = h * 8 + x
+  const uint32_t i = threadIdx.x, h = i / kGapRanges, x = i % kGapRanges;
   uint64_t c1 = 0;
-  if (threadIdx.x < F1 && ns) {
-    uint64_t sh = 0;
-    for (uint32_t i = 0; i < F2 / BPT; ++i) sh += hsum[threadIdx.x * (F2 / BPT) + i];
-    c1 = gap_cap((uint32_t)min<uint64_t>(sh, 0xffffffffull), f);
-  }
+  if (i < NSEG && ns) c1 = gap_cap(hs[FT + x * F1 + h], f);
   uint64_t tot1;
   const uint64_t a1 = block_excl_scan<kGapLayoutNT, uint64_t>(c1, red, &tot1);
   if (tot1 > budget1 && threadIdx.x == 0) atomic_or_agent(err, kErrCapacity);
-  if (threadIdx.x < F1) {
-    rs1[threadIdx.x] = (uint32_t)min(a1, budget1);
-    cur1[threadIdx.x] = (uint32_t)min(a1, budget1);
-    re1[threadIdx.x] = (uint32_t)min(a1 + c1, budget1);
+  if (i < NSEG) {
+    const uint32_t k = x * F1 + h;
+    rs1[k] = (uint32_t)min(a1, budget1);
+    cur1[k] = (uint32_t)min(a1, budget1);
+    re1[k] = (uint32_t)min(a1 + c1, budget1);
   }
 }
 
-// Second-pass work items: segment h (the first-pass region's filled part
-// [rs1[h], min(cur1[h], re1[h]))) splits into about len / L groups; grp
-// receives their exclusive prefix (grp[F1] = total) and grp[F1 + 1] the work
-// ticket, cleared.  Also raises kErrCapacity for a region that overflowed.
-__global__ __launch_bounds__(256) void k_gap_groups(const uint32_t *__restrict__ rs1,
-                                                    const uint32_t *__restrict__ cur1,
-                                                    const uint32_t *__restrict__ re1, uint32_t F1, uint64_t L,
-                                                    uint32_t *__restrict__ grp, uint32_t *err) {
-  __shared__ uint32_t red[256 / kWave + 1];
-  const uint32_t h = threadIdx.x;
+// Second-pass work items: segment i = (h, x) (the first-pass region's filled
+// part [rs1[k], min(cur1[k], re1[k])), k = x F1 + h) splits into about len / L
+// groups; grp receives their exclusive prefix over the segments in (h, x)
+// order (grp[nseg] = total) and grp[nseg + 1] the work ticket, cleared.  Also
+// raises kErrCapacity for a region that overflowed.
+__global__ __launch_bounds__(kGapLayoutNT) void k_gap_groups(const uint32_t *__restrict__ rs1,
+                                                             const uint32_t *__restrict__ cur1,
+                                                             const uint32_t *__restrict__ re1, uint32_t F1,
+                                                             uint64_t L, uint32_t *__restrict__ grp, uint32_t *err) {
+  __shared__ uint32_t red[kGapLayoutNT / kWave + 1];
+  const uint32_t i = threadIdx.x, nseg = F1 * kGapRanges;
   uint32_t J = 0;
-  if (h < F1) {
-    if (cur1[h] > re1[h]) atomic_or_agent(err, kErrCapacity);
-    const uint64_t len = (uint64_t)min(cur1[h], re1[h]) - rs1[h];
+  if (i < nseg) {
+    const uint32_t k = (i % kGapRanges) * F1 + i / kGapRanges;
+    if (cur1[k] > re1[k]) atomic_or_agent(err, kErrCapacity);
+    const uint64_t len = (uint64_t)min(cur1[k], re1[k]) - rs1[k];
     const uint64_t j = (len + L / 2) / L;
     J = (uint32_t)(j < 1 ? 1 : (j > 65536 ? 65536 : j));
   }
   uint32_t tot;
-  const uint32_t ex = block_excl_scan<256, uint32_t>(J, red, &tot);
-  if (h < F1) grp[h] = ex;
-  if (h == 0) {
-    grp[F1] = tot;
-    grp[F1 + 1] = 0;
+  const uint32_t ex = block_excl_scan<kGapLayoutNT, uint32_t>(J, red, &tot);
+  if (i < nseg) grp[i] = ex;
+  if (i == 0) {
+    grp[nseg] = tot;
+    grp[nseg + 1] = 0;
   }
 }
 
 // Second pass into the final partitions, persistent like k_msd_scatter: group
-// w of segment h is the j-th of its J equal parts (even bounds: 16-byte loads),
-// scattered by the low digit with runs reserved from cur2[h * F2 + d].
+// w of segment (h, x) is the j-th of its J equal parts (even bounds: 16-byte
+// loads), scattered by the low digit with runs reserved from cur2[h F2 + d].
 template <int BL>
 __global__ __launch_bounds__(kFastNT) void k_gap_scatter(const uint2 *__restrict__ in,
                                                          const uint32_t *__restrict__ rs1,
@@ -854,16 +851,18 @@ __global__ __launch_bounds__(kFastNT) void k_gap_scatter(const uint2 *__restrict
   constexpr uint32_t F2 = 1u << BL;
   __shared__ tile::ScatterShared<BL, false> sh;
   __shared__ uint32_t s_w;
-  const uint32_t total = grp[F1];
+  const uint32_t nseg = F1 * kGapRanges;
+  const uint32_t total = grp[nseg];
   for (;;) {
     if (threadIdx.x == 0) s_w = atomicAdd(ticket, 1u);
     __syncthreads();
     const uint32_t w = s_w;
     if (w >= total) break;  // uniform: every workgroup leaves once the tickets pass total
-    const uint32_t h = wave_find(grp, F1, w);  // grp[h] <= w < grp[h + 1]
-    const uint32_t J = grp[h + 1] - grp[h], j = w - grp[h];
-    const uint64_t b0 = rs1[h];
-    const uint64_t len = (uint64_t)min(cur1[h], re1[h]) - b0;
+    const uint32_t i = wave_find(grp, nseg, w);  // grp[i] <= w < grp[i + 1]
+    const uint32_t h = i / kGapRanges, k = (i % kGapRanges) * F1 + h;
+    const uint32_t J = grp[i + 1] - grp[i], j = w - grp[i];
+    const uint64_t b0 = rs1[k];
+    const uint64_t len = (uint64_t)min(cur1[k], re1[k]) - b0;
     const uint64_t beg = b0 + (((uint64_t)j * len / J) & ~1ull);
     const uint64_t end = j + 1 == J ? b0 + len : b0 + (((uint64_t)(j + 1) * len / J) & ~1ull);
     for (uint32_t d = threadIdx.x; d < F2; d += kFastNT) {
@@ -883,9 +882,9 @@ __global__ void k_gap_ends(const uint32_t *__restrict__ cur2, const uint32_t *__
   if (b < FT) part_end[b] = min(cur2[b], re2[b]);
 }
 
-// Tuples a relation of n needs in the first-pass (P = 2^bh regions) or final
-// (P = 2^(bh+bl) partitions) buffer for any sample.  With m = s + 1, w <= 1 +
-// z/(3 sqrt(m)) gives m w^3 <= m + z sqrt(m) + z^2/3 + z^3/27, so the
+// Tuples a relation of n needs in a buffer of P sampled regions (the first
+// pass's 8 2^bh, the final 2^(bh+bl)) for any sample.  With m = s + 1, w <= 1
+// + z/(3 sqrt(m)) gives m w^3 <= m + z sqrt(m) + z^2/3 + z^3/27, so the
 // capacities sum to at most f (ns + P + z sum(sqrt(m_b)) + P (z^2/3 +
 // z^3/27)) + P (pad + line + 1), with sum(sqrt(m_b)) <= sqrt(P (ns + P))
 // (Cauchy-Schwarz) and f ns = n; f = n / ns falls as the sample grows, so the
@@ -901,24 +900,25 @@ uint64_t gap_bound(uint64_t n, uint64_t P) {
 
 struct GapLayout {
   size_t o_rows, o_hs, o_re2, o_cur2, o_rs1, o_re1, o_cur1, o_grp, total;
-  uint32_t G;
+  uint32_t G, cols;
 };
 
 GapLayout gap_layout(uint64_t n, int bh, int bl) {
   GapLayout L{};
-  const uint64_t FT = 1ull << (bh + bl), F1 = 1ull << bh;
+  const uint64_t FT = 1ull << (bh + bl), NSEG = (1ull << bh) * kGapRanges;
   const uint64_t T = (n + kFastTile - 1) / kFastTile;
   L.G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(T, kGapSampleWgs));
+  L.cols = (uint32_t)(FT + NSEG);
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   L.o_rows = 0;
-  L.o_hs = L.o_rows + al((size_t)L.G * FT * 4);
-  L.o_re2 = L.o_hs + al(FT * 4);
+  L.o_hs = L.o_rows + al((size_t)L.G * L.cols * 4);
+  L.o_re2 = L.o_hs + al(L.cols * 4);
   L.o_cur2 = L.o_re2 + al(FT * 4);
   L.o_rs1 = L.o_cur2 + al(FT * 4);
-  L.o_re1 = L.o_rs1 + al(F1 * 4);
-  L.o_cur1 = L.o_re1 + al(F1 * 4);
-  L.o_grp = L.o_cur1 + al(F1 * 4);
-  L.total = L.o_grp + al((F1 + 2) * 4);
+  L.o_re1 = L.o_rs1 + al(NSEG * 4);
+  L.o_cur1 = L.o_re1 + al(NSEG * 4);
+  L.o_grp = L.o_cur1 + al(NSEG * 4);
+  L.total = L.o_grp + al((NSEG + 2) * 4);
   return L;
 }
 
@@ -926,7 +926,7 @@ template <int BH, int BL>
 hipError_t launch_gap(uint64_t n, uint64_t group_len, uint32_t workers, const eo_tuple *in, eo_tuple *tmp,
                       eo_tuple *out, uint32_t *part_start, uint32_t *part_end, void *scratch, uint32_t *err,
                       hipStream_t stream) {
-  constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL);
+  constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL), NSEG = F1 * kGapRanges;
   const GapLayout L = gap_layout(n, BH, BL);
   char *sc = (char *)scratch;
   uint32_t *rows = (uint32_t *)(sc + L.o_rows), *hs = (uint32_t *)(sc + L.o_hs);
@@ -934,32 +934,32 @@ hipError_t launch_gap(uint64_t n, uint64_t group_len, uint32_t workers, const eo
   uint32_t *rs1 = (uint32_t *)(sc + L.o_rs1), *re1 = (uint32_t *)(sc + L.o_re1), *cur1 = (uint32_t *)(sc + L.o_cur1);
   uint32_t *grp = (uint32_t *)(sc + L.o_grp);
   const uint32_t T = (uint32_t)((n + kFastTile - 1) / kFastTile);
-  hipError_t e;
-  EO_TIMED("part_sample", stream,
-           hipLaunchKernelGGL((k_gap_sample<BH + BL>), dim3(L.G), dim3(kGapSampleNT), 0, stream,
-                              reinterpret_cast<const uint4 *>(in), n / 2, T, rows));
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_gap_reduce, dim3((FT + 63) / 64), dim3(256), 0, stream, rows, L.G, FT, hs);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n,
-                     gap_bound(n, F1), gap_bound(n, FT), rs1, re1, cur1, part_start, re2, cur2, err);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
   // whole rounds over the XCDs (xcd_local_index is a bijection only on
   // multiples of 8); the blocks past the last tile return at once
   const uint32_t Tg = (T + kXcds - 1) / kXcds * kXcds;
+  hipError_t e;
+  EO_TIMED("part_sample", stream,
+           hipLaunchKernelGGL((k_gap_sample<BH + BL, BH>), dim3(L.G), dim3(kGapSampleNT), 0, stream,
+                              reinterpret_cast<const uint4 *>(in), n / 2, T, Tg, rows));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_gap_reduce, dim3((L.cols + 63) / 64), dim3(256), 0, stream, rows, L.G, L.cols, hs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n,
+                     gap_bound(n, NSEG), gap_bound(n, FT), rs1, re1, cur1, part_start, re2, cur2, err);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   EO_TIMED("part_scatter", stream,
            hipLaunchKernelGGL((k_tile_scatter<BH, false, true>), dim3(Tg), dim3(kFastNT), 0, stream,
                               reinterpret_cast<const uint2 *>(in), n, BL, re1, reinterpret_cast<uint2 *>(tmp), err,
                               cur1));
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_gap_groups, dim3(1), dim3(256), 0, stream, rs1, cur1, re1, F1, group_len, grp, err);
+  hipLaunchKernelGGL(k_gap_groups, dim3(1), dim3(kGapLayoutNT), 0, stream, rs1, cur1, re1, F1, group_len, grp, err);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const uint64_t max_groups = F1 + n / group_len + 1;
+  const uint64_t max_groups = NSEG + n / group_len + 1;
   const uint32_t grid = (uint32_t)std::min<uint64_t>(max_groups, std::max<uint32_t>(workers, 1));
   EO_TIMED("seg_scatter", stream,
            hipLaunchKernelGGL((k_gap_scatter<BL>), dim3(grid), dim3(kFastNT), 0, stream,
-                              reinterpret_cast<const uint2 *>(tmp), rs1, cur1, re1, F1, grp, grp + F1 + 1, re2, cur2,
-                              reinterpret_cast<uint2 *>(out), err));
+                              reinterpret_cast<const uint2 *>(tmp), rs1, cur1, re1, F1, grp, grp + NSEG + 1, re2,
+                              cur2, reinterpret_cast<uint2 *>(out), err));
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_gap_ends, dim3((FT + 255) / 256), dim3(256), 0, stream, cur2, re2, FT, part_end);
   return hipGetLastError();
@@ -1123,10 +1123,14 @@ bool gap_supported(uint64_t n, int bits_hi, int bits_lo) {
   // the 32-bit region cursors start below the buffer size and grow by at
   // most n, so they never wrap
   return gap_bound(n, 1ull << (bits_hi + bits_lo)) + n < (1ull << 32) &&
-         gap_bound(n, 1ull << bits_hi) + n < (1ull << 32);
+         gap_bound(n, (1ull << bits_hi) * kGapRanges) + n < (1ull << 32);
 }
 
-uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int /*bits_lo*/) { return gap_bound(n, 1ull << bits_hi); }
+uint64_t gap_rows(uint64_t n, uint64_t parts) { return gap_bound(n, parts); }
+
+uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int /*bits_lo*/) {
+  return gap_bound(n, (1ull << bits_hi) * kGapRanges);
+}
 
 uint64_t gap_out_rows(uint64_t n, int bits_hi, int bits_lo) { return gap_bound(n, 1ull << (bits_hi + bits_lo)); }
 

@@ -308,6 +308,13 @@ struct ScatterShared : ScatterWaveCounts<BITS, ORDERED> {
   uint32_t pbase[1 << BITS];   // this tile: output position of stage slot i (digit d) = pbase[d] + i (mod 2^32)
 };  // 160 KB at 11 bits: the whole LDS of a gfx950 workgroup
 
+// Set by wave 0 of a reserving scatter when a run of the tile passes its
+// region's end (an LDS word only the reserving kernels use).
+__device__ __forceinline__ uint32_t &scatter_dirty_flag() {
+  __shared__ uint32_t flag;
+  return flag;
+}
+
 // Scatters in[sbeg, send) (in + sbeg 16-byte aligned, or sbeg > 0) by digit
 // into out, digit d starting at sh.cursor[d] (set by the caller, followed by a
 // barrier).  Positions >= nbound are never written (kErrOverflow is raised).
@@ -476,16 +483,37 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
       if (e >= tskip && e < tn) sh.stage[slot(digit_of(cur[j].x, xm, shift, F - 1), rk[2 * j])] = make_uint2(cur[j].x, cur[j].y);
       if (e + 1 < tn) sh.stage[slot(digit_of(cur[j].z, xm, shift, F - 1), rk[2 * j + 1])] = make_uint2(cur[j].z, cur[j].w);
     }
+    // RES: the run bases arrive; a tile all of whose runs fit their regions
+    // (the usual case) writes without a per-tuple bound check
+    bool dirty = false;
     if constexpr (RES) {
       if (threadIdx.x < kWave) {
+        bool over = false;
 #pragma unroll
         for (int k = 0; k < DPL; ++k) {
           const int d = threadIdx.x * DPL + k;
-          if (d < F) sh.pbase[d] = rbase[k] - sh.tbase[d];
+          if (d < F) {
+            const uint32_t tb = sh.tbase[d];
+            const uint32_t cnt = (d + 1 < F ? sh.tbase[d + 1] : tn - tskip) - tb;
+            sh.pbase[d] = rbase[k] - tb;
+            over |= cnt != 0 && (uint64_t)rbase[k] + cnt > sh.cursor[d];
+          }
+        }
+        if constexpr (BITS <= 10) {
+          const uint64_t any = __ballot(over);
+          if (threadIdx.x == 0) scatter_dirty_flag() = any != 0;
+        } else {
+          (void)over;
         }
       }
+      __syncthreads();
+      // (11 bits: the staging tile and the digit arrays fill the whole LDS,
+      // so every tuple is checked)
+      if constexpr (BITS <= 10) dirty = __builtin_amdgcn_readfirstlane((int)scatter_dirty_flag()) != 0;
+      else dirty = true;
+    } else {
+      __syncthreads();
     }
-    __syncthreads();
     // the next tile: its first kPre loads arrived during this tile's rank
     // phase, the rest are read while this tile's runs are written
 #pragma unroll
@@ -498,7 +526,8 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
       const uint32_t d = digit_of(t.x, xm, shift, F - 1);
       const uint64_t pos = (uint32_t)(sh.pbase[d] + i);
       if constexpr (RES) {
-        if (pos < sh.cursor[d]) out[pos] = t;
+        // (the bound is read only when some run of the tile passes its region's end)
+        if (!dirty || pos < sh.cursor[d]) out[pos] = t;
         else atomic_or_agent(err, kErrCapacity);  // the region's estimate was too small
       } else {
         if (pos < nbound) out[pos] = t;
@@ -508,6 +537,42 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
   }
   __syncthreads();
 }
+
+// ---- sampled partition sizes ------------------------------------------------------
+// A relation's partition sizes estimated from a sample: of every kFastTile-
+// tuple tile's 1024 lines of 128 bytes one line in each group of 16 is read,
+// at an offset hashed from (tile, group), so 1/16 of the bytes, spread so that
+// no arrangement that repeats with the tile or the line skews it.
+constexpr uint32_t kGapLineTuples = 16;                                        // 128-byte lines
+constexpr uint32_t kGapGroup = 16;                                             // lines per sampled line
+constexpr uint32_t kGapLinesPerTile = kFastTile / kGapLineTuples / kGapGroup;  // 64
+constexpr int kGapSampleNT = kGapLinesPerTile * 8;  // 8 threads (16-byte pairs) per line
+constexpr int kGapSampleUnroll = 8;                 // tiles in flight per workgroup
+constexpr double kGapZ = 6.0;                       // margin in standard deviations
+constexpr uint32_t kGapPad = 16;                    // plus a fixed margin, in tuples
+
+__host__ __device__ __forceinline__ uint32_t gap_line(uint32_t t, uint32_t j) {
+  uint32_t h = (t * kGapLinesPerTile + j) * 0x9e3779b1u;
+  h ^= h >> 15;
+  h *= 0x85ebca77u;
+  h ^= h >> 13;
+  return j * kGapGroup + (h >> 28);  // line of the tile, in [16 j, 16 j + 16)
+}
+
+// Region capacity for a sampled count s at scale f (tuples per sampled
+// tuple): f times the upper confidence bound of a Poisson mean given s
+// observations, chi^2(2(s + 1)) / 2 at kGapZ standard deviations by the
+// Wilson-Hilferty approximation, (s + 1) (1 - 1/(9 (s + 1)) + z/(3 sqrt(s +
+// 1)))^3 — about s + z sqrt(s) for large s and still a safe bound where a
+// partition was sampled a few times or never (thinning a count by 1/f varies
+// less than a Poisson draw) — plus kGapPad, rounded up to a whole line.
+__device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) {
+  const double m = (double)s + 1.0;
+  const double w = 1.0 - 1.0 / (9.0 * m) + kGapZ / (3.0 * sqrt(m));
+  const uint64_t k = (uint64_t)ceil(f * m * w * w * w) + kGapPad;
+  return (k + kGapLineTuples - 1) / kGapLineTuples * kGapLineTuples;
+}
+
 
 // ---- LDS histogram of one stripe --------------------------------------------------
 // Adds the digit counts of in[beg, end) to the LDS array hist[2^BITS] with LDS

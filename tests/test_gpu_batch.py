@@ -313,3 +313,82 @@ def test_batch_result16_and_left_outer(ctx):
     finally:
         b.free()
         table.free()
+
+
+def test_batch_sampled_partitions_equal_oracle_and_replay(ctx):
+    """Relations of 2^20 tuples or more with up to 8 radix bits, in an inner
+    batch of 256 requests or more, take one-pass partitions sized from a 1/16
+    sample (runs reserved with atomics, gaps between partitions): every
+    request equals the oracle and its solo join, beside small requests, and
+    graph replays repeat the rows."""
+    inputs = [gen.c2(seed=401, nR=(1 << 20) + 333, nS=(1 << 21) + 17, selectivity=0.8),
+              gen.c2(seed=402, nR=(1 << 21) + 5, nS=(1 << 20) + 9, selectivity=1.0),  # built on S
+              gen.c2(seed=403, nR=1 << 20, nS=1 << 22, theta=0.9)]
+    inputs += _mixed_inputs(404, 6, lo=10, hi=18) + _mixed_inputs(406, 250, lo=6, hi=10)
+    reqs = [_Req(ctx, R, S) for R, S in inputs]
+    st = eo.JoinStats()
+    b = ctx.batch()
+    res = b.run([r.request(tag=i) for i, r in enumerate(reqs)], stats=st)
+    assert st.sampled == 1
+    eager = []
+    for i, (r, (tag, n, s)) in enumerate(zip(reqs, res)):
+        assert s == 0 and n == len(r.want), (i, len(r.R), len(r.S))
+        got = r.pairs(n)
+        assert same_multiset(got, r.want), i
+        if i < 3:
+            assert same_multiset(ctx.hash_join(r.R, r.S), got), i
+        eager.append(got)
+    for _ in range(2):
+        b.replay()
+        res2 = b.wait()
+        for i, (r, (tag, n, s)) in enumerate(zip(reqs, res2)):
+            assert s == 0 and n == len(r.want), i
+            assert same_multiset(r.pairs(n), eager[i]), i
+    b.free()
+
+
+def _sampled_line(t, j):
+    """Line of tile t that the partition sample reads for group j (restated to
+    build an input that defeats the sample)."""
+    h = ((t * 64 + j) * 0x9E3779B1) & 0xFFFFFFFF
+    h ^= h >> 15
+    h = (h * 0x85EBCA77) & 0xFFFFFFFF
+    h ^= h >> 13
+    return j * 16 + (h >> 28)
+
+
+def test_batch_sampled_partition_overflow_redoes_the_batch_exactly(ctx):
+    """One request's S holds keys spread over all partitions in every line the
+    sample reads and one hot key everywhere else: its hot partition's region
+    overflows, and waiting for the batch redoes it with exact partitions
+    (stats.sampled == 2); every request still equals the oracle.  A left
+    outer batch keeps exact partitions."""
+    rng = np.random.default_rng(0x5E)
+    n = 1 << 21
+    keys = np.full(n, 5, dtype=np.int32)
+    for t in range(n // 16384):
+        for j in range(64):
+            base = t * 16384 + _sampled_line(t, j) * 16
+            keys[base:base + 16] = rng.integers(0, 1 << 20, size=16, dtype=np.int32)
+    S = np.stack([keys, np.arange(n, dtype=np.int32)], 1)
+    R = np.stack([rng.permutation(1 << 20).astype(np.int32), np.arange(1 << 20, dtype=np.int32)], 1)
+    inputs = [(R, S)] + _mixed_inputs(405, 4, lo=12, hi=16) + _mixed_inputs(407, 251, lo=6, hi=10)
+    reqs = [_Req(ctx, r, s) for r, s in inputs]
+    st = eo.JoinStats()
+    b = ctx.batch()
+    res = b.run([r.request(tag=i) for i, r in enumerate(reqs)], stats=st)
+    assert st.sampled == 2
+    for i, (r, (tag, n_out, s)) in enumerate(zip(reqs, res)):
+        assert s == 0 and n_out == len(r.want), i
+        assert same_multiset(r.pairs(n_out), r.want), i
+    # a left outer batch, and a batch of few requests, keep exact partitions
+    st2 = eo.JoinStats()
+    b.run([r.request(tag=i, count_only=True) for i, r in enumerate(reqs)], stats=st2, left_outer=True)
+    assert st2.sampled == 0
+    st3 = eo.JoinStats()
+    res = b.run([r.request(tag=i) for i, r in enumerate(reqs[:5])], stats=st3)
+    assert st3.sampled == 0
+    for i, (r, (tag, n_out, s)) in enumerate(zip(reqs[:5], res)):
+        assert s == 0 and n_out == len(r.want), i
+        assert same_multiset(r.pairs(n_out), r.want), i
+    b.free()

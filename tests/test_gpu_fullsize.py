@@ -23,12 +23,17 @@ def test_c2_full_size_properties(ctx):
     R, S = gen.c2(seed=0xE01C2, nR=nR, nS=nS)
     dR, dS = eo.DeviceArray.from_host(ctx, R), eo.DeviceArray.from_host(ctx, S)
     dout = eo.DeviceArray(ctx, nS * 8)
-    n, st = ctx.hash_join_device(dR, nR, dS, nS, dout, nS)
-    assert st == 0 and n == nS
-    pairs = dout.to_host(np.int32, (nS, 2))
+    # the default plan (14 bits, two passes) sizes its partitions from a sample;
+    # EO_F_EXACT_PARTITIONS takes the histogram pass: both give every row
+    for flags, sampled in ((0, 1), (_abi.EO_F_EXACT_PARTITIONS, 0)):
+        js = eo.JoinStats()
+        n, st = ctx.hash_join_device(dR, nR, dS, nS, dout, nS, flags=flags, stats=js)
+        assert st == 0 and n == nS
+        assert js.passes == 2 and js.sampled == sampled
+        pairs = dout.to_host(np.int32, (nS, 2))
+        assert np.array_equal(np.bincount(pairs[:, 1], minlength=nS), np.ones(nS, dtype=np.int64))
+        assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])
     dout.close()
-    assert np.array_equal(np.bincount(pairs[:, 1], minlength=nS), np.ones(nS, dtype=np.int64))
-    assert np.array_equal(R[pairs[:, 0], 0], S[pairs[:, 1], 0])
 
 
 def test_c4_full_size_learned_equals_binary(ctx):
@@ -90,7 +95,7 @@ def test_two_pass_half_tables_outer_and_deterministic(ctx):
         st = eo.JoinStats()
         n, status = ctx.hash_join_device(dR, nR, dS, nS, dout, nS, stats=st)
         assert status == 0 and n == int(present.sum())
-        assert st.lds_tuples == 8192 and st.passes == 2 and st.radix_bits == 13
+        assert st.lds_tuples == 8192 and st.passes == 2 and st.radix_bits == 13 and st.sampled == 1
         pairs = dout.to_host(np.int32, (n, 2))
         cnt = np.bincount(pairs[:, 1], minlength=nS)
         assert cnt.max() == 1 and np.array_equal(cnt.astype(bool), present)
@@ -142,6 +147,8 @@ def test_c2_full_size_variants(ctx, theta, sel, bits, passes):
         assert status == 0 and n == n_present
         if bits:
             assert st.radix_bits == bits and st.passes == passes
+        # two-pass plans size their partitions from a sample (skewed keys included)
+        assert st.sampled == (1 if st.passes == 2 else 0)
         pairs = dout.to_host(np.int32, (n, 2))
         cnt = np.bincount(pairs[:, 1], minlength=nS)
         assert cnt.max() == 1 and np.array_equal(cnt.astype(bool), present)
