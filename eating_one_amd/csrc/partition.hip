@@ -809,11 +809,23 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
   }
 }
 
-// Second-pass work items: segment i = (h, x) (the first-pass region's filled
-// part [rs1[k], min(cur1[k], re1[k])), k = x F1 + h) splits into about len / L
-// groups; grp receives their exclusive prefix over the segments in (h, x)
-// order (grp[nseg] = total) and grp[nseg + 1] the work ticket, cleared.  Also
-// raises kErrCapacity for a region that overflowed.
+// Second-pass queues: the segments of high digit h go to queue h % Q (Q =
+// min(8, F1)), whose groups a workgroup on XCD q takes first, so that the runs
+// of one final partition are written from one XCD (their partial lines merge
+// in its L2); a workgroup whose queue is empty takes groups from the others.
+// Segment i in queue order: i = ((h % Q) (F1 / Q) + h / Q) 8 + x, for
+// first-pass region (h, x), k = x F1 + h.
+__host__ __device__ __forceinline__ uint32_t gap_queues(uint32_t F1) { return F1 < kXcds ? F1 : kXcds; }
+__device__ __forceinline__ uint32_t gap_seg_digit(uint32_t i, uint32_t F1) {
+  const uint32_t Q = gap_queues(F1), r = i / kGapRanges;  // r = (h % Q) (F1 / Q) + h / Q
+  return (r % (F1 / Q)) * Q + r / (F1 / Q);
+}
+
+// Second-pass work items: segment i (the first-pass region's filled part
+// [rs1[k], min(cur1[k], re1[k]))) splits into about len / L groups; grp
+// receives their exclusive prefix over the segments in queue order
+// (grp[nseg] = total), and grp[nseg + 1 + q], the ticket of queue q, is
+// cleared.  Also raises kErrCapacity for a region that overflowed.
 __global__ __launch_bounds__(kGapLayoutNT) void k_gap_groups(const uint32_t *__restrict__ rs1,
                                                              const uint32_t *__restrict__ cur1,
                                                              const uint32_t *__restrict__ re1, uint32_t F1,
@@ -822,7 +834,7 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_groups(const uint32_t *__r
   const uint32_t i = threadIdx.x, nseg = F1 * kGapRanges;
   uint32_t J = 0;
   if (i < nseg) {
-    const uint32_t k = (i % kGapRanges) * F1 + i / kGapRanges;
+    const uint32_t k = (i % kGapRanges) * F1 + gap_seg_digit(i, F1);
     if (cur1[k] > re1[k]) atomic_or_agent(err, kErrCapacity);
     const uint64_t len = (uint64_t)min(cur1[k], re1[k]) - rs1[k];
     const uint64_t j = (len + L / 2) / L;
@@ -831,35 +843,44 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_groups(const uint32_t *__r
   uint32_t tot;
   const uint32_t ex = block_excl_scan<kGapLayoutNT, uint32_t>(J, red, &tot);
   if (i < nseg) grp[i] = ex;
-  if (i == 0) {
-    grp[nseg] = tot;
-    grp[nseg + 1] = 0;
-  }
+  if (i == 0) grp[nseg] = tot;
+  if (i < kXcds) grp[nseg + 1 + i] = 0;
 }
 
-// Second pass into the final partitions, persistent like k_msd_scatter: group
-// w of segment (h, x) is the j-th of its J equal parts (even bounds: 16-byte
-// loads), scattered by the low digit with runs reserved from cur2[h F2 + d].
+// Second pass into the final partitions, persistent like k_msd_scatter, with
+// one work queue per XCD (above): group w of segment (h, x) is the j-th of its
+// J equal parts (even bounds: 16-byte loads), scattered by the low digit with
+// runs reserved from cur2[h F2 + d].
 template <int BL>
 __global__ __launch_bounds__(kFastNT) void k_gap_scatter(const uint2 *__restrict__ in,
                                                          const uint32_t *__restrict__ rs1,
                                                          const uint32_t *__restrict__ cur1,
                                                          const uint32_t *__restrict__ re1, uint32_t F1,
-                                                         const uint32_t *__restrict__ grp, uint32_t *ticket,
+                                                         const uint32_t *__restrict__ grp, uint32_t *tickets,
                                                          const uint32_t *__restrict__ re2, uint32_t *cur2,
                                                          uint2 *__restrict__ out, uint32_t *err) {
   constexpr uint32_t F2 = 1u << BL;
   __shared__ tile::ScatterShared<BL, false> sh;
   __shared__ uint32_t s_w;
-  const uint32_t nseg = F1 * kGapRanges;
-  const uint32_t total = grp[nseg];
+  const uint32_t nseg = F1 * kGapRanges, Q = gap_queues(F1), spq = nseg / Q;  // segments per queue
+  const uint32_t home = (blockIdx.x % kXcds) % Q;  // the hardware deals workgroup b to XCD b % 8
+  uint32_t qi = 0;                                 // queues visited: home, home + 1, ...
   for (;;) {
-    if (threadIdx.x == 0) s_w = atomicAdd(ticket, 1u);
+    const uint32_t q = (home + qi) % Q;
+    const uint32_t qb = grp[q * spq], qe = grp[(q + 1) * spq];
+    if (threadIdx.x == 0) {
+      const uint32_t t = atomicAdd(tickets + q, 1u);
+      s_w = t < qe - qb ? qb + t : ~0u;
+    }
     __syncthreads();
     const uint32_t w = s_w;
-    if (w >= total) break;  // uniform: every workgroup leaves once the tickets pass total
+    __syncthreads();  // s_w is written again by the next round
+    if (w == ~0u) {   // uniform: this queue is drained, go on to the next one
+      if (++qi == Q) break;
+      continue;
+    }
     const uint32_t i = wave_find(grp, nseg, w);  // grp[i] <= w < grp[i + 1]
-    const uint32_t h = i / kGapRanges, k = (i % kGapRanges) * F1 + h;
+    const uint32_t h = gap_seg_digit(i, F1), k = (i % kGapRanges) * F1 + h;
     const uint32_t J = grp[i + 1] - grp[i], j = w - grp[i];
     const uint64_t b0 = rs1[k];
     const uint64_t len = (uint64_t)min(cur1[k], re1[k]) - b0;
@@ -871,7 +892,7 @@ __global__ __launch_bounds__(kFastNT) void k_gap_scatter(const uint2 *__restrict
     }
     __syncthreads();
     tile::scatter_stripe<BL, false, true>(sh, in, beg, end, 0, 0, 0u, out, err, cur2 + (uint64_t)h * F2);
-    __syncthreads();  // sh and s_w are reused by the next group
+    __syncthreads();  // sh is reused by the next group
   }
 }
 
@@ -918,7 +939,7 @@ GapLayout gap_layout(uint64_t n, int bh, int bl) {
   L.o_re1 = L.o_rs1 + al(NSEG * 4);
   L.o_cur1 = L.o_re1 + al(NSEG * 4);
   L.o_grp = L.o_cur1 + al(NSEG * 4);
-  L.total = L.o_grp + al((NSEG + 2) * 4);
+  L.total = L.o_grp + al((NSEG + 1 + kXcds) * 4);
   return L;
 }
 
