@@ -94,3 +94,16 @@ def test_underestimated_region_redoes_the_join_exactly(ctx, side):
     want = oracle.hash_join(R, S, nparts=1 << 12)
     assert len(got) == len(want) == join_count_closed_form(R, S)
     assert same_multiset(got, want)
+
+
+@pytest.mark.parametrize("nR,nS", [(300_001, (1 << 22) + 3), ((1 << 21) + 9, 500_003)])
+def test_one_side_sampled_the_other_exact(ctx, nR, nS):
+    """A relation under 2^20 tuples keeps exact (contiguous) partitions while
+    the other one's are sized from the sample: the probe reads each side's
+    partitions through its own [start, end) arrays."""
+    R, S = gen.c3(seed=nR % 97, nR=nR, nS=nS, key_range=1 << 20)
+    got, st = _join(ctx, R, S, radix_bits=14, passes=2)
+    assert st.passes == 2 and st.sampled == 1
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == join_count_closed_form(R, S)
+    assert same_multiset(got, want)
