@@ -217,6 +217,14 @@ def run_c2(a):
     traffic = read_traffic(dom_name) if a.log2_r == 27 else None
     alg_bytes = 24 * (nR + nS) + 8 * n_out
     pipe = alg_bytes / dt / 1e9
+    # HBM bytes a join actually moves: the committed FETCH_SIZE / WRITE_SIZE
+    # figures per launch of each kernel family x this run's launches per join
+    # (the families carry all but a few MB of a join's traffic)
+    join_hbm = None
+    if a.log2_r == 27:
+        per = {k: read_traffic(k) for k in kern}
+        if per and all(per[k] is not None for k in kern):
+            join_hbm = sum(per[k] * kern[k][1] / a.steps for k in kern)
     ceiling = copy_ceiling(ctx)
     res = {
         "metric": "join tuples/s & %HBM peak, 2^27\u22c82^28 hash join; p99 latency, batched joins",
@@ -243,6 +251,9 @@ def run_c2(a):
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
                      "kernel": dom_name, "alg_bytes_per_launch": dom_bytes, "ms_per_launch": dom_mean_ms},
         "pipeline": {"alg_bytes": alg_bytes, "achieved_gbs": pipe, "frac": pipe / HBM_PEAK_GBS,
+                     "hbm_bytes_per_join": join_hbm,
+                     "hbm_gbs": join_hbm / dt / 1e9 if join_hbm else None,
+                     "hbm_frac": join_hbm / dt / 1e9 / HBM_PEAK_GBS if join_hbm else None,
                      "kernels_ms_per_join": {k: v[0] / a.steps for k, v in kern.items()},
                      # every kernel family: algorithmic bytes per launch / mean launch time
                      "kernels_gbs": {k: kernel_alg_bytes(k, nR, nS, n_out) / (v[0] / v[1] * 1e-3) / 1e9

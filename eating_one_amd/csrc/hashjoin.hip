@@ -16,6 +16,7 @@
 
 #include "device.hpp"
 #include "kernels.hpp"
+#include "plan.hpp"
 #include "tile_kernels.hpp"
 
 namespace eo {
@@ -148,6 +149,9 @@ size_t join_plan_scratch_bytes(uint32_t parts) { return scan_scratch_bytes_u32((
 }  // namespace eo
 
 namespace eo {
+
+static_assert(tile::ProbeBig::Cap == kFusedTableFull && tile::ProbeSmall::Cap == kFusedTableHalf,
+              "the planner's table sizes are the probe's");
 
 uint32_t join_fused_cap(uint32_t lds_cap) {
   return lds_cap <= 8192 ? (uint32_t)tile::ProbeSmall::Cap : (uint32_t)tile::ProbeBig::Cap;

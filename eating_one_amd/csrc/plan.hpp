@@ -26,6 +26,16 @@ struct HashPlan {
 
 constexpr int kMaxJoinBits = 22;
 
+// R tuples one fused-probe LDS table holds for a planned lds_cap: the full
+// table (lds_cap 16384) 17408, the half table (lds_cap <= 8192) 8704.
+constexpr uint32_t kFusedTableFull = 17408, kFusedTableHalf = 8704;
+inline uint32_t fused_table_tuples(uint32_t lds_cap) { return lds_cap <= 8192 ? kFusedTableHalf : kFusedTableFull; }
+// An unpartitioned join (every S tile probes each R chunk of one LDS table)
+// when R fills at most this many tables and the S tuples read across them stay
+// within kUnpartitionedSReads.
+constexpr uint64_t kUnpartitionedChunks = 4;
+constexpr uint64_t kUnpartitionedSReads = 1ull << 22;
+
 // Returns EO_OK and fills *out, or EO_EINVAL for inconsistent overrides.
 eo_status plan_hash_join(uint64_t nR, uint64_t nS, const eo_join_opts *opts, HashPlan *out, int cus = 256);
 

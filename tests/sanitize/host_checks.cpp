@@ -281,6 +281,11 @@ static void check_planner() {
       CHECK(hp.s_span >= 8192 && hp.s_span <= (1u << 18) && (hp.s_span & (hp.s_span - 1)) == 0);
       // mean R partition within one LDS table
       if (hp.bits > 0 && hp.bits < eo::kMaxJoinBits) CHECK((nR >> hp.bits) <= hp.lds_cap);
+      // unpartitioned: R in one table, or in a few tables beside a small S
+      if (hp.bits == 0 && nR > hp.lds_cap) {
+        const uint64_t chunks = (nR + eo::fused_table_tuples(hp.lds_cap) - 1) / eo::fused_table_tuples(hp.lds_cap);
+        CHECK(chunks <= eo::kUnpartitionedChunks && chunks * nS <= eo::kUnpartitionedSReads);
+      }
     }
   // overrides: accepted when consistent, refused otherwise
   eo_join_opts o{};
