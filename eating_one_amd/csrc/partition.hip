@@ -765,11 +765,21 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
   const uint32_t b0 = threadIdx.x * BPT;
   uint32_t sv[BPT];
   uint64_t loc = 0;
+  if constexpr (BPT % 4 == 0) {  // whole 16-byte vectors of consecutive bins
 #pragma unroll
-  for (uint32_t k = 0; k < BPT; ++k) {
-    sv[k] = b0 + k < FT ? hs[b0 + k] : 0u;
-    loc += sv[k];
+    for (uint32_t k = 0; k < BPT; k += 4) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(hs + b0 + k);
+      sv[k] = v.x;
+      sv[k + 1] = v.y;
+      sv[k + 2] = v.z;
+      sv[k + 3] = v.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < BPT; ++k) sv[k] = b0 + k < FT ? hs[b0 + k] : 0u;
   }
+#pragma unroll
+  for (uint32_t k = 0; k < BPT; ++k) loc += sv[k];
   const uint64_t ns = block_sum<kGapLayoutNT, uint64_t>(loc, red);
   const double f = ns ? (double)n / (double)ns : 0.0;
   if (ns == 0 && threadIdx.x == 0) atomic_or_agent(err, kErrCapacity);
@@ -784,15 +794,30 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
   uint64_t tot2;
   uint64_t at = block_excl_scan<kGapLayoutNT, uint64_t>(sum2, red, &tot2);
   if (tot2 > budget2 && threadIdx.x == 0) atomic_or_agent(err, kErrCapacity);
+  uint32_t va[BPT], ve[BPT];
 #pragma unroll
   for (uint32_t k = 0; k < BPT; ++k) {
-    if (b0 + k < FT) {
-      const uint64_t a = min(at, budget2), e = min(at + c2[k], budget2);
-      part_start[b0 + k] = (uint32_t)a;
-      cur2[b0 + k] = (uint32_t)a;
-      re2[b0 + k] = (uint32_t)e;
-    }
+    va[k] = (uint32_t)min(at, budget2);
+    ve[k] = (uint32_t)min(at + c2[k], budget2);
     at += c2[k];
+  }
+  if constexpr (BPT % 4 == 0) {
+#pragma unroll
+    for (uint32_t k = 0; k < BPT; k += 4) {
+      const uint4 a4 = make_uint4(va[k], va[k + 1], va[k + 2], va[k + 3]);
+      *reinterpret_cast<uint4 *>(part_start + b0 + k) = a4;
+      *reinterpret_cast<uint4 *>(cur2 + b0 + k) = a4;
+      *reinterpret_cast<uint4 *>(re2 + b0 + k) = make_uint4(ve[k], ve[k + 1], ve[k + 2], ve[k + 3]);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < BPT; ++k) {
+      if (b0 + k < FT) {
+        part_start[b0 + k] = va[k];
+        cur2[b0 + k] = va[k];
+        re2[b0 + k] = ve[k];
+      }
+    }
   }
   if (threadIdx.x == 0) part_start[FT] = (uint32_t)min(tot2, budget2);
   // first-pass regions: thread i = h * 8 + x

@@ -566,9 +566,12 @@ __host__ __device__ __forceinline__ uint32_t gap_line(uint32_t t, uint32_t j) {
 // 1)))^3 — about s + z sqrt(s) for large s and still a safe bound where a
 // partition was sampled a few times or never (thinning a count by 1/f varies
 // less than a Poisson draw) — plus kGapPad, rounded up to a whole line.
+// (w in single precision, raised by 2^-20 to cover its rounding; the
+// product in double)
 __device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) {
-  const double m = (double)s + 1.0;
-  const double w = 1.0 - 1.0 / (9.0 * m) + kGapZ / (3.0 * sqrt(m));
+  const float mf = (float)s + 1.0f;
+  const float wf = (1.0f - 1.0f / (9.0f * mf) + (float)kGapZ / (3.0f * sqrtf(mf))) * (1.0f + 0x1p-20f);
+  const double m = (double)s + 1.0, w = (double)wf;
   const uint64_t k = (uint64_t)ceil(f * m * w * w * w) + kGapPad;
   return (k + kGapLineTuples - 1) / kGapLineTuples * kGapLineTuples;
 }
