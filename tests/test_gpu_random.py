@@ -87,3 +87,37 @@ def test_random_stream_join(ctx, seed):
             continue
         got = ctx.stream_join(R, S, op=op, r_tile=rt, s_chunk=sc, left_outer=outer)
         assert same_multiset(got, want), (op, rt, sc)
+
+
+def _sampled_case(seed):
+    """Relations of 2^20 to 2^21.5 tuples (both sampled), any two-pass split,
+    random key range and skew: the sampled partitions' path."""
+    rng = np.random.default_rng(seed)
+    nR = int(2 ** rng.uniform(20, 21.5))
+    nS = int(2 ** rng.uniform(20, 21.5))
+    kr = int(2 ** rng.uniform(12, 31))
+    kr = max(kr, nR * nS // (1 << 24) + 1)  # at most ~16 M pairs
+    lo = int(rng.integers(KEY_MIN, KEY_MAX - kr)) if rng.random() < 0.5 else 0
+    R = np.stack([rng.integers(lo, lo + kr, nR, dtype=np.int64), np.arange(nR)], 1).astype(np.int32)
+    if rng.random() < 0.5:  # S keys skewed towards a few hot values (few R duplicates: bounded pairs)
+        kr = max(kr, nR // 4)
+        lo = min(lo, KEY_MAX - kr)
+        R[:, 0] = rng.integers(lo, lo + kr, nR, dtype=np.int64).astype(np.int32)
+        hot = rng.integers(lo, lo + kr, 16, dtype=np.int64)
+        pick = np.where(rng.random(nS) < 0.3, hot[rng.integers(0, 16, nS)], rng.integers(lo, lo + kr, nS, dtype=np.int64))
+    else:
+        pick = rng.integers(lo, lo + kr, nS, dtype=np.int64)
+    S = np.stack([pick, np.arange(nS) + 7], 1).astype(np.int32)
+    bits = int(rng.integers(2, 15))
+    return R, S, dict(radix_bits=bits, passes=2, lds_tuples=int(rng.choice([0, 8192, 16384])))
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_sampled_partitions(ctx, seed):
+    import eating_one_amd as eo
+
+    R, S, kw = _sampled_case(5000 + seed)
+    st = eo.JoinStats()
+    got = ctx.hash_join(R, S, stats=st, **kw)
+    assert st.sampled in (1, 2), kw  # sampled, or sampled and redone exactly
+    assert same_multiset(got, oracle.hash_join(R, S, nparts=1 << 10)), kw
