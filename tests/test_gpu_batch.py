@@ -381,6 +381,13 @@ def test_batch_sampled_partition_overflow_redoes_the_batch_exactly(ctx):
     for i, (r, (tag, n_out, s)) in enumerate(zip(reqs, res)):
         assert s == 0 and n_out == len(r.want), i
         assert same_multiset(r.pairs(n_out), r.want), i
+    # replays of the batch (captured from its exact redo) repeat every row
+    for _ in range(2):
+        b.replay()
+        res2 = b.wait()
+        for i, (r, (tag, n_out, s)) in enumerate(zip(reqs, res2)):
+            assert s == 0 and n_out == len(r.want), i
+            assert same_multiset(r.pairs(n_out), r.want), i
     # a left outer batch, and a batch of few requests, keep exact partitions
     st2 = eo.JoinStats()
     b.run([r.request(tag=i, count_only=True) for i, r in enumerate(reqs)], stats=st2, left_outer=True)

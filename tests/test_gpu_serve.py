@@ -209,3 +209,43 @@ def test_serve_cancel_from_another_thread(ctx):
     assert st["wall_ms"] < 350  # the loop ended at the cancellation, not at the last arrival
     for _, t in tables:
         t.free()
+
+
+def _defeating_probe(rng, n, key_range):
+    """Probe keys that defeat the partition sample: every line the sample
+    reads (one 128-byte line in each group of 16, offsets hashed from tile and
+    group) holds spread keys, every other line the key 5."""
+    keys = np.full(n, 5, dtype=np.int32)
+    for t in range(n // 16384):
+        for j in range(64):
+            h = ((t * 64 + j) * 0x9E3779B1) & 0xFFFFFFFF
+            h ^= h >> 15
+            h = (h * 0x85EBCA77) & 0xFFFFFFFF
+            h ^= h >> 13
+            b = t * 16384 + (j * 16 + (h >> 28)) * 16
+            keys[b:b + 16] = rng.integers(0, key_range, size=16, dtype=np.int32)
+    return np.stack([keys, np.arange(n, dtype=np.int32)], 1)
+
+
+def test_serve_redoes_an_overflowing_sampled_batch(ctx):
+    """A served batch of 256+ requests samples its large relations; one
+    request's probe side defeats the sample, the batch is redone with exact
+    partitions inside the loop, and every result still equals the oracle."""
+    rng = np.random.default_rng(0x5F)
+    R = np.stack([rng.permutation(1 << 20).astype(np.int32), np.arange(1 << 20, dtype=np.int32)], 1)
+    S = _defeating_probe(rng, 1 << 21, 1 << 20)
+    want0 = oracle.hash_join(R, S, nparts=1 << 12)
+    dR, dS = DeviceArray.from_host(ctx, R), DeviceArray.from_host(ctx, S)
+    dout = DeviceArray(ctx, len(want0) * 8)
+    tables, items, reqs = _stream(ctx, 51, 259, 6, 11, n_tables=2)
+    reqs = [eo.request(R=dR, nR=len(R), S=dS, nS=len(S), out=dout, out_cap=len(want0), tag=1000)] + reqs
+    res, lat, st = ctx.serve(reqs, None, mode=eo._abi.EO_SERVE_BATCHED)
+    tag, n, status = res[0]
+    assert tag == 1000 and status == 0 and n == len(want0)
+    assert same_multiset(dout.to_host(np.int32, (n, 2)), want0)
+    for i, ((tag, n, status), it) in enumerate(zip(res[1:], items)):
+        assert status == 0 and n == len(it["want"]), i
+        got = it["dout"].to_host(np.int32, (n, 2)) if n else np.empty((0, 2), np.int32)
+        assert same_multiset(got, it["want"]), i
+    for _, t in tables:
+        t.free()
