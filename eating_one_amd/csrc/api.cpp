@@ -623,8 +623,12 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
     }
     A = fx->base;
   } else {
-    EO_TRY_HIP(hipStreamSynchronize(s), "sync");
-    EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+    // growing the arena frees the old one: nothing may still use it; when it
+    // is large enough, this call's work follows earlier work on the stream
+    if (L.total > c->arena.cap) {
+      EO_TRY_HIP(hipStreamSynchronize(s), "sync");
+      EO_TRY_HIP(c->arena.reserve(L.total), "arena");
+    }
     A = c->arena.base;
   }
   const eo_tuple *dR = host ? (eo_tuple *)(A + o_rin) : R;
