@@ -824,6 +824,29 @@ def run_copy(a):
             "sweep": rows if a.copy_all else None}
 
 
+def other_configs(a):
+    """The other configurations' GPU numbers, measured in the same run as the
+    headline (their CPU baselines and full records: bench.py --config ...)."""
+    b = argparse.Namespace(**vars(a))
+    b.no_cpu = True
+    b.steps, b.warmup = 5, 2
+    out = {}
+    c1 = run_c1(b)
+    out["c1"] = {"workload": "C1 hash join 2^16 x 2^20", "ms_per_join": c1["ms_per_step"], "tuples_per_s": c1["value"]}
+    nlj = run_c3_nlj(b)
+    out["c3_nlj"] = {"workload": "C3 block NLJ 2^18 x 2^18", "ms_per_join": nlj["ms_per_step"],
+                     "compares_per_s": nlj["value"], "frac_of_nominal": nlj["roofline"]["frac"],
+                     "frac_of_compare_loop": nlj["roofline"].get("frac_of_compare_loop")}
+    hidx = run_c3_hidx(b)
+    out["c3_hidx"] = {"workload": "C3 hash-index join 2^18 x 2^18", "ms_per_probe_pass": hidx["ms_per_step"],
+                      "probes_per_s": hidx["value"], "build_ms": hidx.get("build_ms")}
+    c4 = run_c4(b)
+    out["c4"] = {"workload": "C4 sorted index 2^26 lognormal keys, 2^27 probes",
+                 "learned_lookup_ms": c4["learned_lookup_ms"], "binary_lookup_ms": c4["binary_lookup_ms"],
+                 "speedup_lookup": c4["speedup_lookup"], "speedup_join": c4["speedup_join"]}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -858,6 +881,8 @@ def main(argv=None) -> int:
                     help="threads of the additional N-core CPU baseline (0/1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="c2: skip the batched C5 stream the default line carries")
+    ap.add_argument("--no-others", action="store_true",
+                    help="c2 with C5: skip the short GPU-only runs of C1, C3 and C4 the default line carries")
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
     ap.add_argument("--copy-all", action="store_true", help="copy: print every sweep point in the JSON line")
     a = ap.parse_args(argv)
@@ -875,6 +900,8 @@ def main(argv=None) -> int:
                           "p99_ms": c5["poisson"]["batched"]["p99_ms"],
                           "per_join_p99_ms": c5["poisson"]["per_join"]["p99_ms"],
                           "roofline": c5["roofline"], "graph": c5["graph"]}
+        if not a.no_others:
+            res["other_configs"] = other_configs(a)
         print(json.dumps(res), flush=True)
         return 0
     fn = {"c1": run_c1, "c2": run_c2, "c3-nlj": run_c3_nlj, "c3-hidx": run_c3_hidx, "c4": run_c4, "c5": run_c5,
