@@ -844,6 +844,9 @@ def other_configs(a):
     out["c4"] = {"workload": "C4 sorted index 2^26 lognormal keys, 2^27 probes",
                  "learned_lookup_ms": c4["learned_lookup_ms"], "binary_lookup_ms": c4["binary_lookup_ms"],
                  "speedup_lookup": c4["speedup_lookup"], "speedup_join": c4["speedup_join"]}
+    dist = run_nlj_dist(b)
+    out["nlj_dist"] = {"workload": "distance NLJ 2^17 x 2^17 3-D points (FP64)", "ms_per_join": dist["ms_per_step"],
+                       "pairs_per_s": dist["value"], "fp64_frac": dist["roofline"]["frac"]}
     return out
 
 
@@ -882,7 +885,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="c2: skip the batched C5 stream the default line carries")
     ap.add_argument("--no-others", action="store_true",
-                    help="c2 with C5: skip the short GPU-only runs of C1, C3 and C4 the default line carries")
+                    help="c2 with C5: skip the short GPU-only runs of C1, C3, C4 and the distance NLJ the default line carries")
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
     ap.add_argument("--copy-all", action="store_true", help="copy: print every sweep point in the JSON line")
     a = ap.parse_args(argv)
