@@ -475,9 +475,19 @@ __global__ __launch_bounds__(kFastNT) void k_tile_scatter(const uint2 *__restric
   // RES: the tile's XCD range x owns regions [x 2^BITS, (x + 1) 2^BITS) of
   // the ends (offsets) and cursors (res)
   const uint32_t rx = RES ? t / (T / kXcds) : 0u;
+  // RES: once the error is raised (the layout found the sampled regions too
+  // large for their buffers, or a region overflowed) the caller redoes the
+  // join exactly, so the tiles still to come move nothing.  Read once per
+  // workgroup beside the cursors, so the decision is the workgroup's.
+  __shared__ uint32_t s_err;
+  if (RES && threadIdx.x == 0) s_err = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
     sh.cursor[d] = RES ? offsets[((uint64_t)rx << BITS) + d] : offsets[(uint64_t)d * T + t];
     sh.cnt[d] = 0;
+  }
+  if constexpr (RES) {
+    __syncthreads();
+    if (s_err) return;
   }
   tile::scatter_stripe<BITS, ORDERED, RES>(sh, in, beg, end, n, shift, 0u, out, err,
                                            RES ? res + ((uint64_t)rx << BITS) : nullptr);
@@ -662,7 +672,10 @@ hipError_t launch_msd2(uint64_t n, const Msd2Geo &geo, uint64_t group_len, uint3
 // line, and the final partitions are [part_start[p], part_end[p]) with gaps
 // between them.
 using tile::gap_cap;
+using tile::gap_kappa;
+using tile::line_same_counts;
 using tile::gap_line;
+using tile::kGapGroup;
 using tile::kGapLinesPerTile;
 using tile::kGapLineTuples;
 using tile::kGapPad;
@@ -680,17 +693,25 @@ constexpr uint32_t kGapRanges = kXcds;
 __device__ __forceinline__ uint32_t gap_range(uint32_t t, uint32_t Tg) { return t / (Tg / kGapRanges); }
 
 // Sampled counts, one row per workgroup (tiles b, b + grid, ...: every
-// workgroup samples the whole relation's spread): 2^TB combined-digit bins,
-// then the high-digit counts of each XCD range, [range][high digit].
+// workgroup samples the whole relation's spread) of gap_cols columns: 2^TB
+// combined-digit bins; the high-digit counts of each XCD range, [range][high
+// digit]; the same regions' sums of same-digit counts (for each sampled tuple,
+// the tuples of its line with its high digit, itself included: the clumping
+// that gap_kappa reads); and the one sum of same-digit counts of the combined
+// digit over all sampled tuples.
+__host__ __device__ constexpr uint32_t gap_cols(uint32_t FT, uint32_t F1) { return FT + 2 * kGapRanges * F1 + 1; }
+
 template <int TB, int BH>
 __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__restrict__ in4, uint64_t npairs,
                                                              uint32_t T, uint32_t Tg, uint32_t *__restrict__ rows) {
   constexpr uint32_t FT = 1u << TB, F1 = 1u << BH, BL = TB - BH;
-  constexpr uint32_t COLS = FT + kGapRanges * F1;
+  constexpr uint32_t COLS = gap_cols(FT, F1), QX = FT + kGapRanges * F1;
+  static_assert(FT <= 0x8000u, "digits packed two to a word, 0xffff free");
   __shared__ uint32_t hist[COLS];
   for (uint32_t b = threadIdx.x; b < COLS; b += kGapSampleNT) hist[b] = 0;
   __syncthreads();
   const uint32_t j = threadIdx.x >> 3, q = threadIdx.x & 7;
+  uint32_t qb = 0;  // same-digit counts of the combined digit
   for (uint32_t t0 = blockIdx.x; t0 < T; t0 += gridDim.x * kGapSampleUnroll) {
     uint4 v[kGapSampleUnroll];
     bool ok[kGapSampleUnroll];
@@ -703,14 +724,25 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
     }
 #pragma unroll
     for (int u = 0; u < kGapSampleUnroll; ++u) {
+      const uint32_t b0 = v[u].x & (FT - 1), b1 = v[u].z & (FT - 1);
+      const uint32_t h0 = (v[u].x >> BL) & (F1 - 1), h1 = (v[u].z >> BL) & (F1 - 1);
+      // the line's 8 lanes compare their digits (all lanes take part)
+      uint32_t cb0, cb1, ch0, ch1;
+      line_same_counts(ok[u] ? b0 | b1 << 16 : ~0u, cb0, cb1);
+      line_same_counts(ok[u] ? h0 | h1 << 16 : ~0u, ch0, ch1);
       if (!ok[u]) continue;
-      const uint32_t hx = FT + gap_range(t0 + (uint32_t)u * gridDim.x, Tg) * F1;
-      atomicAdd(&hist[v[u].x & (FT - 1)], 1u);
-      atomicAdd(&hist[v[u].z & (FT - 1)], 1u);
-      atomicAdd(&hist[hx + ((v[u].x >> BL) & (F1 - 1))], 1u);
-      atomicAdd(&hist[hx + ((v[u].z >> BL) & (F1 - 1))], 1u);
+      const uint32_t x = gap_range(t0 + (uint32_t)u * gridDim.x, Tg) * F1;
+      atomicAdd(&hist[b0], 1u);
+      atomicAdd(&hist[b1], 1u);
+      atomicAdd(&hist[FT + x + h0], 1u);
+      atomicAdd(&hist[FT + x + h1], 1u);
+      atomicAdd(&hist[QX + x + h0], ch0);
+      atomicAdd(&hist[QX + x + h1], ch1);
+      qb += cb0 + cb1;
     }
   }
+  qb = wave_sum(qb);
+  if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(&hist[COLS - 1], qb);
   __syncthreads();
   uint32_t *row = rows + (uint64_t)blockIdx.x * COLS;
   for (uint32_t b = threadIdx.x; b < COLS; b += kGapSampleNT) row[b] = hist[b];
@@ -746,20 +778,24 @@ __global__ __launch_bounds__(256) void k_gap_reduce(const uint32_t *__restrict__
 // Final partition b: [part_start[b], re2[b]), cursor cur2[b] = part_start[b].
 // First-pass region of (high digit h, range x), index k = x F1 + h: [rs1[k],
 // re1[k]), cursor cur1[k], laid out in (h, x) order so that the second pass
-// reads each digit's regions in a row.  The host sized the buffers (budget1,
-// budget2 tuples) for any sample; a total beyond them (or an empty sample)
-// raises kErrCapacity, and every region is clipped to its buffer.
+// reads each digit's regions in a row.  Each estimate's margin is scaled by
+// its clumping (gap_kappa): per region for the first pass (a region is one XCD
+// range, a contiguous part of the input that may be sorted where the rest is
+// not), one for the whole relation for the final partitions.  The host sized
+// the buffers (budget1, budget2 tuples) for any sample of unclumped input; a
+// total beyond them (clumped input, or an empty sample) raises kErrCapacity
+// before any tuple moves, and every region is clipped to its buffer.
 template <int BH, int BL>
 __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__restrict__ hs, uint64_t n,
-                                                              uint64_t budget1, uint64_t budget2,
-                                                              uint32_t *__restrict__ rs1, uint32_t *__restrict__ re1,
-                                                              uint32_t *__restrict__ cur1,
+                                                              uint32_t T, uint32_t Tg, uint64_t budget1,
+                                                              uint64_t budget2, uint32_t *__restrict__ rs1,
+                                                              uint32_t *__restrict__ re1, uint32_t *__restrict__ cur1,
                                                               uint32_t *__restrict__ part_start,
                                                               uint32_t *__restrict__ re2,
                                                               uint32_t *__restrict__ cur2, uint32_t *err) {
   constexpr uint32_t FT = 1u << (BH + BL), F1 = 1u << BH;
   constexpr uint32_t BPT = FT >= kGapLayoutNT ? FT / kGapLayoutNT : 1;  // consecutive bins per thread
-  constexpr uint32_t NSEG = F1 * kGapRanges;
+  constexpr uint32_t NSEG = F1 * kGapRanges, QX = FT + NSEG, QB = gap_cols(FT, F1) - 1;
   static_assert(NSEG <= kGapLayoutNT, "one first-pass region per thread");
   __shared__ uint64_t red[kGapLayoutNT / kWave + 1];
   const uint32_t b0 = threadIdx.x * BPT;
@@ -778,17 +814,34 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
 #pragma unroll
     for (uint32_t k = 0; k < BPT; ++k) sv[k] = b0 + k < FT ? hs[b0 + k] : 0u;
   }
+  uint64_t sq = 0;  // sum of squared counts (< ns^2 <= 2^56)
 #pragma unroll
-  for (uint32_t k = 0; k < BPT; ++k) loc += sv[k];
+  for (uint32_t k = 0; k < BPT; ++k) {
+    loc += sv[k];
+    sq += (uint64_t)sv[k] * sv[k];
+  }
   const uint64_t ns = block_sum<kGapLayoutNT, uint64_t>(loc, red);
+  const uint64_t ns2 = block_sum<kGapLayoutNT, uint64_t>(sq, red);
   const double f = ns ? (double)n / (double)ns : 0.0;
   if (ns == 0 && threadIdx.x == 0) atomic_or_agent(err, kErrCapacity);
-  // final partitions
+  // groups of lines the sample draws from: 64 per tile, and in the last
+  // (partial) tile those of its tail tuples (a group counted with no tuple
+  // only makes gap_kappa's correction smaller)
+  const uint64_t tail = n - (uint64_t)(T - 1) * kFastTile;
+  const uint32_t glast = (uint32_t)((tail + kGapLineTuples * kGapGroup - 1) / (kGapLineTuples * kGapGroup));
+  // final partitions: one kappa for all of them, gap_kappa's with q and s^2 /
+  // g summed over the bins (every bin draws on all the groups)
+  float kb = 1.0f;
+  if (ns) {
+    const double g = (double)(T - 1) * kGapLinesPerTile + glast;
+    const double k = ((double)hs[QB] - (double)ns2 / g) / (double)ns - 1.0;
+    kb = k > 1.0 ? (float)k : 1.0f;
+  }
   uint64_t c2[BPT];
   uint64_t sum2 = 0;
 #pragma unroll
   for (uint32_t k = 0; k < BPT; ++k) {
-    c2[k] = (ns && b0 + k < FT) ? gap_cap(sv[k], f) : 0;
+    c2[k] = (ns && b0 + k < FT) ? gap_cap(sv[k], kb, f) : 0;
     sum2 += c2[k];
   }
   uint64_t tot2;
@@ -823,7 +876,14 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
   // first-pass regions: thread i = h * 8 + x
   const uint32_t i = threadIdx.x, h = i / kGapRanges, x = i % kGapRanges;
   uint64_t c1 = 0;
-  if (i < NSEG && ns) c1 = gap_cap(hs[FT + x * F1 + h], f);
+  if (i < NSEG && ns) {
+    // range x holds tiles [x Tg/8, (x + 1) Tg/8) of the T
+    const uint32_t per = Tg / kGapRanges, lo = x * per;
+    const uint32_t tiles = T > lo ? min(per, T - lo) : 0u;
+    const uint32_t g = tiles * kGapLinesPerTile - ((T - 1) / per == x ? kGapLinesPerTile - glast : 0u);
+    const uint32_t s = hs[FT + x * F1 + h];
+    c1 = gap_cap(s, gap_kappa(s, hs[QX + x * F1 + h], (double)g), f);
+  }
   uint64_t tot1;
   const uint64_t a1 = block_excl_scan<kGapLayoutNT, uint64_t>(c1, red, &tot1);
   if (tot1 > budget1 && threadIdx.x == 0) atomic_or_agent(err, kErrCapacity);
@@ -955,7 +1015,7 @@ GapLayout gap_layout(uint64_t n, int bh, int bl) {
   const uint64_t FT = 1ull << (bh + bl), NSEG = (1ull << bh) * kGapRanges;
   const uint64_t T = (n + kFastTile - 1) / kFastTile;
   L.G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(T, kGapSampleWgs));
-  L.cols = (uint32_t)(FT + NSEG);
+  L.cols = gap_cols((uint32_t)FT, (uint32_t)(1u << bh));
   auto al = [](size_t b) { return (b + 255) / 256 * 256; };
   L.o_rows = 0;
   L.o_hs = L.o_rows + al((size_t)L.G * L.cols * 4);
@@ -991,7 +1051,7 @@ hipError_t launch_gap(uint64_t n, uint64_t group_len, uint32_t workers, const eo
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_gap_reduce, dim3((L.cols + 63) / 64), dim3(256), 0, stream, rows, L.G, L.cols, hs);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n,
+  hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n, T, Tg,
                      gap_bound(n, NSEG), gap_bound(n, FT), rs1, re1, cur1, part_start, re2, cur2, err);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   EO_TIMED("part_scatter", stream,

@@ -551,6 +551,36 @@ constexpr int kGapSampleUnroll = 8;                 // tiles in flight per workg
 constexpr double kGapZ = 6.0;                       // margin in standard deviations
 constexpr uint32_t kGapPad = 16;                    // plus a fixed margin, in tuples
 
+// v from lane (lane ^ R) of the same 32-lane half (R < 8: the 8 lanes of one
+// sampled line), by ds_swizzle's bit-mask mode (and 0x1f, or 0, xor R)
+template <int R>
+__device__ __forceinline__ uint32_t swizzle_xor(uint32_t v) {
+  static_assert(R > 0 && R < 32, "xor within 32 lanes");
+  return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1f | (R << 10));
+}
+
+// For the two digits a and b (< 2^16) a lane holds of a 16-tuple line spread
+// over 8 lanes (two tuples each), packed p = a | b << 16 (0xffffffff for a
+// lane holding none): ca and cb are how many of the line's tuples have digit a
+// and b.  Every lane of the wave executes it.
+__device__ __forceinline__ void line_same_counts(uint32_t p, uint32_t &ca, uint32_t &cb) {
+  const uint32_t a = p & 0xffffu, b = p >> 16;
+  ca = 1u + (a == b);
+  cb = ca;
+  auto add = [&](uint32_t o) {
+    const uint32_t oa = o & 0xffffu, ob = o >> 16;
+    ca += (uint32_t)(a == oa) + (uint32_t)(a == ob);
+    cb += (uint32_t)(b == oa) + (uint32_t)(b == ob);
+  };
+  add(swizzle_xor<1>(p));
+  add(swizzle_xor<2>(p));
+  add(swizzle_xor<3>(p));
+  add(swizzle_xor<4>(p));
+  add(swizzle_xor<5>(p));
+  add(swizzle_xor<6>(p));
+  add(swizzle_xor<7>(p));
+}
+
 __host__ __device__ __forceinline__ uint32_t gap_line(uint32_t t, uint32_t j) {
   uint32_t h = (t * kGapLinesPerTile + j) * 0x9e3779b1u;
   h ^= h >> 15;
@@ -568,12 +598,40 @@ __host__ __device__ __forceinline__ uint32_t gap_line(uint32_t t, uint32_t j) {
 // less than a Poisson draw) — plus kGapPad, rounded up to a whole line.
 // (w in single precision, raised by 2^-20 to cover its rounding; the
 // product in double)
-__device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) {
-  const float mf = (float)s + 1.0f;
+//
+// kappa >= 1 is the sampled count's variance over its mean.  The sample takes
+// whole lines, so a line whose tuples share a digit (sorted or grouped input)
+// adds to that digit's count in steps of more than one: clumps of about kappa
+// tuples, of which s / kappa were seen, stand in for the Poisson observation
+// and the bound is kappa times theirs.  kappa = 1 is the bound above.
+__device__ __forceinline__ uint64_t gap_cap(uint32_t s, float kappa, double f) {
+  const float mf = (float)s / kappa + 1.0f;
   const float wf = (1.0f - 1.0f / (9.0f * mf) + (float)kGapZ / (3.0f * sqrtf(mf))) * (1.0f + 0x1p-20f);
-  const double m = (double)s + 1.0, w = (double)wf;
-  const uint64_t k = (uint64_t)ceil(f * m * w * w * w) + kGapPad;
+  const double m = (double)s / (double)kappa + 1.0, w = (double)wf;
+  const uint64_t k = (uint64_t)ceil(f * (double)kappa * m * w * w * w) + kGapPad;
   return (k + kGapLineTuples - 1) / kGapLineTuples * kGapLineTuples;
+}
+__device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) { return gap_cap(s, 1.0f, f); }
+
+// kappa of a sampled count s from q, the sum over its sampled tuples of how
+// many tuples of their line have the same digit, and g, the number of lines
+// the sample could have taken the count from.  With one line taken from each
+// group of kGapGroup, the count's variance is the sum over the groups of
+// E[c^2] - E[c]^2 over the group's lines (c: the line's tuples of the digit);
+// q estimates the sum of E[c^2], and s^2 / g is at most the sum of E[c]^2
+// (Cauchy-Schwarz over the g groups), so k = (q - s^2 / g) / s bounds the
+// ratio from above in expectation.  Independent tuples give about 1 - 2^-bits
+// (less where a digit fills much of a line), every line alike (all tuples of
+// one digit) gives 0: no sampling error; sorted input gives about 16.  k is
+// itself an estimate (about +-0.25 for a 1-bit digit over 600 lines, and the
+// buffers hold no more than the unclumped margins), so kappa = k - 1, never
+// below 1: no estimate of independent tuples inflates a margin, and a real
+// variance ratio K keeps a margin of z sqrt((K - 1) / K) >= z / sqrt(2)
+// standard deviations (K >= 2), z / sqrt(K) >= z / sqrt(2) below.
+__device__ __forceinline__ float gap_kappa(uint32_t s, uint32_t q, double g) {
+  if (s == 0 || g <= 0.0) return 1.0f;
+  const double ds = (double)s, k = ((double)q - ds * ds / g) / ds - 1.0;
+  return k > 1.0 ? (float)k : 1.0f;
 }
 
 

@@ -107,3 +107,69 @@ def test_one_side_sampled_the_other_exact(ctx, nR, nS):
     want = oracle.hash_join(R, S, nparts=1 << 12)
     assert len(got) == len(want) == join_count_closed_form(R, S)
     assert same_multiset(got, want)
+
+
+@pytest.mark.parametrize("n,sampled", [(1 << 20, 1), ((1 << 20) - 1, 0)])
+def test_sampled_threshold(ctx, n, sampled):
+    """Relations of 2^20 tuples take sampled partitions, one tuple fewer the
+    exact ones (both relations at the size, two passes forced)."""
+    R, S = gen.c2(seed=n, nR=n, nS=n)
+    got, st = _join(ctx, R, S, radix_bits=14, passes=2)
+    assert st.sampled == sampled
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == n
+    assert same_multiset(got, want)
+
+
+def test_sampled_single_hot_key(ctx):
+    """Every probe key equal: one partition holds the whole of S (every line
+    alike, so the sample has no error there), the others are empty; every S
+    tuple pairs with the one R tuple of that key."""
+    nR, nS = 1 << 20, (1 << 21) + 3
+    keys = np.random.default_rng(0x5E).permutation(nR).astype(np.int32)
+    R = np.stack([keys, np.arange(nR, dtype=np.int32)], 1)
+    S = np.stack([np.full(nS, 777_777, dtype=np.int32), np.arange(nS, dtype=np.int32)], 1)
+    got, st = _join(ctx, R, S, radix_bits=14, passes=2)
+    assert st.sampled == 1
+    assert len(got) == nS
+    assert np.array_equal(np.sort(got[:, 1]), np.arange(nS, dtype=np.int32))
+    assert (got[:, 0] == int(np.flatnonzero(keys == 777_777)[0])).all()
+
+
+@pytest.mark.parametrize("layout", ["sorted", "descending", "half_sorted", "sorted_dups"])
+def test_clumped_input_takes_the_exact_partitions(ctx, layout):
+    """Sorted keys put whole 128-byte lines into one high digit: the sample,
+    which reads whole lines, sees each digit in steps of 16 tuples.  The
+    layout measures that clumping and, finding margins too wide for the
+    buffers, hands over to the exact partition before any tuple moves; the
+    rows are the oracle's."""
+    rng = np.random.default_rng(0x5F)
+    nR, nS = (1 << 20) + 5, (1 << 21) + 11
+    keys = np.arange(nR, dtype=np.int32)
+    if layout == "descending":
+        keys = keys[::-1].copy()
+    elif layout == "half_sorted":  # the second half of the tiles in random order
+        keys[nR // 2:] = rng.permutation(keys[nR // 2:])
+    R = np.stack([keys, rng.integers(0, 1 << 30, nR, dtype=np.int32)], 1)
+    skeys = rng.integers(0, nR, nS).astype(np.int32)
+    if layout == "sorted_dups":  # S grouped by key too: each R key about twice in a row
+        skeys.sort()
+    S = np.stack([skeys, np.arange(nS, dtype=np.int32)], 1)
+    got, st = _join(ctx, R, S, radix_bits=14, passes=2)
+    assert st.sampled == 2
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == nS
+    assert same_multiset(got, want)
+
+
+def test_unclumped_input_keeps_sampled_partitions(ctx):
+    """The same sizes in random order: no clumping, sampled partitions."""
+    rng = np.random.default_rng(0x60)
+    nR, nS = (1 << 20) + 5, (1 << 21) + 11
+    R = np.stack([rng.permutation(nR).astype(np.int32), np.arange(nR, dtype=np.int32)], 1)
+    S = np.stack([rng.integers(0, nR, nS).astype(np.int32), np.arange(nS, dtype=np.int32)], 1)
+    got, st = _join(ctx, R, S, radix_bits=14, passes=2)
+    assert st.sampled == 1
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == nS
+    assert same_multiset(got, want)
