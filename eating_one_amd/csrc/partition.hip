@@ -478,19 +478,17 @@ __global__ __launch_bounds__(kFastNT) void k_tile_scatter(const uint2 *__restric
   // RES: once the error is raised (the layout found the sampled regions too
   // large for their buffers, or a region overflowed) the caller redoes the
   // join exactly, so the tiles still to come move nothing.  Read once per
-  // workgroup beside the cursors, so the decision is the workgroup's.
+  // workgroup beside the cursors; scatter_stripe tests it after its first
+  // barrier (the tile's loads already issued), so the decision is the
+  // workgroup's and costs no extra round trip.
   __shared__ uint32_t s_err;
   if (RES && threadIdx.x == 0) s_err = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
     sh.cursor[d] = RES ? offsets[((uint64_t)rx << BITS) + d] : offsets[(uint64_t)d * T + t];
     sh.cnt[d] = 0;
   }
-  if constexpr (RES) {
-    __syncthreads();
-    if (s_err) return;
-  }
   tile::scatter_stripe<BITS, ORDERED, RES>(sh, in, beg, end, n, shift, 0u, out, err,
-                                           RES ? res + ((uint64_t)rx << BITS) : nullptr);
+                                           RES ? res + ((uint64_t)rx << BITS) : nullptr, RES ? &s_err : nullptr);
 }
 
 // counts2[b * G + g] = sum over k of partial[(g * K + k) * FT + b]: a tiled
@@ -675,6 +673,7 @@ using tile::gap_cap;
 using tile::gap_kappa;
 using tile::line_same_counts;
 using tile::gap_line;
+using tile::kGapClumpEvery;
 using tile::kGapGroup;
 using tile::kGapLinesPerTile;
 using tile::kGapLineTuples;
@@ -695,11 +694,16 @@ __device__ __forceinline__ uint32_t gap_range(uint32_t t, uint32_t Tg) { return 
 // Sampled counts, one row per workgroup (tiles b, b + grid, ...: every
 // workgroup samples the whole relation's spread) of gap_cols columns: 2^TB
 // combined-digit bins; the high-digit counts of each XCD range, [range][high
-// digit]; the same regions' sums of same-digit counts (for each sampled tuple,
-// the tuples of its line with its high digit, itself included: the clumping
-// that gap_kappa reads); and the one sum of same-digit counts of the combined
-// digit over all sampled tuples.
-__host__ __device__ constexpr uint32_t gap_cols(uint32_t FT, uint32_t F1) { return FT + 2 * kGapRanges * F1 + 1; }
+// digit]; per XCD range the sum of same-digit counts of the high digit (for
+// each measured tuple, the tuples of its line with its high digit, itself
+// included: the clumping that gap_kappa reads) and the number of lines
+// measured; and the one sum of same-digit counts of the combined digit over
+// all measured tuples.  The clumping is measured on one sampled line in
+// kGapClumpEvery: wave w of a workgroup measures the lines it holds of the
+// round's u = w-th tile, so each wave measures one tile per round.
+__host__ __device__ constexpr uint32_t gap_cols(uint32_t FT, uint32_t F1) {
+  return FT + kGapRanges * F1 + 2 * kGapRanges + 1;
+}
 
 template <int TB, int BH>
 __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__restrict__ in4, uint64_t npairs,
@@ -707,6 +711,8 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
   constexpr uint32_t FT = 1u << TB, F1 = 1u << BH, BL = TB - BH;
   constexpr uint32_t COLS = gap_cols(FT, F1), QX = FT + kGapRanges * F1;
   static_assert(FT <= 0x8000u, "digits packed two to a word, 0xffff free");
+  static_assert(kGapSampleNT / kWave == kGapSampleUnroll && kGapClumpEvery == kGapSampleUnroll,
+                "one wave per tile of a round measures the clumping");
   __shared__ uint32_t hist[COLS];
   for (uint32_t b = threadIdx.x; b < COLS; b += kGapSampleNT) hist[b] = 0;
   __syncthreads();
@@ -725,20 +731,26 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
 #pragma unroll
     for (int u = 0; u < kGapSampleUnroll; ++u) {
       const uint32_t b0 = v[u].x & (FT - 1), b1 = v[u].z & (FT - 1);
-      const uint32_t h0 = (v[u].x >> BL) & (F1 - 1), h1 = (v[u].z >> BL) & (F1 - 1);
+      const uint32_t h0 = b0 >> BL, h1 = b1 >> BL;  // the high digit: the combined one's top BH bits
       // the line's 8 lanes compare their digits (all lanes take part)
-      uint32_t cb0, cb1, ch0, ch1;
-      line_same_counts(ok[u] ? b0 | b1 << 16 : ~0u, cb0, cb1);
-      line_same_counts(ok[u] ? h0 | h1 << 16 : ~0u, ch0, ch1);
+      const uint32_t t = t0 + (uint32_t)u * gridDim.x;
+      if (u == (int)(threadIdx.x / kWave)) {  // wave-uniform: this wave's tile of the round
+        uint32_t cb0, cb1, ch0, ch1;
+        line_same_counts<BL>(ok[u] ? b0 | b1 << 16 : ~0u, cb0, cb1, ch0, ch1);
+        const uint32_t qh = wave_sum(ok[u] ? ch0 + ch1 : 0u);  // the tile's range is the wave's
+        const uint32_t lines = (uint32_t)__popcll(__ballot(ok[u] && q == 0));  // a line's first pair is in
+        if (t < T && (threadIdx.x & (kWave - 1)) == 0) {
+          atomicAdd(&hist[QX + gap_range(t, Tg)], qh);
+          atomicAdd(&hist[QX + kGapRanges + gap_range(t, Tg)], lines);
+        }
+        if (ok[u]) qb += cb0 + cb1;
+      }
       if (!ok[u]) continue;
-      const uint32_t x = gap_range(t0 + (uint32_t)u * gridDim.x, Tg) * F1;
+      const uint32_t x = gap_range(t, Tg) * F1;
       atomicAdd(&hist[b0], 1u);
       atomicAdd(&hist[b1], 1u);
       atomicAdd(&hist[FT + x + h0], 1u);
       atomicAdd(&hist[FT + x + h1], 1u);
-      atomicAdd(&hist[QX + x + h0], ch0);
-      atomicAdd(&hist[QX + x + h1], ch1);
-      qb += cb0 + cb1;
     }
   }
   qb = wave_sum(qb);
@@ -779,9 +791,9 @@ __global__ __launch_bounds__(256) void k_gap_reduce(const uint32_t *__restrict__
 // First-pass region of (high digit h, range x), index k = x F1 + h: [rs1[k],
 // re1[k]), cursor cur1[k], laid out in (h, x) order so that the second pass
 // reads each digit's regions in a row.  Each estimate's margin is scaled by
-// its clumping (gap_kappa): per region for the first pass (a region is one XCD
-// range, a contiguous part of the input that may be sorted where the rest is
-// not), one for the whole relation for the final partitions.  The host sized
+// its clumping (gap_kappa): per XCD range for the first pass (a contiguous
+// part of the input, which may be sorted where the rest is not), one for the
+// whole relation for the final partitions.  The host sized
 // the buffers (budget1, budget2 tuples) for any sample of unclumped input; a
 // total beyond them (clumped input, or an empty sample) raises kErrCapacity
 // before any tuple moves, and every region is clipped to its buffer.
@@ -795,7 +807,7 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
                                                               uint32_t *__restrict__ cur2, uint32_t *err) {
   constexpr uint32_t FT = 1u << (BH + BL), F1 = 1u << BH;
   constexpr uint32_t BPT = FT >= kGapLayoutNT ? FT / kGapLayoutNT : 1;  // consecutive bins per thread
-  constexpr uint32_t NSEG = F1 * kGapRanges, QX = FT + NSEG, QB = gap_cols(FT, F1) - 1;
+  constexpr uint32_t NSEG = F1 * kGapRanges, QX = FT + NSEG, QM = QX + kGapRanges, QB = gap_cols(FT, F1) - 1;
   static_assert(NSEG <= kGapLayoutNT, "one first-pass region per thread");
   __shared__ uint64_t red[kGapLayoutNT / kWave + 1];
   const uint32_t b0 = threadIdx.x * BPT;
@@ -831,12 +843,14 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
   const uint32_t glast = (uint32_t)((tail + kGapLineTuples * kGapGroup - 1) / (kGapLineTuples * kGapGroup));
   // final partitions: one kappa for all of them, gap_kappa's with q and s^2 /
   // g summed over the bins (every bin draws on all the groups)
-  float kb = 1.0f;
-  if (ns) {
-    const double g = (double)(T - 1) * kGapLinesPerTile + glast;
-    const double k = ((double)hs[QB] - (double)ns2 / g) / (double)ns - 1.0;
-    kb = k > 1.0 ? (float)k : 1.0f;
-  }
+  // the clumping sums were measured on some of the lines: scaled by lines
+  // sampled over lines measured (the ranges' line counts of the one in
+  // kGapClumpEvery)
+  const double gall = (double)(T - 1) * kGapLinesPerTile + glast;
+  uint32_t mall = 0;
+#pragma unroll
+  for (uint32_t r = 0; r < kGapRanges; ++r) mall += hs[QM + r];
+  const float kb = mall ? gap_kappa((double)ns, (double)ns2, (double)hs[QB] * gall / mall, gall) : 1.0f;
   uint64_t c2[BPT];
   uint64_t sum2 = 0;
 #pragma unroll
@@ -875,14 +889,33 @@ __global__ __launch_bounds__(kGapLayoutNT) void k_gap_layout(const uint32_t *__r
   if (threadIdx.x == 0) part_start[FT] = (uint32_t)min(tot2, budget2);
   // first-pass regions: thread i = h * 8 + x
   const uint32_t i = threadIdx.x, h = i / kGapRanges, x = i % kGapRanges;
+  const uint32_t s1 = i < NSEG ? hs[FT + x * F1 + h] : 0u;
+  // per range x, the sum of its regions' counts and of their squares: the
+  // lanes of one x are 8 apart in a wave, so xor-shuffles by 8, 16 and 32
+  // leave lane x with the wave's sums, added to LDS once per range per wave
+  __shared__ unsigned long long xs[kGapRanges], xs2[kGapRanges];
+  if (threadIdx.x < kGapRanges) xs[threadIdx.x] = xs2[threadIdx.x] = 0;
+  __syncthreads();
+  unsigned long long a = s1, a2 = (unsigned long long)s1 * s1;  // < 2^56 each, < 2^63 over 128 digits
+#pragma unroll
+  for (int m = kGapRanges; m < kWave; m <<= 1) {
+    a += __shfl_xor(a, m);
+    a2 += __shfl_xor(a2, m);
+  }
+  if ((threadIdx.x & (kWave - 1)) < kGapRanges && (threadIdx.x & ~(kWave - 1)) < NSEG) {
+    atomicAdd(&xs[threadIdx.x & (kWave - 1)], a);
+    atomicAdd(&xs2[threadIdx.x & (kWave - 1)], a2);
+  }
+  __syncthreads();
   uint64_t c1 = 0;
   if (i < NSEG && ns) {
     // range x holds tiles [x Tg/8, (x + 1) Tg/8) of the T
     const uint32_t per = Tg / kGapRanges, lo = x * per;
     const uint32_t tiles = T > lo ? min(per, T - lo) : 0u;
     const uint32_t g = tiles * kGapLinesPerTile - ((T - 1) / per == x ? kGapLinesPerTile - glast : 0u);
-    const uint32_t s = hs[FT + x * F1 + h];
-    c1 = gap_cap(s, gap_kappa(s, hs[QX + x * F1 + h], (double)g), f);
+    const uint32_t m = hs[QM + x];
+    const float kx = m ? gap_kappa((double)xs[x], (double)xs2[x], (double)hs[QX + x] * g / m, (double)g) : 1.0f;
+    c1 = gap_cap(s1, kx, f);
   }
   uint64_t tot1;
   const uint64_t a1 = block_excl_scan<kGapLayoutNT, uint64_t>(c1, red, &tot1);

@@ -327,7 +327,8 @@ __device__ __forceinline__ uint32_t &scatter_dirty_flag() {
 template <int BITS, bool ORDERED = false, bool RES = false>
 __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh, const uint2 *__restrict__ in, uint64_t sbeg,
                                                uint64_t send, uint64_t nbound, int shift, uint32_t xm,
-                                               uint2 *__restrict__ out, uint32_t *err, uint32_t *res = nullptr) {
+                                               uint2 *__restrict__ out, uint32_t *err, uint32_t *res = nullptr,
+                                               const uint32_t *abort_flag = nullptr) {
   static_assert(!(RES && ORDERED), "reserved runs land in atomic order");
   constexpr int F = 1 << BITS;
   constexpr int DPL = (F + kWave - 1) / kWave;  // digits per lane of wave 0
@@ -383,6 +384,9 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
   uint4 pre[kPre > 0 ? kPre : 1];  // the next tile's first loads, in flight while this tile is ranked and staged
   load_part(0, cur, 0, kFastPairs);
   __syncthreads();
+  // abort_flag: an LDS word the caller wrote before this barrier; nonzero
+  // moves nothing (every thread reads the same word after the barrier)
+  if (abort_flag && *abort_flag) return;
 
   for (uint64_t t0 = 0; t0 < npairs; t0 += (uint64_t)kFastNT * kFastPairs) {
     const uint64_t tnext = t0 + (uint64_t)kFastNT * kFastPairs;
@@ -548,6 +552,7 @@ constexpr uint32_t kGapGroup = 16;                                             /
 constexpr uint32_t kGapLinesPerTile = kFastTile / kGapLineTuples / kGapGroup;  // 64
 constexpr int kGapSampleNT = kGapLinesPerTile * 8;  // 8 threads (16-byte pairs) per line
 constexpr int kGapSampleUnroll = 8;                 // tiles in flight per workgroup
+constexpr int kGapClumpEvery = 8;                   // the clumping sums count one sampled line in 8
 constexpr double kGapZ = 6.0;                       // margin in standard deviations
 constexpr uint32_t kGapPad = 16;                    // plus a fixed margin, in tuples
 
@@ -559,18 +564,26 @@ __device__ __forceinline__ uint32_t swizzle_xor(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x1f | (R << 10));
 }
 
-// For the two digits a and b (< 2^16) a lane holds of a 16-tuple line spread
+// For the two digits a and b (< 2^15) a lane holds of a 16-tuple line spread
 // over 8 lanes (two tuples each), packed p = a | b << 16 (0xffffffff for a
 // lane holding none): ca and cb are how many of the line's tuples have digit a
-// and b.  Every lane of the wave executes it.
-__device__ __forceinline__ void line_same_counts(uint32_t p, uint32_t &ca, uint32_t &cb) {
-  const uint32_t a = p & 0xffffu, b = p >> 16;
+// and b, and ha and hb how many have its high part (the digit >> LO; 0xffff
+// >> LO is no high part of a digit under 2^15).  Every lane of the wave
+// executes it.
+template <int LO>
+__device__ __forceinline__ void line_same_counts(uint32_t p, uint32_t &ca, uint32_t &cb, uint32_t &ha,
+                                                 uint32_t &hb) {
+  const uint32_t a = p & 0xffffu, b = p >> 16, a1 = a >> LO, b1 = b >> LO;
   ca = 1u + (a == b);
   cb = ca;
+  ha = 1u + (a1 == b1);
+  hb = ha;
   auto add = [&](uint32_t o) {
-    const uint32_t oa = o & 0xffffu, ob = o >> 16;
+    const uint32_t oa = o & 0xffffu, ob = o >> 16, oa1 = oa >> LO, ob1 = ob >> LO;
     ca += (uint32_t)(a == oa) + (uint32_t)(a == ob);
     cb += (uint32_t)(b == oa) + (uint32_t)(b == ob);
+    ha += (uint32_t)(a1 == oa1) + (uint32_t)(a1 == ob1);
+    hb += (uint32_t)(b1 == oa1) + (uint32_t)(b1 == ob1);
   };
   add(swizzle_xor<1>(p));
   add(swizzle_xor<2>(p));
@@ -613,24 +626,26 @@ __device__ __forceinline__ uint64_t gap_cap(uint32_t s, float kappa, double f) {
 }
 __device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) { return gap_cap(s, 1.0f, f); }
 
-// kappa of a sampled count s from q, the sum over its sampled tuples of how
+// kappa of sampled counts from q, the sum over their sampled tuples of how
 // many tuples of their line have the same digit, and g, the number of lines
-// the sample could have taken the count from.  With one line taken from each
-// group of kGapGroup, the count's variance is the sum over the groups of
-// E[c^2] - E[c]^2 over the group's lines (c: the line's tuples of the digit);
-// q estimates the sum of E[c^2], and s^2 / g is at most the sum of E[c]^2
+// the sample could have taken each count from.  With one line taken from each
+// group of kGapGroup, a count's variance is the sum over the groups of E[c^2]
+// - E[c]^2 over the group's lines (c: the line's tuples of the digit); q
+// estimates the sum of E[c^2], and s^2 / g is at most the sum of E[c]^2
 // (Cauchy-Schwarz over the g groups), so k = (q - s^2 / g) / s bounds the
-// ratio from above in expectation.  Independent tuples give about 1 - 2^-bits
-// (less where a digit fills much of a line), every line alike (all tuples of
-// one digit) gives 0: no sampling error; sorted input gives about 16.  k is
+// variance over the mean from above in expectation; pooled over several
+// digits (s = sum of the counts, s2 = sum of their squares) it is their
+// count-weighted mean.  Independent tuples give about 1 - 2^-bits (less
+// where a digit fills much of a line), every line alike (all tuples of one
+// digit) gives 0: no sampling error; sorted input gives about 16.  k is
 // itself an estimate (about +-0.25 for a 1-bit digit over 600 lines, and the
 // buffers hold no more than the unclumped margins), so kappa = k - 1, never
 // below 1: no estimate of independent tuples inflates a margin, and a real
 // variance ratio K keeps a margin of z sqrt((K - 1) / K) >= z / sqrt(2)
 // standard deviations (K >= 2), z / sqrt(K) >= z / sqrt(2) below.
-__device__ __forceinline__ float gap_kappa(uint32_t s, uint32_t q, double g) {
-  if (s == 0 || g <= 0.0) return 1.0f;
-  const double ds = (double)s, k = ((double)q - ds * ds / g) / ds - 1.0;
+__device__ __forceinline__ float gap_kappa(double s, double s2, double q, double g) {
+  if (s <= 0.0 || g <= 0.0) return 1.0f;
+  const double k = (q - s2 / g) / s - 1.0;
   return k > 1.0 ? (float)k : 1.0f;
 }
 
