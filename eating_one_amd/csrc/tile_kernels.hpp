@@ -399,12 +399,38 @@ __device__ __forceinline__ void scatter_stripe(ScatterShared<BITS, ORDERED> &sh,
         if (t0 + (uint64_t)j * kFastNT + threadIdx.x == nfull) cur[j] = make_uint4(last.x, last.y, 0u, 0u);
     }
     uint32_t rk[2 * kFastPairs];
-    if constexpr (!ORDERED) {
+    if constexpr (!ORDERED && RES) {
 #pragma unroll
       for (int j = 0; j < kFastPairs; ++j) {
         const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
         if (e >= tskip && e < tn) rk[2 * j] = atomicAdd(&sh.cnt[digit_of(cur[j].x, xm, shift, F - 1)], 1u);
         if (e + 1 < tn) rk[2 * j + 1] = atomicAdd(&sh.cnt[digit_of(cur[j].z, xm, shift, F - 1)], 1u);
+      }
+      __syncthreads();
+    } else if constexpr (!ORDERED) {
+      // as above, but a wave whose valid tuples all share one digit (sorted
+      // input: 128 consecutive keys) ranks them with one atomic instead of
+      // 128 on one word.  (The reserving scatter above serves sampled
+      // partitions, which hand sorted input to this one.)
+      const uint64_t lt = lanemask_lt();
+#pragma unroll
+      for (int j = 0; j < kFastPairs; ++j) {
+        const uint32_t e = 2u * (j * kFastNT + threadIdx.x);
+        const bool v0 = e >= tskip && e < tn, v1 = e + 1 < tn;
+        const uint32_t d0 = digit_of(cur[j].x, xm, shift, F - 1), d1 = digit_of(cur[j].z, xm, shift, F - 1);
+        const uint32_t dr = __builtin_amdgcn_readfirstlane(d0);
+        if (__ballot((v0 && d0 != dr) || (v1 && d1 != dr)) == 0) {  // wave-uniform
+          const uint64_t m0 = __ballot(v0), m1 = __ballot(v1);
+          const uint32_t n0 = (uint32_t)__popcll(m0), nv = n0 + (uint32_t)__popcll(m1);
+          uint32_t base = 0;
+          if (lane_id() == 0 && nv) base = atomicAdd(&sh.cnt[dr], nv);
+          base = __builtin_amdgcn_readfirstlane(base);
+          rk[2 * j] = base + (uint32_t)__popcll(m0 & lt);
+          rk[2 * j + 1] = base + n0 + (uint32_t)__popcll(m1 & lt);
+        } else {
+          if (v0) rk[2 * j] = atomicAdd(&sh.cnt[d0], 1u);
+          if (v1) rk[2 * j + 1] = atomicAdd(&sh.cnt[d1], 1u);
+        }
       }
       __syncthreads();
     } else {

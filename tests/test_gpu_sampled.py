@@ -173,3 +173,32 @@ def test_unclumped_input_keeps_sampled_partitions(ctx):
     want = oracle.hash_join(R, S, nparts=1 << 12)
     assert len(got) == len(want) == nS
     assert same_multiset(got, want)
+
+
+@pytest.mark.parametrize("layout", ["sorted", "one_key", "runs_of_64"])
+@pytest.mark.parametrize("bits,passes", [(6, 1), (11, 1), (14, 2)])
+def test_exact_partitions_on_uniform_waves(ctx, layout, bits, passes):
+    """Exact partitions of inputs where whole waves hold one digit: sorted
+    keys (the two-pass first digit is the key's high part), one probe key
+    for every S tuple, S in runs of 64 equal keys.  The histogram and the
+    scatter rank such a wave with one atomic; sizes off the tile grid and
+    odd, so partial waves and the odd last tuple take part.  Rows equal the
+    oracle's."""
+    rng = np.random.default_rng(0x61 + bits)
+    nR, nS = (1 << 19) + 333, (1 << 20) + 4097
+    if layout == "sorted":
+        rk = np.arange(nR, dtype=np.int32)
+        sk = np.sort(rng.integers(0, nR, nS)).astype(np.int32)
+    elif layout == "one_key":
+        rk = np.arange(nR, dtype=np.int32)
+        sk = np.full(nS, 4242, dtype=np.int32)
+    else:  # unique R keys in random order, S in runs of 64 equal keys
+        rk = rng.permutation(nR).astype(np.int32)
+        sk = np.repeat(rng.integers(0, nR, nS // 64 + 1).astype(np.int32), 64)[:nS]
+    R = np.stack([rk, np.arange(nR, dtype=np.int32)], 1)
+    S = np.stack([sk, np.arange(nS, dtype=np.int32)], 1)
+    got, st = _join(ctx, R, S, radix_bits=bits, passes=passes, flags=_abi.EO_F_EXACT_PARTITIONS)
+    assert st.sampled == 0 and st.passes == passes
+    want = oracle.hash_join(R, S, nparts=1 << 12)
+    assert len(got) == len(want) == join_count_closed_form(R, S)
+    assert same_multiset(got, want)
