@@ -715,7 +715,8 @@ __device__ __forceinline__ uint32_t gap_range(uint32_t t, uint32_t Tg) { return 
 // measured; and the one sum of same-digit counts of the combined digit over
 // all measured tuples.  The clumping is measured on one sampled line in
 // kGapClumpEvery: wave w of a workgroup measures the lines it holds of the
-// round's u = w-th tile, so each wave measures one tile per round.
+// u = w-th of the kGapSampleUnroll tiles the workgroup loads at a time, so
+// each wave measures one tile in kGapSampleUnroll.
 __host__ __device__ constexpr uint32_t gap_cols(uint32_t FT, uint32_t F1) {
   return FT + kGapRanges * F1 + 2 * kGapRanges + 1;
 }
@@ -727,7 +728,7 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
   constexpr uint32_t COLS = gap_cols(FT, F1), QX = FT + kGapRanges * F1;
   static_assert(FT <= 0x8000u, "digits packed two to a word, 0xffff free");
   static_assert(kGapSampleNT / kWave == kGapSampleUnroll && kGapClumpEvery == kGapSampleUnroll,
-                "one wave per tile of a round measures the clumping");
+                "one wave per tile of an unrolled step measures the clumping");
   __shared__ uint32_t hist[COLS];
   for (uint32_t b = threadIdx.x; b < COLS; b += kGapSampleNT) hist[b] = 0;
   __syncthreads();
@@ -749,7 +750,7 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
       const uint32_t h0 = b0 >> BL, h1 = b1 >> BL;  // the high digit: the combined one's top BH bits
       // the line's 8 lanes compare their digits (all lanes take part)
       const uint32_t t = t0 + (uint32_t)u * gridDim.x;
-      if (u == (int)(threadIdx.x / kWave)) {  // wave-uniform: this wave's tile of the round
+      if (u == (int)(threadIdx.x / kWave)) {  // wave-uniform: this wave's tile of the step
         uint32_t cb0, cb1, ch0, ch1;
         line_same_counts<BL>(ok[u] ? b0 | b1 << 16 : ~0u, cb0, cb1, ch0, ch1);
         const uint32_t qh = wave_sum(ok[u] ? ch0 + ch1 : 0u);  // the tile's range is the wave's
