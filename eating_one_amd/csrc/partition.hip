@@ -690,6 +690,7 @@ using tile::line_same_counts;
 using tile::gap_line;
 using tile::kGapClumpEvery;
 using tile::kGapGroup;
+using tile::kGapKappaMax;
 using tile::kGapLinesPerTile;
 using tile::kGapLineTuples;
 using tile::kGapPad;
@@ -1039,18 +1040,22 @@ __global__ void k_gap_ends(const uint32_t *__restrict__ cur2, const uint32_t *__
 }
 
 // Tuples a relation of n needs in a buffer of P sampled regions (the first
-// pass's 8 2^bh, the final 2^(bh+bl)) for any sample.  With m = s + 1, w <= 1
-// + z/(3 sqrt(m)) gives m w^3 <= m + z sqrt(m) + z^2/3 + z^3/27, so the
-// capacities sum to at most f (ns + P + z sum(sqrt(m_b)) + P (z^2/3 +
-// z^3/27)) + P (pad + line + 1), with sum(sqrt(m_b)) <= sqrt(P (ns + P))
-// (Cauchy-Schwarz) and f ns = n; f = n / ns falls as the sample grows, so the
-// smallest sample (full tiles only) bounds it.
-uint64_t gap_bound(uint64_t n, uint64_t P) {
+// pass's 8 2^bh, the final 2^(bh+bl)) for any sample whose clumping ratios
+// are at most K.  A region of sampled count s and ratio kappa gets f kappa m w^3
+// with m = s / kappa + 1, and w <= 1 + z/(3 sqrt(m)) gives m w^3 <= m + z
+// sqrt(m) + z^2/3 + z^3/27, so its capacity is at most f (s + kappa + z
+// sqrt(kappa (s + kappa)) + kappa (z^2/3 + z^3/27)) + pad + line; over the P
+// regions, with kappa <= K and sum(sqrt(s_b + K)) <= sqrt(P (ns + K P))
+// (Cauchy-Schwarz), at most f (ns + K P (1 + z^2/3 + z^3/27) + z sqrt(K P (ns
+// + K P))) + P (pad + line + 1), and f ns = n; f = n / ns falls as the sample
+// grows, so the smallest sample (full tiles only) bounds it.  A layout whose
+// ratios pass K may not fit: it raises kErrCapacity before any tuple moves.
+uint64_t gap_bound(uint64_t n, uint64_t P, double K) {
   const uint64_t full = n / kFastTile;
   const double ns = (double)full * kGapLinesPerTile * kGapLineTuples;
   if (ns <= 0.0) return ~0ull;
   const double f = (double)n / ns, z = kGapZ, p = (double)P;
-  const double extra = f * (p * (1.0 + z * z / 3.0 + z * z * z / 27.0) + z * std::sqrt(p * (ns + p)));
+  const double extra = f * (K * p * (1.0 + z * z / 3.0 + z * z * z / 27.0) + z * std::sqrt(K * p * (ns + K * p)));
   return n + (uint64_t)std::ceil(extra) + P * (kGapPad + kGapLineTuples + 2);
 }
 
@@ -1101,7 +1106,8 @@ hipError_t launch_gap(uint64_t n, uint64_t group_len, uint32_t workers, const eo
   hipLaunchKernelGGL(k_gap_reduce, dim3((L.cols + 63) / 64), dim3(256), 0, stream, rows, L.G, L.cols, hs);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n, T, Tg,
-                     gap_bound(n, NSEG), gap_bound(n, FT), rs1, re1, cur1, part_start, re2, cur2, err);
+                     gap_bound(n, NSEG, kGapKappaMax), gap_bound(n, FT, kGapKappaMax), rs1, re1, cur1, part_start, re2,
+                     cur2, err);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   EO_TIMED("part_scatter", stream,
            hipLaunchKernelGGL((k_tile_scatter<BH, false, true>), dim3(Tg), dim3(kFastNT), 0, stream,
@@ -1287,17 +1293,20 @@ bool gap_supported(uint64_t n, int bits_hi, int bits_lo) {
   if (!msd2_supported(bits_hi, bits_lo) || bits_hi + bits_lo < 2 || n < kGapMinRows) return false;
   // the 32-bit region cursors start below the buffer size and grow by at
   // most n, so they never wrap
-  return gap_bound(n, 1ull << (bits_hi + bits_lo)) + n < (1ull << 32) &&
-         gap_bound(n, (1ull << bits_hi) * kGapRanges) + n < (1ull << 32);
+  return gap_bound(n, 1ull << (bits_hi + bits_lo), kGapKappaMax) + n < (1ull << 32) &&
+         gap_bound(n, (1ull << bits_hi) * kGapRanges, kGapKappaMax) + n < (1ull << 32);
 }
 
-uint64_t gap_rows(uint64_t n, uint64_t parts) { return gap_bound(n, parts); }
+// batch groups size their regions without a clumping ratio (kappa 1)
+uint64_t gap_rows(uint64_t n, uint64_t parts) { return gap_bound(n, parts, 1.0); }
 
 uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int /*bits_lo*/) {
-  return gap_bound(n, (1ull << bits_hi) * kGapRanges);
+  return gap_bound(n, (1ull << bits_hi) * kGapRanges, kGapKappaMax);
 }
 
-uint64_t gap_out_rows(uint64_t n, int bits_hi, int bits_lo) { return gap_bound(n, 1ull << (bits_hi + bits_lo)); }
+uint64_t gap_out_rows(uint64_t n, int bits_hi, int bits_lo) {
+  return gap_bound(n, 1ull << (bits_hi + bits_lo), kGapKappaMax);
+}
 
 size_t gap_scratch_bytes(uint64_t n, int bits_hi, int bits_lo) { return gap_layout(n, bits_hi, bits_lo).total; }
 

@@ -579,6 +579,8 @@ constexpr uint32_t kGapLinesPerTile = kFastTile / kGapLineTuples / kGapGroup;  /
 constexpr int kGapSampleNT = kGapLinesPerTile * 8;  // 8 threads (16-byte pairs) per line
 constexpr int kGapSampleUnroll = 8;                 // tiles in flight per workgroup
 constexpr int kGapClumpEvery = 8;                   // the clumping sums count one sampled line in 8
+constexpr double kGapKappaMax = 2.0;                // clumping the single-join buffers are sized for
+constexpr double kGapKappaNoise = 0.25;             // taken off a measured clumping ratio (its noise)
 constexpr double kGapZ = 6.0;                       // margin in standard deviations
 constexpr uint32_t kGapPad = 16;                    // plus a fixed margin, in tuples
 
@@ -663,15 +665,15 @@ __device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) { return gap_c
 // digits (s = sum of the counts, s2 = sum of their squares) it is their
 // count-weighted mean.  Independent tuples give about 1 - 2^-bits (less
 // where a digit fills much of a line), every line alike (all tuples of one
-// digit) gives 0: no sampling error; sorted input gives about 16.  k is
-// itself an estimate (about +-0.25 for a 1-bit digit over 600 lines, and the
-// buffers hold no more than the unclumped margins), so kappa = k - 1, never
-// below 1: no estimate of independent tuples inflates a margin, and a real
-// variance ratio K keeps a margin of z sqrt((K - 1) / K) >= z / sqrt(2)
-// standard deviations (K >= 2), z / sqrt(K) >= z / sqrt(2) below.
+// digit) gives 0: no sampling error; sorted input gives about 16, keys in
+// adjacent pairs about 2.  k is itself an estimate (pooled over a range's
+// digits, random keys stay within 1.03 at 2^20 tuples), so kappa = k -
+// kGapKappaNoise, never below 1: no estimate of independent tuples inflates a
+// margin, and a real ratio K keeps a margin of at least z sqrt((K - 0.25) /
+// K) standard deviations (5.6 at K = 2).
 __device__ __forceinline__ float gap_kappa(double s, double s2, double q, double g) {
   if (s <= 0.0 || g <= 0.0) return 1.0f;
-  const double k = (q - s2 / g) / s - 1.0;
+  const double k = (q - s2 / g) / s - kGapKappaNoise;
   return k > 1.0 ? (float)k : 1.0f;
 }
 

@@ -136,7 +136,7 @@ def test_sampled_single_hot_key(ctx):
     assert (got[:, 0] == int(np.flatnonzero(keys == 777_777)[0])).all()
 
 
-@pytest.mark.parametrize("layout", ["sorted", "descending", "half_sorted", "sorted_dups"])
+@pytest.mark.parametrize("layout", ["sorted", "descending", "half_sorted", "sorted_dups", "runs_of_4"])
 def test_clumped_input_takes_the_exact_partitions(ctx, layout):
     """Sorted keys put whole 128-byte lines into one high digit: the sample,
     which reads whole lines, sees each digit in steps of 16 tuples.  The
@@ -154,6 +154,10 @@ def test_clumped_input_takes_the_exact_partitions(ctx, layout):
     skeys = rng.integers(0, nR, nS).astype(np.int32)
     if layout == "sorted_dups":  # S grouped by key too: each R key about twice in a row
         skeys.sort()
+    elif layout == "runs_of_4":  # R in random order, S keys in runs of four (variance ratio ~4)
+        keys[:] = rng.permutation(nR)
+        R[:, 0] = keys
+        skeys = np.repeat(skeys[: nS // 4 + 1], 4)[:nS]
     S = np.stack([skeys, np.arange(nS, dtype=np.int32)], 1)
     got, st = _join(ctx, R, S, radix_bits=14, passes=2)
     assert st.sampled == 2
@@ -162,12 +166,18 @@ def test_clumped_input_takes_the_exact_partitions(ctx, layout):
     assert same_multiset(got, want)
 
 
-def test_unclumped_input_keeps_sampled_partitions(ctx):
-    """The same sizes in random order: no clumping, sampled partitions."""
+@pytest.mark.parametrize("layout", ["random", "adjacent_pairs"])
+def test_unclumped_input_keeps_sampled_partitions(ctx, layout):
+    """The same sizes in random order: no clumping, or S keys in adjacent
+    pairs (variance ratio ~2, within what the buffers are sized for, so the
+    margins widen and no redo follows): sampled partitions."""
     rng = np.random.default_rng(0x60)
     nR, nS = (1 << 20) + 5, (1 << 21) + 11
     R = np.stack([rng.permutation(nR).astype(np.int32), np.arange(nR, dtype=np.int32)], 1)
-    S = np.stack([rng.integers(0, nR, nS).astype(np.int32), np.arange(nS, dtype=np.int32)], 1)
+    skeys = rng.integers(0, nR, nS).astype(np.int32)
+    if layout == "adjacent_pairs":
+        skeys = np.repeat(skeys[: nS // 2 + 1], 2)[:nS]
+    S = np.stack([skeys, np.arange(nS, dtype=np.int32)], 1)
     got, st = _join(ctx, R, S, radix_bits=14, passes=2)
     assert st.sampled == 1
     want = oracle.hash_join(R, S, nparts=1 << 12)
