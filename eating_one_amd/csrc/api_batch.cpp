@@ -278,7 +278,7 @@ eo_status plan_batch(eo_batch *b, const eo_request *reqs, uint32_t n, const eo_j
     G.g.total_stripes = (uint32_t)stripes;
     G.g.ncounts = counts;
     G.g.total_samp = (uint32_t)samp;
-    if (sampled) G.o_hs = L.add(mem.size() * (sizeof(uint32_t) << bits));
+    if (sampled) G.o_hs = L.add(mem.size() * sizeof(uint32_t) * batch_hs_words(bits));
     max_counts = std::max(max_counts, counts);
     b->groups.push_back(G);
   }
@@ -391,7 +391,7 @@ void fill_blob(eo_batch *b, const std::vector<std::vector<std::pair<uint32_t, in
         const uint64_t unit = kPartTile * kBatchSampleTiles;
         sg.samp_base = samp_base;
         sg.nsamp = (uint32_t)((sd.n + unit - 1) / unit);
-        sg.hs = hs_all + ((uint64_t)k << bits);
+        sg.hs = hs_all + (uint64_t)k * batch_hs_words(bits);
         sg.re = reinterpret_cast<uint32_t *>(A + sd.o_re);
         sg.cur = reinterpret_cast<uint32_t *>(A + sd.o_cur);
         sg.part_end = reinterpret_cast<uint32_t *>(A + sd.o_end);

@@ -128,7 +128,10 @@ __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__rest
 // Workgroup u samples kBatchSampleTiles tiles of one relation (the tile's
 // sampled lines, as the single-join sampled partition reads them) into an LDS
 // histogram and adds it to the relation's counts hs with one atomic per
-// non-empty digit.
+// non-empty digit.  It also measures the lines' clumping as the single-join
+// sample does: wave w takes the w-th of every kGapSampleUnroll tiles and sums,
+// for each of its tuples, the tuples of its line with the same digit, added
+// with the number of lines measured to hs[2^bits] and hs[2^bits + 1].
 __device__ __forceinline__ uint32_t find_samp(const BatchSeg *segs, uint32_t nseg, uint32_t u) {
   uint32_t lo = 0, hi = nseg;  // segs[lo].samp_base <= u < segs[hi].samp_base
   while (hi - lo > 1) {
@@ -154,6 +157,8 @@ __global__ __launch_bounds__(tile::kGapSampleNT) void k_bsamp(const BatchSeg *__
   const uint64_t npairs = sg.n / 2;
   const uint4 *in4 = reinterpret_cast<const uint4 *>(sg.in);
   const uint32_t j = threadIdx.x >> 3, q = threadIdx.x & 7;
+  static_assert(NT / kWave == U && F <= 0x8000u, "one wave per unrolled tile; digits packed two to a word");
+  uint32_t qsum = 0, lines = 0;  // clumping sum (per lane), lines measured (per wave)
   for (uint32_t tb = t0; tb < t1; tb += U) {
     uint4 v[U];
     bool ok[U];
@@ -167,20 +172,33 @@ __global__ __launch_bounds__(tile::kGapSampleNT) void k_bsamp(const BatchSeg *__
     }
 #pragma unroll
     for (int k = 0; k < U; ++k) {
+      const uint32_t d0 = v[k].x & (F - 1), d1 = v[k].z & (F - 1);
+      if (k == (int)(threadIdx.x / kWave)) {  // wave-uniform: this wave's tile of the step
+        uint32_t c0, c1;
+        tile::line_same_counts(ok[k] ? d0 | d1 << 16 : ~0u, c0, c1);
+        if (ok[k]) qsum += c0 + c1;
+        lines += (uint32_t)__popcll(__ballot(ok[k] && q == 0));  // a line's first pair is in
+      }
       if (!ok[k]) continue;
-      atomicAdd(&hist[v[k].x & (F - 1)], 1u);
-      atomicAdd(&hist[v[k].z & (F - 1)], 1u);
+      atomicAdd(&hist[d0], 1u);
+      atomicAdd(&hist[d1], 1u);
     }
   }
+  qsum = wave_sum(qsum);
   __syncthreads();
   for (uint32_t d = threadIdx.x; d < F; d += NT)
     if (hist[d]) atomic_add_agent(sg.hs + d, hist[d]);
+  if ((threadIdx.x & (kWave - 1)) == 0 && lines) {
+    atomic_add_agent(sg.hs + F, qsum);
+    atomic_add_agent(sg.hs + F + 1, lines);
+  }
 }
 
 // One workgroup per relation: region d = [part_start[d], re[d]) sized by
-// gap_cap from the sampled counts, cursor cur[d] = part_start[d]; a total
-// beyond the relation's buffer (or an empty sample) raises kErrCapacity and
-// every region is clipped to the buffer.
+// gap_cap from the sampled counts and the relation's clumping ratio
+// (gap_kappa), cursor cur[d] = part_start[d]; a total beyond the relation's
+// buffer (clumped input, or an empty sample) raises kErrCapacity and every
+// region is clipped to the buffer.
 template <int BITS>
 __global__ __launch_bounds__(1024) void k_blayout(const BatchSeg *__restrict__ segs, uint32_t *err) {
   constexpr uint32_t F = 1u << BITS, NT = 1024, BPT = F > NT ? F / NT : 1;
@@ -188,19 +206,29 @@ __global__ __launch_bounds__(1024) void k_blayout(const BatchSeg *__restrict__ s
   const BatchSeg sg = load_seg(segs, blockIdx.x);
   const uint32_t b0 = threadIdx.x * BPT;
   uint32_t sv[BPT];
-  uint64_t loc = 0;
+  uint64_t loc = 0, sq = 0;
 #pragma unroll
   for (uint32_t k = 0; k < BPT; ++k) {
     sv[k] = b0 + k < F ? sg.hs[b0 + k] : 0u;
     loc += sv[k];
+    sq += (uint64_t)sv[k] * sv[k];
   }
   const uint64_t ns = block_sum<NT, uint64_t>(loc, red);
+  const uint64_t ns2 = block_sum<NT, uint64_t>(sq, red);
   const double f = ns ? (double)sg.n / (double)ns : 0.0;
+  // groups of lines the sample draws from: 64 per tile, those of the tail
+  // tuples in the last; the clumping sum scaled from the lines measured
+  const uint64_t T = (sg.n + tile::kFastTile - 1) / tile::kFastTile;
+  const uint64_t tail = sg.n - (T - 1) * tile::kFastTile;
+  const double g = (double)(T - 1) * tile::kGapLinesPerTile +
+                   (double)((tail + tile::kGapLineTuples * tile::kGapGroup - 1) / (tile::kGapLineTuples * tile::kGapGroup));
+  const uint32_t qs = sg.hs[F], m = sg.hs[F + 1];
+  const float kap = (ns && m) ? tile::gap_kappa((double)ns, (double)ns2, (double)qs * g / m, g) : 1.0f;
   uint64_t c[BPT];
   uint64_t sum = 0;
 #pragma unroll
   for (uint32_t k = 0; k < BPT; ++k) {
-    c[k] = (ns && b0 + k < F) ? tile::gap_cap(sv[k], f) : 0;
+    c[k] = (ns && b0 + k < F) ? tile::gap_cap(sv[k], kap, f) : 0;
     sum += c[k];
   }
   uint64_t tot;
@@ -385,7 +413,7 @@ hipError_t launch_group(const BatchGroup &g, uint64_t *counts, void *scan_scratc
   const uint32_t grid = (g.total_stripes + kXcds - 1) / kXcds * kXcds;
   if (g.sampled) {
     // sample -> region layout -> scatter with reserved runs -> partition ends
-    hipError_t e = fill_u32(g.hs_all, 0u, (uint64_t)g.nseg << BITS, stream);
+    hipError_t e = fill_u32(g.hs_all, 0u, (uint64_t)g.nseg * batch_hs_words(BITS), stream);
     if (e != hipSuccess) return e;
     EO_TIMED("batch_sample", stream,
              hipLaunchKernelGGL(k_bsamp<BITS>, dim3(g.total_samp), dim3(tile::kGapSampleNT), 0, stream, g.segs,

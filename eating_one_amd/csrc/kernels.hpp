@@ -231,7 +231,8 @@ struct BatchSeg {
   // sampled groups: partitions sized from a 1/16 sample (gaps between them)
   uint32_t samp_base;        // first sample workgroup of this relation in the group
   uint32_t nsamp;
-  uint32_t *hs;              // 2^bits sampled counts (zero on entry)
+  uint32_t *hs;              // batch_hs_words(bits): 2^bits sampled counts, then the clumping sum and the
+                             // lines it was measured on (zero on entry)
   uint32_t *re, *cur;        // 2^bits region ends and cursors
   uint32_t *part_end;        // 2^bits partition ends (output)
   uint64_t budget;           // tuples of out
@@ -251,6 +252,9 @@ struct BatchGroup {
 // tiles one sample workgroup of a sampled batch relation covers.
 constexpr uint64_t kPartTile = 16384;
 constexpr uint32_t kBatchSampleTiles = 16;
+// words of a sampled batch relation's counts: the 2^bits digits, the sum of
+// same-digit line counts and the number of lines it was measured on
+inline constexpr uint32_t batch_hs_words(int bits) { return (1u << bits) + 2u; }
 // One join of a batch, as the persistent probe sees it.
 struct BatchJoin {
   const uint2 *R, *S;        // partitioned (or raw when P == 1)

@@ -1297,8 +1297,7 @@ bool gap_supported(uint64_t n, int bits_hi, int bits_lo) {
          gap_bound(n, (1ull << bits_hi) * kGapRanges, kGapKappaMax) + n < (1ull << 32);
 }
 
-// batch groups size their regions without a clumping ratio (kappa 1)
-uint64_t gap_rows(uint64_t n, uint64_t parts) { return gap_bound(n, parts, 1.0); }
+uint64_t gap_rows(uint64_t n, uint64_t parts) { return gap_bound(n, parts, kGapKappaMax); }
 
 uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int /*bits_lo*/) {
   return gap_bound(n, (1ull << bits_hi) * kGapRanges, kGapKappaMax);

@@ -595,9 +595,26 @@ __device__ __forceinline__ uint32_t swizzle_xor(uint32_t v) {
 // For the two digits a and b (< 2^15) a lane holds of a 16-tuple line spread
 // over 8 lanes (two tuples each), packed p = a | b << 16 (0xffffffff for a
 // lane holding none): ca and cb are how many of the line's tuples have digit a
-// and b, and ha and hb how many have its high part (the digit >> LO; 0xffff
-// >> LO is no high part of a digit under 2^15).  Every lane of the wave
-// executes it.
+// and b, and (second form) ha and hb how many have its high part (the digit
+// >> LO; 0xffff >> LO is no high part of a digit under 2^15).  Every lane of
+// the wave executes it.
+__device__ __forceinline__ void line_same_counts(uint32_t p, uint32_t &ca, uint32_t &cb) {
+  const uint32_t a = p & 0xffffu, b = p >> 16;
+  ca = 1u + (a == b);
+  cb = ca;
+  auto add = [&](uint32_t o) {
+    const uint32_t oa = o & 0xffffu, ob = o >> 16;
+    ca += (uint32_t)(a == oa) + (uint32_t)(a == ob);
+    cb += (uint32_t)(b == oa) + (uint32_t)(b == ob);
+  };
+  add(swizzle_xor<1>(p));
+  add(swizzle_xor<2>(p));
+  add(swizzle_xor<3>(p));
+  add(swizzle_xor<4>(p));
+  add(swizzle_xor<5>(p));
+  add(swizzle_xor<6>(p));
+  add(swizzle_xor<7>(p));
+}
 template <int LO>
 __device__ __forceinline__ void line_same_counts(uint32_t p, uint32_t &ca, uint32_t &cb, uint32_t &ha,
                                                  uint32_t &hb) {
@@ -652,7 +669,6 @@ __device__ __forceinline__ uint64_t gap_cap(uint32_t s, float kappa, double f) {
   const uint64_t k = (uint64_t)ceil(f * (double)kappa * m * w * w * w) + kGapPad;
   return (k + kGapLineTuples - 1) / kGapLineTuples * kGapLineTuples;
 }
-__device__ __forceinline__ uint64_t gap_cap(uint32_t s, double f) { return gap_cap(s, 1.0f, f); }
 
 // kappa of sampled counts from q, the sum over their sampled tuples of how
 // many tuples of their line have the same digit, and g, the number of lines

@@ -399,3 +399,29 @@ def test_batch_sampled_partition_overflow_redoes_the_batch_exactly(ctx):
         assert s == 0 and n_out == len(r.want), i
         assert same_multiset(r.pairs(n_out), r.want), i
     b.free()
+
+
+@pytest.mark.parametrize("run,sampled", [(2, 1), (8, 2)])
+def test_batch_sampled_clumping(ctx, run, sampled):
+    """A large relation whose S keys come in runs of equal keys: the batch
+    sample measures the clumping.  Pairs (variance ratio ~2) keep sampled
+    partitions with widened margins; runs of 8 exceed what the relation's
+    buffer is sized for, the layout raises the capacity flag and waiting for
+    the batch redoes it exactly.  Every request equals the oracle."""
+    rng = np.random.default_rng(0x70 + run)
+    nR, nS = 1 << 20, (1 << 21) + 37
+    R = np.stack([rng.permutation(nR).astype(np.int32), np.arange(nR, dtype=np.int32)], 1)
+    skeys = np.repeat(rng.integers(0, nR, nS // run + 1).astype(np.int32), run)[:nS]
+    S = np.stack([skeys, np.arange(nS, dtype=np.int32)], 1)
+    inputs = [(R, S)] + _mixed_inputs(0x71 + run, 255, lo=6, hi=10)
+    reqs = [_Req(ctx, r, s) for r, s in inputs]
+    st = eo.JoinStats()
+    b = ctx.batch()
+    try:
+        res = b.run([r.request(tag=i) for i, r in enumerate(reqs)], stats=st)
+        assert st.sampled == sampled
+        for i, (r, (tag, n_out, s)) in enumerate(zip(reqs, res)):
+            assert s == 0 and n_out == len(r.want), i
+            assert same_multiset(r.pairs(n_out), r.want), i
+    finally:
+        b.free()
