@@ -749,9 +749,9 @@ __global__ __launch_bounds__(kGapSampleNT) void k_gap_sample(const uint4 *__rest
     for (int u = 0; u < kGapSampleUnroll; ++u) {
       const uint32_t b0 = v[u].x & (FT - 1), b1 = v[u].z & (FT - 1);
       const uint32_t h0 = b0 >> BL, h1 = b1 >> BL;  // the high digit: the combined one's top BH bits
-      // the line's 8 lanes compare their digits (all lanes take part)
       const uint32_t t = t0 + (uint32_t)u * gridDim.x;
       if (u == (int)(threadIdx.x / kWave)) {  // wave-uniform: this wave's tile of the step
+        // each line's 8 lanes compare their digits (all lanes of the wave take part)
         uint32_t cb0, cb1, ch0, ch1;
         line_same_counts<BL>(ok[u] ? b0 | b1 << 16 : ~0u, cb0, cb1, ch0, ch1);
         const uint32_t qh = wave_sum(ok[u] ? ch0 + ch1 : 0u);  // the tile's range is the wave's
