@@ -209,6 +209,25 @@ def run_c2(a):
     ctx.sync()
     det_dt = (time.perf_counter() - t0) / a.steps
     log(f"deterministic join: {det_dt * 1e3:.3f} ms")
+    # the configuration's other partition plans (one pass at 8 and 11 bits, two
+    # passes at 12 and 13), a few joins each, beside the default plan's `value`
+    plans = []
+    if not a.no_plans and a.log2_r == 27:
+        for bits, passes in ((8, 1), (11, 1), (12, 2), (13, 2)):
+            pkw = dict(kw, radix_bits=bits, passes=passes)
+            pst = _abi.JoinStats()
+            for it in range(4):
+                if it == 1:
+                    ctx.sync()
+                    t0 = time.perf_counter()
+                n, st = ctx.hash_join_device(dR, nR, dS, nS, dout, n_out, stats=pst if it == 0 else None, **pkw)
+                if st != _abi.EO_OK or n != n_out:
+                    raise RuntimeError(f"{bits}-bit {passes}-pass join failed: status {st}, {n} pairs")
+            ctx.sync()
+            pdt = (time.perf_counter() - t0) / 3
+            plans.append({"radix_bits": bits, "passes": passes, "ms_per_join": pdt * 1e3,
+                          "tuples_per_s": (nR + nS) / pdt, "sampled": int(pst.sampled)})
+            log(f"plan {bits} bits, {passes} pass(es): {pdt * 1e3:.3f} ms")
     dom = max(kern.items(), key=lambda x: x[1][0])
     dom_name, (dom_ms, dom_launches) = dom
     dom_mean_ms = dom_ms / dom_launches
@@ -262,6 +281,8 @@ def run_c2(a):
         "deterministic": {"ms_per_join": det_dt * 1e3, "tuples_per_s": (nR + nS) / det_dt,
                           "note": "same rows in the same order on every run; not the headline value"},
     }
+    if plans:
+        res["plans"] = plans
     if a.host_leg:
         # the PCIe-inclusive rate: host R and S in, pairs back to host memory (not `value`)
         hR, hS = keep
@@ -884,6 +905,7 @@ def main(argv=None) -> int:
                     help="threads of the additional N-core CPU baseline (0/1: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="c2: skip the batched C5 stream the default line carries")
+    ap.add_argument("--no-plans", action="store_true", help="c2: skip the other partition plans (8-13 bits)")
     ap.add_argument("--no-others", action="store_true",
                     help="c2 with C5: skip the short GPU-only runs of C1, C3, C4 and the distance NLJ the default line carries")
     ap.add_argument("--host-leg", action="store_true", help="also time the host-buffer (PCIe-inclusive) path")
