@@ -29,7 +29,6 @@ namespace {
 
 constexpr int kMaxGroupBits = kMaxPassBits;  // one fast pass per group
 constexpr uint64_t kMaxTileDescs = 1ull << 21;  // 64 MiB of probe tile descriptors per batch
-constexpr int kBatchSampleMaxBits = 8;          // sampled partitions up to 8 radix bits
 constexpr uint32_t kBatchSampleMinJoins = 256;  // and in batches of at least 256 requests
 
 // Where one relation of one join comes from and goes to.

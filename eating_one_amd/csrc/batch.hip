@@ -104,7 +104,13 @@ __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__rest
   const uint32_t s = g - sg.stripe_base;
   const uint64_t beg = (uint64_t)s * len;
   const uint64_t end = beg + len < sg.n ? beg + len : sg.n;
+  // RES (sampled groups; err is the batch's own capacity flag): once a layout
+  // has found a relation's regions too large for its buffer, or a region
+  // overflowed, the batch is redone exactly, so the stripes still to come
+  // move nothing (read beside the cursors, tested after the first barrier)
+  __shared__ uint32_t s_err;
   if constexpr (RES) {
+    if (threadIdx.x == 0) s_err = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (int d = threadIdx.x; d < (1 << BITS); d += kFastNT) {
       sh.cursor[d] = sg.re[d];
       sh.cnt[d] = 0;
@@ -121,7 +127,8 @@ __global__ __launch_bounds__(kFastNT) void k_bseg_scatter(const BatchSeg *__rest
     }
     if (s == 0 && threadIdx.x == 0) sg.part_start[1 << BITS] = (uint32_t)sg.n;
   }
-  tile::scatter_stripe<BITS, false, RES>(sh, sg.in, beg, end, sg.n, 0, 0u, sg.out, err, RES ? sg.cur : nullptr);
+  tile::scatter_stripe<BITS, false, RES>(sh, sg.in, beg, end, sg.n, 0, 0u, sg.out, err, RES ? sg.cur : nullptr,
+                                         RES ? &s_err : nullptr);
 }
 
 // ---- sampled groups ----------------------------------------------------------------
@@ -413,21 +420,26 @@ hipError_t launch_group(const BatchGroup &g, uint64_t *counts, void *scan_scratc
   const uint32_t grid = (g.total_stripes + kXcds - 1) / kXcds * kXcds;
   if (g.sampled) {
     // sample -> region layout -> scatter with reserved runs -> partition ends
-    hipError_t e = fill_u32(g.hs_all, 0u, (uint64_t)g.nseg * batch_hs_words(BITS), stream);
-    if (e != hipSuccess) return e;
-    EO_TIMED("batch_sample", stream,
-             hipLaunchKernelGGL(k_bsamp<BITS>, dim3(g.total_samp), dim3(tile::kGapSampleNT), 0, stream, g.segs,
-                                g.nseg, g.total_samp));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_blayout<BITS>, dim3(g.nseg), dim3(1024), 0, stream, g.segs, err);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    EO_TIMED("batch_scatter", stream,
-             hipLaunchKernelGGL((k_bseg_scatter<BITS, true>), dim3(grid), dim3(kFastNT), 0, stream, g.segs, g.nseg,
-                                g.total_stripes, g.stripe_len, nullptr, err));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    const uint64_t ends = (uint64_t)g.nseg << BITS;
-    hipLaunchKernelGGL(k_bends<BITS>, dim3((uint32_t)((ends + 255) / 256)), dim3(256), 0, stream, g.segs, g.nseg);
-    return hipGetLastError();
+    // (planned up to kBatchSampleMaxBits; wider kernels are not built)
+    if constexpr (BITS > kBatchSampleMaxBits) {
+      return hipErrorInvalidValue;
+    } else {
+      hipError_t e = fill_u32(g.hs_all, 0u, (uint64_t)g.nseg * batch_hs_words(BITS), stream);
+      if (e != hipSuccess) return e;
+      EO_TIMED("batch_sample", stream,
+               hipLaunchKernelGGL(k_bsamp<BITS>, dim3(g.total_samp), dim3(tile::kGapSampleNT), 0, stream, g.segs,
+                                  g.nseg, g.total_samp));
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      hipLaunchKernelGGL(k_blayout<BITS>, dim3(g.nseg), dim3(1024), 0, stream, g.segs, err);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      EO_TIMED("batch_scatter", stream,
+               hipLaunchKernelGGL((k_bseg_scatter<BITS, true>), dim3(grid), dim3(kFastNT), 0, stream, g.segs, g.nseg,
+                                  g.total_stripes, g.stripe_len, nullptr, err));
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      const uint64_t ends = (uint64_t)g.nseg << BITS;
+      hipLaunchKernelGGL(k_bends<BITS>, dim3((uint32_t)((ends + 255) / 256)), dim3(256), 0, stream, g.segs, g.nseg);
+      return hipGetLastError();
+    }
   }
   EO_TIMED("batch_hist", stream,
            hipLaunchKernelGGL(k_bseg_hist<BITS>, dim3(grid), dim3(kBHistNT), 0, stream, g.segs, g.nseg,

@@ -252,6 +252,7 @@ struct BatchGroup {
 // tiles one sample workgroup of a sampled batch relation covers.
 constexpr uint64_t kPartTile = 16384;
 constexpr uint32_t kBatchSampleTiles = 16;
+constexpr int kBatchSampleMaxBits = 8;  // sampled batch groups: up to 8 radix bits
 // words of a sampled batch relation's counts: the 2^bits digits, the sum of
 // same-digit line counts and the number of lines it was measured on
 inline constexpr uint32_t batch_hs_words(int bits) { return (1u << bits) + 2u; }
