@@ -608,9 +608,11 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
   const uint64_t ublocks = outer ? unmatched_blocks(nS) : 0;  // deterministic outer rows: block offsets
   const size_t o_ub = L.add(ublocks * sizeof(uint64_t));
   const size_t o_ubs = L.add(ublocks ? scan_scratch_bytes_u64(ublocks) : 0);
-  const size_t o_pscr = L.add(std::max({partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus),
-                                        gapR ? gap_scratch_bytes(nR, hp.bits_pass2, hp.bits_pass1) : (size_t)0,
-                                        gapS ? gap_scratch_bytes(nS, hp.bits_pass2, hp.bits_pass1) : (size_t)0}));
+  const size_t o_pscr = L.add(std::max(partition_scratch(nR, hp, c->props.cus), partition_scratch(nS, hp, c->props.cus)));
+  // sampled relations keep their layouts from their prepare step to their
+  // scatters, so each has scratch of its own
+  const size_t o_gr = L.add(gapR ? gap_scratch_bytes(nR, hp.bits_pass2, hp.bits_pass1) : 0);
+  const size_t o_gs = L.add(gapS ? gap_scratch_bytes(nS, hp.bits_pass2, hp.bits_pass1) : 0);
   const size_t o_jscr = L.add(join_plan_scratch_bytes(P));
   const uint64_t tiles_est = (uint64_t)P + nR / hp.lds_cap + nS / hp.s_span + 1024;
   const size_t o_tc = L.add(tiles_est * sizeof(uint64_t) + scan_scratch_bytes_u64(tiles_est));
@@ -655,22 +657,33 @@ eo_status hash_join_impl(eo_ctx *c, const eo_tuple *R, uint64_t nR, const eo_tup
     // stable passes, which a deterministic left outer join takes so that its
     // null rows come in S order within each partition
     const bool stable = deterministic && outer, fixed = deterministic && !outer;
-    auto part = [&](const eo_tuple *in, uint64_t n, bool gap, eo_tuple *dst, uint32_t *start, uint32_t *end,
-                    const uint32_t **end_out) -> eo_status {
-      if (gap && aligned16(in) && aligned16(tmp) && aligned16(dst)) {
-        EO_TRY_HIP(partition_gapped(n, hp.bits_pass2, hp.bits_pass1, in, tmp, dst, start, end, A + o_pscr, c->d_err,
-                                    c->props.cus, s),
-                   "partition (sampled regions)");
-        *end_out = end;
-        used_gap = true;
-        return EO_OK;
-      }
-      return partition_relation(c, in, n, hp, tmp, dst, start, A + o_pscr, s, stable, fixed);
+    const bool gr = gapR && aligned16(dR) && aligned16(tmp) && aligned16(rp);
+    const bool gs = gapS && aligned16(dS) && aligned16(tmp) && aligned16(sp);
+    // sampled relations: both samples and layouts first, so that a layout
+    // that finds its relation clumped stops both relations' scatters
+    auto gap = [&](const eo_tuple *in, uint64_t n, eo_tuple *dst, uint32_t *start, uint32_t *end, size_t o_scr,
+                   unsigned phases) -> hipError_t {
+      return partition_gapped(n, hp.bits_pass2, hp.bits_pass1, in, tmp, dst, start, end, A + o_scr, c->d_err,
+                              c->props.cus, s, phases);
     };
-    st = part(dR, nR, gapR, rp, r_start, (uint32_t *)(A + o_re), &r_end);
-    if (st != EO_OK) return st;
-    st = part(dS, nS, gapS, sp, s_start, (uint32_t *)(A + o_se), &s_end);
-    if (st != EO_OK) return st;
+    uint32_t *r_end_g = (uint32_t *)(A + o_re), *s_end_g = (uint32_t *)(A + o_se);
+    if (gr) EO_TRY_HIP(gap(dR, nR, rp, r_start, r_end_g, o_gr, kGapPrepare), "partition sample (R)");
+    if (gs) EO_TRY_HIP(gap(dS, nS, sp, s_start, s_end_g, o_gs, kGapPrepare), "partition sample (S)");
+    if (gr) {
+      EO_TRY_HIP(gap(dR, nR, rp, r_start, r_end_g, o_gr, kGapScatter), "partition (sampled regions, R)");
+      r_end = r_end_g;
+    } else {
+      st = partition_relation(c, dR, nR, hp, tmp, rp, r_start, A + o_pscr, s, stable, fixed);
+      if (st != EO_OK) return st;
+    }
+    if (gs) {
+      EO_TRY_HIP(gap(dS, nS, sp, s_start, s_end_g, o_gs, kGapScatter), "partition (sampled regions, S)");
+      s_end = s_end_g;
+    } else {
+      st = partition_relation(c, dS, nS, hp, tmp, sp, s_start, A + o_pscr, s, stable, fixed);
+      if (st != EO_OK) return st;
+    }
+    used_gap = gr || gs;
     Rp = rp;
     Sp = sp;
   } else {

@@ -213,9 +213,15 @@ uint64_t gap_rows(uint64_t n, uint64_t parts);
 uint64_t gap_tmp_rows(uint64_t n, int bits_hi, int bits_lo);
 uint64_t gap_out_rows(uint64_t n, int bits_hi, int bits_lo);
 size_t gap_scratch_bytes(uint64_t n, int bits_hi, int bits_lo);
+// phases: kGapPrepare (sample, reduce, region layout: writes part_start and
+// the scratch) then kGapScatter (both passes and the partition ends, from the
+// same scratch), in two calls or one.  Preparing both relations of a join
+// before scattering either lets a layout that finds its relation clumped
+// (kErrCapacity) stop both relations' scatters.
+constexpr unsigned kGapPrepare = 1u, kGapScatter = 2u;
 hipError_t partition_gapped(uint64_t n, int bits_hi, int bits_lo, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
                             uint32_t *part_start, uint32_t *part_end, void *scratch, uint32_t *err, int cus,
-                            hipStream_t stream);
+                            hipStream_t stream, unsigned phases = kGapPrepare | kGapScatter);
 
 // ---- batch.hip --------------------------------------------------------------
 // One relation of one join to partition in a batch group (all of a group's

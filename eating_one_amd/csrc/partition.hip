@@ -1086,7 +1086,7 @@ GapLayout gap_layout(uint64_t n, int bh, int bl) {
 template <int BH, int BL>
 hipError_t launch_gap(uint64_t n, uint64_t group_len, uint32_t workers, const eo_tuple *in, eo_tuple *tmp,
                       eo_tuple *out, uint32_t *part_start, uint32_t *part_end, void *scratch, uint32_t *err,
-                      hipStream_t stream) {
+                      unsigned phases, hipStream_t stream) {
   constexpr uint32_t F1 = 1u << BH, FT = 1u << (BH + BL), NSEG = F1 * kGapRanges;
   const GapLayout L = gap_layout(n, BH, BL);
   char *sc = (char *)scratch;
@@ -1099,16 +1099,19 @@ hipError_t launch_gap(uint64_t n, uint64_t group_len, uint32_t workers, const eo
   // multiples of 8); the blocks past the last tile return at once
   const uint32_t Tg = (T + kXcds - 1) / kXcds * kXcds;
   hipError_t e;
-  EO_TIMED("part_sample", stream,
-           hipLaunchKernelGGL((k_gap_sample<BH + BL, BH>), dim3(L.G), dim3(kGapSampleNT), 0, stream,
-                              reinterpret_cast<const uint4 *>(in), n / 2, T, Tg, rows));
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_gap_reduce, dim3((L.cols + 63) / 64), dim3(256), 0, stream, rows, L.G, L.cols, hs);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n, T, Tg,
-                     gap_bound(n, NSEG, kGapKappaMax), gap_bound(n, FT, kGapKappaMax), rs1, re1, cur1, part_start, re2,
-                     cur2, err);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (phases & kGapPrepare) {
+    EO_TIMED("part_sample", stream,
+             hipLaunchKernelGGL((k_gap_sample<BH + BL, BH>), dim3(L.G), dim3(kGapSampleNT), 0, stream,
+                                reinterpret_cast<const uint4 *>(in), n / 2, T, Tg, rows));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gap_reduce, dim3((L.cols + 63) / 64), dim3(256), 0, stream, rows, L.G, L.cols, hs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gap_layout<BH, BL>), dim3(1), dim3(kGapLayoutNT), 0, stream, hs, n, T, Tg,
+                       gap_bound(n, NSEG, kGapKappaMax), gap_bound(n, FT, kGapKappaMax), rs1, re1, cur1, part_start, re2,
+                       cur2, err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  if (!(phases & kGapScatter)) return hipSuccess;
   EO_TIMED("part_scatter", stream,
            hipLaunchKernelGGL((k_tile_scatter<BH, false, true>), dim3(Tg), dim3(kFastNT), 0, stream,
                               reinterpret_cast<const uint2 *>(in), n, BL, re1, reinterpret_cast<uint2 *>(tmp), err,
@@ -1311,7 +1314,8 @@ size_t gap_scratch_bytes(uint64_t n, int bits_hi, int bits_lo) { return gap_layo
 
 hipError_t partition_gapped(uint64_t n, int bh, int bl, const eo_tuple *in, eo_tuple *tmp, eo_tuple *out,
                             uint32_t *part_start, uint32_t *part_end, void *scratch, uint32_t *err, int cus,
-                            hipStream_t stream) {
+                            hipStream_t stream, unsigned phases) {
+  if (!phases || (phases & ~(kGapPrepare | kGapScatter))) return hipErrorInvalidValue;
   if (!gap_supported(n, bh, bl)) return hipErrorInvalidValue;
   if (((uintptr_t)in & 15) || ((uintptr_t)tmp & 15) || ((uintptr_t)out & 15)) return hipErrorInvalidValue;
   const uint64_t L = gap_group_len(n, cus);
@@ -1319,7 +1323,7 @@ hipError_t partition_gapped(uint64_t n, int bh, int bl, const eo_tuple *in, eo_t
   switch (bh * 16 + bl) {
 #define EO_GAP_CASE(BH, BL) \
   case BH * 16 + BL:        \
-    return launch_gap<BH, BL>(n, L, workers, in, tmp, out, part_start, part_end, scratch, err, stream);
+    return launch_gap<BH, BL>(n, L, workers, in, tmp, out, part_start, part_end, scratch, err, phases, stream);
 #define EO_GAP_ROW(BH) \
   EO_GAP_CASE(BH, 1) EO_GAP_CASE(BH, 2) EO_GAP_CASE(BH, 3) EO_GAP_CASE(BH, 4) EO_GAP_CASE(BH, 5) EO_GAP_CASE(BH, 6) \
   EO_GAP_CASE(BH, 7)
