@@ -679,11 +679,16 @@ hipError_t launch_msd2(uint64_t n, const Msd2Geo &geo, uint64_t group_len, uint3
 // atomic per digit.  The first pass keeps one region per (high digit, XCD):
 // the tiles of one XCD (a contiguous range of tiles) fill their own region of
 // each digit, so each cursor takes an eighth of the atomics and the runs that
-// meet in a cache line come from one L2.  A region whose estimate proves too
-// small takes no tuple past its end and raises kErrCapacity; the caller then
-// redoes the join with the exact partition.  Every region starts on a 128-byte
-// line, and the final partitions are [part_start[p], part_end[p]) with gaps
-// between them.
+// meet in a cache line come from one L2.  The margins widen with the sample's
+// measured line clumping (sorted or grouped keys); a layout too large for the
+// buffers raises kErrCapacity before any tuple moves, and the first-pass
+// scatter then moves nothing.  A region whose estimate proves too small takes
+// no tuple past its end and raises kErrCapacity too; either way the caller
+// redoes the join with the exact partition.  Every region starts on a
+// 128-byte line, and the final partitions are [part_start[p], part_end[p])
+// with gaps between them.  The work runs in two phases (kGapPrepare:
+// sample, reduce, layout; kGapScatter: both passes and the ends) so that a
+// join can lay out both relations before scattering either.
 using tile::gap_cap;
 using tile::gap_kappa;
 using tile::line_same_counts;
